@@ -1,0 +1,245 @@
+#!/usr/bin/env python3
+"""Benchmark: FP64 projection time steps per second on one MI355X.
+
+Workload (one step = 3 RK3 substeps, each: upwind nonlinear + explicit
+Helmholtz + rhs, divergence, 200 JOR sweeps (omega 0.8) + BiCGstab to
+||r||_inf <= 1e-8 ||b||_inf, gradient correction; then IBM mask and pressure
+levelling): grid 257 x 129 x 129 points (256 x 128 x 128 cells), walls, tanh
+stretching factor 1.5 in y and z, a stenosed tube (radius 0.3, 50 % area
+reduction at mid-length) as an immersed-boundary mask, pulsatile inflow,
+Re = 500, dt = 5e-4.  Inputs are synthetic and deterministic; all fields are
+resident in HBM when the timed region starts.
+
+Prints one JSON line (rank 0).  Progress goes to stderr.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "gpu-impact-ct_amd"))
+
+METRIC = "time steps/s + HBM TB/s at 256×128² FP64; mixed-prec speedup; p50/p99 latency"
+HBM_PEAK_GBS = 8000.0
+JOR_BYTES_PER_POINT = 24  # read x, read b, write x' (FP64)
+
+N = (257, 129, 129)
+BC = (1, 1, 1, 1, 1, 1)
+LEN = (2.0, 1.0, 1.0)
+STRETCH = (0.0, 1.5, 1.5)
+RE, DT = 500.0, 5e-4
+JOR_SWEEPS, JOR_OMEGA = 200, 0.8
+TOL, MAXIT = 1e-8, 5000
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def tube_radius(x, R=0.3, x0=1.0, ell=0.4, delta=1.0 - 1.0 / math.sqrt(2.0)):
+    r = np.full_like(x, R)
+    s = np.abs(x - x0) <= ell
+    r[s] = R * (1.0 - delta / 2.0 * (1.0 + np.cos(np.pi * (x[s] - x0) / ell)))
+    return r
+
+
+def read_traffic():
+    """HBM bytes per JOR launch from the committed rocprofv3 counter summary."""
+    p = ROOT / "profiles" / "jor_sweep_256x128x128_counters.txt"
+    if not p.exists():
+        return None
+    vals = {}
+    for line in p.read_text().splitlines():
+        if "=" in line and not line.lstrip().startswith("#"):
+            k, v = line.split("=", 1)
+            vals[k.strip()] = v.strip()
+    try:
+        return float(vals["hbm_bytes_per_launch"])
+    except (KeyError, ValueError):
+        return None
+
+
+def setup_case(ih, jor_fp32=False):
+    c = ih.Case(N, BC, LEN, STRETCH, device=0)
+    r_u = tube_radius(c.coords(0, 1))
+    r_p = tube_radius(c.coords(0, 0))
+    c.set_tube_mask(0.5, 0.5, r_u, r_p)
+    m = c.get_mask()
+    vel = np.zeros((3,) + c.shape)
+    vel[0] = m[0].astype(np.float64)
+    for q in range(3):
+        c.upload(ih.VEL1 + q, np.ascontiguousarray(vel[q]))
+        c.fill(ih.NL1 + q, 0.0)
+    c.fill(ih.PRE, 0.0)
+    p = ih.StepParams(Re=RE, dtime=DT, jor_sweeps=JOR_SWEEPS, jor_omega=JOR_OMEGA, bicg_rel_tol=TOL,
+                      bicg_max_it=MAXIT, level_pressure=1, inflow_kind=2, inflow_amp=1.0, use_mask=1,
+                      jor_fp32=1 if jor_fp32 else 0)
+    return c, p
+
+
+def cpu_baseline(iters_per_step):
+    """Oracle (single-threaded C restatement) on a bounded sample at the full size:
+    3 JOR sweeps, 2 BiCGstab iterations and one step with the Poisson solve cut to
+    one iteration; the per-step time is composed with the GPU run's iteration counts."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    import ctypes
+
+    import oracle_lib as O
+
+    oc = O.OracleCase(N, BC, LEN, STRETCH)
+    oc.set_reduction_mode(0)  # the reference's sequential loop order
+    r_u = tube_radius(oc.coords(0, 1))
+    r_p = tube_radius(oc.coords(0, 0))
+    mask = oc.tube_mask(0.5, 0.5, r_u, r_p)
+    vel = oc.zeros(3)
+    vel[0] = mask[0].astype(np.float64)
+    vel = np.ascontiguousarray(vel)
+    pre, nl = oc.zeros(), oc.zeros(3)
+    rng = np.random.default_rng(0)
+    bb = np.ascontiguousarray(rng.standard_normal(oc.shape))
+    rel = oc.zeros()
+    t0 = time.perf_counter()
+    oc.jor(3, JOR_OMEGA, oc.Sp, oc.Np, rel, bb, oc.zeros())
+    t_jor = (time.perf_counter() - t0) / 3
+    log(f"cpu: {t_jor * 1e3:.1f} ms per JOR sweep")
+    t0 = time.perf_counter()
+    st = oc.bicgstab(0.0, 2, oc.Sp, oc.Np, bb, rel)
+    t_bi = (time.perf_counter() - t0) / max(st.iterations, 1)
+    log(f"cpu: {t_bi * 1e3:.1f} ms per BiCGstab iteration (incl. start-up)")
+    po = O.StepParams(Re=RE, dtime=DT, jor_sweeps=0, jor_omega=JOR_OMEGA, bicg_rel_tol=TOL, bicg_max_it=1,
+                      level_pressure=1, inflow_kind=2, inflow_amp=1.0,
+                      mask3=mask.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)))
+    t0 = time.perf_counter()
+    oc.timestep(po, 0, 0.0, vel, pre, nl)
+    t_rest = time.perf_counter() - t0
+    log(f"cpu: {t_rest:.2f} s for a step with one BiCGstab iteration per substep")
+    step = t_rest + 3 * JOR_SWEEPS * t_jor + max(iters_per_step - 3, 0) * t_bi
+    sample = t_rest + 3 * t_jor + 2 * t_bi
+    return {
+        "value": 1.0 / step,
+        "unit": "time steps/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": (f"oracle at 257x129x129 on 1 core: 3 JOR sweeps ({t_jor * 1e3:.0f} ms each), 2 BiCGstab "
+                   f"iterations ({t_bi:.2f} s each), one step with 1 BiCGstab iteration per substep "
+                   f"({t_rest:.1f} s); step = that + 600 sweeps + {iters_per_step - 3} iterations "
+                   f"(the GPU run's count); {sample:.0f} s of CPU work"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=4)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--mixed", action="store_true", help="FP32 JOR sweeps with FP64 residual and update")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs one GPU; --gpus {args.gpus} is not supported "
+              "(the block decomposition across GPUs is described in DESIGN.md, not built).", file=sys.stderr)
+        sys.exit(2)
+    rank = int(os.environ.get("RANK", "0"))
+
+    import impact_hip as ih
+
+    log(f"bench: case {N}, walls, tanh {STRETCH[1]}, stenosed tube, Re {RE}, dt {DT}")
+    c, p = setup_case(ih, jor_fp32=args.mixed)
+    log(f"bench: {c.bytes() / 2**30:.2f} GiB resident")
+    t = 0.0
+    for w in range(args.warmup):
+        st = c.timestep(p, t, stats=True)
+        t += DT
+        log(f"warmup step {w}: BiCGstab iterations {list(st.bicg_iterations)}, max|div u| {st.div_max_after:.3e}")
+    c.lib.ih_kernel_timing_enable(c.h, 1)
+    iters = []
+    c.sync()
+    c.timer_start()
+    t_wall0 = time.perf_counter()
+    for s in range(args.steps):
+        st = c.timestep(p, t, stats=False)
+        t += DT
+        log(f"timed step {s} done")
+    ms = c.timer_stop()
+    wall = time.perf_counter() - t_wall0
+    import ctypes
+
+    def fam(name):
+        tot, n = ctypes.c_double(0), ctypes.c_int64(0)
+        c.lib.ih_kernel_timing_get(c.h, name.encode(), ctypes.byref(tot), ctypes.byref(n))
+        return tot.value, n.value
+
+    jor_name = "jor_f32" if args.mixed else "jor"
+    jor_ms, jor_n = fam(jor_name)
+    bi_ms, bi_n = fam("bicgstab")
+    c.lib.ih_kernel_timing_enable(c.h, 0)
+    # one more step with statistics for the iteration count of this workload
+    st = c.timestep(p, t, stats=True)
+    iters_per_step = int(sum(st.bicg_iterations))
+    ms_per_step = ms / args.steps
+    value = 1000.0 / ms_per_step
+    P = N[0] * N[1] * N[2]
+    jor_bytes = (12 if args.mixed else JOR_BYTES_PER_POINT) * P
+    jor_avg_ms = jor_ms / max(jor_n, 1)
+    achieved = jor_bytes / (jor_avg_ms * 1e-3) / 1e9
+    traffic = read_traffic()
+    result = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "time steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64" if args.mixed else "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "stenosed-tube projection step, 257x129x129 points (256x128x128 cells), tanh 1.5 in y,z, "
+                        "Re 500, dt 5e-4, 200 JOR sweeps + BiCGstab to 1e-8",
+            "grid": list(N),
+            "jor_sweeps_per_substep": JOR_SWEEPS,
+            "bicgstab_iterations_per_step": iters_per_step,
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "kernel": "k_jor_f32" if args.mixed else "k_jor",
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic,
+            "bytes_per_launch": jor_bytes,
+            "avg_launch_ms": jor_avg_ms,
+        },
+        "phases_ms_per_step": {
+            "jor": jor_ms / args.steps,
+            "bicgstab": bi_ms / args.steps,
+            "other": ms_per_step - (jor_ms + bi_ms) / args.steps,
+        },
+        "bicgstab_ms_per_iteration": bi_ms / max(bi_n, 1),
+        "wall_s_timed": wall,
+    }
+    if not args.no_cpu_baseline and rank == 0:
+        log("cpu baseline: timing the oracle on a bounded sample ...")
+        result["cpu_baseline"] = cpu_baseline(iters_per_step)
+    else:
+        result["cpu_baseline"] = None
+    c.close()
+    if rank == 0:
+        print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

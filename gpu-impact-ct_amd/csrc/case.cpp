@@ -1,0 +1,1220 @@
+// C ABI of libimpact_hip: case lifetime, field transfer, operators, Poisson
+// solvers and the RK3 projection step, all ordered on the case's HIP stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "ih_host.h"
+
+using namespace ih;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                                   \
+    do {                                                                                               \
+        hipError_t e_ = (expr);                                                                        \
+        if (e_ != hipSuccess) return fail(IH_EHIP, "%s: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, __LINE__); \
+    } while (0)
+
+#define LAUNCHCHK()                                                                                    \
+    do {                                                                                               \
+        hipError_t e_ = hipGetLastError();                                                             \
+        if (e_ != hipSuccess) return fail(IH_EHIP, "kernel launch: %s (%s:%d)", hipGetErrorString(e_), __FILE__, __LINE__); \
+    } while (0)
+
+// RK3 coefficients (src/mod_vars.f90:774-775), evaluated in double precision.
+const double aRK[3] = {8.0 / 15.0, 5.0 / 12.0, 3.0 / 4.0};
+const double bRK[3] = {0.0, -17.0 / 60.0, -5.0 / 12.0};
+
+// Scalar slots in ih_case::dscal / hscal
+enum { S_DOT0 = 0, S_DOT1, S_DOT2, S_DOT3, S_MAX, S_BETA, S_OM2, S_ALPHA, S_OMEGA, S_ALPHA2, S_SUM, S_NSLOTS = 32 };
+
+bool valid_field(int f) { return f >= 0 && f < IH_NFIELDS; }
+
+Box pbox(const Geo &g) {
+    Box b;
+    for (int d = 0; d < 3; ++d) {
+        b.lo[d] = g.Sp[d];
+        b.hi[d] = g.Np[d];
+    }
+    return b;
+}
+
+bool box_ok(const Geo &g, const Box &b) {
+    for (int d = 0; d < 3; ++d) {
+        if (b.lo[d] < 0 || b.hi[d] > g.N[d] || b.lo[d] > b.hi[d]) return false;
+        // a stencil of the box reads one layer beyond it
+        if (b.lo[d] - 1 < -3 || b.hi[d] + 1 > g.N[d] + 3) return false;
+    }
+    return true;
+}
+
+// init_limits, src/mod_setup.f90:465-700 (single block, ls = -1)
+void init_limits(Geo &g, const int bc[6]) {
+    for (int d = 0; d < 3; ++d) {
+        const int N = g.N[d];
+        g.per[d] = bc[2 * d] == -1 ? 1 : 0;
+        g.Sp[d] = 1;
+        g.Np[d] = N - 1;
+        for (int m = 0; m < 3; ++m) {
+            g.SB[m][d] = 1;
+            g.NB[m][d] = N - 1;
+            g.SI[m][d] = 1;
+            g.NI[m][d] = N - 1;
+        }
+        if (bc[2 * d] > 0)
+            for (int m = 0; m < 3; ++m) {
+                g.SB[m][d] = (m == d) ? 0 : 1;
+                g.SI[m][d] = (m == d) ? 1 : 2;
+            }
+        if (bc[2 * d + 1] > 0) {
+            g.Np[d] = N;
+            for (int m = 0; m < 3; ++m) {
+                g.NB[m][d] = N;
+                g.NI[m][d] = N - 1;
+            }
+        }
+    }
+}
+
+int ensure_part(ih_case *c, int64_t nb) {
+    if (nb <= c->part_cap) return IH_OK;
+    if (c->part) (void)hipFree(c->part);
+    c->part = nullptr;
+    HIPCHK(hipMalloc(&c->part, sizeof(double) * (size_t)nb));
+    c->part_cap = nb;
+    return IH_OK;
+}
+
+// canonical reductions into dscal[slot] (device); host reads with sync_scalars
+int dot_to(ih_case *c, const Box &b, const double *a, const double *bb, int slot) {
+    const int64_t nb = red_blocks(b);
+    int rc = ensure_part(c, nb);
+    if (rc) return rc;
+    launch_dot_partial(c->stream, c->g, b, a, bb, c->part);
+    launch_final_sum(c->stream, c->part, nb, c->dscal + slot);
+    LAUNCHCHK();
+    return IH_OK;
+}
+
+int sum_to(ih_case *c, const Box &b, const double *a, int slot) {
+    const int64_t nb = red_blocks(b);
+    int rc = ensure_part(c, nb);
+    if (rc) return rc;
+    launch_sum_partial(c->stream, c->g, b, a, c->part);
+    launch_final_sum(c->stream, c->part, nb, c->dscal + slot);
+    LAUNCHCHK();
+    return IH_OK;
+}
+
+int max_to(ih_case *c, const Box &b, const double *a, int slot) {
+    const int64_t nb = red_blocks(b);
+    int rc = ensure_part(c, nb);
+    if (rc) return rc;
+    launch_max_partial(c->stream, c->g, b, a, c->part);
+    launch_final_max(c->stream, c->part, nb, c->dscal + slot);
+    LAUNCHCHK();
+    return IH_OK;
+}
+
+int read_scalars(ih_case *c, int first, int count) {
+    HIPCHK(hipMemcpyAsync(c->hscal + first, c->dscal + first, sizeof(double) * (size_t)count, hipMemcpyDeviceToHost,
+                          c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return IH_OK;
+}
+
+int write_scalars(ih_case *c, int first, int count) {
+    HIPCHK(hipMemcpyAsync(c->dscal + first, c->hscal + first, sizeof(double) * (size_t)count, hipMemcpyHostToDevice,
+                          c->stream));
+    return IH_OK;
+}
+
+double *F(ih_case *c, int f) { return c->fields[f]; }
+
+// Event pair around a phase of launches on the case's stream (when enabled).
+struct PhaseTimer {
+    ih_case *c;
+    PendingTiming p;
+    const char *family;
+    PhaseTimer(ih_case *cc, const char *fam) : c(cc), family(fam) {
+        if (!c->ktime) return;
+        if (hipEventCreate(&p.a) != hipSuccess || hipEventCreate(&p.b) != hipSuccess) {
+            p.a = p.b = nullptr;
+            return;
+        }
+        (void)hipEventRecord(p.a, c->stream);
+    }
+    void done(int64_t launches) {
+        if (!c->ktime || !p.a) return;
+        (void)hipEventRecord(p.b, c->stream);
+        p.launches = launches;
+        c->kpending[family].push_back(p);
+        p.a = p.b = nullptr;
+    }
+};
+
+void resolve_timings(ih_case *c) {
+    for (auto &kv : c->kpending) {
+        KernelTiming &t = c->ktimes[kv.first];
+        for (PendingTiming &p : kv.second) {
+            (void)hipEventSynchronize(p.b);
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+                t.total_ms += ms;
+                t.launches += p.launches;
+            }
+            (void)hipEventDestroy(p.a);
+            (void)hipEventDestroy(p.b);
+        }
+        kv.second.clear();
+    }
+}
+
+// exchange of one field along one direction
+int exch(ih_case *c, int dir, int vel_dir, double *f) {
+    double *fs[1] = {f};
+    int vd[1] = {vel_dir};
+    launch_exchange(c->stream, c->g, dir, 1, fs, vd);
+    LAUNCHCHK();
+    return IH_OK;
+}
+
+// product_div_grad (src/mod_laplace.f90:38-197): gradient then divergence per
+// direction, through the DIG field
+int apply_div_grad(ih_case *c, double *phi, double *lap) {
+    double *dig = F(c, IH_DIG);
+    for (int m = 1; m <= 3; ++m) {
+        int rc = exch(c, m, 0, phi);
+        if (rc) return rc;
+        launch_gradient(c->stream, c->g, c->c, m - 1, phi, dig);
+        rc = exch(c, m, m, dig);
+        if (rc) return rc;
+        launch_div_dir(c->stream, c->g, c->c, m - 1, dig, lap, m == 1 ? 1 : 0);
+        LAUNCHCHK();
+    }
+    return IH_OK;
+}
+
+int upload_coeffs(ih_case *c) {
+    const Geo &g = c->g;
+    // SoA layout: per direction 10 arrays of up to 7 taps plus cdg (3 taps) and coordinates
+    std::vector<double> h;
+    std::vector<int64_t> off_G(3), off_Ip(3), off_D(3), off_Iu(3), off_NpD(3), off_NpU(3), off_NuD(3), off_NuU(3),
+        off_p2(3), off_u2(3), off_dg(3), off_xp(3), off_xu(3);
+    auto push = [&](const std::vector<double> &src, int taps, int cst, int N, bool is_dg) -> int64_t {
+        const int64_t o = (int64_t)h.size();
+        h.resize(h.size() + (size_t)taps * cst, 0.0);
+        if (!is_dg) {
+            for (int i = 0; i <= N; ++i)
+                for (int t = 0; t < taps; ++t) h[(size_t)o + (size_t)t * cst + i] = src[(size_t)t + (size_t)taps * i];
+        } else {
+            for (int i = 1; i <= N; ++i)
+                for (int t = 0; t < 3; ++t) h[(size_t)o + (size_t)t * cst + i] = src[(size_t)t + 3 * (size_t)(i - 1)];
+        }
+        return o;
+    };
+    const HostCoeffs &hc = c->hc;
+    for (int d = 0; d < 3; ++d) {
+        const int N = g.N[d], cst = g.cst[d];
+        off_G[d] = push(hc.G[d], 6, cst, N, false);
+        off_Ip[d] = push(hc.Ip[d], 6, cst, N, false);
+        off_D[d] = push(hc.D[d], 6, cst, N, false);
+        off_Iu[d] = push(hc.Iu[d], 6, cst, N, false);
+        off_NpD[d] = push(hc.NpD[d], 7, cst, N, false);
+        off_NpU[d] = push(hc.NpU[d], 7, cst, N, false);
+        off_NuD[d] = push(hc.NuD[d], 7, cst, N, false);
+        off_NuU[d] = push(hc.NuU[d], 7, cst, N, false);
+        off_p2[d] = push(hc.p2[d], 7, cst, N, false);
+        off_u2[d] = push(hc.u2[d], 7, cst, N, false);
+        off_dg[d] = push(hc.dg[d], 3, cst, N, true);
+        off_xp[d] = (int64_t)h.size();
+        h.insert(h.end(), c->hg.xp[d].begin(), c->hg.xp[d].end());
+        off_xu[d] = (int64_t)h.size();
+        h.insert(h.end(), c->hg.xu[d].begin(), c->hg.xu[d].end());
+    }
+    if (c->dcoef) (void)hipFree(c->dcoef);
+    if (c->dcoef32) (void)hipFree(c->dcoef32);
+    c->dcoef = nullptr;
+    c->dcoef32 = nullptr;
+    HIPCHK(hipMalloc(&c->dcoef, sizeof(double) * h.size()));
+    HIPCHK(hipMemcpy(c->dcoef, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice));
+    std::vector<float> h32;
+    std::vector<int64_t> off32(3);
+    for (int d = 0; d < 3; ++d) {
+        off32[d] = (int64_t)h32.size();
+        for (int t = 0; t < 3 * g.cst[d]; ++t) h32.push_back((float)h[(size_t)off_dg[d] + t]);
+    }
+    HIPCHK(hipMalloc(&c->dcoef32, sizeof(float) * h32.size()));
+    HIPCHK(hipMemcpy(c->dcoef32, h32.data(), sizeof(float) * h32.size(), hipMemcpyHostToDevice));
+    for (int d = 0; d < 3; ++d) {
+        c->c.G[d] = c->dcoef + off_G[d];
+        c->c.Ip[d] = c->dcoef + off_Ip[d];
+        c->c.D[d] = c->dcoef + off_D[d];
+        c->c.Iu[d] = c->dcoef + off_Iu[d];
+        c->c.NpD[d] = c->dcoef + off_NpD[d];
+        c->c.NpU[d] = c->dcoef + off_NpU[d];
+        c->c.NuD[d] = c->dcoef + off_NuD[d];
+        c->c.NuU[d] = c->dcoef + off_NuU[d];
+        c->c.p2[d] = c->dcoef + off_p2[d];
+        c->c.u2[d] = c->dcoef + off_u2[d];
+        c->c.dg[d] = c->dcoef + off_dg[d];
+        c->c.dgf[d] = c->dcoef32 + off32[d];
+        c->c.xp[d] = c->dcoef + off_xp[d];
+        c->c.xu[d] = c->dcoef + off_xu[d];
+    }
+    c->has_coeffs = true;
+    return IH_OK;
+}
+
+int need_coeffs(ih_case *c) {
+    if (!c) return fail(IH_EINVAL, "null case");
+    if (!c->has_coeffs) return fail(IH_ESTATE, "coefficients not set (call ih_case_set_coeffs or ih_case_generate_coeffs)");
+    return IH_OK;
+}
+
+int to_box(const ih_case *c, const int ss[3], const int nn[3], Box &b) {
+    for (int d = 0; d < 3; ++d) {
+        b.lo[d] = ss[d];
+        b.hi[d] = nn[d];
+    }
+    if (!box_ok(c->g, b)) return fail(IH_EINVAL, "index range outside the block");
+    return IH_OK;
+}
+
+double absmax_host(ih_case *c, const Box &b, const double *a, int *rc) {
+    *rc = max_to(c, b, a, S_MAX);
+    if (*rc) return 0.0;
+    *rc = read_scalars(c, S_MAX, 1);
+    return c->hscal[S_MAX];
+}
+
+}  // namespace
+
+static int grid_from_desc(const ih_grid_desc *gd, HostGrid &hg);
+
+// ===========================================================================
+extern "C" {
+
+int ih_abi_version(void) { return IH_ABI_VERSION; }
+
+int ih_last_error(char *buf, size_t n) {
+    if (!buf || n == 0) return (int)g_err.size();
+    std::snprintf(buf, n, "%s", g_err.c_str());
+    return (int)g_err.size();
+}
+
+int ih_device_count(int *count) {
+    if (!count) return fail(IH_EINVAL, "null count");
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) n = 0;
+    *count = n;
+    return IH_OK;
+}
+
+int ih_case_create(const ih_grid_desc *gd, int device, ih_case **out) {
+    if (!gd || !out) return fail(IH_EINVAL, "null argument");
+    *out = nullptr;
+    {
+        HostGrid probe;
+        int rc0 = grid_from_desc(gd, probe);
+        if (rc0) return rc0;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(IH_EHIP, "no HIP device available");
+    if (device < 0 || device >= ndev) return fail(IH_EINVAL, "device %d out of range (%d devices)", device, ndev);
+    HIPCHK(hipSetDevice(device));
+
+    ih_case *c = new ih_case();
+    c->device = device;
+    Geo &g = c->g;
+    for (int d = 0; d < 3; ++d) g.N[d] = gd->N[d];
+    init_limits(g, gd->bc);
+    g.lead = 12;  // puts i = 1 of every row on a 128-byte boundary
+    g.px = ((g.N[0] + 7) + 15) / 16 * 16;
+    g.pxy = (int64_t)g.px * (g.N[1] + 7);
+    g.fe = ((g.lead + g.pxy * (g.N[2] + 7)) + 63) / 64 * 64;
+    for (int d = 0; d < 3; ++d) g.cst[d] = ((g.N[d] + 1) + 7) / 8 * 8;
+    for (int d = 0; d < 3; ++d) {
+        c->hg.N[d] = gd->N[d];
+        c->hg.bc[d][0] = gd->bc[2 * d];
+        c->hg.bc[d][1] = gd->bc[2 * d + 1];
+        c->hg.L[d] = gd->L[d];
+        c->hg.stretch[d] = gd->stretch[d];
+    }
+    auto cleanup = [&](int rc) {
+        ih_case_destroy(c);
+        return rc;
+    };
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+        return cleanup(fail(IH_EHIP, "hipStreamCreate failed"));
+    const size_t fbytes = sizeof(double) * (size_t)g.fe;
+    for (int f = 0; f < IH_NFIELDS; ++f) {
+        if (hipMalloc(&c->fields[f], fbytes) != hipSuccess)
+            return cleanup(fail(IH_ENOMEM, "out of device memory allocating field %d (%zu bytes)", f, fbytes));
+        if (hipMemsetAsync(c->fields[f], 0, fbytes, c->stream) != hipSuccess) return cleanup(fail(IH_EHIP, "memset"));
+        c->bytes += (int64_t)fbytes;
+    }
+    for (int q = 0; q < 3; ++q) {
+        if (hipMalloc(&c->f32[q], sizeof(float) * (size_t)g.fe) != hipSuccess)
+            return cleanup(fail(IH_ENOMEM, "out of device memory (fp32 field)"));
+        if (hipMemsetAsync(c->f32[q], 0, sizeof(float) * (size_t)g.fe, c->stream) != hipSuccess)
+            return cleanup(fail(IH_EHIP, "memset"));
+        c->bytes += (int64_t)(sizeof(float) * (size_t)g.fe);
+    }
+    if (hipMalloc(&c->mask, 3 * (size_t)g.fe) != hipSuccess) return cleanup(fail(IH_ENOMEM, "mask"));
+    if (hipMemsetAsync(c->mask, 1, 3 * (size_t)g.fe, c->stream) != hipSuccess) return cleanup(fail(IH_EHIP, "memset"));
+    c->bytes += 3 * (int64_t)g.fe;
+    if (hipMalloc(&c->dscal, sizeof(double) * S_NSLOTS) != hipSuccess) return cleanup(fail(IH_ENOMEM, "scalars"));
+    if (hipHostMalloc((void **)&c->hscal, sizeof(double) * S_NSLOTS, hipHostMallocDefault) != hipSuccess)
+        return cleanup(fail(IH_ENOMEM, "pinned scalars"));
+    std::memset(c->hscal, 0, sizeof(double) * S_NSLOTS);
+    if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess)
+        return cleanup(fail(IH_EHIP, "event create"));
+    Box pb = pbox(g);
+    int rc = ensure_part(c, std::max<int64_t>(red_blocks(pb), 4096));
+    if (rc) return cleanup(rc);
+    if (hipStreamSynchronize(c->stream) != hipSuccess) return cleanup(fail(IH_EHIP, "stream sync"));
+    *out = c;
+    return IH_OK;
+}
+
+void ih_case_destroy(ih_case *c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (int f = 0; f < IH_NFIELDS; ++f)
+        if (c->fields[f]) (void)hipFree(c->fields[f]);
+    for (int q = 0; q < 3; ++q)
+        if (c->f32[q]) (void)hipFree(c->f32[q]);
+    if (c->mask) (void)hipFree(c->mask);
+    if (c->dcoef) (void)hipFree(c->dcoef);
+    if (c->dcoef32) (void)hipFree(c->dcoef32);
+    if (c->part) (void)hipFree(c->part);
+    if (c->dscal) (void)hipFree(c->dscal);
+    if (c->hscal) (void)hipHostFree(c->hscal);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    resolve_timings(c);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int ih_case_set_coeffs(ih_case *c, const ih_coeff_set *cs) {
+    if (!c || !cs) return fail(IH_EINVAL, "null argument");
+    HIPCHK(hipSetDevice(c->device));
+    for (int d = 0; d < 3; ++d) {
+        const int N = c->g.N[d];
+        auto take = [&](std::vector<double> &dst, const double *src, size_t n) -> bool {
+            if (!src) return false;
+            dst.assign(src, src + n);
+            return true;
+        };
+        const size_t n6 = 6 * (size_t)(N + 1), n7 = 7 * (size_t)(N + 1);
+        bool ok = take(c->hc.G[d], cs->cGp[d], n6) && take(c->hc.D[d], cs->cD[d], n6) &&
+                  take(c->hc.Ip[d], cs->cIp[d], n6) && take(c->hc.Iu[d], cs->cIu[d], n6) &&
+                  take(c->hc.NpD[d], cs->cNpD[d], n7) && take(c->hc.NpU[d], cs->cNpU[d], n7) &&
+                  take(c->hc.NuD[d], cs->cNuD[d], n7) && take(c->hc.NuU[d], cs->cNuU[d], n7) &&
+                  take(c->hc.p2[d], cs->cp2[d], n7) && take(c->hc.u2[d], cs->cu2[d], n7) &&
+                  take(c->hc.dg[d], cs->cdg[d], 3 * (size_t)N);
+        if (!ok) return fail(IH_EINVAL, "coefficient set: missing array in direction %d", d);
+    }
+    // coordinates for the inflow profile: generated from L/stretch
+    host_coords(c->hg);
+    return upload_coeffs(c);
+}
+
+int ih_case_generate_coeffs(ih_case *c) {
+    if (!c) return fail(IH_EINVAL, "null case");
+    for (int d = 0; d < 3; ++d)
+        if (!(c->hg.L[d] > 0.0)) return fail(IH_EINVAL, "L[%d] must be positive", d);
+    HIPCHK(hipSetDevice(c->device));
+    host_coords(c->hg);
+    std::string err;
+    if (!host_coeffs(c->hg, c->hc, err)) return fail(IH_EINVAL, "coefficients: %s", err.c_str());
+    return upload_coeffs(c);
+}
+
+static const std::vector<double> *coeff_by_name(const HostCoeffs &hc, const char *name) {
+    static const char *uv[3] = {"u1", "v2", "w3"};
+    static const char *uvw2[3] = {"u11", "v22", "w33"};
+    const std::vector<double> *src = nullptr;
+    for (int d = 0; d < 3 && !src; ++d) {
+        char b[16];
+        const char dc = (char)('1' + d);
+        auto is = [&](const char *s) { return std::strcmp(name, s) == 0; };
+        std::snprintf(b, sizeof b, "cGp%c", dc);
+        if (is(b)) src = &hc.G[d];
+        std::snprintf(b, sizeof b, "cD%s", uv[d]);
+        if (is(b)) src = &hc.D[d];
+        std::snprintf(b, sizeof b, "cIp%c", "uvw"[d]);
+        if (is(b)) src = &hc.Ip[d];
+        std::snprintf(b, sizeof b, "cI%cp", "uvw"[d]);
+        if (is(b)) src = &hc.Iu[d];
+        std::snprintf(b, sizeof b, "cNp%cD", dc);
+        if (is(b)) src = &hc.NpD[d];
+        std::snprintf(b, sizeof b, "cNp%cU", dc);
+        if (is(b)) src = &hc.NpU[d];
+        std::snprintf(b, sizeof b, "cN%sD", uv[d]);
+        if (is(b)) src = &hc.NuD[d];
+        std::snprintf(b, sizeof b, "cN%sU", uv[d]);
+        if (is(b)) src = &hc.NuU[d];
+        std::snprintf(b, sizeof b, "cp%c%c", dc, dc);
+        if (is(b)) src = &hc.p2[d];
+        std::snprintf(b, sizeof b, "c%s", uvw2[d]);
+        if (is(b)) src = &hc.u2[d];
+        std::snprintf(b, sizeof b, "cdg%c", dc);
+        if (is(b)) src = &hc.dg[d];
+    }
+    return src;
+}
+
+static int grid_from_desc(const ih_grid_desc *gd, HostGrid &hg) {
+    if (!gd) return fail(IH_EINVAL, "null grid");
+    if (gd->version != IH_ABI_VERSION) return fail(IH_EINVAL, "grid desc version %d, library %d", gd->version, IH_ABI_VERSION);
+    for (int d = 0; d < 3; ++d) {
+        if (gd->N[d] < 8 || gd->N[d] > 8192) return fail(IH_EINVAL, "N[%d] = %d out of range [8, 8192]", d, gd->N[d]);
+        for (int s = 0; s < 2; ++s)
+            if (!(gd->bc[2 * d + s] == -1 || gd->bc[2 * d + s] == 1))
+                return fail(IH_EINVAL, "bc[%d] = %d: only -1 (periodic) and 1 (wall) are supported", 2 * d + s,
+                            gd->bc[2 * d + s]);
+        if ((gd->bc[2 * d] == -1) != (gd->bc[2 * d + 1] == -1))
+            return fail(IH_EINVAL, "direction %d: periodic must be set on both sides", d);
+        hg.N[d] = gd->N[d];
+        hg.bc[d][0] = gd->bc[2 * d];
+        hg.bc[d][1] = gd->bc[2 * d + 1];
+        hg.L[d] = gd->L[d];
+        hg.stretch[d] = gd->stretch[d];
+    }
+    return IH_OK;
+}
+
+int ih_case_get_coeff(ih_case *c, const char *name, double *out, int64_t *n) {
+    if (!c || !name || !n) return fail(IH_EINVAL, "null argument");
+    if (!c->has_coeffs) return fail(IH_ESTATE, "no coefficients");
+    const std::vector<double> *src = coeff_by_name(c->hc, name);
+    if (!src) return fail(IH_EINVAL, "unknown coefficient '%s'", name);
+    *n = (int64_t)src->size();
+    if (out) std::memcpy(out, src->data(), sizeof(double) * src->size());
+    return IH_OK;
+}
+
+int ih_grid_coeff(const ih_grid_desc *gd, const char *name, double *out, int64_t *n) {
+    if (!name || !n) return fail(IH_EINVAL, "null argument");
+    HostGrid hg;
+    int rc = grid_from_desc(gd, hg);
+    if (rc) return rc;
+    host_coords(hg);
+    HostCoeffs hc;
+    std::string err;
+    if (!host_coeffs(hg, hc, err)) return fail(IH_EINVAL, "coefficients: %s", err.c_str());
+    const std::vector<double> *src = coeff_by_name(hc, name);
+    if (!src) return fail(IH_EINVAL, "unknown coefficient '%s'", name);
+    *n = (int64_t)src->size();
+    if (out) std::memcpy(out, src->data(), sizeof(double) * src->size());
+    return IH_OK;
+}
+
+int ih_grid_coords(const ih_grid_desc *gd, int d, int which, double *out) {
+    if (!out || d < 0 || d > 2) return fail(IH_EINVAL, "bad argument");
+    HostGrid hg;
+    int rc = grid_from_desc(gd, hg);
+    if (rc) return rc;
+    host_coords(hg);
+    const std::vector<double> &v = which ? hg.xu[d] : hg.xp[d];
+    std::memcpy(out, v.data(), sizeof(double) * v.size());
+    return IH_OK;
+}
+
+int ih_case_get_coords(ih_case *c, int d, int which, double *out) {
+    if (!c || !out || d < 0 || d > 2) return fail(IH_EINVAL, "bad argument");
+    const std::vector<double> &v = which ? c->hg.xu[d] : c->hg.xp[d];
+    if (v.empty()) return fail(IH_ESTATE, "coordinates not generated");
+    std::memcpy(out, v.data(), sizeof(double) * v.size());
+    return IH_OK;
+}
+
+int ih_case_get_ranges(ih_case *c, int out[48]) {
+    if (!c || !out) return fail(IH_EINVAL, "null argument");
+    const Geo &g = c->g;
+    int q = 0;
+    for (int d = 0; d < 3; ++d) out[q++] = g.Sp[d];
+    for (int d = 0; d < 3; ++d) out[q++] = g.Np[d];
+    for (int m = 0; m < 3; ++m) for (int d = 0; d < 3; ++d) out[q++] = g.SB[m][d];
+    for (int m = 0; m < 3; ++m) for (int d = 0; d < 3; ++d) out[q++] = g.NB[m][d];
+    for (int m = 0; m < 3; ++m) for (int d = 0; d < 3; ++d) out[q++] = g.SI[m][d];
+    for (int m = 0; m < 3; ++m) for (int d = 0; d < 3; ++d) out[q++] = g.NI[m][d];
+    for (int d = 0; d < 3; ++d) out[q++] = g.per[d] ? 1 : 2;  // relaxation range SNF (mod_setup.f90:850-867)
+    for (int d = 0; d < 3; ++d) out[q++] = g.per[d] ? g.N[d] - 1 : g.N[d] - 1;
+    return IH_OK;
+}
+
+int64_t ih_field_elems(const ih_case *c) {
+    if (!c) return 0;
+    return (int64_t)(c->g.N[0] + 7) * (c->g.N[1] + 7) * (c->g.N[2] + 7);
+}
+
+int64_t ih_case_bytes(const ih_case *c) { return c ? c->bytes : 0; }
+
+int ih_field_upload(ih_case *c, ih_field f, const double *host) {
+    if (!c || !host || !valid_field(f)) return fail(IH_EINVAL, "bad argument");
+    HIPCHK(hipSetDevice(c->device));
+    const Geo &g = c->g;
+    const size_t w = sizeof(double) * (size_t)(g.N[0] + 7);
+    HIPCHK(hipMemcpy2DAsync(c->fields[f] + g.lead, sizeof(double) * (size_t)g.px, host, w, w,
+                            (size_t)(g.N[1] + 7) * (g.N[2] + 7), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return IH_OK;
+}
+
+int ih_field_download(ih_case *c, ih_field f, double *host) {
+    if (!c || !host || !valid_field(f)) return fail(IH_EINVAL, "bad argument");
+    HIPCHK(hipSetDevice(c->device));
+    const Geo &g = c->g;
+    const size_t w = sizeof(double) * (size_t)(g.N[0] + 7);
+    HIPCHK(hipMemcpy2DAsync(host, w, c->fields[f] + g.lead, sizeof(double) * (size_t)g.px, w,
+                            (size_t)(g.N[1] + 7) * (g.N[2] + 7), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return IH_OK;
+}
+
+int ih_field_fill(ih_case *c, ih_field f, double value) {
+    if (!c || !valid_field(f)) return fail(IH_EINVAL, "bad argument");
+    launch_fill(c->stream, c->fields[f], value, c->g.fe);
+    LAUNCHCHK();
+    return IH_OK;
+}
+
+int ih_field_copy(ih_case *c, ih_field dst, ih_field src) {
+    if (!c || !valid_field(dst) || !valid_field(src)) return fail(IH_EINVAL, "bad argument");
+    HIPCHK(hipMemcpyAsync(c->fields[dst], c->fields[src], sizeof(double) * (size_t)c->g.fe, hipMemcpyDeviceToDevice,
+                          c->stream));
+    return IH_OK;
+}
+
+int ih_synchronize(ih_case *c) {
+    if (!c) return fail(IH_EINVAL, "null case");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return IH_OK;
+}
+
+// ---------------------------------------------------------------------------
+int ih_exchange(ih_case *c, int dir, int vel_dir, ih_field f) {
+    if (!c || !valid_field(f) || dir < 1 || dir > 3 || vel_dir < 0 || vel_dir > 3) return fail(IH_EINVAL, "bad argument");
+    return exch(c, dir, vel_dir, F(c, f));
+}
+
+int ih_exchange_relax(ih_case *c, ih_field f) {
+    if (!c || !valid_field(f)) return fail(IH_EINVAL, "bad argument");
+    launch_exchange_relax(c->stream, c->g, F(c, f));
+    LAUNCHCHK();
+    return IH_OK;
+}
+
+int ih_divergence2(ih_case *c, ih_field u, ih_field v, ih_field w, ih_field div) {
+    int rc = need_coeffs(c);
+    if (rc) return rc;
+    if (!valid_field(u) || !valid_field(v) || !valid_field(w) || !valid_field(div)) return fail(IH_EINVAL, "bad field");
+    if ((rc = exch(c, 1, 1, F(c, u)))) return rc;
+    if ((rc = exch(c, 2, 2, F(c, v)))) return rc;
+    if ((rc = exch(c, 3, 3, F(c, w)))) return rc;
+    launch_divergence2(c->stream, c->g, c->c, F(c, u), F(c, v), F(c, w), F(c, div));
+    LAUNCHCHK();
+    return IH_OK;
+}
+
+int ih_gradient(ih_case *c, int m, ih_field phi, ih_field grad) {
+    int rc = need_coeffs(c);
+    if (rc) return rc;
+    if (m < 1 || m > 3 || !valid_field(phi) || !valid_field(grad)) return fail(IH_EINVAL, "bad argument");
+    if ((rc = exch(c, m, 0, F(c, phi)))) return rc;
+    launch_gradient(c->stream, c->g, c->c, m - 1, F(c, phi), F(c, grad));
+    LAUNCHCHK();
+    return IH_OK;
+}
+
+int ih_interpolate2_vel_pre(ih_case *c, int m, ih_field phi, ih_field inter) {
+    int rc = need_coeffs(c);
+    if (rc) return rc;
+    if (m < 1 || m > 3 || !valid_field(phi) || !valid_field(inter)) return fail(IH_EINVAL, "bad argument");
+    launch_interp_vel_pre(c->stream, c->g, c->c, m - 1, F(c, phi), F(c, inter));
+    LAUNCHCHK();
+    return IH_OK;
+}
+
+int ih_interpolate2_pre_vel(ih_case *c, int m, ih_field phi, ih_field inter) {
+    int rc = need_coeffs(c);
+    if (rc) return rc;
+    if (m < 1 || m > 3 || !valid_field(phi) || !valid_field(inter)) return fail(IH_EINVAL, "bad argument");
+    launch_interp_pre_vel(c->stream, c->g, c->c, m - 1, F(c, phi), F(c, inter));
+    LAUNCHCHK();
+    return IH_OK;
+}
+
+int ih_interpolate_vel(ih_case *c) {
+    int rc = need_coeffs(c);
+    if (rc) return rc;
+    launch_interp_vel_pre3(c->stream, c->g, c->c, F(c, IH_VEL1), F(c, IH_VEL2), F(c, IH_VEL3), F(c, IH_WORK1),
+                           F(c, IH_WORK2), F(c, IH_WORK3));
+    LAUNCHCHK();
+    double *ws[3] = {F(c, IH_WORK1), F(c, IH_WORK2), F(c, IH_WORK3)};
+    int vd[3] = {0, 0, 0};
+    for (int dir = 1; dir <= 3; ++dir) {
+        launch_exchange(c->stream, c->g, dir, 3, ws, vd);
+        LAUNCHCHK();
+    }
+    return IH_OK;
+}
+
+int ih_nonlinear_upwind(ih_case *c, int component_id, ih_field pp, ih_field vel_c, ih_field nl_c) {
+    int rc = need_coeffs(c);
+    if (rc) return rc;
+    if (component_id < 1 || component_id > 9 || !valid_field(pp) || !valid_field(vel_c) || !valid_field(nl_c))
+        return fail(IH_EINVAL, "bad argument");
+    const int q = (component_id - 1) / 3, d = (component_id - 1) % 3;
+    launch_nonlinear_one(c->stream, c->g, c->c, q, d, F(c, pp), F(c, vel_c), F(c, nl_c), d == 0 ? 1 : 0);
+    LAUNCHCHK();
+    return IH_OK;
+}
+
+int ih_helmholtz(ih_case *c, int component, double multL, ih_field vel_c, ih_field nl_c) {
+    int rc = need_coeffs(c);
+    if (rc) return rc;
+    if (component < 1 || component > 3 || !valid_field(vel_c) || !valid_field(nl_c)) return fail(IH_EINVAL, "bad argument");
+    launch_helmholtz_one(c->stream, c->g, c->c, component - 1, multL, F(c, vel_c), F(c, nl_c));
+    LAUNCHCHK();
+    return IH_OK;
+}
+
+int ih_axpby(ih_case *c, double a, double b, ih_field x, ih_field y, ih_field z) {
+    if (!c || !valid_field(x) || !valid_field(y) || !valid_field(z)) return fail(IH_EINVAL, "bad argument");
+    launch_axpby(c->stream, c->g, a, b, F(c, x), F(c, y), F(c, z), c->g.fe);
+    LAUNCHCHK();
+    return IH_OK;
+}
+
+int ih_product_div_grad(ih_case *c, ih_field phi, ih_field lap) {
+    int rc = need_coeffs(c);
+    if (rc) return rc;
+    if (!valid_field(phi) || !valid_field(lap) || phi == IH_DIG || lap == IH_DIG) return fail(IH_EINVAL, "bad field");
+    return apply_div_grad(c, F(c, phi), F(c, lap));
+}
+
+int ih_dot(ih_case *c, const int ss[3], const int nn[3], ih_field a, ih_field b, double *out) {
+    if (!c || !ss || !nn || !out || !valid_field(a) || !valid_field(b)) return fail(IH_EINVAL, "bad argument");
+    Box bx;
+    int rc = to_box(c, ss, nn, bx);
+    if (rc) return rc;
+    if ((rc = dot_to(c, bx, F(c, a), F(c, b), S_DOT0))) return rc;
+    if ((rc = read_scalars(c, S_DOT0, 1))) return rc;
+    *out = c->hscal[S_DOT0];
+    return IH_OK;
+}
+
+int ih_jor(ih_case *c, int n_sweeps, double omega, const int ss[3], const int nn[3], ih_field rel, ih_field bb) {
+    int rc = need_coeffs(c);
+    if (rc) return rc;
+    if (!ss || !nn || n_sweeps < 0 || !valid_field(rel) || !valid_field(bb) || rel == IH_COMP || bb == IH_COMP)
+        return fail(IH_EINVAL, "bad argument");
+    Box b;
+    if ((rc = to_box(c, ss, nn, b))) return rc;
+    double *x = F(c, rel), *y = F(c, IH_COMP);
+    const double *B = F(c, bb);
+    PhaseTimer pt(c, "jor");
+    for (int s = 0; s < n_sweeps; ++s) {
+        launch_jor(c->stream, c->g, c->c, b, omega, x, B, y);
+        std::swap(x, y);
+    }
+    pt.done(n_sweeps);
+    LAUNCHCHK();
+    if (x != F(c, rel)) {
+        launch_copy_box(c->stream, c->g, b, x, F(c, rel));
+        LAUNCHCHK();
+    }
+    launch_exchange_relax(c->stream, c->g, F(c, rel));
+    LAUNCHCHK();
+    return IH_OK;
+}
+
+int ih_residual_cdg(ih_case *c, const int ss[3], const int nn[3], ih_field rel, ih_field bb, double *out) {
+    int rc = need_coeffs(c);
+    if (rc) return rc;
+    if (!ss || !nn || !out || !valid_field(rel) || !valid_field(bb)) return fail(IH_EINVAL, "bad argument");
+    Box b;
+    if ((rc = to_box(c, ss, nn, b))) return rc;
+    const int64_t nb = red_blocks(b);
+    if ((rc = ensure_part(c, nb))) return rc;
+    launch_exchange_relax(c->stream, c->g, F(c, rel));
+    launch_residual_cdg(c->stream, c->g, c->c, b, F(c, rel), F(c, bb), nullptr, c->part);
+    launch_final_max(c->stream, c->part, nb, c->dscal + S_MAX);
+    LAUNCHCHK();
+    if ((rc = read_scalars(c, S_MAX, 1))) return rc;
+    *out = c->hscal[S_MAX];
+    return IH_OK;
+}
+
+// BiCGstab_orig (src/mod_solvers.f90:1509-1935), preconditioner 0,
+// init_yes = .FALSE.; the scalar recurrences run on the host in the
+// reference's order, every dot product is a device reduction.
+int ih_bicgstab(ih_case *c, double eps, int n_it_max, const int ss[3], const int nn[3], ih_field bbf, ih_field phif,
+                ih_bicgstab_stats *st) {
+    int rc = need_coeffs(c);
+    if (rc) return rc;
+    if (!ss || !nn || !valid_field(bbf) || !valid_field(phif) || n_it_max < 0) return fail(IH_EINVAL, "bad argument");
+    for (int f : {IH_RR, IH_RH, IH_PK, IH_AP, IH_AR, IH_DIG})
+        if (bbf == f || phif == f) return fail(IH_EINVAL, "bb/phi may not be a BiCGstab work field");
+    Box b;
+    if ((rc = to_box(c, ss, nn, b))) return rc;
+    double *rr = F(c, IH_RR), *rh = F(c, IH_RH), *pk = F(c, IH_PK), *Ap = F(c, IH_AP), *Ar = F(c, IH_AR);
+    double *phi = F(c, phif);
+    const double *bb = F(c, bbf);
+    const Geo &g = c->g;
+    hipStream_t s = c->stream;
+
+    if ((rc = apply_div_grad(c, phi, rr))) return rc;
+    launch_bicg_init(s, g, b, bb, rr, rh);
+    LAUNCHCHK();
+    double norm_inf = absmax_host(c, b, rr, &rc);
+    if (rc) return rc;
+    if (st) {
+        st->norm_inf_initial = norm_inf;
+        st->exit_reason = 0;
+    }
+    int counter = 0;
+    double rhr = 0.0, rhr_prev, alpha = 0.0, beta, omega = 0.0;
+    for (;;) {
+        bool exit_yes = norm_inf < eps;
+        if (!exit_yes && counter == n_it_max) {
+            exit_yes = true;
+            if (st) st->exit_reason = 1;
+        }
+        if (counter < 1 && norm_inf != 0.0) exit_yes = false;
+        if (exit_yes) break;
+        counter = counter + 1;
+        rhr_prev = rhr;
+        if ((rc = dot_to(c, b, rr, rh, S_DOT0))) return rc;
+        if ((rc = read_scalars(c, S_DOT0, 1))) return rc;
+        rhr = c->hscal[S_DOT0];
+        if (std::fabs(rhr) == 0.0) {
+            launch_copy_box(s, g, b, rr, rh);
+            if ((rc = dot_to(c, b, rr, rh, S_DOT0))) return rc;
+            if ((rc = read_scalars(c, S_DOT0, 1))) return rc;
+            rhr = c->hscal[S_DOT0];
+        }
+        if (counter >= 2) {
+            if (omega == 0.0 || rhr_prev == 0.0) {
+                if (st) st->exit_reason = 2;
+                break;
+            }
+            beta = (alpha / omega) * (rhr / rhr_prev);
+            omega = -beta * omega;
+            c->hscal[S_BETA] = beta;
+            c->hscal[S_OM2] = omega;
+            if ((rc = write_scalars(c, S_BETA, 2))) return rc;
+            launch_bicg_p(s, g, b, rr, pk, Ap, c->dscal + S_BETA);
+        } else {
+            launch_copy_box(s, g, b, rr, pk);
+        }
+        LAUNCHCHK();
+        if ((rc = apply_div_grad(c, pk, Ap))) return rc;
+        if ((rc = dot_to(c, b, Ap, rh, S_DOT1))) return rc;
+        if ((rc = read_scalars(c, S_DOT1, 1))) return rc;
+        const double rhAp = c->hscal[S_DOT1];
+        if (std::fabs(rhAp) == 0.0) {
+            if (st) st->exit_reason = 2;
+            break;
+        }
+        alpha = rhr / rhAp;
+        c->hscal[S_ALPHA] = alpha;
+        if ((rc = write_scalars(c, S_ALPHA, 1))) return rc;
+        launch_bicg_s(s, g, b, rr, Ap, c->dscal + S_ALPHA);
+        LAUNCHCHK();
+        if ((rc = apply_div_grad(c, rr, Ar))) return rc;
+        if ((rc = dot_to(c, b, rr, Ar, S_DOT2))) return rc;
+        if ((rc = dot_to(c, b, Ar, Ar, S_DOT3))) return rc;
+        if ((rc = read_scalars(c, S_DOT2, 2))) return rc;
+        const double rAr = c->hscal[S_DOT2], ArAr = c->hscal[S_DOT3];
+        if (std::fabs(ArAr) == 0.0) {
+            if (st) st->exit_reason = 2;
+            break;
+        }
+        omega = rAr / ArAr;
+        c->hscal[S_OMEGA] = omega;
+        c->hscal[S_ALPHA2] = alpha;
+        if ((rc = write_scalars(c, S_OMEGA, 2))) return rc;
+        const int64_t nb = red_blocks(b);
+        launch_bicg_x(s, g, b, phi, rr, pk, Ar, c->dscal + S_OMEGA, c->part);
+        launch_final_max(s, c->part, nb, c->dscal + S_MAX);
+        LAUNCHCHK();
+        if ((rc = read_scalars(c, S_MAX, 1))) return rc;
+        norm_inf = c->hscal[S_MAX];
+    }
+    if (st) {
+        st->iterations = counter;
+        st->norm_inf_final = norm_inf;
+    }
+    return IH_OK;
+}
+
+int ih_fill_corners(ih_case *c, ih_field f) {
+    if (!c || !valid_field(f)) return fail(IH_EINVAL, "bad argument");
+    launch_fill_corners(c->stream, c->g, F(c, f));
+    LAUNCHCHK();
+    return IH_OK;
+}
+
+// level_pressure, src/mod_lib.f90:507-530
+int ih_level_pressure(ih_case *c, ih_field pre) {
+    if (!c || !valid_field(pre)) return fail(IH_EINVAL, "bad argument");
+    const Geo &g = c->g;
+    Box b = pbox(g);
+    int rc = sum_to(c, b, F(c, pre), S_SUM);
+    if (rc) return rc;
+    if ((rc = read_scalars(c, S_SUM, 1))) return rc;
+    const int dim1 = g.per[0] ? g.N[0] - 1 : g.N[0];
+    const int dim2 = g.per[1] ? g.N[1] - 1 : g.N[1];
+    const int dim3 = g.per[2] ? g.N[2] - 1 : g.N[2];
+    c->hscal[S_SUM + 1] = c->hscal[S_SUM] / (double)dim1 / (double)dim2 / (double)dim3;
+    if ((rc = write_scalars(c, S_SUM + 1, 1))) return rc;
+    launch_sub_box(c->stream, g, b, F(c, pre), c->dscal + S_SUM + 1);
+    LAUNCHCHK();
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return IH_OK;
+}
+
+int ih_set_tube_mask(ih_case *c, double yc, double zc, const double *r_u, const double *r_p) {
+    if (!c || !r_u || !r_p) return fail(IH_EINVAL, "null argument");
+    if (c->hg.xp[0].empty()) return fail(IH_ESTATE, "coordinates not generated");
+    const Geo &g = c->g;
+    const int64_t fe = g.fe;
+    std::vector<uint8_t> m((size_t)(3 * fe), 1);
+    for (int q = 0; q < 3; ++q)
+        for (int k = g.SB[q][2]; k <= g.NB[q][2]; ++k)
+            for (int j = g.SB[q][1]; j <= g.NB[q][1]; ++j)
+                for (int i = g.SB[q][0]; i <= g.NB[q][0]; ++i) {
+                    const double y = (q == 1) ? c->hg.xu[1][j + 3] : c->hg.xp[1][j + 3];
+                    const double z = (q == 2) ? c->hg.xu[2][k + 3] : c->hg.xp[2][k + 3];
+                    const double r = (q == 0) ? r_u[i + 3] : r_p[i + 3];
+                    const double dy = y - yc, dz = z - zc;
+                    m[(size_t)(q * fe + gidx(g, i, j, k))] = (std::sqrt(dy * dy + dz * dz) <= r) ? 1 : 0;
+                }
+    HIPCHK(hipMemcpy(c->mask, m.data(), m.size(), hipMemcpyHostToDevice));
+    c->has_mask = 1;
+    return IH_OK;
+}
+
+int ih_get_mask(ih_case *c, uint8_t *host3) {
+    if (!c || !host3) return fail(IH_EINVAL, "null argument");
+    const Geo &g = c->g;
+    std::vector<uint8_t> m((size_t)(3 * g.fe));
+    HIPCHK(hipMemcpy(m.data(), c->mask, m.size(), hipMemcpyDeviceToHost));
+    const int64_t nx = g.N[0] + 7, ny = g.N[1] + 7, nz = g.N[2] + 7;
+    for (int q = 0; q < 3; ++q)
+        for (int64_t k = 0; k < nz; ++k)
+            for (int64_t j = 0; j < ny; ++j)
+                for (int64_t i = 0; i < nx; ++i)
+                    host3[q * nx * ny * nz + i + nx * (j + ny * k)] =
+                        m[(size_t)(q * g.fe + gidx(g, (int)i - 3, (int)j - 3, (int)k - 3))];
+    return IH_OK;
+}
+
+int ih_apply_mask(ih_case *c) {
+    if (!c) return fail(IH_EINVAL, "null case");
+    launch_apply_mask(c->stream, c->g, c->mask, F(c, IH_VEL1), F(c, IH_VEL2), F(c, IH_VEL3));
+    LAUNCHCHK();
+    return IH_OK;
+}
+
+// ---------------------------------------------------------------------------
+// One time step: 3 RK substeps of src/mod_timeint.f90:275-377 with rhs_vel
+// (src/mod_rhs.f90:35-241) and explicit (src/mod_solvers.f90:64-346); after
+// the step the mask (apply_CT_forcing pattern) and level_pressure.
+// ---------------------------------------------------------------------------
+static int poisson_solve(ih_case *c, const ih_step_params *p, const Box &b, ih_bicgstab_stats *bst) {
+    int rc;
+    const int ss[3] = {b.lo[0], b.lo[1], b.lo[2]};
+    const int nn[3] = {b.hi[0], b.hi[1], b.hi[2]};
+    if (p->jor_sweeps > 0) {
+        if (p->jor_fp32) {
+            // FP32 correction sweeps: A e = r solved approximately in FP32 from
+            // e = 0, with r = b - A p and the update p += e kept in FP64.
+            const Geo &g = c->g;
+            const int64_t nb = red_blocks(b);
+            if ((rc = ensure_part(c, nb))) return rc;
+            launch_exchange_relax(c->stream, g, F(c, IH_PRE));
+            launch_residual_cdg(c->stream, g, c->c, b, F(c, IH_PRE), F(c, IH_RES), F(c, IH_TMP1), c->part);
+            launch_to_f32(c->stream, g, b, F(c, IH_TMP1), c->f32[1]);
+            launch_fill_f32(c->stream, c->f32[0], 0.0f, g.fe);
+            launch_fill_f32(c->stream, c->f32[2], 0.0f, g.fe);
+            float *x = c->f32[0], *y = c->f32[2];
+            PhaseTimer pt(c, "jor_f32");
+            for (int s = 0; s < p->jor_sweeps; ++s) {
+                launch_jor_f32(c->stream, g, c->c, b, (float)p->jor_omega, x, c->f32[1], y);
+                std::swap(x, y);
+            }
+            pt.done(p->jor_sweeps);
+            launch_correct_f32(c->stream, g, b, x, F(c, IH_PRE));
+            launch_exchange_relax(c->stream, g, F(c, IH_PRE));
+            LAUNCHCHK();
+        } else {
+            rc = ih_jor(c, p->jor_sweeps, p->jor_omega, ss, nn, IH_PRE, IH_RES);
+            if (rc) return rc;
+        }
+    }
+    int rc2;
+    const double bnorm = absmax_host(c, b, F(c, IH_RES), &rc2);
+    if (rc2) return rc2;
+    const double eps = p->bicg_rel_tol * bnorm;
+    PhaseTimer pt(c, "bicgstab");
+    rc = ih_bicgstab(c, eps, p->bicg_max_it, ss, nn, IH_RES, IH_PRE, bst);
+    pt.done(bst ? bst->iterations : 0);
+    return rc;
+}
+
+int ih_timestep(ih_case *c, const ih_step_params *p, double time, ih_step_stats *st) {
+    int rc = need_coeffs(c);
+    if (rc) return rc;
+    if (!p || !(p->dtime > 0.0) || !(p->Re > 0.0)) return fail(IH_EINVAL, "bad step parameters");
+    if (p->inflow_kind == 2 && !c->has_mask) return fail(IH_ESTATE, "pulsatile inflow needs a mask");
+    HIPCHK(hipSetDevice(c->device));
+    const Geo &g = c->g;
+    hipStream_t s = c->stream;
+    const double dtime = p->dtime;
+    const Box pb = pbox(g);
+    for (int sub = 0; sub < 3; ++sub) {
+        double subtime;
+        if (sub == 0)
+            subtime = time + dtime * aRK[0];
+        else if (sub == 1)
+            subtime = time + dtime * (aRK[0] + aRK[1] + bRK[1]);
+        else
+            subtime = time + dtime;
+
+        // exchange_all_all(.TRUE., vel)
+        double *vel[3] = {F(c, IH_VEL1), F(c, IH_VEL2), F(c, IH_VEL3)};
+        for (int dir = 1; dir <= 3; ++dir) {
+            int vd[3] = {1, 2, 3};
+            launch_exchange(s, g, dir, 3, vel, vd);
+        }
+        LAUNCHCHK();
+        if ((rc = ih_interpolate_vel(c))) return rc;
+
+        // rhs_vel: nonlinear + Helmholtz + rhs assembly fused per component
+        const double mult_b = dtime * bRK[sub];
+        const double mult_a = dtime * aRK[sub];
+        for (int q = 0; q < 3; ++q)
+            launch_nl_helm_rhs(s, g, c->c, q, F(c, IH_WORK1), F(c, IH_WORK2), F(c, IH_WORK3), F(c, IH_VEL1 + q),
+                               F(c, IH_NL1 + q), F(c, IH_RHS1 + q), 1.0 / p->Re, mult_a, mult_b, sub != 0 ? 1 : 0);
+        LAUNCHCHK();
+        double amp = p->inflow_amp;
+        if (p->inflow_kind == 2) {
+            const double pi = 3.14159265358979323846;
+            amp = p->inflow_amp *
+                  (1.0 + 0.5 * std::sin(2.0 * pi * subtime) + 0.2 * std::sin(4.0 * pi * subtime + 0.7) +
+                   0.1 * std::sin(6.0 * pi * subtime + 1.3));
+        }
+        launch_bc_planes(s, g, c->c, F(c, IH_RHS1), F(c, IH_RHS2), F(c, IH_RHS3), p->inflow_kind, amp,
+                         p->inflow_kind == 2 ? c->mask : nullptr);
+        LAUNCHCHK();
+
+        // pressure scaled as the increment's initial guess
+        launch_scale_box(s, g, pb, F(c, IH_PRE), aRK[sub] + bRK[sub], dtime, 0);
+
+        // explicit: vel = rhs (on the B range: swap the buffers), div, Poisson, correction
+        for (int q = 0; q < 3; ++q) std::swap(c->fields[IH_VEL1 + q], c->fields[IH_RHS1 + q]);
+        if ((rc = ih_divergence2(c, IH_VEL1, IH_VEL2, IH_VEL3, IH_RES))) return rc;
+        ih_bicgstab_stats bst{};
+        if ((rc = poisson_solve(c, p, pb, &bst))) return rc;
+        if (st) {
+            st->bicg_iterations[sub] = bst.iterations;
+            st->bicg_final[sub] = bst.norm_inf_final;
+        }
+        for (int m = 1; m <= 3; ++m) {
+            if ((rc = exch(c, m, 0, F(c, IH_PRE)))) return rc;
+            launch_grad_correct(s, g, c->c, m - 1, F(c, IH_PRE), F(c, IH_VEL1 + m - 1));
+        }
+        LAUNCHCHK();
+        launch_scale_box(s, g, pb, F(c, IH_PRE), aRK[sub] + bRK[sub], dtime, 1);
+        launch_fill_corners(s, g, F(c, IH_PRE));
+        LAUNCHCHK();
+    }
+    if (p->use_mask) {
+        if (!c->has_mask) return fail(IH_ESTATE, "no mask set");
+        if ((rc = ih_apply_mask(c))) return rc;
+    }
+    if (p->level_pressure)
+        if ((rc = ih_level_pressure(c, IH_PRE))) return rc;
+    if (st) {
+        if ((rc = ih_divergence2(c, IH_VEL1, IH_VEL2, IH_VEL3, IH_TMP2))) return rc;
+        st->div_max_after = absmax_host(c, pb, F(c, IH_TMP2), &rc);
+        if (rc) return rc;
+    }
+    HIPCHK(hipStreamSynchronize(s));
+    return IH_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Reference-signature entry points: host arrays in and out (PCIe each call),
+// as the reference's *_host.f90 wrappers do.
+// ---------------------------------------------------------------------------
+int ih_device_gradient(ih_case *c, int dimension_id, double *phi, double *grad) {
+    int rc;
+    if ((rc = ih_field_upload(c, IH_TMP1, phi))) return rc;
+    if ((rc = ih_field_upload(c, IH_TMP2, grad))) return rc;
+    if ((rc = ih_gradient(c, dimension_id, IH_TMP1, IH_TMP2))) return rc;
+    if ((rc = ih_field_download(c, IH_TMP1, phi))) return rc;
+    return ih_field_download(c, IH_TMP2, grad);
+}
+
+int ih_device_divergence2(ih_case *c, double *phi3, double *div) {
+    int rc;
+    if (!c || !phi3) return fail(IH_EINVAL, "null argument");
+    const int64_t n = ih_field_elems(c);
+    for (int q = 0; q < 3; ++q)
+        if ((rc = ih_field_upload(c, (ih_field)(IH_TMP1 + q), phi3 + q * n))) return rc;
+    if ((rc = ih_field_upload(c, IH_TMP4, div))) return rc;
+    if ((rc = ih_divergence2(c, IH_TMP1, IH_TMP2, IH_TMP3, IH_TMP4))) return rc;
+    for (int q = 0; q < 3; ++q)
+        if ((rc = ih_field_download(c, (ih_field)(IH_TMP1 + q), phi3 + q * n))) return rc;
+    return ih_field_download(c, IH_TMP4, div);
+}
+
+int ih_device_nonlinear_upwind(ih_case *c, int component_id, double *pp, double *vel_component, double *nl_component) {
+    int rc;
+    if ((rc = ih_field_upload(c, IH_TMP1, pp))) return rc;
+    if ((rc = ih_field_upload(c, IH_TMP2, vel_component))) return rc;
+    if ((rc = ih_field_upload(c, IH_TMP3, nl_component))) return rc;
+    if ((rc = ih_nonlinear_upwind(c, component_id, IH_TMP1, IH_TMP2, IH_TMP3))) return rc;
+    return ih_field_download(c, IH_TMP3, nl_component);
+}
+
+int ih_device_helmholtz(ih_case *c, int component_id, double multL, double *vel_component, double *helm_component) {
+    int rc;
+    if ((rc = ih_field_upload(c, IH_TMP1, vel_component))) return rc;
+    if ((rc = ih_field_upload(c, IH_TMP2, helm_component))) return rc;
+    if ((rc = ih_helmholtz(c, component_id, multL, IH_TMP1, IH_TMP2))) return rc;
+    return ih_field_download(c, IH_TMP2, helm_component);
+}
+
+int ih_device_axpby(ih_case *c, double a, double b, double *x, double *y, double *z) {
+    int rc;
+    if ((rc = ih_field_upload(c, IH_TMP1, x))) return rc;
+    if ((rc = ih_field_upload(c, IH_TMP2, y))) return rc;
+    if ((rc = ih_axpby(c, a, b, IH_TMP1, IH_TMP2, IH_TMP3))) return rc;
+    return ih_field_download(c, IH_TMP3, z);
+}
+
+int ih_device_interpolate(ih_case *c, int dimension_id, double *phi, double *inter) {
+    int rc;
+    if ((rc = ih_field_upload(c, IH_TMP1, phi))) return rc;
+    if ((rc = ih_field_upload(c, IH_TMP2, inter))) return rc;
+    if ((rc = ih_interpolate2_pre_vel(c, dimension_id, IH_TMP1, IH_TMP2))) return rc;
+    return ih_field_download(c, IH_TMP2, inter);
+}
+
+int ih_device_interpolate2(ih_case *c, int dimension_id, double *phi, double *inter) {
+    int rc;
+    if ((rc = ih_field_upload(c, IH_TMP1, phi))) return rc;
+    if ((rc = ih_field_upload(c, IH_TMP2, inter))) return rc;
+    if ((rc = ih_interpolate2_vel_pre(c, dimension_id, IH_TMP1, IH_TMP2))) return rc;
+    return ih_field_download(c, IH_TMP2, inter);
+}
+
+// device_jacobi_solver (src/jacobi_solver_host_v4.f90:5-717): JOR sweeps with
+// the reference's stopping test every 20 sweeps (:308-414); grid_id must be 1.
+int ih_device_jacobi_solver(ih_case *c, int grid_id, double omega, int SS1, int SS2, int SS3, int NN1, int NN2,
+                            int NN3, double *rel, double *bb) {
+    int rc = need_coeffs(c);
+    if (rc) return rc;
+    if (grid_id != 1) return fail(IH_EINVAL, "only the finest grid (grid_id 1) is supported");
+    if ((rc = ih_field_upload(c, IH_TMP1, rel))) return rc;
+    if ((rc = ih_field_upload(c, IH_TMP2, bb))) return rc;
+    const int ss[3] = {SS1, SS2, SS3}, nn[3] = {NN1, NN2, NN3};
+    Box b;
+    if ((rc = to_box(c, ss, nn, b))) return rc;
+    const int mode_j = 20;
+    const double thresh = 20. * mode_j * 8. * 1. / 3.0 * 0.8 * 3.1415 * 3.1415 *
+                          (1. / (double)(c->g.N[0] * c->g.N[0]) + 1. / (double)(c->g.N[1] * c->g.N[1]) +
+                           1. / (double)(c->g.N[2] * c->g.N[2]));
+    double err_tot_old = 0.0;
+    launch_fill(c->stream, F(c, IH_TMP3), 0.0, c->g.fe);
+    const int max_iter = 200000;
+    for (int iter = 1; iter <= max_iter; ++iter) {
+        if (iter % mode_j == 1) {
+            // err_tot = sum |rel - rel_old| (canonical tree), stop on a stalled log-ratio
+            const int64_t nb = red_blocks(b);
+            if ((rc = ensure_part(c, nb))) return rc;
+            launch_absdiff_partial(c->stream, c->g, b, F(c, IH_TMP1), F(c, IH_TMP3), c->part);
+            launch_final_sum(c->stream, c->part, nb, c->dscal + S_SUM);
+            LAUNCHCHK();
+            if ((rc = read_scalars(c, S_SUM, 1))) return rc;
+            const double err_tot = c->hscal[S_SUM];
+            const double rel_err_tot = std::fabs(std::log(err_tot_old / err_tot));
+            if (rel_err_tot <= thresh) break;
+            HIPCHK(hipMemcpyAsync(F(c, IH_TMP3), F(c, IH_TMP1), sizeof(double) * (size_t)c->g.fe,
+                                  hipMemcpyDeviceToDevice, c->stream));
+            err_tot_old = err_tot;
+        }
+        if ((rc = ih_jor(c, 1, omega, ss, nn, IH_TMP1, IH_TMP2))) return rc;
+    }
+    return ih_field_download(c, IH_TMP1, rel);
+}
+
+// ---------------------------------------------------------------------------
+int ih_timer_start(ih_case *c) {
+    if (!c) return fail(IH_EINVAL, "null case");
+    HIPCHK(hipEventRecord(c->ev0, c->stream));
+    return IH_OK;
+}
+
+int ih_timer_stop(ih_case *c, double *ms) {
+    if (!c || !ms) return fail(IH_EINVAL, "null argument");
+    HIPCHK(hipEventRecord(c->ev1, c->stream));
+    HIPCHK(hipEventSynchronize(c->ev1));
+    float f = 0.f;
+    HIPCHK(hipEventElapsedTime(&f, c->ev0, c->ev1));
+    *ms = f;
+    return IH_OK;
+}
+
+int ih_kernel_timing_enable(ih_case *c, int on) {
+    if (!c) return fail(IH_EINVAL, "null case");
+    resolve_timings(c);
+    c->ktime = on != 0;
+    c->ktimes.clear();
+    return IH_OK;
+}
+
+int ih_kernel_timing_get(ih_case *c, const char *family, double *total_ms, int64_t *launches) {
+    if (!c || !family || !total_ms || !launches) return fail(IH_EINVAL, "null argument");
+    resolve_timings(c);
+    auto it = c->ktimes.find(family);
+    if (it == c->ktimes.end()) {
+        *total_ms = 0.0;
+        *launches = 0;
+        return IH_OK;
+    }
+    *total_ms = it->second.total_ms;
+    *launches = it->second.launches;
+    return IH_OK;
+}
+
+}  // extern "C"

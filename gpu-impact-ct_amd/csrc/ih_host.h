@@ -1,0 +1,65 @@
+// Host-side structures of libimpact_hip (not part of the public ABI).
+#pragma once
+
+#include <cstdint>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/impact_hip.h"
+#include "ih_internal.h"
+
+namespace ih {
+
+struct HostGrid {
+    int N[3];
+    int bc[3][2];
+    double L[3];
+    double stretch[3];
+    std::vector<double> xp[3], xu[3];  // (-3:N+3) at [i+3]
+};
+
+// Coefficients in the reference layout c(lo:hi, 0:N), cdg (-1:1, 1:N).
+struct HostCoeffs {
+    std::vector<double> G[3], Ip[3], D[3], Iu[3], NpD[3], NpU[3], NuD[3], NuU[3], p2[3], u2[3], dg[3];
+};
+
+void host_coords(HostGrid &hg);
+bool host_coeffs(const HostGrid &hg, HostCoeffs &hc, std::string &err);
+
+struct KernelTiming {
+    double total_ms = 0.0;
+    int64_t launches = 0;
+};
+
+struct PendingTiming {
+    hipEvent_t a = nullptr, b = nullptr;
+    int64_t launches = 0;
+};
+
+}  // namespace ih
+
+struct ih_case {
+    int device = 0;
+    ih::Geo g{};
+    ih::Coef c{};
+    hipStream_t stream = nullptr;
+    double *fields[IH_NFIELDS] = {};
+    float *f32[3] = {};        // mixed precision: relaxed correction, right-hand side, ping-pong
+    uint8_t *mask = nullptr;   // 3 * fe bytes
+    int has_mask = 0;
+    double *dcoef = nullptr;   // coefficient tables (SoA) + coordinates
+    float *dcoef32 = nullptr;  // cdg in FP32
+    double *part = nullptr;    // reduction partials
+    int64_t part_cap = 0;
+    double *dscal = nullptr;   // device scalars
+    double *hscal = nullptr;   // pinned mirror
+    int64_t bytes = 0;
+    ih::HostGrid hg{};
+    ih::HostCoeffs hc{};
+    bool has_coeffs = false;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    bool ktime = false;
+    std::map<std::string, ih::KernelTiming> ktimes;
+    std::map<std::string, std::vector<ih::PendingTiming>> kpending;
+};

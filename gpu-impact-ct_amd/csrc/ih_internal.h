@@ -1,0 +1,120 @@
+// Internal declarations shared by the HIP kernels and the host side of
+// libimpact_hip.  Not part of the public ABI (include/impact_hip.h).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace ih {
+
+// Geometry of one block as the kernels see it.  Field element (i,j,k) with the
+// reference's Fortran indices (-3..N+3) lives at lead + (i+3) + (j+3)*px + (k+3)*pxy.
+struct Geo {
+    int N[3];
+    int px;
+    int64_t pxy;
+    int lead;
+    int64_t fe;          // elements allocated per field (padded)
+    int per[3];          // 1 if the direction is periodic
+    int Sp[3], Np[3];    // pressure range
+    int SB[3][3], NB[3][3];  // [component][direction] velocity ranges incl. boundary
+    int SI[3][3], NI[3][3];  // velocity ranges without boundary
+    int cst[3];          // coefficient stride per direction (>= N+1)
+};
+
+__host__ __device__ inline int64_t gidx(const Geo &g, int i, int j, int k) {
+    return (int64_t)g.lead + (i + 3) + (int64_t)(j + 3) * g.px + (int64_t)(k + 3) * g.pxy;
+}
+
+// Device coefficient tables, structure-of-arrays: c(ii, i) at base[(ii - lo) * stride + i].
+struct Coef {
+    const double *G[3];    // cGp    lo -2, 6 taps
+    const double *Ip[3];   // cIpu.. lo -2, 6 taps
+    const double *D[3];    // cDu1.. lo -3, 6 taps
+    const double *Iu[3];   // cIup.. lo -3, 6 taps
+    const double *NpD[3];  // lo -3, 7 taps
+    const double *NpU[3];
+    const double *NuD[3];
+    const double *NuU[3];
+    const double *p2[3];   // cp11..
+    const double *u2[3];   // cu11..
+    const double *dg[3];   // cdg: 3 taps (lo -1), index i = 1..N
+    const float *dgf[3];   // cdg in FP32 for the mixed-precision sweep
+    const double *xp[3];   // coordinates (-3:N+3) at [i+3]
+    const double *xu[3];
+};
+
+// Reduction tree constants, shared with the oracle's restatement.
+constexpr int RED_TX = 64;
+constexpr int RED_TY = 4;
+constexpr int RED_KC = 8;
+constexpr int RED_THREADS = 256;
+
+// Launch-side helpers implemented in kernels.hip ------------------------------
+struct Box {
+    int lo[3], hi[3];
+};
+
+void launch_exchange(hipStream_t s, const Geo &g, int dir, int nf, double *const *f, const int *vel_dir);
+void launch_exchange_relax(hipStream_t s, const Geo &g, double *f);
+void launch_interp_vel_pre3(hipStream_t s, const Geo &g, const Coef &c, const double *u, const double *v,
+                            const double *w, double *w1, double *w2, double *w3);
+void launch_interp_vel_pre(hipStream_t s, const Geo &g, const Coef &c, int q, const double *phi, double *inter);
+void launch_interp_pre_vel(hipStream_t s, const Geo &g, const Coef &c, int q, const double *phi, double *inter);
+void launch_nl_helm_rhs(hipStream_t s, const Geo &g, const Coef &c, int q, const double *w1, const double *w2,
+                        const double *w3, const double *vel, double *nl, double *rhs, double multL, double mult_a,
+                        double mult_b, int use_b);
+void launch_nonlinear_one(hipStream_t s, const Geo &g, const Coef &c, int q, int d, const double *pp,
+                          const double *vel, double *nl, int assign);
+void launch_helmholtz_one(hipStream_t s, const Geo &g, const Coef &c, int q, double multL, const double *vel,
+                          double *nl);
+void launch_bc_planes(hipStream_t s, const Geo &g, const Coef &c, double *rhs1, double *rhs2, double *rhs3,
+                      int inflow_kind, double amp, const uint8_t *mask_u);
+void launch_scale_box(hipStream_t s, const Geo &g, const Box &b, double *f, double a, double bb, int divide);
+void launch_divergence2(hipStream_t s, const Geo &g, const Coef &c, const double *u, const double *v,
+                        const double *w, double *div);
+void launch_gradient(hipStream_t s, const Geo &g, const Coef &c, int q, const double *phi, double *grad);
+void launch_grad_walls(hipStream_t s, const Geo &g, int q, double *grad);
+void launch_grad_correct(hipStream_t s, const Geo &g, const Coef &c, int q, const double *phi, double *vel);
+void launch_div_dir(hipStream_t s, const Geo &g, const Coef &c, int q, const double *dig, double *lap, int assign);
+void launch_jor(hipStream_t s, const Geo &g, const Coef &c, const Box &b, double omega, const double *rel,
+                const double *bb, double *out);
+void launch_jor_f32(hipStream_t s, const Geo &g, const Coef &c, const Box &b, float omega, const float *rel,
+                    const float *bb, float *out);
+void launch_residual_cdg(hipStream_t s, const Geo &g, const Coef &c, const Box &b, const double *rel,
+                         const double *bb, double *out_res /* may be null */, double *partial_max);
+void launch_copy_box(hipStream_t s, const Geo &g, const Box &b, const double *src, double *dst);
+void launch_axpby(hipStream_t s, const Geo &g, double a, double b, const double *x, const double *y, double *z,
+                  int64_t n);
+void launch_fill(hipStream_t s, double *f, double v, int64_t n);
+void launch_fill_f32(hipStream_t s, float *f, float v, int64_t n);
+void launch_to_f32(hipStream_t s, const Geo &g, const Box &b, const double *src, float *dst);
+void launch_correct_f32(hipStream_t s, const Geo &g, const Box &b, const float *e, double *x);
+
+// canonical reductions: returns number of partial blocks written
+int64_t red_blocks(const Box &b);
+void launch_dot_partial(hipStream_t s, const Geo &g, const Box &b, const double *a, const double *c, double *part);
+void launch_sum_partial(hipStream_t s, const Geo &g, const Box &b, const double *a, double *part);
+void launch_final_sum(hipStream_t s, const double *part, int64_t nb, double *out);
+void launch_max_partial(hipStream_t s, const Geo &g, const Box &b, const double *a, double *part);
+void launch_absdiff_partial(hipStream_t s, const Geo &g, const Box &b, const double *a, const double *c,
+                            double *part);
+void launch_final_max(hipStream_t s, const double *part, int64_t nb, double *out);
+
+// BiCGstab vector updates
+void launch_bicg_init(hipStream_t s, const Geo &g, const Box &b, const double *bb, double *rr, double *rh);
+void launch_bicg_p(hipStream_t s, const Geo &g, const Box &b, const double *rr, double *pk, const double *Ap,
+                   const double *scal /* beta at [0], omega2 at [1] on device */);
+void launch_bicg_s(hipStream_t s, const Geo &g, const Box &b, double *rr, const double *Ap, const double *alpha);
+void launch_bicg_x(hipStream_t s, const Geo &g, const Box &b, double *phi, double *rr, const double *pk,
+                   const double *Ar, const double *scal /* omega [0], alpha [1] */, double *part_max);
+void launch_copy_field_box(hipStream_t s, const Geo &g, const Box &b, const double *src, double *dst);
+
+void launch_fill_corners(hipStream_t s, const Geo &g, double *f);
+void launch_sub_box(hipStream_t s, const Geo &g, const Box &b, double *f, const double *val);
+void launch_apply_mask(hipStream_t s, const Geo &g, const uint8_t *mask, double *u, double *v, double *w);
+
+}  // namespace ih

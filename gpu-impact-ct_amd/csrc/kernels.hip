@@ -1,0 +1,1024 @@
+// HIP kernels (gfx950) of the projection-step hot path.
+//
+// Every FP64 expression keeps the operation order of the reference Fortran
+// loops it replaces (cited per kernel) and the library is built with
+// -ffp-contract=off, so each result is bit-identical to a serial evaluation of
+// the same loop.  Threads run along the unit-stride x index (64 per wavefront)
+// so every field access is a coalesced 512-byte row segment.
+#include "ih_internal.h"
+
+namespace ih {
+
+namespace {
+
+constexpr int BX = 64;
+constexpr int BY = 4;
+
+struct Range3 {
+    int lo0, lo1, lo2, n0, n1, n2;
+};
+
+inline Range3 make_range(const Box &b) {
+    return Range3{b.lo[0], b.lo[1], b.lo[2], b.hi[0] - b.lo[0] + 1, b.hi[1] - b.lo[1] + 1, b.hi[2] - b.lo[2] + 1};
+}
+
+inline dim3 grid_for(const Range3 &r) {
+    return dim3((unsigned)((r.n0 + BX - 1) / BX), (unsigned)((r.n1 + BY - 1) / BY), (unsigned)r.n2);
+}
+
+inline bool empty(const Range3 &r) { return r.n0 <= 0 || r.n1 <= 0 || r.n2 <= 0; }
+
+#define IH_POINT(r)                                                  \
+    const int i = (r).lo0 + (int)(blockIdx.x * BX + threadIdx.x);    \
+    const int j = (r).lo1 + (int)(blockIdx.y * BY + threadIdx.y);    \
+    const int k = (r).lo2 + (int)blockIdx.z;                         \
+    if (i >= (r).lo0 + (r).n0 || j >= (r).lo1 + (r).n1) return;
+
+__device__ inline int64_t sh(const Geo &g, int d) { return d == 0 ? 1 : (d == 1 ? (int64_t)g.px : g.pxy); }
+
+__device__ inline int pick(int d, int i, int j, int k) { return d == 0 ? i : (d == 1 ? j : k); }
+
+// ---------------------------------------------------------------------------
+// ghost fill: exchange(dir, vel_dir, phi), src/mod_exchange.f90:47-294.
+// Layers 0..2 lower (periodic source N?p+1+s), 3 = index 0 on walls,
+// 4..6 upper.  Transverse extent 1..N of the two other directions.
+// ---------------------------------------------------------------------------
+struct ExchArgs {
+    double *f[3];
+    int vel_dir[3];
+    int nf;
+};
+
+__global__ void k_exchange(Geo g, int dir, ExchArgs a) {
+    const int d = dir - 1;
+    const int ea = d == 0 ? 1 : 0;
+    const int eb = d == 2 ? 1 : 2;
+    const int ta = 1 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const int tb = 1 + (int)blockIdx.y;
+    const int layer = (int)blockIdx.z % 7;
+    const int fi = (int)blockIdx.z / 7;
+    if (ta > g.N[ea] || fi >= a.nf) return;
+    double *phi = a.f[fi];
+    auto at = [&](int s) -> int64_t {
+        int ijk[3];
+        ijk[d] = s;
+        ijk[ea] = ta;
+        ijk[eb] = tb;
+        return gidx(g, ijk[0], ijk[1], ijk[2]);
+    };
+    if (g.per[d]) {
+        if (layer < 3) {
+            const int s = -3 + layer;
+            phi[at(g.Sp[d] + s)] = phi[at(g.Np[d] + 1 + s)];
+        } else if (layer >= 4) {
+            const int s = layer - 3;
+            phi[at(g.Np[d] + s)] = phi[at(g.Sp[d] - 1 + s)];
+        }
+        return;
+    }
+    // walls (both sides, single block)
+    if (layer <= 3) {
+        const int idx = -3 + layer;
+        const int lo_end = (a.vel_dir[fi] == dir) ? -1 : 0;
+        if (idx <= lo_end) phi[at(idx)] = 0.0;
+    } else {
+        const int idx = g.N[d] + 1 + (layer - 4);
+        phi[at(idx)] = 0.0;
+    }
+}
+
+// exchange_relax (one layer, periodic only), src/mod_exchange.f90:1273-1480
+__global__ void k_exchange_relax(Geo g, int d, double *phi) {
+    const int ea = d == 0 ? 1 : 0;
+    const int eb = d == 2 ? 1 : 2;
+    const int ta = 1 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const int tb = 1 + (int)blockIdx.y;
+    if (ta > g.N[ea]) return;
+    int lo[3], hi[3], sl[3], sh_[3];
+    lo[ea] = hi[ea] = sl[ea] = sh_[ea] = ta;
+    lo[eb] = hi[eb] = sl[eb] = sh_[eb] = tb;
+    lo[d] = g.Sp[d] - 1;
+    sl[d] = g.Np[d];
+    hi[d] = g.Np[d] + 1;
+    sh_[d] = g.Sp[d];
+    const double a = phi[gidx(g, sl[0], sl[1], sl[2])];
+    const double b = phi[gidx(g, sh_[0], sh_[1], sh_[2])];
+    phi[gidx(g, lo[0], lo[1], lo[2])] = a;
+    phi[gidx(g, hi[0], hi[1], hi[2])] = b;
+}
+
+// ---------------------------------------------------------------------------
+// interpolate2_vel_pre, src/mod_diff.f90:1740-1825: 6 taps (-3..2) along m.
+// ---------------------------------------------------------------------------
+__device__ inline double interp_up(const Geo &g, const double *c, int cst, int q, const double *phi, int64_t x,
+                                   int idx) {
+    const int64_t s = sh(g, q);
+    double v = c[0 * cst + idx] * phi[x - 3 * s];
+#pragma unroll
+    for (int t = -2; t <= 2; ++t) v = v + c[(t + 3) * cst + idx] * phi[x + t * s];
+    return v;
+}
+
+__global__ void k_interp_vel_pre3(Geo g, Coef c, Range3 r, const double *u, const double *v, const double *w,
+                                  double *w1, double *w2, double *w3) {
+    IH_POINT(r);
+    const int64_t x = gidx(g, i, j, k);
+    w1[x] = interp_up(g, c.Iu[0], g.cst[0], 0, u, x, i);
+    w2[x] = interp_up(g, c.Iu[1], g.cst[1], 1, v, x, j);
+    w3[x] = interp_up(g, c.Iu[2], g.cst[2], 2, w, x, k);
+}
+
+__global__ void k_interp_vel_pre(Geo g, Coef c, Range3 r, int q, const double *phi, double *inter) {
+    IH_POINT(r);
+    const int64_t x = gidx(g, i, j, k);
+    inter[x] = interp_up(g, c.Iu[q], g.cst[q], q, phi, x, pick(q, i, j, k));
+}
+
+// interpolate2_pre_vel, src/mod_diff.f90:1567-1648: 6 taps (-2..3) along m.
+__device__ inline double interp_pu(const Geo &g, const double *c, int cst, int q, const double *phi, int64_t x,
+                                   int idx) {
+    const int64_t s = sh(g, q);
+    double v = c[0 * cst + idx] * phi[x - 2 * s];
+#pragma unroll
+    for (int t = -1; t <= 3; ++t) v = v + c[(t + 2) * cst + idx] * phi[x + t * s];
+    return v;
+}
+
+__global__ void k_interp_pre_vel(Geo g, Coef c, Range3 r, int q, const double *phi, double *inter) {
+    IH_POINT(r);
+    const int64_t x = gidx(g, i, j, k);
+    inter[x] = interp_pu(g, c.Ip[q], g.cst[q], q, phi, x, pick(q, i, j, k));
+}
+
+// 7-tap (-3..3) stencil along direction d with coefficient index idx.
+__device__ inline double tap7(const double *c, int cst, int idx, const double *f, int64_t x, int64_t s) {
+    double v = c[0 * cst + idx] * f[x - 3 * s];
+#pragma unroll
+    for (int t = -2; t <= 3; ++t) v = v + c[(t + 3) * cst + idx] * f[x + t * s];
+    return v;
+}
+
+// ---------------------------------------------------------------------------
+// Fused nonlinear (upwind, src/mod_diff.f90:784-1424) + Helmholtz_explicit
+// (src/mod_diff.f90:549-762) + rhs assembly (src/mod_rhs.f90:56-130) for one
+// velocity component over its interior range.  The advection velocity
+// pp = I_m(work_d) is interpolated in registers instead of a 39 MB field.
+// ---------------------------------------------------------------------------
+__global__ void k_nl_helm_rhs(Geo g, Coef c, Range3 r, int q, const double *w1, const double *w2, const double *w3,
+                              const double *vel, double *nl, double *rhs, double multL, double mult_a, double mult_b,
+                              int use_b) {
+    IH_POINT(r);
+    const int64_t x = gidx(g, i, j, k);
+    const int iq = pick(q, i, j, k);
+    const double *work[3] = {w1, w2, w3};
+    double acc = 0.0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const double pp = interp_pu(g, c.Ip[q], g.cst[q], q, work[d], x, iq);
+        const double *cU = (d == q) ? c.NuU[d] : c.NpU[d];
+        const double *cD = (d == q) ? c.NuD[d] : c.NpD[d];
+        const double *cc = (pp >= 0.0) ? cU : cD;
+        const double dd1 = tap7(cc, g.cst[d], pick(d, i, j, k), vel, x, sh(g, d));
+        acc = (d == 0) ? dd1 * pp : acc + dd1 * pp;
+    }
+    // Helmholtz: 21 taps accumulated in one sum
+    {
+        const double *c1 = (q == 0) ? c.u2[0] : c.p2[0];
+        const double *c2 = (q == 1) ? c.u2[1] : c.p2[1];
+        const double *c3 = (q == 2) ? c.u2[2] : c.p2[2];
+        const int64_t s1 = 1, s2 = g.px, s3 = g.pxy;
+        double dd1 = c1[0 * g.cst[0] + i] * vel[x - 3 * s1];
+#pragma unroll
+        for (int t = -2; t <= 3; ++t) dd1 = dd1 + c1[(t + 3) * g.cst[0] + i] * vel[x + t * s1];
+#pragma unroll
+        for (int t = -3; t <= 3; ++t) dd1 = dd1 + c2[(t + 3) * g.cst[1] + j] * vel[x + t * s2];
+#pragma unroll
+        for (int t = -3; t <= 3; ++t) dd1 = dd1 + c3[(t + 3) * g.cst[2] + k] * vel[x + t * s3];
+        acc = acc - multL * dd1;
+    }
+    double rr = vel[x];
+    if (use_b) rr = rr - mult_b * nl[x];
+    rr = rr - mult_a * acc;
+    nl[x] = acc;
+    rhs[x] = rr;
+}
+
+// one (component, direction) pass of nonlinear, for device_Nonlinear_upwind
+__global__ void k_nonlinear_one(Geo g, Coef c, Range3 r, int q, int d, const double *pp_f, const double *vel,
+                                double *nl, int assign) {
+    IH_POINT(r);
+    const int64_t x = gidx(g, i, j, k);
+    const double pp = pp_f[x];
+    const double *cU = (d == q) ? c.NuU[d] : c.NpU[d];
+    const double *cD = (d == q) ? c.NuD[d] : c.NpD[d];
+    const double *cc = (pp >= 0.0) ? cU : cD;
+    const double dd1 = tap7(cc, g.cst[d], pick(d, i, j, k), vel, x, sh(g, d));
+    nl[x] = assign ? dd1 * pp : nl[x] + dd1 * pp;
+}
+
+__global__ void k_helmholtz_one(Geo g, Coef c, Range3 r, int q, double multL, const double *vel, double *nl) {
+    IH_POINT(r);
+    const int64_t x = gidx(g, i, j, k);
+    const double *c1 = (q == 0) ? c.u2[0] : c.p2[0];
+    const double *c2 = (q == 1) ? c.u2[1] : c.p2[1];
+    const double *c3 = (q == 2) ? c.u2[2] : c.p2[2];
+    double dd1 = c1[0 * g.cst[0] + i] * vel[x - 3];
+#pragma unroll
+    for (int t = -2; t <= 3; ++t) dd1 = dd1 + c1[(t + 3) * g.cst[0] + i] * vel[x + t];
+#pragma unroll
+    for (int t = -3; t <= 3; ++t) dd1 = dd1 + c2[(t + 3) * g.cst[1] + j] * vel[x + t * (int64_t)g.px];
+#pragma unroll
+    for (int t = -3; t <= 3; ++t) dd1 = dd1 + c3[(t + 3) * g.cst[2] + k] * vel[x + t * g.pxy];
+    nl[x] = nl[x] - multL * dd1;
+}
+
+// ---------------------------------------------------------------------------
+// Dirichlet planes of rhs (src/mod_rhs.f90:213-241).  Each wall plane of a
+// component's B range gets its boundary value; where planes of one component
+// meet, the plane written last by the reference's order wins, so a thread
+// skips a point that a later plane also covers.
+// ---------------------------------------------------------------------------
+__device__ inline double inflow_u(const Coef &c, int kind, double amp, const uint8_t *mask_u, const Geo &g, int ib,
+                                  int j, int k) {
+    if (kind == 1) {
+        const double y = c.xp[1][j + 3], z = c.xp[2][k + 3];
+        return amp * (16.0 * y * (1.0 - y) * z * (1.0 - z));
+    }
+    if (kind == 2) {
+        const uint8_t fl = mask_u ? mask_u[gidx(g, ib, j, k)] : (uint8_t)1;
+        return fl ? amp : 0.0;
+    }
+    return 0.0;
+}
+
+__global__ void k_bc_planes(Geo g, Coef c, double *r1, double *r2, double *r3, int kind, double amp,
+                            const uint8_t *mask_u) {
+    // blockIdx.z: comp * 6 + face (e*2 + side); threads span the plane
+    const int q = (int)blockIdx.z / 6;
+    const int face = (int)blockIdx.z % 6;
+    const int e = face / 2, side = face % 2;
+    // reference order of the faces per component (last wins)
+    const int order[3][3] = {{1, 2, 0}, {0, 2, 1}, {0, 1, 2}};
+    if (g.per[e]) return;
+    const int ea = e == 0 ? 1 : 0;
+    const int eb = e == 2 ? 1 : 2;
+    const int ta = g.SB[q][ea] + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const int tb = g.SB[q][eb] + (int)blockIdx.y;
+    if (ta > g.NB[q][ea] || tb > g.NB[q][eb]) return;
+    int ijk[3];
+    ijk[ea] = ta;
+    ijk[eb] = tb;
+    ijk[e] = side == 0 ? (e == q ? 0 : 1) : g.N[e];
+    // rank of this face in the order
+    int rank = 0;
+    for (int o = 0; o < 3; ++o)
+        if (order[q][o] == e) rank = o;
+    for (int o = rank + 1; o < 3; ++o) {
+        const int f = order[q][o];
+        if (g.per[f]) continue;
+        const int lo = (f == q) ? 0 : 1;
+        if (ijk[f] == lo || ijk[f] == g.N[f]) return;  // a later plane owns this point
+    }
+    double *rhs = q == 0 ? r1 : (q == 1 ? r2 : r3);
+    double val = 0.0;
+    if (q == 0 && e == 0) val = inflow_u(c, kind, amp, mask_u, g, ijk[0], ijk[1], ijk[2]);
+    rhs[gidx(g, ijk[0], ijk[1], ijk[2])] = val;
+}
+
+// f = (f * a) * b  or  f = (f / a) / b over a box (pressure scaling, mod_timeint.f90:318,372)
+__global__ void k_scale_box(Geo g, Range3 r, double *f, double a, double b, int divide) {
+    IH_POINT(r);
+    const int64_t x = gidx(g, i, j, k);
+    f[x] = divide ? f[x] / a / b : f[x] * a * b;
+}
+
+// divergence2, src/mod_diff.f90:130-215
+__global__ void k_divergence2(Geo g, Coef c, Range3 r, const double *u, const double *v, const double *w,
+                              double *div) {
+    IH_POINT(r);
+    const int64_t x = gidx(g, i, j, k);
+    const int c0 = g.cst[0], c1 = g.cst[1], c2 = g.cst[2];
+    const double *d1 = c.D[0], *d2 = c.D[1], *d3 = c.D[2];
+    double s = d1[0 * c0 + i] * u[x - 3];
+#pragma unroll
+    for (int t = -2; t <= 2; ++t) s = s + d1[(t + 3) * c0 + i] * u[x + t];
+#pragma unroll
+    for (int t = -3; t <= 2; ++t) s = s + d2[(t + 3) * c1 + j] * v[x + t * (int64_t)g.px];
+#pragma unroll
+    for (int t = -3; t <= 2; ++t) s = s + d3[(t + 3) * c2 + k] * w[x + t * g.pxy];
+    div[x] = s;
+}
+
+// gradient, src/mod_diff.f90:309-414: 6 taps (-2..3) along m.  For m = 1 the
+// reference multiplies phi*c, for m = 2, 3 c*phi: the products are identical.
+__device__ inline double grad_at(const Geo &g, const Coef &c, int q, const double *phi, int64_t x, int idx) {
+    const int64_t s = sh(g, q);
+    const double *cg = c.G[q];
+    const int cst = g.cst[q];
+    double v = cg[0 * cst + idx] * phi[x - 2 * s];
+#pragma unroll
+    for (int t = -1; t <= 3; ++t) v = v + cg[(t + 2) * cst + idx] * phi[x + t * s];
+    return v;
+}
+
+__global__ void k_gradient(Geo g, Coef c, Range3 r, int q, const double *phi, double *grad) {
+    IH_POINT(r);
+    const int64_t x = gidx(g, i, j, k);
+    grad[x] = grad_at(g, c, q, phi, x, pick(q, i, j, k));
+}
+
+// velocity correction vel_m = vel_m - G_m p (src/mod_solvers.f90:273-330);
+// outside the interior range the reference's gradient is zero on the walls.
+__global__ void k_grad_correct(Geo g, Coef c, Range3 r, int q, const double *phi, double *vel) {
+    IH_POINT(r);
+    const int64_t x = gidx(g, i, j, k);
+    vel[x] = vel[x] - grad_at(g, c, q, phi, x, pick(q, i, j, k));
+}
+
+// zero the wall planes of a gradient field (src/mod_diff.f90:339-346 etc.)
+__global__ void k_grad_walls(Geo g, int q, double *grad) {
+    const int face = (int)blockIdx.z;
+    const int e = face / 2, side = face % 2;
+    if (g.per[e]) return;
+    const int ea = e == 0 ? 1 : 0;
+    const int eb = e == 2 ? 1 : 2;
+    const int ta = g.SB[q][ea] + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const int tb = g.SB[q][eb] + (int)blockIdx.y;
+    if (ta > g.NB[q][ea] || tb > g.NB[q][eb]) return;
+    int ijk[3];
+    ijk[ea] = ta;
+    ijk[eb] = tb;
+    ijk[e] = side == 0 ? (e == q ? 0 : 1) : g.N[e];
+    grad[gidx(g, ijk[0], ijk[1], ijk[2])] = 0.0;
+}
+
+// divergence in one direction, src/mod_diff.f90:52-118 (assign for m = 1)
+__global__ void k_div_dir(Geo g, Coef c, Range3 r, int q, const double *dig, double *lap, int assign) {
+    IH_POINT(r);
+    const int64_t x = gidx(g, i, j, k);
+    const int64_t s = sh(g, q);
+    const int idx = pick(q, i, j, k);
+    const double *cd = c.D[q];
+    const int cst = g.cst[q];
+    double v;
+    int t0;
+    if (assign) {
+        v = cd[0 * cst + idx] * dig[x - 3 * s];
+        t0 = -2;
+    } else {
+        v = lap[x];
+        t0 = -3;
+    }
+    for (int t = t0; t <= 2; ++t) v = v + cd[(t + 3) * cst + idx] * dig[x + t * s];
+    lap[x] = v;
+}
+
+// ---------------------------------------------------------------------------
+// JOR sweep on the compact cdg operator (src/mod_laplace.f90:514-518), out of
+// place.  The one-layer ghost of a periodic direction is read from the
+// opposite side of the block (what exchange_relax would have copied there),
+// so no ghost-fill pass runs between sweeps.
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ inline void jor_neighbours(const Geo &g, int i, int j, int k, int64_t x, int64_t &xm1, int64_t &xp1,
+                                      int64_t &xm2, int64_t &xp2, int64_t &xm3, int64_t &xp3) {
+    xm1 = x - 1;
+    xp1 = x + 1;
+    xm2 = x - g.px;
+    xp2 = x + g.px;
+    xm3 = x - g.pxy;
+    xp3 = x + g.pxy;
+    if (g.per[0]) {
+        if (i == g.Sp[0]) xm1 = x + (g.Np[0] - g.Sp[0]);
+        if (i == g.Np[0]) xp1 = x - (g.Np[0] - g.Sp[0]);
+    }
+    if (g.per[1]) {
+        if (j == g.Sp[1]) xm2 = x + (int64_t)(g.Np[1] - g.Sp[1]) * g.px;
+        if (j == g.Np[1]) xp2 = x - (int64_t)(g.Np[1] - g.Sp[1]) * g.px;
+    }
+    if (g.per[2]) {
+        if (k == g.Sp[2]) xm3 = x + (int64_t)(g.Np[2] - g.Sp[2]) * g.pxy;
+        if (k == g.Np[2]) xp3 = x - (int64_t)(g.Np[2] - g.Sp[2]) * g.pxy;
+    }
+}
+
+__global__ void k_jor(Geo g, Coef c, Range3 r, double omega, const double *rel, const double *bb, double *out) {
+    IH_POINT(r);
+    const int64_t x = gidx(g, i, j, k);
+    int64_t xm1, xp1, xm2, xp2, xm3, xp3;
+    jor_neighbours<double>(g, i, j, k, x, xm1, xp1, xm2, xp2, xm3, xp3);
+    const double *g1 = c.dg[0], *g2 = c.dg[1], *g3 = c.dg[2];
+    const int c0 = g.cst[0], c1 = g.cst[1], c2 = g.cst[2];
+    double t = bb[x];
+    t = t - g1[0 * c0 + i] * rel[xm1];
+    t = t - g1[2 * c0 + i] * rel[xp1];
+    t = t - g2[0 * c1 + j] * rel[xm2];
+    t = t - g2[2 * c1 + j] * rel[xp2];
+    t = t - g3[0 * c2 + k] * rel[xm3];
+    t = t - g3[2 * c2 + k] * rel[xp3];
+    const double diag = (g1[1 * c0 + i] + g2[1 * c1 + j]) + g3[1 * c2 + k];
+    out[x] = omega * t / diag + (1.0 - omega) * rel[x];
+}
+
+__global__ void k_jor_f32(Geo g, Coef c, Range3 r, float omega, const float *rel, const float *bb, float *out) {
+    IH_POINT(r);
+    const int64_t x = gidx(g, i, j, k);
+    int64_t xm1, xp1, xm2, xp2, xm3, xp3;
+    jor_neighbours<float>(g, i, j, k, x, xm1, xp1, xm2, xp2, xm3, xp3);
+    const float *g1 = c.dgf[0], *g2 = c.dgf[1], *g3 = c.dgf[2];
+    const int c0 = g.cst[0], c1 = g.cst[1], c2 = g.cst[2];
+    float t = bb[x];
+    t = t - g1[0 * c0 + i] * rel[xm1];
+    t = t - g1[2 * c0 + i] * rel[xp1];
+    t = t - g2[0 * c1 + j] * rel[xm2];
+    t = t - g2[2 * c1 + j] * rel[xp2];
+    t = t - g3[0 * c2 + k] * rel[xm3];
+    t = t - g3[2 * c2 + k] * rel[xp3];
+    const float diag = (g1[1 * c0 + i] + g2[1 * c1 + j]) + g3[1 * c2 + k];
+    out[x] = omega * t / diag + (1.0f - omega) * rel[x];
+}
+
+// ---------------------------------------------------------------------------
+// Reductions.  Block tile 64 x 4 threads, each thread adds RED_KC planes in
+// order; the wavefront combines by an xor butterfly (offsets 32..1) and the
+// four wave sums as (w0+w1)+(w2+w3).  The result does not depend on the
+// launch or on scheduling, so repeated solves are bit-reproducible.
+// ---------------------------------------------------------------------------
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v = v + __shfl_xor(v, off, 64);
+    return v;
+}
+
+__device__ inline double wave_max(double v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        const double o = __shfl_xor(v, off, 64);
+        v = (o > v) ? o : v;
+    }
+    return v;
+}
+
+__device__ inline double block_sum(double v, double *sm) {
+    v = wave_sum(v);
+    const int tid = threadIdx.x + threadIdx.y * blockDim.x;
+    if ((tid & 63) == 0) sm[tid >> 6] = v;
+    __syncthreads();
+    double r = 0.0;
+    if (tid == 0) r = (sm[0] + sm[1]) + (sm[2] + sm[3]);
+    return r;
+}
+
+__device__ inline double block_max(double v, double *sm) {
+    v = wave_max(v);
+    const int tid = threadIdx.x + threadIdx.y * blockDim.x;
+    if ((tid & 63) == 0) sm[tid >> 6] = v;
+    __syncthreads();
+    double r = 0.0;
+    if (tid == 0) {
+        r = sm[0];
+        for (int q = 1; q < 4; ++q) r = (sm[q] > r) ? sm[q] : r;
+    }
+    return r;
+}
+
+template <typename F>
+__device__ inline double tile_sum(const Geo &g, Range3 r, F f) {
+    const int i = r.lo0 + (int)(blockIdx.x * RED_TX + threadIdx.x);
+    const int j = r.lo1 + (int)(blockIdx.y * RED_TY + threadIdx.y);
+    double acc = 0.0;
+    if (i < r.lo0 + r.n0 && j < r.lo1 + r.n1) {
+        for (int t = 0; t < RED_KC; ++t) {
+            const int k = r.lo2 + (int)blockIdx.z * RED_KC + t;
+            if (k < r.lo2 + r.n2) acc = acc + f(gidx(g, i, j, k));
+        }
+    }
+    return acc;
+}
+
+__device__ inline int64_t part_index() {
+    return (int64_t)blockIdx.x + (int64_t)gridDim.x * ((int64_t)blockIdx.y + (int64_t)gridDim.y * blockIdx.z);
+}
+
+__global__ __launch_bounds__(256) void k_dot_partial(Geo g, Range3 r, const double *a, const double *b,
+                                                     double *part) {
+    __shared__ double sm[4];
+    const double acc = tile_sum(g, r, [&](int64_t x) { return a[x] * b[x]; });
+    const double s = block_sum(acc, sm);
+    if (threadIdx.x == 0 && threadIdx.y == 0) part[part_index()] = s;
+}
+
+__global__ __launch_bounds__(256) void k_sum_partial(Geo g, Range3 r, const double *a, double *part) {
+    __shared__ double sm[4];
+    const double acc = tile_sum(g, r, [&](int64_t x) { return a[x]; });
+    const double s = block_sum(acc, sm);
+    if (threadIdx.x == 0 && threadIdx.y == 0) part[part_index()] = s;
+}
+
+__global__ __launch_bounds__(256) void k_absdiff_partial(Geo g, Range3 r, const double *a, const double *b,
+                                                         double *part) {
+    __shared__ double sm[4];
+    const double acc = tile_sum(g, r, [&](int64_t x) { return fabs(a[x] - b[x]); });
+    const double s = block_sum(acc, sm);
+    if (threadIdx.x == 0 && threadIdx.y == 0) part[part_index()] = s;
+}
+
+__global__ __launch_bounds__(256) void k_max_partial(Geo g, Range3 r, const double *a, double *part) {
+    __shared__ double sm[4];
+    const int i = r.lo0 + (int)(blockIdx.x * RED_TX + threadIdx.x);
+    const int j = r.lo1 + (int)(blockIdx.y * RED_TY + threadIdx.y);
+    double m = 0.0;
+    if (i < r.lo0 + r.n0 && j < r.lo1 + r.n1) {
+        for (int t = 0; t < RED_KC; ++t) {
+            const int k = r.lo2 + (int)blockIdx.z * RED_KC + t;
+            if (k < r.lo2 + r.n2) {
+                const double v = fabs(a[gidx(g, i, j, k)]);
+                m = (v > m) ? v : m;
+            }
+        }
+    }
+    const double s = block_max(m, sm);
+    if (threadIdx.x == 0 && threadIdx.y == 0) part[part_index()] = s;
+}
+
+__global__ __launch_bounds__(256) void k_final_sum(const double *part, int64_t nb, double *out) {
+    __shared__ double sm[4];
+    const int tid = threadIdx.x + threadIdx.y * blockDim.x;
+    double acc = 0.0;
+    for (int64_t q = tid; q < nb; q += RED_THREADS) acc = acc + part[q];
+    const double s = block_sum(acc, sm);
+    if (tid == 0) *out = s;
+}
+
+__global__ __launch_bounds__(256) void k_final_max(const double *part, int64_t nb, double *out) {
+    __shared__ double sm[4];
+    const int tid = threadIdx.x + threadIdx.y * blockDim.x;
+    double m = 0.0;
+    for (int64_t q = tid; q < nb; q += RED_THREADS) m = (part[q] > m) ? part[q] : m;
+    const double s = block_max(m, sm);
+    if (tid == 0) *out = s;
+}
+
+// max |bb - A_cdg rel| (src/jacobi_solver_host_v4.f90:357-371) as block partials
+__global__ __launch_bounds__(256) void k_residual_cdg(Geo g, Coef c, Range3 r, const double *rel, const double *bb,
+                                                      double *res_out, double *part) {
+    __shared__ double sm[4];
+    const int i = r.lo0 + (int)(blockIdx.x * RED_TX + threadIdx.x);
+    const int j = r.lo1 + (int)(blockIdx.y * RED_TY + threadIdx.y);
+    double m = 0.0;
+    if (i < r.lo0 + r.n0 && j < r.lo1 + r.n1) {
+        const double *g1 = c.dg[0], *g2 = c.dg[1], *g3 = c.dg[2];
+        const int c0 = g.cst[0], c1 = g.cst[1], c2 = g.cst[2];
+        for (int tt = 0; tt < RED_KC; ++tt) {
+            const int k = r.lo2 + (int)blockIdx.z * RED_KC + tt;
+            if (k >= r.lo2 + r.n2) break;
+            const int64_t x = gidx(g, i, j, k);
+            int64_t xm1, xp1, xm2, xp2, xm3, xp3;
+            jor_neighbours<double>(g, i, j, k, x, xm1, xp1, xm2, xp2, xm3, xp3);
+            double t = bb[x];
+            t = t - g1[0 * c0 + i] * rel[xm1];
+            t = t - g1[2 * c0 + i] * rel[xp1];
+            t = t - g2[0 * c1 + j] * rel[xm2];
+            t = t - g2[2 * c1 + j] * rel[xp2];
+            t = t - g3[0 * c2 + k] * rel[xm3];
+            t = t - g3[2 * c2 + k] * rel[xp3];
+            t = t - ((g1[1 * c0 + i] + g2[1 * c1 + j]) + g3[1 * c2 + k]) * rel[x];
+            if (res_out) res_out[x] = t;
+            const double a = fabs(t);
+            m = (a > m) ? a : m;
+        }
+    }
+    const double s = block_max(m, sm);
+    if (threadIdx.x == 0 && threadIdx.y == 0) part[part_index()] = s;
+}
+
+// ---------------------------------------------------------------------------
+// BiCGstab vector updates (src/mod_solvers.f90:1576-1930)
+// ---------------------------------------------------------------------------
+__global__ void k_bicg_init(Geo g, Range3 r, const double *bb, double *rr, double *rh) {
+    IH_POINT(r);
+    const int64_t x = gidx(g, i, j, k);
+    const double v = bb[x] - rr[x];
+    rr[x] = v;
+    rh[x] = v;
+}
+
+__global__ void k_bicg_p(Geo g, Range3 r, const double *rr, double *pk, const double *Ap, const double *scal) {
+    IH_POINT(r);
+    const int64_t x = gidx(g, i, j, k);
+    const double beta = scal[0], om = scal[1];
+    pk[x] = rr[x] + beta * pk[x] + om * Ap[x];
+}
+
+__global__ void k_bicg_s(Geo g, Range3 r, double *rr, const double *Ap, const double *alpha) {
+    IH_POINT(r);
+    const int64_t x = gidx(g, i, j, k);
+    rr[x] = rr[x] - (*alpha) * Ap[x];
+}
+
+__global__ __launch_bounds__(256) void k_bicg_x(Geo g, Range3 r, double *phi, double *rr, const double *pk,
+                                                const double *Ar, const double *scal, double *part) {
+    __shared__ double sm[4];
+    const double om = scal[0], alpha = scal[1];
+    const int i = r.lo0 + (int)(blockIdx.x * RED_TX + threadIdx.x);
+    const int j = r.lo1 + (int)(blockIdx.y * RED_TY + threadIdx.y);
+    double m = 0.0;
+    if (i < r.lo0 + r.n0 && j < r.lo1 + r.n1) {
+        for (int t = 0; t < RED_KC; ++t) {
+            const int k = r.lo2 + (int)blockIdx.z * RED_KC + t;
+            if (k >= r.lo2 + r.n2) break;
+            const int64_t x = gidx(g, i, j, k);
+            phi[x] = phi[x] + om * rr[x] + alpha * pk[x];
+            const double v = rr[x] - om * Ar[x];
+            rr[x] = v;
+            const double a = fabs(v);
+            m = (a > m) ? a : m;
+        }
+    }
+    const double s = block_max(m, sm);
+    if (threadIdx.x == 0 && threadIdx.y == 0) part[part_index()] = s;
+}
+
+__global__ void k_copy_box(Geo g, Range3 r, const double *src, double *dst) {
+    IH_POINT(r);
+    const int64_t x = gidx(g, i, j, k);
+    dst[x] = src[x];
+}
+
+__global__ void k_axpby(double a, double b, const double *x, const double *y, double *z, int64_t n) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) z[t] = a * x[t] + b * y[t];
+}
+
+__global__ void k_fill(double *f, double v, int64_t n) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) f[t] = v;
+}
+
+__global__ void k_fill_f32(float *f, float v, int64_t n) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) f[t] = v;
+}
+
+__global__ void k_to_f32(Geo g, Range3 r, const double *src, float *dst) {
+    IH_POINT(r);
+    const int64_t x = gidx(g, i, j, k);
+    dst[x] = (float)src[x];
+}
+
+__global__ void k_correct_f32(Geo g, Range3 r, const float *e, double *xx) {
+    IH_POINT(r);
+    const int64_t x = gidx(g, i, j, k);
+    xx[x] = xx[x] + (double)e[x];
+}
+
+// fill_corners, src/mod_lib.f90:549-575: the twelve edges in the reference's order
+__global__ void k_fill_corners(Geo g, double *f, int edge) {
+    const int t = 1 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const int N1 = g.N[0], N2 = g.N[1], N3 = g.N[2];
+    auto F = [&](int i, int j, int k) -> double & { return f[gidx(g, i, j, k)]; };
+    const int grp = edge / 4, e = edge % 4;
+    if (grp == 0) {
+        if (t > N3) return;
+        const int k = t;
+        const int i0 = (e < 2) ? 1 : N1, id = (e < 2) ? 2 : N1 - 1;
+        const int j0 = (e % 2 == 0) ? 1 : N2, jd = (e % 2 == 0) ? 2 : N2 - 1;
+        F(i0, j0, k) = (F(id, j0, k) + F(i0, jd, k) + F(id, jd, k)) / 3.0;
+    } else if (grp == 1) {
+        if (t > N2) return;
+        const int j = t;
+        const int i0 = (e < 2) ? 1 : N1, id = (e < 2) ? 2 : N1 - 1;
+        const int k0 = (e % 2 == 0) ? 1 : N3, kd = (e % 2 == 0) ? 2 : N3 - 1;
+        F(i0, j, k0) = (F(id, j, k0) + F(i0, j, kd) + F(id, j, kd)) / 3.0;
+    } else {
+        if (t > N1) return;
+        const int i = t;
+        const int j0 = (e < 2) ? 1 : N2, jd = (e < 2) ? 2 : N2 - 1;
+        const int k0 = (e % 2 == 0) ? 1 : N3, kd = (e % 2 == 0) ? 2 : N3 - 1;
+        F(i, j0, k0) = (F(i, jd, k0) + F(i, j0, kd) + F(i, jd, kd)) / 3.0;
+    }
+}
+
+__global__ void k_sub_box(Geo g, Range3 r, double *f, const double *val) {
+    IH_POINT(r);
+    const int64_t x = gidx(g, i, j, k);
+    f[x] = f[x] - *val;
+}
+
+__global__ void k_apply_mask(Geo g, Range3 r, int q, const uint8_t *mask, double *v) {
+    IH_POINT(r);
+    const int64_t x = gidx(g, i, j, k);
+    if (!mask[x]) v[x] = 0.0;
+}
+
+}  // namespace
+
+// ===========================================================================
+// launchers
+// ===========================================================================
+static inline Range3 R(const Box &b) { return make_range(b); }
+
+void launch_exchange(hipStream_t s, const Geo &g, int dir, int nf, double *const *f, const int *vel_dir) {
+    const int d = dir - 1;
+    const int ea = d == 0 ? 1 : 0;
+    const int eb = d == 2 ? 1 : 2;
+    ExchArgs a{};
+    a.nf = nf;
+    for (int q = 0; q < nf; ++q) {
+        a.f[q] = f[q];
+        a.vel_dir[q] = vel_dir[q];
+    }
+    dim3 blk(64, 1, 1);
+    dim3 grd((unsigned)((g.N[ea] + 63) / 64), (unsigned)g.N[eb], (unsigned)(7 * nf));
+    hipLaunchKernelGGL(k_exchange, grd, blk, 0, s, g, dir, a);
+}
+
+void launch_exchange_relax(hipStream_t s, const Geo &g, double *f) {
+    // directions in the reference's order; each reads only interior values
+    for (int d = 0; d < 3; ++d) {
+        if (!g.per[d]) continue;
+        const int ea = d == 0 ? 1 : 0;
+        const int eb = d == 2 ? 1 : 2;
+        dim3 grd((unsigned)((g.N[ea] + 63) / 64), (unsigned)g.N[eb], 1);
+        hipLaunchKernelGGL(k_exchange_relax, grd, dim3(64, 1, 1), 0, s, g, d, f);
+    }
+}
+
+static Box pbox(const Geo &g) {
+    Box b;
+    for (int d = 0; d < 3; ++d) {
+        b.lo[d] = g.Sp[d];
+        b.hi[d] = g.Np[d];
+    }
+    return b;
+}
+
+static Box ibox(const Geo &g, int q) {
+    Box b;
+    for (int d = 0; d < 3; ++d) {
+        b.lo[d] = g.SI[q][d];
+        b.hi[d] = g.NI[q][d];
+    }
+    return b;
+}
+
+void launch_interp_vel_pre3(hipStream_t s, const Geo &g, const Coef &c, const double *u, const double *v,
+                            const double *w, double *w1, double *w2, double *w3) {
+    Range3 r = R(pbox(g));
+    if (empty(r)) return;
+    hipLaunchKernelGGL(k_interp_vel_pre3, grid_for(r), dim3(BX, BY, 1), 0, s, g, c, r, u, v, w, w1, w2, w3);
+}
+
+void launch_interp_vel_pre(hipStream_t s, const Geo &g, const Coef &c, int q, const double *phi, double *inter) {
+    Range3 r = R(pbox(g));
+    if (empty(r)) return;
+    hipLaunchKernelGGL(k_interp_vel_pre, grid_for(r), dim3(BX, BY, 1), 0, s, g, c, r, q, phi, inter);
+}
+
+void launch_interp_pre_vel(hipStream_t s, const Geo &g, const Coef &c, int q, const double *phi, double *inter) {
+    Range3 r = R(ibox(g, q));
+    if (empty(r)) return;
+    hipLaunchKernelGGL(k_interp_pre_vel, grid_for(r), dim3(BX, BY, 1), 0, s, g, c, r, q, phi, inter);
+}
+
+void launch_nl_helm_rhs(hipStream_t s, const Geo &g, const Coef &c, int q, const double *w1, const double *w2,
+                        const double *w3, const double *vel, double *nl, double *rhs, double multL, double mult_a,
+                        double mult_b, int use_b) {
+    Range3 r = R(ibox(g, q));
+    if (empty(r)) return;
+    hipLaunchKernelGGL(k_nl_helm_rhs, grid_for(r), dim3(BX, BY, 1), 0, s, g, c, r, q, w1, w2, w3, vel, nl, rhs,
+                       multL, mult_a, mult_b, use_b);
+}
+
+void launch_nonlinear_one(hipStream_t s, const Geo &g, const Coef &c, int q, int d, const double *pp,
+                          const double *vel, double *nl, int assign) {
+    Range3 r = R(ibox(g, q));
+    if (empty(r)) return;
+    hipLaunchKernelGGL(k_nonlinear_one, grid_for(r), dim3(BX, BY, 1), 0, s, g, c, r, q, d, pp, vel, nl, assign);
+}
+
+void launch_helmholtz_one(hipStream_t s, const Geo &g, const Coef &c, int q, double multL, const double *vel,
+                          double *nl) {
+    Range3 r = R(ibox(g, q));
+    if (empty(r)) return;
+    hipLaunchKernelGGL(k_helmholtz_one, grid_for(r), dim3(BX, BY, 1), 0, s, g, c, r, q, multL, vel, nl);
+}
+
+void launch_bc_planes(hipStream_t s, const Geo &g, const Coef &c, double *rhs1, double *rhs2, double *rhs3,
+                      int inflow_kind, double amp, const uint8_t *mask_u) {
+    int maxa = 0, maxb = 0;
+    for (int e = 0; e < 3; ++e) {
+        const int ea = e == 0 ? 1 : 0;
+        const int eb = e == 2 ? 1 : 2;
+        maxa = std::max(maxa, g.N[ea] + 1);
+        maxb = std::max(maxb, g.N[eb] + 1);
+    }
+    dim3 grd((unsigned)((maxa + 63) / 64), (unsigned)maxb, 18);
+    hipLaunchKernelGGL(k_bc_planes, grd, dim3(64, 1, 1), 0, s, g, c, rhs1, rhs2, rhs3, inflow_kind, amp, mask_u);
+}
+
+void launch_scale_box(hipStream_t s, const Geo &g, const Box &b, double *f, double a, double bb, int divide) {
+    Range3 r = R(b);
+    if (empty(r)) return;
+    hipLaunchKernelGGL(k_scale_box, grid_for(r), dim3(BX, BY, 1), 0, s, g, r, f, a, bb, divide);
+}
+
+void launch_divergence2(hipStream_t s, const Geo &g, const Coef &c, const double *u, const double *v,
+                        const double *w, double *div) {
+    Range3 r = R(pbox(g));
+    if (empty(r)) return;
+    hipLaunchKernelGGL(k_divergence2, grid_for(r), dim3(BX, BY, 1), 0, s, g, c, r, u, v, w, div);
+}
+
+void launch_gradient(hipStream_t s, const Geo &g, const Coef &c, int q, const double *phi, double *grad) {
+    Range3 r = R(ibox(g, q));
+    if (!empty(r)) hipLaunchKernelGGL(k_gradient, grid_for(r), dim3(BX, BY, 1), 0, s, g, c, r, q, phi, grad);
+    launch_grad_walls(s, g, q, grad);
+}
+
+void launch_grad_walls(hipStream_t s, const Geo &g, int q, double *grad) {
+    int maxa = 0, maxb = 0;
+    for (int e = 0; e < 3; ++e) {
+        const int ea = e == 0 ? 1 : 0;
+        const int eb = e == 2 ? 1 : 2;
+        maxa = std::max(maxa, g.N[ea] + 1);
+        maxb = std::max(maxb, g.N[eb] + 1);
+    }
+    dim3 grd((unsigned)((maxa + 63) / 64), (unsigned)maxb, 6);
+    hipLaunchKernelGGL(k_grad_walls, grd, dim3(64, 1, 1), 0, s, g, q, grad);
+}
+
+void launch_grad_correct(hipStream_t s, const Geo &g, const Coef &c, int q, const double *phi, double *vel) {
+    Range3 r = R(ibox(g, q));
+    if (empty(r)) return;
+    hipLaunchKernelGGL(k_grad_correct, grid_for(r), dim3(BX, BY, 1), 0, s, g, c, r, q, phi, vel);
+}
+
+void launch_div_dir(hipStream_t s, const Geo &g, const Coef &c, int q, const double *dig, double *lap, int assign) {
+    Range3 r = R(pbox(g));
+    if (empty(r)) return;
+    hipLaunchKernelGGL(k_div_dir, grid_for(r), dim3(BX, BY, 1), 0, s, g, c, r, q, dig, lap, assign);
+}
+
+void launch_jor(hipStream_t s, const Geo &g, const Coef &c, const Box &b, double omega, const double *rel,
+                const double *bb, double *out) {
+    Range3 r = R(b);
+    if (empty(r)) return;
+    hipLaunchKernelGGL(k_jor, grid_for(r), dim3(BX, BY, 1), 0, s, g, c, r, omega, rel, bb, out);
+}
+
+void launch_jor_f32(hipStream_t s, const Geo &g, const Coef &c, const Box &b, float omega, const float *rel,
+                    const float *bb, float *out) {
+    Range3 r = R(b);
+    if (empty(r)) return;
+    hipLaunchKernelGGL(k_jor_f32, grid_for(r), dim3(BX, BY, 1), 0, s, g, c, r, omega, rel, bb, out);
+}
+
+int64_t red_blocks(const Box &b) {
+    const int64_t nI = b.hi[0] - b.lo[0] + 1, nJ = b.hi[1] - b.lo[1] + 1, nK = b.hi[2] - b.lo[2] + 1;
+    if (nI <= 0 || nJ <= 0 || nK <= 0) return 0;
+    return ((nI + RED_TX - 1) / RED_TX) * ((nJ + RED_TY - 1) / RED_TY) * ((nK + RED_KC - 1) / RED_KC);
+}
+
+static dim3 red_grid(const Range3 &r) {
+    return dim3((unsigned)((r.n0 + RED_TX - 1) / RED_TX), (unsigned)((r.n1 + RED_TY - 1) / RED_TY),
+                (unsigned)((r.n2 + RED_KC - 1) / RED_KC));
+}
+
+void launch_residual_cdg(hipStream_t s, const Geo &g, const Coef &c, const Box &b, const double *rel,
+                         const double *bb, double *out_res, double *partial_max) {
+    Range3 r = R(b);
+    if (empty(r)) return;
+    hipLaunchKernelGGL(k_residual_cdg, red_grid(r), dim3(RED_TX, RED_TY, 1), 0, s, g, c, r, rel, bb, out_res,
+                       partial_max);
+}
+
+void launch_dot_partial(hipStream_t s, const Geo &g, const Box &b, const double *a, const double *c, double *part) {
+    Range3 r = R(b);
+    if (empty(r)) return;
+    hipLaunchKernelGGL(k_dot_partial, red_grid(r), dim3(RED_TX, RED_TY, 1), 0, s, g, r, a, c, part);
+}
+
+void launch_sum_partial(hipStream_t s, const Geo &g, const Box &b, const double *a, double *part) {
+    Range3 r = R(b);
+    if (empty(r)) return;
+    hipLaunchKernelGGL(k_sum_partial, red_grid(r), dim3(RED_TX, RED_TY, 1), 0, s, g, r, a, part);
+}
+
+void launch_absdiff_partial(hipStream_t s, const Geo &g, const Box &b, const double *a, const double *c,
+                            double *part) {
+    Range3 r = R(b);
+    if (empty(r)) return;
+    hipLaunchKernelGGL(k_absdiff_partial, red_grid(r), dim3(RED_TX, RED_TY, 1), 0, s, g, r, a, c, part);
+}
+
+void launch_max_partial(hipStream_t s, const Geo &g, const Box &b, const double *a, double *part) {
+    Range3 r = R(b);
+    if (empty(r)) return;
+    hipLaunchKernelGGL(k_max_partial, red_grid(r), dim3(RED_TX, RED_TY, 1), 0, s, g, r, a, part);
+}
+
+void launch_final_sum(hipStream_t s, const double *part, int64_t nb, double *out) {
+    hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(RED_TX, RED_TY, 1), 0, s, part, nb, out);
+}
+
+void launch_final_max(hipStream_t s, const double *part, int64_t nb, double *out) {
+    hipLaunchKernelGGL(k_final_max, dim3(1), dim3(RED_TX, RED_TY, 1), 0, s, part, nb, out);
+}
+
+void launch_bicg_init(hipStream_t s, const Geo &g, const Box &b, const double *bb, double *rr, double *rh) {
+    Range3 r = R(b);
+    if (empty(r)) return;
+    hipLaunchKernelGGL(k_bicg_init, grid_for(r), dim3(BX, BY, 1), 0, s, g, r, bb, rr, rh);
+}
+
+void launch_bicg_p(hipStream_t s, const Geo &g, const Box &b, const double *rr, double *pk, const double *Ap,
+                   const double *scal) {
+    Range3 r = R(b);
+    if (empty(r)) return;
+    hipLaunchKernelGGL(k_bicg_p, grid_for(r), dim3(BX, BY, 1), 0, s, g, r, rr, pk, Ap, scal);
+}
+
+void launch_bicg_s(hipStream_t s, const Geo &g, const Box &b, double *rr, const double *Ap, const double *alpha) {
+    Range3 r = R(b);
+    if (empty(r)) return;
+    hipLaunchKernelGGL(k_bicg_s, grid_for(r), dim3(BX, BY, 1), 0, s, g, r, rr, Ap, alpha);
+}
+
+void launch_bicg_x(hipStream_t s, const Geo &g, const Box &b, double *phi, double *rr, const double *pk,
+                   const double *Ar, const double *scal, double *part_max) {
+    Range3 r = R(b);
+    if (empty(r)) return;
+    hipLaunchKernelGGL(k_bicg_x, red_grid(r), dim3(RED_TX, RED_TY, 1), 0, s, g, r, phi, rr, pk, Ar, scal, part_max);
+}
+
+void launch_copy_box(hipStream_t s, const Geo &g, const Box &b, const double *src, double *dst) {
+    Range3 r = R(b);
+    if (empty(r)) return;
+    hipLaunchKernelGGL(k_copy_box, grid_for(r), dim3(BX, BY, 1), 0, s, g, r, src, dst);
+}
+
+void launch_copy_field_box(hipStream_t s, const Geo &g, const Box &b, const double *src, double *dst) {
+    launch_copy_box(s, g, b, src, dst);
+}
+
+void launch_axpby(hipStream_t s, const Geo &g, double a, double b, const double *x, const double *y, double *z,
+                  int64_t n) {
+    (void)g;
+    hipLaunchKernelGGL(k_axpby, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a, b, x, y, z, n);
+}
+
+void launch_fill(hipStream_t s, double *f, double v, int64_t n) {
+    hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, f, v, n);
+}
+
+void launch_fill_f32(hipStream_t s, float *f, float v, int64_t n) {
+    hipLaunchKernelGGL(k_fill_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, f, v, n);
+}
+
+void launch_to_f32(hipStream_t s, const Geo &g, const Box &b, const double *src, float *dst) {
+    Range3 r = R(b);
+    if (empty(r)) return;
+    hipLaunchKernelGGL(k_to_f32, grid_for(r), dim3(BX, BY, 1), 0, s, g, r, src, dst);
+}
+
+void launch_correct_f32(hipStream_t s, const Geo &g, const Box &b, const float *e, double *x) {
+    Range3 r = R(b);
+    if (empty(r)) return;
+    hipLaunchKernelGGL(k_correct_f32, grid_for(r), dim3(BX, BY, 1), 0, s, g, r, e, x);
+}
+
+void launch_fill_corners(hipStream_t s, const Geo &g, double *f) {
+    const bool w1L = !g.per[0], w2L = !g.per[1], w3L = !g.per[2];
+    // edges parallel to z (1-2 walls), y (1-3 walls), x (2-3 walls); 4 each, in order
+    for (int edge = 0; edge < 12; ++edge) {
+        const int grp = edge / 4;
+        bool on = grp == 0 ? (w1L && w2L) : (grp == 1 ? (w1L && w3L) : (w2L && w3L));
+        if (!on) continue;
+        const int n = grp == 0 ? g.N[2] : (grp == 1 ? g.N[1] : g.N[0]);
+        hipLaunchKernelGGL(k_fill_corners, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, s, g, f, edge);
+    }
+}
+
+void launch_sub_box(hipStream_t s, const Geo &g, const Box &b, double *f, const double *val) {
+    Range3 r = R(b);
+    if (empty(r)) return;
+    hipLaunchKernelGGL(k_sub_box, grid_for(r), dim3(BX, BY, 1), 0, s, g, r, f, val);
+}
+
+void launch_apply_mask(hipStream_t s, const Geo &g, const uint8_t *mask, double *u, double *v, double *w) {
+    double *f[3] = {u, v, w};
+    const int64_t ntot = g.fe;
+    for (int q = 0; q < 3; ++q) {
+        Box b;
+        for (int d = 0; d < 3; ++d) {
+            b.lo[d] = g.SB[q][d];
+            b.hi[d] = g.NB[q][d];
+        }
+        Range3 r = R(b);
+        if (empty(r)) continue;
+        hipLaunchKernelGGL(k_apply_mask, grid_for(r), dim3(BX, BY, 1), 0, s, g, r, q, mask + q * ntot, f[q]);
+    }
+}
+
+}  // namespace ih

@@ -1,0 +1,369 @@
+"""Python binding of libimpact_hip (the MI355X projection-step library).
+
+The library is the product; this module only loads it with ctypes and mirrors
+its C ABI (include/impact_hip.h) with numpy arrays in the reference's Fortran
+layout phi(-3:N1+3, -3:N2+3, -3:N3+3), viewed from numpy as shape
+(N3+7, N2+7, N1+7) so that element (i, j, k) is ``a[k+3, j+3, i+3]``.
+
+There is no CPU fallback: if the built library is missing, importing the
+binding's loader raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+PKG_ROOT = Path(__file__).resolve().parent.parent
+LIB_PATH = PKG_ROOT / "lib" / "libimpact_hip.so"
+
+IH_ABI_VERSION = 1
+
+# field ids (include/impact_hip.h)
+(VEL1, VEL2, VEL3, NL1, NL2, NL3, RHS1, RHS2, RHS3, WORK1, WORK2, WORK3, PRE, RES, GPRE, PP, COMP, DIG,
+ RR, RH, PK, AP, AR, TMP1, TMP2, TMP3, TMP4, NFIELDS) = range(28)
+
+
+class IHError(RuntimeError):
+    pass
+
+
+class GridDesc(C.Structure):
+    _fields_ = [("version", C.c_int), ("N", C.c_int * 3), ("bc", C.c_int * 6), ("L", C.c_double * 3),
+                ("stretch", C.c_double * 3)]
+
+
+class CoeffSet(C.Structure):
+    _fields_ = [(name, C.POINTER(C.c_double) * 3) for name in
+                ("cGp", "cD", "cIp", "cIu", "cNpD", "cNpU", "cNuD", "cNuU", "cp2", "cu2", "cdg")]
+
+
+class BicgStats(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("norm_inf_initial", C.c_double), ("norm_inf_final", C.c_double),
+                ("exit_reason", C.c_int)]
+
+
+class StepParams(C.Structure):
+    _fields_ = [("Re", C.c_double), ("dtime", C.c_double), ("jor_sweeps", C.c_int), ("jor_omega", C.c_double),
+                ("bicg_rel_tol", C.c_double), ("bicg_max_it", C.c_int), ("level_pressure", C.c_int),
+                ("inflow_kind", C.c_int), ("inflow_amp", C.c_double), ("use_mask", C.c_int), ("jor_fp32", C.c_int)]
+
+
+class StepStats(C.Structure):
+    _fields_ = [("bicg_iterations", C.c_int * 3), ("bicg_final", C.c_double * 3), ("div_max_after", C.c_double)]
+
+
+_lib = None
+
+
+def load(path: os.PathLike | None = None):
+    """Load libimpact_hip.so (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    p = Path(path) if path else LIB_PATH
+    if not p.exists():
+        raise IHError(f"{p} is missing: build it with `make -C {PKG_ROOT}` (or __graft_entry__.build())")
+    L = C.CDLL(str(p))
+    vp, dp, ip, i64p = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_int64)
+    sig = {
+        "ih_abi_version": (C.c_int, []),
+        "ih_last_error": (C.c_int, [C.c_char_p, C.c_size_t]),
+        "ih_device_count": (C.c_int, [ip]),
+        "ih_case_create": (C.c_int, [C.POINTER(GridDesc), C.c_int, C.POINTER(vp)]),
+        "ih_case_destroy": (None, [vp]),
+        "ih_case_set_coeffs": (C.c_int, [vp, C.POINTER(CoeffSet)]),
+        "ih_case_generate_coeffs": (C.c_int, [vp]),
+        "ih_case_get_coeff": (C.c_int, [vp, C.c_char_p, dp, i64p]),
+        "ih_case_get_coords": (C.c_int, [vp, C.c_int, C.c_int, dp]),
+        "ih_grid_coeff": (C.c_int, [C.POINTER(GridDesc), C.c_char_p, dp, i64p]),
+        "ih_grid_coords": (C.c_int, [C.POINTER(GridDesc), C.c_int, C.c_int, dp]),
+        "ih_case_get_ranges": (C.c_int, [vp, ip]),
+        "ih_field_elems": (C.c_int64, [vp]),
+        "ih_case_bytes": (C.c_int64, [vp]),
+        "ih_field_upload": (C.c_int, [vp, C.c_int, dp]),
+        "ih_field_download": (C.c_int, [vp, C.c_int, dp]),
+        "ih_field_fill": (C.c_int, [vp, C.c_int, C.c_double]),
+        "ih_field_copy": (C.c_int, [vp, C.c_int, C.c_int]),
+        "ih_synchronize": (C.c_int, [vp]),
+        "ih_exchange": (C.c_int, [vp, C.c_int, C.c_int, C.c_int]),
+        "ih_exchange_relax": (C.c_int, [vp, C.c_int]),
+        "ih_divergence2": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int]),
+        "ih_gradient": (C.c_int, [vp, C.c_int, C.c_int, C.c_int]),
+        "ih_interpolate2_vel_pre": (C.c_int, [vp, C.c_int, C.c_int, C.c_int]),
+        "ih_interpolate2_pre_vel": (C.c_int, [vp, C.c_int, C.c_int, C.c_int]),
+        "ih_interpolate_vel": (C.c_int, [vp]),
+        "ih_nonlinear_upwind": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int]),
+        "ih_helmholtz": (C.c_int, [vp, C.c_int, C.c_double, C.c_int, C.c_int]),
+        "ih_axpby": (C.c_int, [vp, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int]),
+        "ih_product_div_grad": (C.c_int, [vp, C.c_int, C.c_int]),
+        "ih_dot": (C.c_int, [vp, ip, ip, C.c_int, C.c_int, dp]),
+        "ih_jor": (C.c_int, [vp, C.c_int, C.c_double, ip, ip, C.c_int, C.c_int]),
+        "ih_residual_cdg": (C.c_int, [vp, ip, ip, C.c_int, C.c_int, dp]),
+        "ih_bicgstab": (C.c_int, [vp, C.c_double, C.c_int, ip, ip, C.c_int, C.c_int, C.POINTER(BicgStats)]),
+        "ih_fill_corners": (C.c_int, [vp, C.c_int]),
+        "ih_level_pressure": (C.c_int, [vp, C.c_int]),
+        "ih_set_tube_mask": (C.c_int, [vp, C.c_double, C.c_double, dp, dp]),
+        "ih_get_mask": (C.c_int, [vp, C.POINTER(C.c_uint8)]),
+        "ih_apply_mask": (C.c_int, [vp]),
+        "ih_timestep": (C.c_int, [vp, C.POINTER(StepParams), C.c_double, C.POINTER(StepStats)]),
+        "ih_device_jacobi_solver": (C.c_int, [vp, C.c_int, C.c_double] + [C.c_int] * 6 + [dp, dp]),
+        "ih_device_gradient": (C.c_int, [vp, C.c_int, dp, dp]),
+        "ih_device_divergence2": (C.c_int, [vp, dp, dp]),
+        "ih_device_nonlinear_upwind": (C.c_int, [vp, C.c_int, dp, dp, dp]),
+        "ih_device_helmholtz": (C.c_int, [vp, C.c_int, C.c_double, dp, dp]),
+        "ih_device_axpby": (C.c_int, [vp, C.c_double, C.c_double, dp, dp, dp]),
+        "ih_device_interpolate": (C.c_int, [vp, C.c_int, dp, dp]),
+        "ih_device_interpolate2": (C.c_int, [vp, C.c_int, dp, dp]),
+        "ih_timer_start": (C.c_int, [vp]),
+        "ih_timer_stop": (C.c_int, [vp, dp]),
+        "ih_kernel_timing_enable": (C.c_int, [vp, C.c_int]),
+        "ih_kernel_timing_get": (C.c_int, [vp, C.c_char_p, dp, i64p]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    if L.ih_abi_version() != IH_ABI_VERSION:
+        raise IHError("ABI version mismatch")
+    _lib = L
+    return L
+
+
+def exported_symbols():
+    """Names the binding expects the library to export."""
+    load()
+    return [n for n in dir(_lib) if n.startswith("ih_")]
+
+
+def last_error() -> str:
+    buf = C.create_string_buffer(1024)
+    load().ih_last_error(buf, 1024)
+    return buf.value.decode(errors="replace")
+
+
+def check(rc: int, what: str = ""):
+    if rc != 0:
+        raise IHError(f"{what} failed ({rc}): {last_error()}")
+
+
+def _dp(a: np.ndarray):
+    if a.dtype != np.float64 or not a.flags.c_contiguous:
+        raise TypeError("expected a contiguous float64 array")
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _ints(seq):
+    return (C.c_int * len(seq))(*[int(x) for x in seq])
+
+
+def grid_desc(N, bc, L=(1.0, 1.0, 1.0), stretch=(0.0, 0.0, 0.0)) -> GridDesc:
+    g = GridDesc()
+    g.version = IH_ABI_VERSION
+    for d in range(3):
+        g.N[d] = int(N[d])
+        g.L[d] = float(L[d])
+        g.stretch[d] = float(stretch[d])
+    for s in range(6):
+        g.bc[s] = int(bc[s])
+    return g
+
+
+def grid_coeff(N, bc, L, stretch, name: str) -> np.ndarray:
+    """Coefficient array the library generates for a grid (host only, no device)."""
+    L_ = load()
+    g = grid_desc(N, bc, L, stretch)
+    n = C.c_int64(0)
+    check(L_.ih_grid_coeff(C.byref(g), name.encode(), None, C.byref(n)), "ih_grid_coeff")
+    out = np.zeros(n.value)
+    check(L_.ih_grid_coeff(C.byref(g), name.encode(), _dp(out), C.byref(n)), "ih_grid_coeff")
+    return out
+
+
+def grid_coords(N, bc, L, stretch, d: int, which: int) -> np.ndarray:
+    L_ = load()
+    g = grid_desc(N, bc, L, stretch)
+    out = np.zeros(int(N[d]) + 7)
+    check(L_.ih_grid_coords(C.byref(g), d, which, _dp(out)), "ih_grid_coords")
+    return out
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    check(load().ih_device_count(C.byref(n)), "ih_device_count")
+    return n.value
+
+
+class Case:
+    """One grid block resident on a GPU (ih_case)."""
+
+    def __init__(self, N, bc, L=(1.0, 1.0, 1.0), stretch=(0.0, 0.0, 0.0), device: int = 0, generate: bool = True):
+        self.lib = load()
+        self.N = tuple(int(n) for n in N)
+        self.bc = tuple(int(b) for b in bc)
+        self.L = tuple(float(x) for x in L)
+        self.stretch = tuple(float(x) for x in stretch)
+        g = grid_desc(self.N, self.bc, self.L, self.stretch)
+        h = C.c_void_p()
+        check(self.lib.ih_case_create(C.byref(g), device, C.byref(h)), "ih_case_create")
+        self.h = h
+        self.shape = (self.N[2] + 7, self.N[1] + 7, self.N[0] + 7)
+        r = (C.c_int * 48)()
+        check(self.lib.ih_case_get_ranges(self.h, r), "ranges")
+        r = list(r)
+        self.Sp, self.Np = r[0:3], r[3:6]
+        self.SB = [r[6 + 3 * m:9 + 3 * m] for m in range(3)]
+        self.NB = [r[15 + 3 * m:18 + 3 * m] for m in range(3)]
+        self.SI = [r[24 + 3 * m:27 + 3 * m] for m in range(3)]
+        self.NI = [r[33 + 3 * m:36 + 3 * m] for m in range(3)]
+        if generate:
+            check(self.lib.ih_case_generate_coeffs(self.h), "ih_case_generate_coeffs")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.ih_case_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # --- fields -------------------------------------------------------------
+    def upload(self, f: int, a: np.ndarray):
+        assert a.shape == self.shape
+        check(self.lib.ih_field_upload(self.h, f, _dp(np.ascontiguousarray(a))), "upload")
+
+    def download(self, f: int) -> np.ndarray:
+        a = np.zeros(self.shape)
+        check(self.lib.ih_field_download(self.h, f, _dp(a)), "download")
+        return a
+
+    def fill(self, f: int, v: float):
+        check(self.lib.ih_field_fill(self.h, f, float(v)), "fill")
+
+    def sync(self):
+        check(self.lib.ih_synchronize(self.h), "sync")
+
+    def bytes(self) -> int:
+        return int(self.lib.ih_case_bytes(self.h))
+
+    def coeff(self, name: str) -> np.ndarray:
+        n = C.c_int64(0)
+        check(self.lib.ih_case_get_coeff(self.h, name.encode(), None, C.byref(n)), "coeff")
+        out = np.zeros(n.value)
+        check(self.lib.ih_case_get_coeff(self.h, name.encode(), _dp(out), C.byref(n)), "coeff")
+        return out
+
+    def coords(self, d: int, which: int) -> np.ndarray:
+        out = np.zeros(self.N[d] + 7)
+        check(self.lib.ih_case_get_coords(self.h, d, which, _dp(out)), "coords")
+        return out
+
+    # --- operators (field ids) ---------------------------------------------
+    def exchange(self, d, vel_dir, f):
+        check(self.lib.ih_exchange(self.h, d, vel_dir, f), "exchange")
+
+    def divergence2(self, u=VEL1, v=VEL2, w=VEL3, div=RES):
+        check(self.lib.ih_divergence2(self.h, u, v, w, div), "divergence2")
+
+    def gradient(self, m, phi, grad):
+        check(self.lib.ih_gradient(self.h, m, phi, grad), "gradient")
+
+    def interpolate_vel(self):
+        check(self.lib.ih_interpolate_vel(self.h), "interpolate_vel")
+
+    def interpolate2_pre_vel(self, m, phi, inter):
+        check(self.lib.ih_interpolate2_pre_vel(self.h, m, phi, inter), "interpolate2_pre_vel")
+
+    def interpolate2_vel_pre(self, m, phi, inter):
+        check(self.lib.ih_interpolate2_vel_pre(self.h, m, phi, inter), "interpolate2_vel_pre")
+
+    def nonlinear_upwind(self, cid, pp, vel_c, nl_c):
+        check(self.lib.ih_nonlinear_upwind(self.h, cid, pp, vel_c, nl_c), "nonlinear_upwind")
+
+    def helmholtz(self, comp, multL, vel_c, nl_c):
+        check(self.lib.ih_helmholtz(self.h, comp, float(multL), vel_c, nl_c), "helmholtz")
+
+    def product_div_grad(self, phi, lap):
+        check(self.lib.ih_product_div_grad(self.h, phi, lap), "product_div_grad")
+
+    def dot(self, ss, nn, a, b) -> float:
+        out = C.c_double(0.0)
+        check(self.lib.ih_dot(self.h, _ints(ss), _ints(nn), a, b, C.byref(out)), "dot")
+        return out.value
+
+    def jor(self, n, omega, ss, nn, rel=PRE, bb=RES):
+        check(self.lib.ih_jor(self.h, int(n), float(omega), _ints(ss), _ints(nn), rel, bb), "jor")
+
+    def residual_cdg(self, ss, nn, rel=PRE, bb=RES) -> float:
+        out = C.c_double(0.0)
+        check(self.lib.ih_residual_cdg(self.h, _ints(ss), _ints(nn), rel, bb, C.byref(out)), "residual_cdg")
+        return out.value
+
+    def bicgstab(self, eps, nmax, ss, nn, bb=RES, phi=PRE) -> BicgStats:
+        st = BicgStats()
+        check(self.lib.ih_bicgstab(self.h, float(eps), int(nmax), _ints(ss), _ints(nn), bb, phi, C.byref(st)),
+              "bicgstab")
+        return st
+
+    def fill_corners(self, f=PRE):
+        check(self.lib.ih_fill_corners(self.h, f), "fill_corners")
+
+    def level_pressure(self, f=PRE):
+        check(self.lib.ih_level_pressure(self.h, f), "level_pressure")
+
+    def set_tube_mask(self, yc, zc, r_u, r_p):
+        check(self.lib.ih_set_tube_mask(self.h, float(yc), float(zc), _dp(np.ascontiguousarray(r_u, dtype=np.float64)),
+                                        _dp(np.ascontiguousarray(r_p, dtype=np.float64))), "set_tube_mask")
+
+    def get_mask(self) -> np.ndarray:
+        m = np.zeros((3,) + self.shape, dtype=np.uint8)
+        check(self.lib.ih_get_mask(self.h, m.ctypes.data_as(C.POINTER(C.c_uint8))), "get_mask")
+        return m
+
+    def apply_mask(self):
+        check(self.lib.ih_apply_mask(self.h), "apply_mask")
+
+    def timestep(self, params: StepParams, time: float, stats: bool = True) -> StepStats | None:
+        st = StepStats()
+        check(self.lib.ih_timestep(self.h, C.byref(params), float(time), C.byref(st) if stats else None), "timestep")
+        return st if stats else None
+
+    # --- reference-signature calls (host arrays) ---------------------------
+    def device_gradient(self, m, phi, grad):
+        check(self.lib.ih_device_gradient(self.h, m, _dp(phi), _dp(grad)), "device_gradient")
+
+    def device_divergence2(self, phi3, div):
+        check(self.lib.ih_device_divergence2(self.h, _dp(phi3), _dp(div)), "device_divergence2")
+
+    def device_nonlinear_upwind(self, cid, pp, vel_c, nl_c):
+        check(self.lib.ih_device_nonlinear_upwind(self.h, cid, _dp(pp), _dp(vel_c), _dp(nl_c)), "device_nonlinear")
+
+    def device_helmholtz(self, cid, multL, vel_c, nl_c):
+        check(self.lib.ih_device_helmholtz(self.h, cid, float(multL), _dp(vel_c), _dp(nl_c)), "device_helmholtz")
+
+    def device_axpby(self, a, b, x, y, z):
+        check(self.lib.ih_device_axpby(self.h, float(a), float(b), _dp(x), _dp(y), _dp(z)), "device_axpby")
+
+    def device_interpolate(self, m, phi, inter):
+        check(self.lib.ih_device_interpolate(self.h, m, _dp(phi), _dp(inter)), "device_interpolate")
+
+    def device_interpolate2(self, m, phi, inter):
+        check(self.lib.ih_device_interpolate2(self.h, m, _dp(phi), _dp(inter)), "device_interpolate2")
+
+    def device_jacobi_solver(self, omega, ss, nn, rel, bb):
+        check(self.lib.ih_device_jacobi_solver(self.h, 1, float(omega), *[int(x) for x in ss],
+                                               *[int(x) for x in nn], _dp(rel), _dp(bb)), "device_jacobi_solver")
+
+    # --- timing ------------------------------------------------------------
+    def timer_start(self):
+        check(self.lib.ih_timer_start(self.h), "timer_start")
+
+    def timer_stop(self) -> float:
+        ms = C.c_double(0.0)
+        check(self.lib.ih_timer_stop(self.h, C.byref(ms)), "timer_stop")
+        return ms.value

@@ -1,0 +1,214 @@
+/*
+ * impact_hip -- MI355X (gfx950) implementation of IMPACT's projection-step hot
+ * path behind a plain C ABI.
+ *
+ * A case (ih_case) owns one grid block and all of its fields resident in HBM.
+ * Fields are addressed by id; host arrays cross the boundary in the reference's
+ * Fortran layout phi(-3:N1+3,-3:N2+3,-3:N3+3) (REAL(8), column-major, dense).
+ * All work of a case is ordered on the case's own HIP stream.  Calls on distinct
+ * cases may run concurrently from different threads; calls on one case must not.
+ *
+ * Every function returns IH_OK (0) or a negative status; ih_last_error() returns
+ * the message of the last failure on the calling thread.
+ *
+ * Reference interfaces replaced (paths relative to the reference tree):
+ *   device_jacobi_solver     src/jacobi_solver_host_v4.f90:5-40   -> ih_device_jacobi_solver, ih_jor
+ *   device_gradient          src/grad_host.f90:39-50              -> ih_device_gradient, ih_gradient
+ *   device_divergence2       src/divergence2_host.f90:6-20        -> ih_device_divergence2, ih_divergence2
+ *   device_Nonlinear_upwind  src/Nonlinear_upwind_host.f90:6-30   -> ih_device_nonlinear_upwind
+ *   device_helmholtz         src/helmholtz_explicit_host.f90:6-30 -> ih_device_helmholtz
+ *   device_axpby             src/axpby_host.f90:5-30              -> ih_device_axpby
+ *   device_interpolate       src/interpolate2_pre_vel_host.f90    -> ih_device_interpolate
+ *   device_interpolate2      src/interpolate2_vel_pre_host.f90    -> ih_device_interpolate2
+ *   GPU_malloc / GPU_memcopy src/mod_mem_GPU.f90:152-434          -> ih_case_create, ih_case_set_coeffs
+ *   GPU_memfree              src/mod_mem_GPU.f90:437-610          -> ih_case_destroy
+ *   launcher ABI, e.g. jor_launcher_ src/jor_kernel_launcher.cu:12 (replaced wholesale)
+ */
+#ifndef IMPACT_HIP_H
+#define IMPACT_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define IH_ABI_VERSION 1
+
+enum {
+    IH_OK = 0,
+    IH_EINVAL = -22,
+    IH_ENOMEM = -12,
+    IH_EHIP = -1000,
+    IH_ESTATE = -1001
+};
+
+/* Field ids.  Vectors occupy three consecutive ids. */
+typedef enum {
+    IH_VEL1 = 0, IH_VEL2, IH_VEL3,
+    IH_NL1, IH_NL2, IH_NL3,
+    IH_RHS1, IH_RHS2, IH_RHS3,
+    IH_WORK1, IH_WORK2, IH_WORK3,
+    IH_PRE, IH_RES, IH_GPRE, IH_PP, IH_COMP, IH_DIG,
+    IH_RR, IH_RH, IH_PK, IH_AP, IH_AR,
+    IH_TMP1, IH_TMP2, IH_TMP3, IH_TMP4,
+    IH_NFIELDS
+} ih_field;
+
+typedef struct ih_case ih_case;
+
+/* Grid of one block: N[d] points per direction (IMPACT N1..N3; for a single
+ * block N = M), bc[2d], bc[2d+1] the lower/upper BC: -1 periodic, 1 wall.
+ * L[d] extents and stretch[d] (0 equidistant, else the tanh factor) are used
+ * only by ih_case_generate_coeffs. */
+typedef struct {
+    int version;
+    int N[3];
+    int bc[6];
+    double L[3];
+    double stretch[3];
+} ih_grid_desc;
+
+/* Coefficient arrays exactly as the reference keeps them (mod_vars.f90:280-370):
+ * c(lo:hi, 0:N) column-major.  cdg?(-1:1, 1:N) for the finest grid. */
+typedef struct {
+    const double *cGp[3];   /* cGp1..3   (-2:3, 0:N)  gradient                 */
+    const double *cD[3];    /* cDu1,cDv2,cDw3 (-3:2, 0:N) divergence           */
+    const double *cIp[3];   /* cIpu,cIpv,cIpw (-2:3, 0:N) pressure -> velocity  */
+    const double *cIu[3];   /* cIup,cIvp,cIwp (-3:2, 0:N) velocity -> pressure  */
+    const double *cNpD[3];  /* cNp1D..   (-3:3, 0:N)  upwind, pressure grid     */
+    const double *cNpU[3];
+    const double *cNuD[3];  /* cNu1D,cNv2D,cNw3D (-3:3, 0:N) velocity grid      */
+    const double *cNuU[3];
+    const double *cp2[3];   /* cp11,cp22,cp33 (-3:3, 0:N)                       */
+    const double *cu2[3];   /* cu11,cv22,cw33 (-3:3, 0:N)                       */
+    const double *cdg[3];   /* cdg1..3 (-1:1, 1:N)                              */
+} ih_coeff_set;
+
+/* ---- library / case lifetime ---------------------------------------------- */
+int  ih_abi_version(void);
+int  ih_last_error(char *buf, size_t n);
+int  ih_device_count(int *count);
+int  ih_case_create(const ih_grid_desc *g, int device, ih_case **out);
+void ih_case_destroy(ih_case *c);
+/* Upload coefficients computed by the host (GPU_memcopy's role). */
+int  ih_case_set_coeffs(ih_case *c, const ih_coeff_set *cs);
+/* Compute coordinates and coefficients on the host from g->L, g->stretch
+ * (FD_coeffs + get_stencil with mapping, src/mod_coeffs.f90:37-120,1048-1258)
+ * and upload them. */
+int  ih_case_generate_coeffs(ih_case *c);
+/* Export one generated coefficient array by reference name ("cGp1", "cdg2", ...);
+ * *n receives the element count.  out may be NULL to query the size. */
+int  ih_case_get_coeff(ih_case *c, const char *name, double *out, int64_t *n);
+/* Coordinates x?p (which = 0) or x?u (which = 1) of direction d, (-3:N+3). */
+int  ih_case_get_coords(ih_case *c, int d, int which, double *out);
+/* Host-only forms of the two above (no device needed): coefficients and
+ * coordinates the library would generate for the grid g. */
+int  ih_grid_coeff(const ih_grid_desc *g, const char *name, double *out, int64_t *n);
+int  ih_grid_coords(const ih_grid_desc *g, int d, int which, double *out);
+/* Index ranges in the layout of the reference's mod_vars (48 ints, see docs). */
+int  ih_case_get_ranges(ih_case *c, int out[48]);
+/* Elements of one host field, (N1+7)(N2+7)(N3+7). */
+int64_t ih_field_elems(const ih_case *c);
+/* Bytes of HBM held by the case. */
+int64_t ih_case_bytes(const ih_case *c);
+
+/* ---- field transfer (Fortran-dense host arrays) -------------------------- */
+int ih_field_upload(ih_case *c, ih_field f, const double *host);
+int ih_field_download(ih_case *c, ih_field f, double *host);
+int ih_field_fill(ih_case *c, ih_field f, double value);
+int ih_field_copy(ih_case *c, ih_field dst, ih_field src);
+int ih_synchronize(ih_case *c);
+
+/* ---- device-resident operators ------------------------------------------ */
+/* exchange(dir, vel_dir, phi): src/mod_exchange.f90:47-294 as an on-device fill. */
+int ih_exchange(ih_case *c, int dir, int vel_dir, ih_field f);
+int ih_exchange_relax(ih_case *c, ih_field f);
+int ih_divergence2(ih_case *c, ih_field u, ih_field v, ih_field w, ih_field div);
+int ih_gradient(ih_case *c, int m, ih_field phi, ih_field grad);
+int ih_interpolate2_vel_pre(ih_case *c, int m, ih_field phi, ih_field inter);
+int ih_interpolate2_pre_vel(ih_case *c, int m, ih_field phi, ih_field inter);
+/* work1..3 = interpolated velocities + ghost fill (interpolate_vel). */
+int ih_interpolate_vel(ih_case *c);
+/* component_id 1..9 as in the reference (1,4,7 assign; others accumulate). */
+int ih_nonlinear_upwind(ih_case *c, int component_id, ih_field pp, ih_field vel_c, ih_field nl_c);
+int ih_helmholtz(ih_case *c, int component, double multL, ih_field vel_c, ih_field nl_c);
+int ih_axpby(ih_case *c, double a, double b, ih_field x, ih_field y, ih_field z);
+int ih_product_div_grad(ih_case *c, ih_field phi, ih_field lap);
+int ih_dot(ih_case *c, const int ss[3], const int nn[3], ih_field a, ih_field b, double *out);
+
+typedef struct {
+    int iterations;
+    double norm_inf_initial;
+    double norm_inf_final;
+    int exit_reason; /* 0 converged, 1 max iterations, 2 breakdown */
+} ih_bicgstab_stats;
+
+/* JOR on the compact cdg operator, out of place, n_sweeps sweeps. */
+int ih_jor(ih_case *c, int n_sweeps, double omega, const int ss[3], const int nn[3], ih_field rel, ih_field bb);
+/* max |bb - A_cdg rel| over [ss, nn]. */
+int ih_residual_cdg(ih_case *c, const int ss[3], const int nn[3], ih_field rel, ih_field bb, double *out);
+/* BiCGstab_orig (preconditioner 0) on product_div_grad; stop at ||r||_inf < eps. */
+int ih_bicgstab(ih_case *c, double eps, int n_it_max, const int ss[3], const int nn[3], ih_field bb, ih_field phi,
+                ih_bicgstab_stats *st);
+
+int ih_fill_corners(ih_case *c, ih_field f);
+int ih_level_pressure(ih_case *c, ih_field pre);
+
+/* ---- immersed-boundary mask ---------------------------------------------- */
+/* Stenosed tube along x: radius per x-index on the u grid (r_u) and on the
+ * pressure grid (r_p), arrays (-3:N1+3).  The mask is built on the host and
+ * uploaded (uint8, 1 = fluid). */
+int ih_set_tube_mask(ih_case *c, double yc, double zc, const double *r_u, const double *r_p);
+int ih_get_mask(ih_case *c, uint8_t *host3);
+int ih_apply_mask(ih_case *c);
+
+/* ---- projection time step ------------------------------------------------ */
+typedef struct {
+    double Re, dtime;
+    int jor_sweeps;
+    double jor_omega;
+    double bicg_rel_tol;
+    int bicg_max_it;
+    int level_pressure;
+    int inflow_kind; /* 0 none, 1 parabola 16y(1-y)z(1-z), 2 pulsatile plug in the mask */
+    double inflow_amp;
+    int use_mask;
+    int jor_fp32;   /* 1: JOR sweeps in FP32 (residual and update kept FP64) */
+} ih_step_params;
+
+typedef struct {
+    int bicg_iterations[3];
+    double bicg_final[3];
+    double div_max_after;
+} ih_step_stats;
+
+/* One time step (3 RK substeps) of the fields VEL, PRE, NL in place.
+ * stats may be NULL; when given, div_max_after costs one extra divergence. */
+int ih_timestep(ih_case *c, const ih_step_params *p, double time, ih_step_stats *st);
+
+/* ---- reference-signature entry points (host arrays in, host arrays out) --- */
+int ih_device_jacobi_solver(ih_case *c, int grid_id, double omega, int SS1, int SS2, int SS3, int NN1, int NN2,
+                            int NN3, double *rel, double *bb);
+int ih_device_gradient(ih_case *c, int dimension_id, double *phi, double *grad);
+int ih_device_divergence2(ih_case *c, double *phi3, double *div);
+int ih_device_nonlinear_upwind(ih_case *c, int component_id, double *pp, double *vel_component,
+                               double *nl_component);
+int ih_device_helmholtz(ih_case *c, int component_id, double multL, double *vel_component,
+                        double *helm_component);
+int ih_device_axpby(ih_case *c, double a, double b, double *x, double *y, double *z);
+int ih_device_interpolate(ih_case *c, int dimension_id, double *phi, double *inter);
+int ih_device_interpolate2(ih_case *c, int dimension_id, double *phi, double *inter);
+
+/* ---- timing of the case's stream (for benchmarks) ------------------------ */
+int ih_timer_start(ih_case *c);
+int ih_timer_stop(ih_case *c, double *ms);
+/* Per-kernel-family launch statistics collected with HIP events when enabled. */
+int ih_kernel_timing_enable(ih_case *c, int on);
+int ih_kernel_timing_get(ih_case *c, const char *family, double *total_ms, int64_t *launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

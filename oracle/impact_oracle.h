@@ -1,0 +1,137 @@
+/*
+ * CPU oracle for the IMPACT projection-step hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product links, loads or calls this
+ * code; only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg
+ * use it, and only as the checker (or as the timed CPU baseline).
+ *
+ * It restates, loop by loop and in the same floating-point operation order,
+ * the reference Fortran CPU path (single MPI block, walls or periodic BCs):
+ *   - index ranges        src/mod_setup.f90:465-700, 828-867
+ *   - coordinates         src/usr_geometry.f90:93-157, src/mod_geometry.f90:37-430,
+ *                         src/usr_func.f90:1964-2000 (coord_tanh)
+ *   - FD coefficients     src/mod_coeffs.f90:37-120, 200-596, 598-772
+ *   - Poisson stencil     src/mod_coeffs.f90:1048-1258 (get_stencil -> cdg)
+ *   - ghost fill          src/mod_exchange.f90:47-294, 565-817, 1273-1480
+ *   - stencils            src/mod_diff.f90:130-215, 309-414, 549-762, 784-1424,
+ *                         1436-1472, 1567-1648, 1740-1825
+ *   - JOR relaxation      src/mod_laplace.f90:466-597, src/jacobi_solver_host_v4.f90:283-420
+ *   - BiCGstab            src/mod_solvers.f90:1509-1935, 2319-2515;
+ *                         operator src/mod_laplace.f90:38-197
+ *   - projection / RK3    src/mod_solvers.f90:64-346, src/mod_rhs.f90:35-420,
+ *                         src/mod_timeint.f90:275-377, src/mod_lib.f90:507-575
+ *   - IBM volume mask     src/mod_sharp_interface_ibm.f90:585-640
+ *
+ * Arrays are Fortran-ordered with halo 3: phi(-3:N1+3,-3:N2+3,-3:N3+3),
+ * element (i,j,k) at (i+3) + (j+3)*nx + (k+3)*nx*ny, nx = N1+7, ny = N2+7.
+ * Coefficient arrays c(lo:hi,0:N) at (ii-lo) + (hi-lo+1)*i.
+ * cdg?(-1:1,1:N) at (ii+1) + 3*(i-1).
+ *
+ * Compiled with -ffp-contract=off so no multiply-add is fused.
+ */
+#ifndef IMPACT_ORACLE_H
+#define IMPACT_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_case orc_case;
+
+/* Create a single-block case.  N[d] grid points per direction (IMPACT N1..N3),
+ * bc[2*d+0/1] lower/upper BC (-1 periodic, 1 wall), L[d] extents,
+ * stretch[d] = 0 for an equidistant grid, else the tanh factor ii0L=ii0U. */
+orc_case *orc_case_create(const int N[3], const int bc[6], const double L[3], const double stretch[3]);
+void orc_case_destroy(orc_case *c);
+
+/* Field size in doubles, (N1+7)(N2+7)(N3+7). */
+int64_t orc_field_size(const orc_case *c);
+
+/* Index ranges: out[0..2]=S?p, [3..5]=N?p, then for m=1..3: S1m B..(9 values
+ * SB[m][d]), NB (9), SI (9), NI (9), then SNF lower (3) and upper (3). */
+void orc_ranges(const orc_case *c, int out[48]);
+
+/* Coordinates, (-3:N+3) each: which = 0 x?p, 1 x?u  (direction d). */
+void orc_coords(const orc_case *c, int d, int which, double *out);
+
+/* Coefficient export by name ("cGp1", "cDu1", "cIpu", "cIup", "cNu1U", "cNp2D",
+ * "cu11", "cp22", "cdg1", ...).  Returns number of doubles written, or -1. */
+int orc_coeff(const orc_case *c, const char *name, double *out);
+
+/* One diff_coeffs_exact table lookup; returns 1 if the table has the entry. */
+int orc_exact_table(int abl, int n_coeff, double deltaX, double *cc);
+
+/* ---------------- ghost fill ---------------- */
+void orc_exchange(const orc_case *c, int dir, int vel_dir, double *phi);
+void orc_exchange_all_all(const orc_case *c, double *vel /* 3 fields, contiguous */);
+void orc_exchange_relax(const orc_case *c, double *phi);
+
+/* ---------------- stencils ---------------- */
+void orc_divergence2(const orc_case *c, double *vel3, double *div);
+void orc_gradient(const orc_case *c, int m, double *phi, double *grad);
+void orc_interpolate2_vel_pre(const orc_case *c, int m, double *phi, double *inter);
+void orc_interpolate2_pre_vel(const orc_case *c, int m, double *phi, double *inter);
+void orc_interpolate_vel(const orc_case *c, double *vel3, double *work3);
+/* nonlinear (upwind) into nl3; reads work3 (exchanged), vel3 */
+void orc_nonlinear(const orc_case *c, double *vel3, double *work3, double *nl3, double *pp);
+void orc_helmholtz_explicit(const orc_case *c, double multL, double *vel3, double *nl3);
+void orc_product_div_grad(const orc_case *c, double *phi, double *lap, double *dig);
+
+/* ---------------- reductions ---------------- */
+/* mode 0: sequential k,j,i loop order (reference order);
+ * mode 1: the fixed two-level tree the device kernels use. */
+void orc_set_reduction_mode(orc_case *c, int mode);
+double orc_dot(const orc_case *c, const int ss[3], const int nn[3], const double *a, const double *b);
+
+/* ---------------- Poisson ---------------- */
+void orc_jor(const orc_case *c, int n_sweeps, double omega, const int ss[3], const int nn[3],
+             double *rel, const double *bb, double *comp);
+double orc_residual_cdg(const orc_case *c, const int ss[3], const int nn[3], double *rel, const double *bb);
+typedef struct {
+    int iterations;
+    double norm_inf_initial;
+    double norm_inf_final;
+    int exit_reason; /* 0 converged, 1 max iterations, 2 breakdown */
+} orc_bicgstab_stats;
+void orc_bicgstab(const orc_case *c, double eps, int n_it_max, const int ss[3], const int nn[3],
+                  const double *bb, double *phi, double *work /* 7 fields */, orc_bicgstab_stats *st);
+
+/* ---------------- projection step ---------------- */
+typedef struct {
+    double Re, dtime;
+    int jor_sweeps;
+    double jor_omega;
+    double bicg_rel_tol;   /* stop when ||r||_inf < rel_tol * ||b||_inf */
+    int bicg_max_it;
+    int level_pressure;    /* 1: subtract mean pressure after each step */
+    int inflow_kind;       /* 0: none, 1: parabola(y,z), 2: pulsatile plug inside mask */
+    double inflow_amp;
+    const uint8_t *mask3;  /* optional: 3 masks (u,v,w grids), 1 = fluid; NULL = none */
+} orc_step_params;
+
+typedef struct {
+    int bicg_iterations[3];
+    double bicg_final[3];
+    double div_max_after;  /* max |div u| after the last substep */
+} orc_step_stats;
+
+/* Boundary values bcXY used by rhs_vel: filled from the inflow model at subtime. */
+void orc_timestep(orc_case *c, const orc_step_params *p, int step_index, double time,
+                  double *vel3, double *pre, double *nl3, orc_step_stats *st);
+
+/* IBM volume mask of a stenosed tube along x (axis (yc,zc), radius profile r(x_i)
+ * given per i on the three staggered grids).  mask3 = 3 uint8 fields. */
+void orc_tube_mask(const orc_case *c, double yc, double zc, const double *r_u /* (-3:N1+3) at x1u */,
+                   const double *r_p /* at x1p */, uint8_t *mask3);
+void orc_apply_mask(const orc_case *c, const uint8_t *mask3, double *vel3);
+void orc_level_pressure(const orc_case *c, double *pre);
+/* inflow amplitude at time t: kind 1 constant amp, kind 2 amp*(1 + 0.5 sin 2pi t + 0.2 sin(4pi t+0.7) + 0.1 sin(6pi t+1.3)) */
+double orc_inflow_amplitude(int kind, double amp, double t);
+void orc_fill_corners(const orc_case *c, double *phi);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,436 @@
+/*
+ * Oracle (test infrastructure only): Poisson solvers and the projection step.
+ * Build with -ffp-contract=off.
+ */
+#include "orc_internal.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define A(p, i, j, k) (p)[ORC_IDX(c, i, j, k)]
+
+/* RK3 coefficients, src/mod_vars.f90:774-775 (evaluated in double precision,
+ * as the reference's CPU builds do with -fdefault-real-8). */
+static const double aRK[3] = {8.0 / 15.0, 5.0 / 12.0, 3.0 / 4.0};
+static const double bRK[3] = {0.0, -17.0 / 60.0, -5.0 / 12.0};
+
+/* ------------------------------------------------------------------------ */
+/* JOR on the compact cdg operator: src/mod_laplace.f90:514-518 (out of      */
+/* place into comp, then copied back over the relaxed range), ghost layer    */
+/* refreshed before every sweep as src/jacobi_solver_host_v4.f90:298 does.   */
+/* ------------------------------------------------------------------------ */
+void orc_jor(const orc_case *c, int n_sweeps, double omega, const int ss[3], const int nn[3], double *rel,
+             const double *bb, double *comp)
+{
+    const double *g1 = c->cdg[0], *g2 = c->cdg[1], *g3 = c->cdg[2];
+    for (int r = 0; r < n_sweeps; ++r) {
+        orc_exchange_relax(c, rel);
+        for (int k = ss[2]; k <= nn[2]; ++k)
+            for (int j = ss[1]; j <= nn[1]; ++j)
+                for (int i = ss[0]; i <= nn[0]; ++i) {
+                    double t = A(bb, i, j, k);
+                    t = t - CDG(g1, -1, i) * A(rel, i - 1, j, k);
+                    t = t - CDG(g1, 1, i) * A(rel, i + 1, j, k);
+                    t = t - CDG(g2, -1, j) * A(rel, i, j - 1, k);
+                    t = t - CDG(g2, 1, j) * A(rel, i, j + 1, k);
+                    t = t - CDG(g3, -1, k) * A(rel, i, j, k - 1);
+                    t = t - CDG(g3, 1, k) * A(rel, i, j, k + 1);
+                    const double diag = (CDG(g1, 0, i) + CDG(g2, 0, j)) + CDG(g3, 0, k);
+                    A(comp, i, j, k) = omega * t / diag + (1.0 - omega) * A(rel, i, j, k);
+                }
+        for (int k = ss[2]; k <= nn[2]; ++k)
+            for (int j = ss[1]; j <= nn[1]; ++j)
+                for (int i = ss[0]; i <= nn[0]; ++i) A(rel, i, j, k) = A(comp, i, j, k);
+    }
+    orc_exchange_relax(c, rel);
+}
+
+/* max |bb - A_cdg rel|: src/jacobi_solver_host_v4.f90:357-371 */
+double orc_residual_cdg(const orc_case *c, const int ss[3], const int nn[3], double *rel, const double *bb)
+{
+    const double *g1 = c->cdg[0], *g2 = c->cdg[1], *g3 = c->cdg[2];
+    orc_exchange_relax(c, rel);
+    double m = 0.0;
+    for (int k = ss[2]; k <= nn[2]; ++k)
+        for (int j = ss[1]; j <= nn[1]; ++j)
+            for (int i = ss[0]; i <= nn[0]; ++i) {
+                double t = A(bb, i, j, k);
+                t = t - CDG(g1, -1, i) * A(rel, i - 1, j, k);
+                t = t - CDG(g1, 1, i) * A(rel, i + 1, j, k);
+                t = t - CDG(g2, -1, j) * A(rel, i, j - 1, k);
+                t = t - CDG(g2, 1, j) * A(rel, i, j + 1, k);
+                t = t - CDG(g3, -1, k) * A(rel, i, j, k - 1);
+                t = t - CDG(g3, 1, k) * A(rel, i, j, k + 1);
+                t = t - ((CDG(g1, 0, i) + CDG(g2, 0, j)) + CDG(g3, 0, k)) * A(rel, i, j, k);
+                const double a = fabs(t);
+                m = (a > m) ? a : m;
+            }
+    return m;
+}
+
+static double absmax(const orc_case *c, const int ss[3], const int nn[3], const double *a)
+{
+    double m = 0.0;
+    for (int k = ss[2]; k <= nn[2]; ++k)
+        for (int j = ss[1]; j <= nn[1]; ++j)
+            for (int i = ss[0]; i <= nn[0]; ++i) {
+                const double v = fabs(A(a, i, j, k));
+                m = (v > m) ? v : m;
+            }
+    return m;
+}
+
+/* ------------------------------------------------------------------------ */
+/* BiCGstab_orig, preconditioner 0, init_yes = .FALSE., problem_type 2:      */
+/* src/mod_solvers.f90:1509-1935 with product_scalar (:2319-2354) and       */
+/* status_iteration (:2482-2515).                                            */
+/* ------------------------------------------------------------------------ */
+void orc_bicgstab(const orc_case *c, double eps, int n_it_max, const int ss[3], const int nn[3], const double *bb,
+                  double *phi, double *work, orc_bicgstab_stats *st)
+{
+    const int64_t n = c->ntot;
+    double *own = NULL;
+    if (!work) {
+        own = calloc((size_t)7 * n, sizeof(double));
+        work = own;
+    }
+    double *rr = work, *rh = work + n, *pk = work + 2 * n, *Ap = work + 3 * n, *Ar = work + 4 * n,
+           *dig = work + 5 * n;
+
+    orc_product_div_grad(c, phi, rr, dig);
+    for (int k = ss[2]; k <= nn[2]; ++k)
+        for (int j = ss[1]; j <= nn[1]; ++j)
+            for (int i = ss[0]; i <= nn[0]; ++i) {
+                A(rr, i, j, k) = A(bb, i, j, k) - A(rr, i, j, k);
+                A(rh, i, j, k) = A(rr, i, j, k);
+            }
+    double norm_inf = absmax(c, ss, nn, rr);
+    if (st) {
+        st->norm_inf_initial = norm_inf;
+        st->exit_reason = 0;
+    }
+
+    int counter = 0;
+    double rhr = 0.0, rhr_prev, alpha = 0.0, beta, omega = 0.0;
+    for (;;) {
+        int exit_yes = 0;
+        if (norm_inf < eps)
+            exit_yes = 1;
+        if (!exit_yes && counter == n_it_max) {
+            exit_yes = 1;
+            if (st) st->exit_reason = 1;
+        }
+        if (counter < 1 && norm_inf != 0.0)
+            exit_yes = 0; /* number_poisson == 1: at least one iteration */
+        if (exit_yes)
+            break;
+        counter = counter + 1;
+
+        rhr_prev = rhr;
+        rhr = orc_dot(c, ss, nn, rr, rh);
+        if (fabs(rhr) == 0.0) {
+            for (int k = ss[2]; k <= nn[2]; ++k)
+                for (int j = ss[1]; j <= nn[1]; ++j)
+                    for (int i = ss[0]; i <= nn[0]; ++i) A(rh, i, j, k) = A(rr, i, j, k);
+            rhr = orc_dot(c, ss, nn, rr, rh);
+        }
+        if (counter >= 2) {
+            if (omega == 0.0 || rhr_prev == 0.0) {
+                if (st) st->exit_reason = 2;
+                break;
+            }
+            beta = (alpha / omega) * (rhr / rhr_prev);
+            omega = -beta * omega;
+            for (int k = ss[2]; k <= nn[2]; ++k)
+                for (int j = ss[1]; j <= nn[1]; ++j)
+                    for (int i = ss[0]; i <= nn[0]; ++i)
+                        A(pk, i, j, k) = A(rr, i, j, k) + beta * A(pk, i, j, k) + omega * A(Ap, i, j, k);
+        } else {
+            for (int k = ss[2]; k <= nn[2]; ++k)
+                for (int j = ss[1]; j <= nn[1]; ++j)
+                    for (int i = ss[0]; i <= nn[0]; ++i) A(pk, i, j, k) = A(rr, i, j, k);
+        }
+        orc_product_div_grad(c, pk, Ap, dig);
+        const double rhAp = orc_dot(c, ss, nn, Ap, rh);
+        if (fabs(rhAp) == 0.0) {
+            if (st) st->exit_reason = 2;
+            break;
+        }
+        alpha = rhr / rhAp;
+        for (int k = ss[2]; k <= nn[2]; ++k)
+            for (int j = ss[1]; j <= nn[1]; ++j)
+                for (int i = ss[0]; i <= nn[0]; ++i) A(rr, i, j, k) = A(rr, i, j, k) - alpha * A(Ap, i, j, k);
+        orc_product_div_grad(c, rr, Ar, dig);
+        const double rAr = orc_dot(c, ss, nn, rr, Ar);
+        const double ArAr = orc_dot(c, ss, nn, Ar, Ar);
+        if (fabs(ArAr) == 0.0) {
+            if (st) st->exit_reason = 2;
+            break;
+        }
+        omega = rAr / ArAr;
+        norm_inf = 0.0;
+        for (int k = ss[2]; k <= nn[2]; ++k)
+            for (int j = ss[1]; j <= nn[1]; ++j)
+                for (int i = ss[0]; i <= nn[0]; ++i) {
+                    A(phi, i, j, k) = A(phi, i, j, k) + omega * A(rr, i, j, k) + alpha * A(pk, i, j, k);
+                    A(rr, i, j, k) = A(rr, i, j, k) - omega * A(Ar, i, j, k);
+                    const double a = fabs(A(rr, i, j, k));
+                    norm_inf = (a > norm_inf) ? a : norm_inf;
+                }
+    }
+    if (st) {
+        st->iterations = counter;
+        st->norm_inf_final = norm_inf;
+    }
+    free(own);
+}
+
+/* ------------------------------------------------------------------------ */
+/* IBM volume mask: src/mod_sharp_interface_ibm.f90:585-640 pattern --       */
+/* outside sqrt((y-yc)**2+(z-zc)**2) <= r the velocity is zeroed.  Each      */
+/* component is evaluated at its own staggered position over its B range.   */
+/* ------------------------------------------------------------------------ */
+void orc_tube_mask(const orc_case *c, double yc, double zc, const double *r_u, const double *r_p, uint8_t *mask3)
+{
+    for (int64_t q = 0; q < 3 * c->ntot; ++q) mask3[q] = 1;
+    for (int q = 0; q < 3; ++q) {
+        uint8_t *mk = mask3 + (int64_t)q * c->ntot;
+        for (int k = c->SB[q][2]; k <= c->NB[q][2]; ++k)
+            for (int j = c->SB[q][1]; j <= c->NB[q][1]; ++j)
+                for (int i = c->SB[q][0]; i <= c->NB[q][0]; ++i) {
+                    const double y = (q == 1) ? c->xu[1][j + 3] : c->xp[1][j + 3];
+                    const double z = (q == 2) ? c->xu[2][k + 3] : c->xp[2][k + 3];
+                    const double r = (q == 0) ? r_u[i + 3] : r_p[i + 3];
+                    const double dy = y - yc, dz = z - zc;
+                    mk[ORC_IDX(c, i, j, k)] = (sqrt(dy * dy + dz * dz) <= r) ? 1 : 0;
+                }
+    }
+}
+
+void orc_apply_mask(const orc_case *c, const uint8_t *mask3, double *vel)
+{
+    for (int q = 0; q < 3; ++q) {
+        const uint8_t *mk = mask3 + (int64_t)q * c->ntot;
+        double *v = vel + (int64_t)q * c->ntot;
+        for (int k = c->SB[q][2]; k <= c->NB[q][2]; ++k)
+            for (int j = c->SB[q][1]; j <= c->NB[q][1]; ++j)
+                for (int i = c->SB[q][0]; i <= c->NB[q][0]; ++i)
+                    if (!mk[ORC_IDX(c, i, j, k)]) A(v, i, j, k) = 0.0;
+    }
+}
+
+/* level_pressure: src/mod_lib.f90:507-530 */
+typedef struct {
+    const double *p;
+} pre_ctx;
+static double pre_pt(const void *ctx, int64_t idx) { return ((const pre_ctx *)ctx)->p[idx]; }
+
+void orc_level_pressure(const orc_case *c, double *pre)
+{
+    const int ss[3] = {c->Sp[0], c->Sp[1], c->Sp[2]};
+    const int nn[3] = {c->Np[0], c->Np[1], c->Np[2]};
+    pre_ctx pc = {pre};
+    double pre0 = orc_reduce_sum(c, ss, nn, pre_pt, &pc);
+    const int dim1 = c->bc[0][0] == -1 ? c->N[0] - 1 : c->N[0];
+    const int dim2 = c->bc[1][0] == -1 ? c->N[1] - 1 : c->N[1];
+    const int dim3 = c->bc[2][0] == -1 ? c->N[2] - 1 : c->N[2];
+    pre0 = pre0 / (double)dim1 / (double)dim2 / (double)dim3;
+    for (int k = ss[2]; k <= nn[2]; ++k)
+        for (int j = ss[1]; j <= nn[1]; ++j)
+            for (int i = ss[0]; i <= nn[0]; ++i) A(pre, i, j, k) = A(pre, i, j, k) - pre0;
+}
+
+/* fill_corners: src/mod_lib.f90:549-575 */
+void orc_fill_corners(const orc_case *c, double *phi)
+{
+    const int N1 = c->N[0], N2 = c->N[1], N3 = c->N[2];
+    const int w1L = c->bc[0][0] > 0, w1U = c->bc[0][1] > 0;
+    const int w2L = c->bc[1][0] > 0, w2U = c->bc[1][1] > 0;
+    const int w3L = c->bc[2][0] > 0, w3U = c->bc[2][1] > 0;
+    for (int k = 1; k <= N3; ++k) {
+        if (w1L && w2L) A(phi, 1, 1, k) = (A(phi, 2, 1, k) + A(phi, 1, 2, k) + A(phi, 2, 2, k)) / 3.0;
+        if (w1L && w2U) A(phi, 1, N2, k) = (A(phi, 2, N2, k) + A(phi, 1, N2 - 1, k) + A(phi, 2, N2 - 1, k)) / 3.0;
+        if (w1U && w2L) A(phi, N1, 1, k) = (A(phi, N1 - 1, 1, k) + A(phi, N1, 2, k) + A(phi, N1 - 1, 2, k)) / 3.0;
+        if (w1U && w2U)
+            A(phi, N1, N2, k) = (A(phi, N1 - 1, N2, k) + A(phi, N1, N2 - 1, k) + A(phi, N1 - 1, N2 - 1, k)) / 3.0;
+    }
+    for (int j = 1; j <= N2; ++j) {
+        if (w1L && w3L) A(phi, 1, j, 1) = (A(phi, 2, j, 1) + A(phi, 1, j, 2) + A(phi, 2, j, 2)) / 3.0;
+        if (w1L && w3U) A(phi, 1, j, N3) = (A(phi, 2, j, N3) + A(phi, 1, j, N3 - 1) + A(phi, 2, j, N3 - 1)) / 3.0;
+        if (w1U && w3L) A(phi, N1, j, 1) = (A(phi, N1 - 1, j, 1) + A(phi, N1, j, 2) + A(phi, N1 - 1, j, 2)) / 3.0;
+        if (w1U && w3U)
+            A(phi, N1, j, N3) = (A(phi, N1 - 1, j, N3) + A(phi, N1, j, N3 - 1) + A(phi, N1 - 1, j, N3 - 1)) / 3.0;
+    }
+    for (int i = 1; i <= N1; ++i) {
+        if (w2L && w3L) A(phi, i, 1, 1) = (A(phi, i, 2, 1) + A(phi, i, 1, 2) + A(phi, i, 2, 2)) / 3.0;
+        if (w2L && w3U) A(phi, i, 1, N3) = (A(phi, i, 2, N3) + A(phi, i, 1, N3 - 1) + A(phi, i, 2, N3 - 1)) / 3.0;
+        if (w2U && w3L) A(phi, i, N2, 1) = (A(phi, i, N2 - 1, 1) + A(phi, i, N2, 2) + A(phi, i, N2 - 1, 2)) / 3.0;
+        if (w2U && w3U)
+            A(phi, i, N2, N3) = (A(phi, i, N2 - 1, N3) + A(phi, i, N2, N3 - 1) + A(phi, i, N2 - 1, N3 - 1)) / 3.0;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* boundary values for the Dirichlet planes of rhs (src/mod_rhs.f90:213-241):*/
+/* every wall value is 0 except bc11 (u on the x-faces), which carries the   */
+/* inflow profile of the case at both x = 0 and x = L1.                      */
+/* ------------------------------------------------------------------------ */
+double orc_inflow_amplitude(int kind, double amp, double t)
+{
+    const double pi = 3.14159265358979323846;
+    if (kind == 2)
+        return amp * (1.0 + 0.5 * sin(2.0 * pi * t) + 0.2 * sin(4.0 * pi * t + 0.7) + 0.1 * sin(6.0 * pi * t + 1.3));
+    return amp;
+}
+
+static double bc11_value(const orc_case *c, const orc_step_params *p, double t, int side, int j, int k)
+{
+    if (p->inflow_kind == 1) {
+        const double y = c->xp[1][j + 3], z = c->xp[2][k + 3];
+        return p->inflow_amp * (16.0 * y * (1.0 - y) * z * (1.0 - z));
+    }
+    if (p->inflow_kind == 2) {
+        const int i = side == 0 ? 0 : c->N[0];
+        const uint8_t fluid = p->mask3 ? p->mask3[ORC_IDX(c, i, j, k)] : 1;
+        return fluid ? orc_inflow_amplitude(2, p->inflow_amp, t) : 0.0;
+    }
+    return 0.0;
+}
+
+static void set_plane(const orc_case *c, double *f, int q, int e, int plane, double value_or_nan, const orc_step_params *p,
+                      double t, int side)
+{
+    int lo[3], hi[3];
+    for (int g = 0; g < 3; ++g) {
+        lo[g] = c->SB[q][g];
+        hi[g] = c->NB[q][g];
+    }
+    lo[e] = hi[e] = plane;
+    for (int k = lo[2]; k <= hi[2]; ++k)
+        for (int j = lo[1]; j <= hi[1]; ++j)
+            for (int i = lo[0]; i <= hi[0]; ++i)
+                A(f, i, j, k) = (q == 0 && e == 0) ? bc11_value(c, p, t, side, j, k) : value_or_nan;
+}
+
+/* "RB einsetzen", src/mod_rhs.f90:213-241: the order of the planes decides the
+ * edge values (the last plane written wins). */
+static void apply_bc_planes(const orc_case *c, double *rhs, const orc_step_params *p, double t)
+{
+    static const int order[3][3] = {{1, 2, 0}, {0, 2, 1}, {0, 1, 2}};
+    for (int q = 0; q < 3; ++q) {
+        double *f = rhs + (int64_t)q * c->ntot;
+        for (int o = 0; o < 3; ++o) {
+            const int e = order[q][o];
+            for (int side = 0; side < 2; ++side) {
+                if (!(c->bc[e][side] > 0))
+                    continue;
+                const int plane = side == 0 ? (e == q ? 0 : 1) : c->N[e];
+                set_plane(c, f, q, e, plane, 0.0, p, t, side);
+            }
+        }
+    }
+}
+
+static void range_op(const orc_case *c, int q, double *dst, const double *src, double mult, int kind)
+{
+    for (int k = c->SB[q][2]; k <= c->NB[q][2]; ++k)
+        for (int j = c->SB[q][1]; j <= c->NB[q][1]; ++j)
+            for (int i = c->SB[q][0]; i <= c->NB[q][0]; ++i) {
+                const int64_t x = ORC_IDX(c, i, j, k);
+                if (kind == 0)
+                    dst[x] = src[x];
+                else
+                    dst[x] = dst[x] - mult * src[x];
+            }
+}
+
+/* one time step = 3 RK substeps: src/mod_timeint.f90:275-377 with rhs_vel
+ * (src/mod_rhs.f90:35-241) and explicit (src/mod_solvers.f90:64-346); after the
+ * step the mask (apply_CT_forcing pattern, :416) and level_pressure (:552). */
+void orc_timestep(orc_case *c, const orc_step_params *p, int step_index, double time, double *vel, double *pre,
+                  double *nl, orc_step_stats *st)
+{
+    (void)step_index;
+    const int64_t n = c->ntot;
+    double *work = calloc((size_t)3 * n, sizeof(double));
+    double *rhs = calloc((size_t)3 * n, sizeof(double));
+    double *res = calloc((size_t)n, sizeof(double));
+    double *gpre = calloc((size_t)n, sizeof(double));
+    double *pp = calloc((size_t)n, sizeof(double));
+    double *comp = calloc((size_t)n, sizeof(double));
+    double *bw = calloc((size_t)7 * n, sizeof(double));
+    const int ss[3] = {c->Sp[0], c->Sp[1], c->Sp[2]};
+    const int nn[3] = {c->Np[0], c->Np[1], c->Np[2]};
+    const double dtime = p->dtime;
+
+    for (int s = 0; s < 3; ++s) {
+        double subtime;
+        if (s == 0)
+            subtime = time + dtime * aRK[0];
+        else if (s == 1)
+            subtime = time + dtime * (aRK[0] + aRK[1] + bRK[1]);
+        else
+            subtime = time + dtime;
+
+        orc_exchange_all_all(c, vel);
+        orc_interpolate_vel(c, vel, work);
+
+        /* rhs_vel */
+        for (int q = 0; q < 3; ++q) range_op(c, q, rhs + q * n, vel + q * n, 0.0, 0);
+        if (s != 0) {
+            const double mult = dtime * bRK[s];
+            for (int q = 0; q < 3; ++q) range_op(c, q, rhs + q * n, nl + q * n, mult, 1);
+        }
+        orc_nonlinear(c, vel, work, nl, pp);
+        orc_helmholtz_explicit(c, 1.0 / p->Re, vel, nl);
+        {
+            const double mult = dtime * aRK[s];
+            for (int q = 0; q < 3; ++q) range_op(c, q, rhs + q * n, nl + q * n, mult, 1);
+        }
+        apply_bc_planes(c, rhs, p, subtime);
+
+        /* pressure as the increment's initial guess */
+        for (int k = ss[2]; k <= nn[2]; ++k)
+            for (int j = ss[1]; j <= nn[1]; ++j)
+                for (int i = ss[0]; i <= nn[0]; ++i) A(pre, i, j, k) = A(pre, i, j, k) * (aRK[s] + bRK[s]) * dtime;
+
+        /* explicit */
+        for (int q = 0; q < 3; ++q) range_op(c, q, vel + q * n, rhs + q * n, 0.0, 0);
+        orc_divergence2(c, vel, res);
+        if (p->jor_sweeps > 0)
+            orc_jor(c, p->jor_sweeps, p->jor_omega, ss, nn, pre, res, comp);
+        orc_bicgstab_stats bst;
+        const double eps = p->bicg_rel_tol * absmax(c, ss, nn, res);
+        orc_bicgstab(c, eps, p->bicg_max_it, ss, nn, res, pre, bw, &bst);
+        if (st) {
+            st->bicg_iterations[s] = bst.iterations;
+            st->bicg_final[s] = bst.norm_inf_final;
+        }
+        for (int m = 1; m <= 3; ++m) {
+            orc_gradient(c, m, pre, gpre);
+            range_op(c, m - 1, vel + (m - 1) * n, gpre, 1.0, 1);
+        }
+
+        for (int k = ss[2]; k <= nn[2]; ++k)
+            for (int j = ss[1]; j <= nn[1]; ++j)
+                for (int i = ss[0]; i <= nn[0]; ++i)
+                    A(pre, i, j, k) = A(pre, i, j, k) / (aRK[s] + bRK[s]) / dtime;
+        orc_fill_corners(c, pre);
+    }
+    if (p->mask3)
+        orc_apply_mask(c, p->mask3, vel);
+    if (p->level_pressure)
+        orc_level_pressure(c, pre);
+    if (st) {
+        orc_divergence2(c, vel, res);
+        st->div_max_after = absmax(c, ss, nn, res);
+    }
+    free(work);
+    free(rhs);
+    free(res);
+    free(gpre);
+    free(pp);
+    free(comp);
+    free(bw);
+}

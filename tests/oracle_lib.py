@@ -1,0 +1,234 @@
+"""ctypes wrapper of the CPU oracle (oracle/_build/liboracle.so).
+
+Test infrastructure: only tests, ``__graft_entry__.smoke()`` and bench.py's
+``cpu_baseline`` leg use this module, and only as the checker.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent.parent
+ORACLE_DIR = ROOT / "oracle"
+LIB_PATH = ORACLE_DIR / "_build" / "liboracle.so"
+
+_lib = None
+
+
+def build(force: bool = False) -> Path:
+    if force or not LIB_PATH.exists():
+        subprocess.run(["make", "-s", "-C", str(ORACLE_DIR)], check=True)
+    return LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            build()
+        L = C.CDLL(str(LIB_PATH))
+        dp = C.POINTER(C.c_double)
+        ip = C.POINTER(C.c_int)
+        vp = C.c_void_p
+        L.orc_case_create.restype = vp
+        L.orc_case_create.argtypes = [ip, ip, dp, dp]
+        L.orc_case_destroy.argtypes = [vp]
+        L.orc_field_size.restype = C.c_int64
+        L.orc_field_size.argtypes = [vp]
+        L.orc_ranges.argtypes = [vp, ip]
+        L.orc_coords.argtypes = [vp, C.c_int, C.c_int, dp]
+        L.orc_coeff.restype = C.c_int
+        L.orc_coeff.argtypes = [vp, C.c_char_p, dp]
+        L.orc_exact_table.restype = C.c_int
+        L.orc_exact_table.argtypes = [C.c_int, C.c_int, C.c_double, dp]
+        L.orc_exchange.argtypes = [vp, C.c_int, C.c_int, dp]
+        L.orc_exchange_all_all.argtypes = [vp, dp]
+        L.orc_exchange_relax.argtypes = [vp, dp]
+        L.orc_divergence2.argtypes = [vp, dp, dp]
+        L.orc_gradient.argtypes = [vp, C.c_int, dp, dp]
+        L.orc_interpolate2_vel_pre.argtypes = [vp, C.c_int, dp, dp]
+        L.orc_interpolate2_pre_vel.argtypes = [vp, C.c_int, dp, dp]
+        L.orc_interpolate_vel.argtypes = [vp, dp, dp]
+        L.orc_nonlinear.argtypes = [vp, dp, dp, dp, dp]
+        L.orc_helmholtz_explicit.argtypes = [vp, C.c_double, dp, dp]
+        L.orc_product_div_grad.argtypes = [vp, dp, dp, dp]
+        L.orc_set_reduction_mode.argtypes = [vp, C.c_int]
+        L.orc_dot.restype = C.c_double
+        L.orc_dot.argtypes = [vp, ip, ip, dp, dp]
+        L.orc_jor.argtypes = [vp, C.c_int, C.c_double, ip, ip, dp, dp, dp]
+        L.orc_residual_cdg.restype = C.c_double
+        L.orc_residual_cdg.argtypes = [vp, ip, ip, dp, dp]
+        L.orc_bicgstab.argtypes = [vp, C.c_double, C.c_int, ip, ip, dp, dp, dp, C.POINTER(BicgStats)]
+        L.orc_timestep.argtypes = [vp, C.POINTER(StepParams), C.c_int, C.c_double, dp, dp, dp, C.POINTER(StepStats)]
+        L.orc_tube_mask.argtypes = [vp, C.c_double, C.c_double, dp, dp, C.POINTER(C.c_uint8)]
+        L.orc_apply_mask.argtypes = [vp, C.POINTER(C.c_uint8), dp]
+        L.orc_level_pressure.argtypes = [vp, dp]
+        L.orc_fill_corners.argtypes = [vp, dp]
+        L.orc_inflow_amplitude.restype = C.c_double
+        L.orc_inflow_amplitude.argtypes = [C.c_int, C.c_double, C.c_double]
+        _lib = L
+    return _lib
+
+
+class BicgStats(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("norm_inf_initial", C.c_double),
+                ("norm_inf_final", C.c_double), ("exit_reason", C.c_int)]
+
+
+class StepParams(C.Structure):
+    _fields_ = [("Re", C.c_double), ("dtime", C.c_double), ("jor_sweeps", C.c_int),
+                ("jor_omega", C.c_double), ("bicg_rel_tol", C.c_double), ("bicg_max_it", C.c_int),
+                ("level_pressure", C.c_int), ("inflow_kind", C.c_int), ("inflow_amp", C.c_double),
+                ("mask3", C.POINTER(C.c_uint8))]
+
+
+class StepStats(C.Structure):
+    _fields_ = [("bicg_iterations", C.c_int * 3), ("bicg_final", C.c_double * 3),
+                ("div_max_after", C.c_double)]
+
+
+def _dp(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _ip(seq):
+    arr = (C.c_int * len(seq))(*seq)
+    return arr
+
+
+class OracleCase:
+    """A single-block grid of the oracle with its coefficients."""
+
+    def __init__(self, N, bc, L=(1.0, 1.0, 1.0), stretch=(0.0, 0.0, 0.0)):
+        self.L = lib()
+        self.N = tuple(int(n) for n in N)
+        self.bc = tuple(int(b) for b in bc)
+        Ld = (C.c_double * 3)(*L)
+        Sd = (C.c_double * 3)(*stretch)
+        self.h = self.L.orc_case_create(_ip(self.N), _ip(self.bc), Ld, Sd)
+        if not self.h:
+            raise ValueError("oracle rejected the case")
+        self.shape = (self.N[2] + 7, self.N[1] + 7, self.N[0] + 7)  # numpy (k, j, i)
+        self.size = int(self.L.orc_field_size(self.h))
+        r = (C.c_int * 48)()
+        self.L.orc_ranges(self.h, r)
+        r = list(r)
+        self.Sp, self.Np = r[0:3], r[3:6]
+        self.SB = [r[6 + 3 * m:9 + 3 * m] for m in range(3)]
+        self.NB = [r[15 + 3 * m:18 + 3 * m] for m in range(3)]
+        self.SI = [r[24 + 3 * m:27 + 3 * m] for m in range(3)]
+        self.NI = [r[33 + 3 * m:36 + 3 * m] for m in range(3)]
+        self.SNF = (r[42:45], r[45:48])
+
+    def __del__(self):
+        try:
+            if self.h:
+                self.L.orc_case_destroy(self.h)
+        except Exception:
+            pass
+
+    # --- fields -------------------------------------------------------------
+    def zeros(self, n=1):
+        if n == 1:
+            return np.zeros(self.shape, dtype=np.float64)
+        return np.zeros((n,) + self.shape, dtype=np.float64)
+
+    @staticmethod
+    def at(f, i, j, k):
+        """Fortran-index view helper: f[k+3, j+3, i+3]."""
+        return f[k + 3, j + 3, i + 3]
+
+    def coords(self, d, which):
+        out = np.zeros(self.N[d] + 7)
+        self.L.orc_coords(self.h, d, which, _dp(out))
+        return out
+
+    def coeff(self, name):
+        n = max(self.N) + 1
+        buf = np.zeros(7 * n + 16)
+        cnt = self.L.orc_coeff(self.h, name.encode(), _dp(buf))
+        if cnt < 0:
+            raise KeyError(name)
+        return buf[:cnt].copy()
+
+    # --- operators ----------------------------------------------------------
+    def exchange(self, d, vel_dir, f):
+        self.L.orc_exchange(self.h, d, vel_dir, _dp(f))
+
+    def exchange_all_all(self, vel3):
+        self.L.orc_exchange_all_all(self.h, _dp(vel3))
+
+    def exchange_relax(self, f):
+        self.L.orc_exchange_relax(self.h, _dp(f))
+
+    def divergence2(self, vel3, div):
+        self.L.orc_divergence2(self.h, _dp(vel3), _dp(div))
+
+    def gradient(self, m, phi, grad):
+        self.L.orc_gradient(self.h, m, _dp(phi), _dp(grad))
+
+    def interpolate_vel(self, vel3, work3):
+        self.L.orc_interpolate_vel(self.h, _dp(vel3), _dp(work3))
+
+    def interpolate2_vel_pre(self, m, phi, inter):
+        self.L.orc_interpolate2_vel_pre(self.h, m, _dp(phi), _dp(inter))
+
+    def interpolate2_pre_vel(self, m, phi, inter):
+        self.L.orc_interpolate2_pre_vel(self.h, m, _dp(phi), _dp(inter))
+
+    def nonlinear(self, vel3, work3, nl3, pp):
+        self.L.orc_nonlinear(self.h, _dp(vel3), _dp(work3), _dp(nl3), _dp(pp))
+
+    def helmholtz(self, multL, vel3, nl3):
+        self.L.orc_helmholtz_explicit(self.h, multL, _dp(vel3), _dp(nl3))
+
+    def product_div_grad(self, phi, lap, dig):
+        self.L.orc_product_div_grad(self.h, _dp(phi), _dp(lap), _dp(dig))
+
+    def set_reduction_mode(self, mode):
+        self.L.orc_set_reduction_mode(self.h, int(mode))
+
+    def dot(self, ss, nn, a, b):
+        return self.L.orc_dot(self.h, _ip(ss), _ip(nn), _dp(a), _dp(b))
+
+    def jor(self, n, omega, ss, nn, rel, bb, comp):
+        self.L.orc_jor(self.h, n, omega, _ip(ss), _ip(nn), _dp(rel), _dp(bb), _dp(comp))
+
+    def residual_cdg(self, ss, nn, rel, bb):
+        return self.L.orc_residual_cdg(self.h, _ip(ss), _ip(nn), _dp(rel), _dp(bb))
+
+    def bicgstab(self, eps, nmax, ss, nn, bb, phi):
+        st = BicgStats()
+        self.L.orc_bicgstab(self.h, eps, nmax, _ip(ss), _ip(nn), _dp(bb), _dp(phi), None, C.byref(st))
+        return st
+
+    def timestep(self, params: StepParams, step, time, vel3, pre, nl3):
+        st = StepStats()
+        self.L.orc_timestep(self.h, C.byref(params), step, time, _dp(vel3), _dp(pre), _dp(nl3), C.byref(st))
+        return st
+
+    def tube_mask(self, yc, zc, r_u, r_p):
+        m = np.zeros((3,) + self.shape, dtype=np.uint8)
+        self.L.orc_tube_mask(self.h, yc, zc, _dp(np.ascontiguousarray(r_u)), _dp(np.ascontiguousarray(r_p)),
+                             m.ctypes.data_as(C.POINTER(C.c_uint8)))
+        return m
+
+    def apply_mask(self, mask3, vel3):
+        self.L.orc_apply_mask(self.h, mask3.ctypes.data_as(C.POINTER(C.c_uint8)), _dp(vel3))
+
+    def level_pressure(self, pre):
+        self.L.orc_level_pressure(self.h, _dp(pre))
+
+    def fill_corners(self, pre):
+        self.L.orc_fill_corners(self.h, _dp(pre))
+
+
+def exact_table(abl, n, dx):
+    cc = np.zeros(7)
+    ok = lib().orc_exact_table(abl, n, dx, _dp(cc))
+    return cc[:n].copy() if ok else None

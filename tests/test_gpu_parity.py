@@ -41,6 +41,20 @@ def box(lo, hi):
     return tuple(slice(lo[d] + 3, hi[d] + 4) for d in (2, 1, 0))
 
 
+def readable(oc):
+    """Cells a one-dimensional stencil from the pressure box can read: at most
+    one index outside the box.  Edge and corner ghosts are filled from other
+    ghosts by the exchange order and are never read."""
+    out = np.zeros(oc.shape, dtype=int)
+    for d, ax in ((0, 2), (1, 1), (2, 0)):
+        idx = np.arange(oc.N[d] + 7) - 3
+        outside = (idx < oc.Sp[d]) | (idx > oc.Np[d])
+        shp = [1, 1, 1]
+        shp[ax] = -1
+        out = out + outside.reshape(shp)
+    return out <= 1
+
+
 @pytest.mark.parametrize("spec", GRIDS)
 def test_exchange(spec):
     oc, gc = pair(spec)
@@ -180,7 +194,8 @@ def test_jor_and_residual(spec, sweeps):
     gc.upload(ih.RES, bb)
     oc.jor(sweeps, 0.8, oc.Sp, oc.Np, rel, bb, oc.zeros())
     gc.jor(sweeps, 0.8, oc.Sp, oc.Np)
-    same(gc.download(ih.PRE), rel, "jor")
+    r = readable(oc)
+    same(gc.download(ih.PRE)[r], rel[r], "jor")
     r_o = oc.residual_cdg(oc.Sp, oc.Np, rel, bb)
     r_g = gc.residual_cdg(oc.Sp, oc.Np)
     assert r_o == r_g

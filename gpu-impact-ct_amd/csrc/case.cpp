@@ -94,11 +94,16 @@ void init_limits(Geo &g, const int bc[6]) {
     }
 }
 
+// both partial buffers hold at least nb block partials
 int ensure_part(ih_case *c, int64_t nb) {
-    if (nb <= c->part_cap) return IH_OK;
+    if (nb <= c->part_cap && c->part2) return IH_OK;
+    nb = std::max(nb, c->part_cap);
+    HIPCHK(hipStreamSynchronize(c->stream));
     if (c->part) (void)hipFree(c->part);
-    c->part = nullptr;
+    if (c->part2) (void)hipFree(c->part2);
+    c->part = c->part2 = nullptr;
     HIPCHK(hipMalloc(&c->part, sizeof(double) * (size_t)nb));
+    HIPCHK(hipMalloc(&c->part2, sizeof(double) * (size_t)nb));
     c->part_cap = nb;
     return IH_OK;
 }
@@ -197,19 +202,17 @@ int exch(ih_case *c, int dir, int vel_dir, double *f) {
     return IH_OK;
 }
 
-// product_div_grad (src/mod_laplace.f90:38-197): gradient then divergence per
-// direction, through the DIG field
+// product_div_grad (src/mod_laplace.f90:38-197) with the gradients kept on chip.
+// The reference also refreshes phi's ghost layers (exchange(m, 0, phi)); the
+// fused kernel reads them through the same rules, and the exchange is kept so
+// phi leaves the call in the same state.
 int apply_div_grad(ih_case *c, double *phi, double *lap) {
-    double *dig = F(c, IH_DIG);
     for (int m = 1; m <= 3; ++m) {
         int rc = exch(c, m, 0, phi);
         if (rc) return rc;
-        launch_gradient(c->stream, c->g, c->c, m - 1, phi, dig);
-        rc = exch(c, m, m, dig);
-        if (rc) return rc;
-        launch_div_dir(c->stream, c->g, c->c, m - 1, dig, lap, m == 1 ? 1 : 0);
-        LAUNCHCHK();
     }
+    launch_apply_A(c->stream, c->g, c->c, pbox(c->g), phi, lap, nullptr, KDOT_NONE, nullptr, nullptr, nullptr);
+    LAUNCHCHK();
     return IH_OK;
 }
 
@@ -392,6 +395,12 @@ int ih_case_create(const ih_grid_desc *gd, int device, ih_case **out) {
     Box pb = pbox(g);
     int rc = ensure_part(c, std::max<int64_t>(red_blocks(pb), 4096));
     if (rc) return cleanup(rc);
+    if (hipMalloc(&c->ksc, sizeof(double) * KS_NSLOTS) != hipSuccess) return cleanup(fail(IH_ENOMEM, "ksc"));
+    if (hipMalloc(&c->kfl, sizeof(int) * KF_NSLOTS) != hipSuccess) return cleanup(fail(IH_ENOMEM, "kfl"));
+    if (hipHostMalloc((void **)&c->hksc, sizeof(double) * KS_NSLOTS, hipHostMallocDefault) != hipSuccess)
+        return cleanup(fail(IH_ENOMEM, "pinned"));
+    if (hipHostMalloc((void **)&c->hkfl, sizeof(int) * KF_NSLOTS, hipHostMallocDefault) != hipSuccess)
+        return cleanup(fail(IH_ENOMEM, "pinned"));
     if (hipStreamSynchronize(c->stream) != hipSuccess) return cleanup(fail(IH_EHIP, "stream sync"));
     *out = c;
     return IH_OK;
@@ -409,6 +418,11 @@ void ih_case_destroy(ih_case *c) {
     if (c->dcoef) (void)hipFree(c->dcoef);
     if (c->dcoef32) (void)hipFree(c->dcoef32);
     if (c->part) (void)hipFree(c->part);
+    if (c->part2) (void)hipFree(c->part2);
+    if (c->ksc) (void)hipFree(c->ksc);
+    if (c->kfl) (void)hipFree(c->kfl);
+    if (c->hksc) (void)hipHostFree(c->hksc);
+    if (c->hkfl) (void)hipHostFree(c->hkfl);
     if (c->dscal) (void)hipFree(c->dscal);
     if (c->hscal) (void)hipHostFree(c->hscal);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -772,8 +786,9 @@ int ih_residual_cdg(ih_case *c, const int ss[3], const int nn[3], ih_field rel, 
 }
 
 // BiCGstab_orig (src/mod_solvers.f90:1509-1935), preconditioner 0,
-// init_yes = .FALSE.; the scalar recurrences run on the host in the
-// reference's order, every dot product is a device reduction.
+// init_yes = .FALSE.: the iteration runs on the device (scalars, stop test and
+// breakdown exits in one-block control kernels); the host enqueues batches of
+// iterations and reads the flags once per batch.
 int ih_bicgstab(ih_case *c, double eps, int n_it_max, const int ss[3], const int nn[3], ih_field bbf, ih_field phif,
                 ih_bicgstab_stats *st) {
     int rc = need_coeffs(c);
@@ -783,93 +798,40 @@ int ih_bicgstab(ih_case *c, double eps, int n_it_max, const int ss[3], const int
         if (bbf == f || phif == f) return fail(IH_EINVAL, "bb/phi may not be a BiCGstab work field");
     Box b;
     if ((rc = to_box(c, ss, nn, b))) return rc;
-    double *rr = F(c, IH_RR), *rh = F(c, IH_RH), *pk = F(c, IH_PK), *Ap = F(c, IH_AP), *Ar = F(c, IH_AR);
-    double *phi = F(c, phif);
-    const double *bb = F(c, bbf);
-    const Geo &g = c->g;
+    const Box pb = pbox(c->g);
+    for (int d = 0; d < 3; ++d)
+        if (b.lo[d] != pb.lo[d] || b.hi[d] != pb.hi[d])
+            return fail(IH_EINVAL, "BiCGstab runs on the pressure range S?p..N?p");
+    const int64_t nb = red_blocks(b);
+    if ((rc = ensure_part(c, nb))) return rc;
     hipStream_t s = c->stream;
-
-    if ((rc = apply_div_grad(c, phi, rr))) return rc;
-    launch_bicg_init(s, g, b, bb, rr, rh);
+    KrylovPtrs k{F(c, IH_RR), F(c, IH_RH), F(c, IH_PK), F(c, IH_AP), F(c, IH_AR), F(c, phif), c->part, c->part2,
+                 c->ksc, c->kfl};
+    // rr = bb - A phi, rh = rr, norm and <rr, rh>
+    if ((rc = apply_div_grad(c, F(c, phif), k.rr))) return rc;
+    launch_krylov_init(s, c->g, b, F(c, bbf), k.rr, k.rh, k.part1, k.part2, k.sc, k.fl, eps, n_it_max);
     LAUNCHCHK();
-    double norm_inf = absmax_host(c, b, rr, &rc);
-    if (rc) return rc;
-    if (st) {
-        st->norm_inf_initial = norm_inf;
-        st->exit_reason = 0;
-    }
-    int counter = 0;
-    double rhr = 0.0, rhr_prev, alpha = 0.0, beta, omega = 0.0;
+    const int batch = 8;
     for (;;) {
-        bool exit_yes = norm_inf < eps;
-        if (!exit_yes && counter == n_it_max) {
-            exit_yes = true;
-            if (st) st->exit_reason = 1;
-        }
-        if (counter < 1 && norm_inf != 0.0) exit_yes = false;
-        if (exit_yes) break;
-        counter = counter + 1;
-        rhr_prev = rhr;
-        if ((rc = dot_to(c, b, rr, rh, S_DOT0))) return rc;
-        if ((rc = read_scalars(c, S_DOT0, 1))) return rc;
-        rhr = c->hscal[S_DOT0];
-        if (std::fabs(rhr) == 0.0) {
-            launch_copy_box(s, g, b, rr, rh);
-            if ((rc = dot_to(c, b, rr, rh, S_DOT0))) return rc;
-            if ((rc = read_scalars(c, S_DOT0, 1))) return rc;
-            rhr = c->hscal[S_DOT0];
-        }
-        if (counter >= 2) {
-            if (omega == 0.0 || rhr_prev == 0.0) {
-                if (st) st->exit_reason = 2;
-                break;
-            }
-            beta = (alpha / omega) * (rhr / rhr_prev);
-            omega = -beta * omega;
-            c->hscal[S_BETA] = beta;
-            c->hscal[S_OM2] = omega;
-            if ((rc = write_scalars(c, S_BETA, 2))) return rc;
-            launch_bicg_p(s, g, b, rr, pk, Ap, c->dscal + S_BETA);
-        } else {
-            launch_copy_box(s, g, b, rr, pk);
-        }
+        for (int q = 0; q < batch; ++q) launch_krylov_iteration(s, c->g, c->c, b, k, 0);
         LAUNCHCHK();
-        if ((rc = apply_div_grad(c, pk, Ap))) return rc;
-        if ((rc = dot_to(c, b, Ap, rh, S_DOT1))) return rc;
-        if ((rc = read_scalars(c, S_DOT1, 1))) return rc;
-        const double rhAp = c->hscal[S_DOT1];
-        if (std::fabs(rhAp) == 0.0) {
-            if (st) st->exit_reason = 2;
-            break;
+        HIPCHK(hipMemcpyAsync(c->hkfl, c->kfl, sizeof(int) * KF_NSLOTS, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        if (c->hkfl[KF_DONE]) break;
+        if (c->hkfl[KF_RESET]) {
+            // rhr == 0: new reference vector rh = rr, then finish this iteration
+            launch_krylov_resume(s, c->g, b, k);
+            launch_krylov_iteration(s, c->g, c->c, b, k, 1);
+            LAUNCHCHK();
         }
-        alpha = rhr / rhAp;
-        c->hscal[S_ALPHA] = alpha;
-        if ((rc = write_scalars(c, S_ALPHA, 1))) return rc;
-        launch_bicg_s(s, g, b, rr, Ap, c->dscal + S_ALPHA);
-        LAUNCHCHK();
-        if ((rc = apply_div_grad(c, rr, Ar))) return rc;
-        if ((rc = dot_to(c, b, rr, Ar, S_DOT2))) return rc;
-        if ((rc = dot_to(c, b, Ar, Ar, S_DOT3))) return rc;
-        if ((rc = read_scalars(c, S_DOT2, 2))) return rc;
-        const double rAr = c->hscal[S_DOT2], ArAr = c->hscal[S_DOT3];
-        if (std::fabs(ArAr) == 0.0) {
-            if (st) st->exit_reason = 2;
-            break;
-        }
-        omega = rAr / ArAr;
-        c->hscal[S_OMEGA] = omega;
-        c->hscal[S_ALPHA2] = alpha;
-        if ((rc = write_scalars(c, S_OMEGA, 2))) return rc;
-        const int64_t nb = red_blocks(b);
-        launch_bicg_x(s, g, b, phi, rr, pk, Ar, c->dscal + S_OMEGA, c->part);
-        launch_final_max(s, c->part, nb, c->dscal + S_MAX);
-        LAUNCHCHK();
-        if ((rc = read_scalars(c, S_MAX, 1))) return rc;
-        norm_inf = c->hscal[S_MAX];
     }
+    HIPCHK(hipMemcpyAsync(c->hksc, c->ksc, sizeof(double) * KS_NSLOTS, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
     if (st) {
-        st->iterations = counter;
-        st->norm_inf_final = norm_inf;
+        st->iterations = c->hkfl[KF_COUNTER];
+        st->norm_inf_initial = c->hksc[KS_NORM0];
+        st->norm_inf_final = c->hksc[KS_NORM];
+        st->exit_reason = c->hkfl[KF_EXIT];
     }
     return IH_OK;
 }

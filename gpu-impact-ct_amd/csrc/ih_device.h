@@ -1,0 +1,108 @@
+// Device-side helpers shared by the kernel files of libimpact_hip.
+#pragma once
+
+#include "ih_internal.h"
+
+namespace ih {
+namespace dev {
+
+constexpr int BX = 64;
+constexpr int BY = 4;
+
+struct Range3 {
+    int lo0, lo1, lo2, n0, n1, n2;
+};
+
+inline Range3 make_range(const Box &b) {
+    return Range3{b.lo[0], b.lo[1], b.lo[2], b.hi[0] - b.lo[0] + 1, b.hi[1] - b.lo[1] + 1, b.hi[2] - b.lo[2] + 1};
+}
+
+inline dim3 grid_for(const Range3 &r) {
+    return dim3((unsigned)((r.n0 + BX - 1) / BX), (unsigned)((r.n1 + BY - 1) / BY), (unsigned)r.n2);
+}
+
+inline bool empty(const Range3 &r) { return r.n0 <= 0 || r.n1 <= 0 || r.n2 <= 0; }
+
+inline dim3 red_grid(const Range3 &r) {
+    return dim3((unsigned)((r.n0 + RED_TX - 1) / RED_TX), (unsigned)((r.n1 + RED_TY - 1) / RED_TY),
+                (unsigned)((r.n2 + RED_KC - 1) / RED_KC));
+}
+
+#define IH_POINT(r)                                                  \
+    const int i = (r).lo0 + (int)(blockIdx.x * BX + threadIdx.x);    \
+    const int j = (r).lo1 + (int)(blockIdx.y * BY + threadIdx.y);    \
+    const int k = (r).lo2 + (int)blockIdx.z;                         \
+    if (i >= (r).lo0 + (r).n0 || j >= (r).lo1 + (r).n1) return;
+
+__device__ inline int64_t sh(const Geo &g, int d) { return d == 0 ? 1 : (d == 1 ? (int64_t)g.px : g.pxy); }
+
+__device__ inline int pick(int d, int i, int j, int k) { return d == 0 ? i : (d == 1 ? j : k); }
+
+// Reduction tree: xor butterfly within the 64-lane wavefront (offsets 32..1),
+// then the four wave sums of a 256-thread block as (w0+w1)+(w2+w3).
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v = v + __shfl_xor(v, off, 64);
+    return v;
+}
+
+__device__ inline double wave_max(double v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        const double o = __shfl_xor(v, off, 64);
+        v = (o > v) ? o : v;
+    }
+    return v;
+}
+
+__device__ inline double block_sum(double v, double *sm) {
+    v = wave_sum(v);
+    const int tid = threadIdx.x + threadIdx.y * blockDim.x;
+    if ((tid & 63) == 0) sm[tid >> 6] = v;
+    __syncthreads();
+    double r = 0.0;
+    if (tid == 0) r = (sm[0] + sm[1]) + (sm[2] + sm[3]);
+    return r;
+}
+
+__device__ inline double block_max(double v, double *sm) {
+    v = wave_max(v);
+    const int tid = threadIdx.x + threadIdx.y * blockDim.x;
+    if ((tid & 63) == 0) sm[tid >> 6] = v;
+    __syncthreads();
+    double r = 0.0;
+    if (tid == 0) {
+        r = sm[0];
+        for (int q = 1; q < 4; ++q) r = (sm[q] > r) ? sm[q] : r;
+    }
+    return r;
+}
+
+__device__ inline int64_t part_index() {
+    return (int64_t)blockIdx.x + (int64_t)gridDim.x * ((int64_t)blockIdx.y + (int64_t)gridDim.y * blockIdx.z);
+}
+
+// XCD-aware tile order.  The command processor deals consecutive workgroups of
+// a launch round robin to the 8 XCDs (each with its own L2).  Launched as a
+// flat grid padded to a multiple of 8, workgroup id maps to a logical tile so
+// that every XCD walks one contiguous range of tiles, and tiles that share
+// halo rows and planes hit the same L2.  Returns false for the padding.
+constexpr int NXCD = 8;
+
+inline unsigned xcd_grid(int64_t tiles) { return (unsigned)((tiles + NXCD - 1) / NXCD * NXCD); }
+
+__device__ inline bool xcd_tile(int nbx, int nby, int nbz, int &bx, int &by, int &bz) {
+    const int64_t total = (int64_t)nbx * nby * nbz;
+    const int64_t padded = (int64_t)gridDim.x;
+    const int64_t per = padded / NXCD;
+    const int64_t id = blockIdx.x;
+    const int64_t logical = (id % NXCD) * per + id / NXCD;
+    if (logical >= total) return false;
+    bx = (int)(logical % nbx);
+    by = (int)((logical / nbx) % nby);
+    bz = (int)(logical / ((int64_t)nbx * nby));
+    return true;
+}
+
+}  // namespace dev
+}  // namespace ih

@@ -8,6 +8,7 @@
 
 #include "../../include/impact_hip.h"
 #include "ih_internal.h"
+#include "ih_krylov.h"
 
 namespace ih {
 
@@ -54,6 +55,11 @@ struct ih_case {
     int64_t part_cap = 0;
     double *dscal = nullptr;   // device scalars
     double *hscal = nullptr;   // pinned mirror
+    double *part2 = nullptr;   // second partial buffer (fused dots)
+    double *ksc = nullptr;     // BiCGstab scalars (device)
+    int *kfl = nullptr;        // BiCGstab flags (device)
+    double *hksc = nullptr;    // pinned mirrors
+    int *hkfl = nullptr;
     int64_t bytes = 0;
     ih::HostGrid hg{};
     ih::HostCoeffs hc{};

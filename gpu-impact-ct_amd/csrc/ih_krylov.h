@@ -1,0 +1,44 @@
+// Device-controlled BiCGstab: scalar and flag slots in device memory and the
+// launch entry points (krylov.hip).
+#pragma once
+
+#include "ih_internal.h"
+
+namespace ih {
+
+// double slots
+enum {
+    KS_RHR = 0,
+    KS_RHR_PREV,
+    KS_RHR_NEXT,
+    KS_ALPHA,
+    KS_BETA,
+    KS_OMEGA,
+    KS_OM2,
+    KS_NORM,
+    KS_NORM0,
+    KS_EPS,
+    KS_NSLOTS = 16
+};
+// int slots
+enum { KF_ACTIVE = 0, KF_COUNTER, KF_NMAX, KF_EXIT, KF_RESET, KF_MODE, KF_DONE, KF_NSLOTS = 8 };
+
+struct KrylovPtrs {
+    double *rr, *rh, *pk, *Ap, *Ar, *phi;
+    double *part1, *part2;
+    double *sc;
+    int *fl;
+};
+
+// dot modes of the operator kernel
+enum { KDOT_NONE = 0, KDOT_W = 1, KDOT_SELF = 2 };
+
+void launch_apply_A(hipStream_t s, const Geo &g, const Coef &c, const Box &pb, const double *src, double *dst,
+                    const double *w, int dot_mode, double *part1, double *part2, const int *active);
+void launch_krylov_init(hipStream_t s, const Geo &g, const Box &b, const double *bb, double *rr, double *rh,
+                        double *pmax, double *pdot, double *sc, int *fl, double eps, int nmax);
+void launch_krylov_iteration(hipStream_t s, const Geo &g, const Coef &c, const Box &b, const KrylovPtrs &k,
+                             int from_stage);
+void launch_krylov_resume(hipStream_t s, const Geo &g, const Box &b, const KrylovPtrs &k);
+
+}  // namespace ih

@@ -5,38 +5,13 @@
 // -ffp-contract=off, so each result is bit-identical to a serial evaluation of
 // the same loop.  Threads run along the unit-stride x index (64 per wavefront)
 // so every field access is a coalesced 512-byte row segment.
-#include "ih_internal.h"
+#include "ih_device.h"
 
 namespace ih {
 
 namespace {
 
-constexpr int BX = 64;
-constexpr int BY = 4;
-
-struct Range3 {
-    int lo0, lo1, lo2, n0, n1, n2;
-};
-
-inline Range3 make_range(const Box &b) {
-    return Range3{b.lo[0], b.lo[1], b.lo[2], b.hi[0] - b.lo[0] + 1, b.hi[1] - b.lo[1] + 1, b.hi[2] - b.lo[2] + 1};
-}
-
-inline dim3 grid_for(const Range3 &r) {
-    return dim3((unsigned)((r.n0 + BX - 1) / BX), (unsigned)((r.n1 + BY - 1) / BY), (unsigned)r.n2);
-}
-
-inline bool empty(const Range3 &r) { return r.n0 <= 0 || r.n1 <= 0 || r.n2 <= 0; }
-
-#define IH_POINT(r)                                                  \
-    const int i = (r).lo0 + (int)(blockIdx.x * BX + threadIdx.x);    \
-    const int j = (r).lo1 + (int)(blockIdx.y * BY + threadIdx.y);    \
-    const int k = (r).lo2 + (int)blockIdx.z;                         \
-    if (i >= (r).lo0 + (r).n0 || j >= (r).lo1 + (r).n1) return;
-
-__device__ inline int64_t sh(const Geo &g, int d) { return d == 0 ? 1 : (d == 1 ? (int64_t)g.px : g.pxy); }
-
-__device__ inline int pick(int d, int i, int j, int k) { return d == 0 ? i : (d == 1 ? j : k); }
+using namespace dev;
 
 // ---------------------------------------------------------------------------
 // ghost fill: exchange(dir, vel_dir, phi), src/mod_exchange.f90:47-294.
@@ -402,40 +377,69 @@ __device__ inline void jor_neighbours(const Geo &g, int i, int j, int k, int64_t
     }
 }
 
-__global__ void k_jor(Geo g, Coef c, Range3 r, double omega, const double *rel, const double *bb, double *out) {
-    IH_POINT(r);
-    const int64_t x = gidx(g, i, j, k);
-    int64_t xm1, xp1, xm2, xp2, xm3, xp3;
-    jor_neighbours<double>(g, i, j, k, x, xm1, xp1, xm2, xp2, xm3, xp3);
-    const double *g1 = c.dg[0], *g2 = c.dg[1], *g3 = c.dg[2];
+// One thread per (x, y) column walks JZ planes, keeping rel(k-1), rel(k),
+// rel(k+1) in registers, so each plane of rel is fetched once per tile column
+// instead of three times; tiles are dealt to the XCDs in contiguous ranges.
+constexpr int JZ = 16;
+
+template <typename T>
+__device__ inline void jor_column(const Geo &g, const Coef &c, const Range3 &r, int nbx, int nby, int nbz, T omega,
+                                  const T *__restrict__ rel, const T *__restrict__ bb, T *__restrict__ out,
+                                  const T *g1, const T *g2, const T *g3) {
+    int bx, by, bz;
+    if (!xcd_tile(nbx, nby, nbz, bx, by, bz)) return;
+    const int i = r.lo0 + bx * BX + (int)threadIdx.x;
+    const int j = r.lo1 + by * BY + (int)threadIdx.y;
+    if (i >= r.lo0 + r.n0 || j >= r.lo1 + r.n1) return;
+    const int k0 = r.lo2 + bz * JZ;
+    const int k1 = min(k0 + JZ - 1, r.lo2 + r.n2 - 1);
     const int c0 = g.cst[0], c1 = g.cst[1], c2 = g.cst[2];
-    double t = bb[x];
-    t = t - g1[0 * c0 + i] * rel[xm1];
-    t = t - g1[2 * c0 + i] * rel[xp1];
-    t = t - g2[0 * c1 + j] * rel[xm2];
-    t = t - g2[2 * c1 + j] * rel[xp2];
-    t = t - g3[0 * c2 + k] * rel[xm3];
-    t = t - g3[2 * c2 + k] * rel[xp3];
-    const double diag = (g1[1 * c0 + i] + g2[1 * c1 + j]) + g3[1 * c2 + k];
-    out[x] = omega * t / diag + (1.0 - omega) * rel[x];
+    // x and y neighbours: periodic columns read the opposite side of the block
+    int64_t dxm = -1, dxp = 1, dym = -(int64_t)g.px, dyp = g.px;
+    if (g.per[0]) {
+        if (i == g.Sp[0]) dxm = g.Np[0] - g.Sp[0];
+        if (i == g.Np[0]) dxp = -(g.Np[0] - g.Sp[0]);
+    }
+    if (g.per[1]) {
+        if (j == g.Sp[1]) dym = (int64_t)(g.Np[1] - g.Sp[1]) * g.px;
+        if (j == g.Np[1]) dyp = -(int64_t)(g.Np[1] - g.Sp[1]) * g.px;
+    }
+    const T a1m = g1[0 * c0 + i], a1p = g1[2 * c0 + i], a1d = g1[1 * c0 + i];
+    const T a2m = g2[0 * c1 + j], a2p = g2[2 * c1 + j], a2d = g2[1 * c1 + j];
+    auto zidx = [&](int k) -> int64_t {
+        if (g.per[2]) {
+            if (k < g.Sp[2]) k += g.Np[2] - g.Sp[2] + 1;
+            else if (k > g.Np[2]) k -= g.Np[2] - g.Sp[2] + 1;
+        }
+        return gidx(g, i, j, k);
+    };
+    T zm = rel[zidx(k0 - 1)];
+    T zc = rel[zidx(k0)];
+    for (int k = k0; k <= k1; ++k) {
+        const T zp = rel[zidx(k + 1)];
+        const int64_t x = gidx(g, i, j, k);
+        T t = bb[x];
+        t = t - a1m * rel[x + dxm];
+        t = t - a1p * rel[x + dxp];
+        t = t - a2m * rel[x + dym];
+        t = t - a2p * rel[x + dyp];
+        t = t - g3[0 * c2 + k] * zm;
+        t = t - g3[2 * c2 + k] * zp;
+        const T diag = (a1d + a2d) + g3[1 * c2 + k];
+        out[x] = omega * t / diag + (T(1) - omega) * zc;
+        zm = zc;
+        zc = zp;
+    }
 }
 
-__global__ void k_jor_f32(Geo g, Coef c, Range3 r, float omega, const float *rel, const float *bb, float *out) {
-    IH_POINT(r);
-    const int64_t x = gidx(g, i, j, k);
-    int64_t xm1, xp1, xm2, xp2, xm3, xp3;
-    jor_neighbours<float>(g, i, j, k, x, xm1, xp1, xm2, xp2, xm3, xp3);
-    const float *g1 = c.dgf[0], *g2 = c.dgf[1], *g3 = c.dgf[2];
-    const int c0 = g.cst[0], c1 = g.cst[1], c2 = g.cst[2];
-    float t = bb[x];
-    t = t - g1[0 * c0 + i] * rel[xm1];
-    t = t - g1[2 * c0 + i] * rel[xp1];
-    t = t - g2[0 * c1 + j] * rel[xm2];
-    t = t - g2[2 * c1 + j] * rel[xp2];
-    t = t - g3[0 * c2 + k] * rel[xm3];
-    t = t - g3[2 * c2 + k] * rel[xp3];
-    const float diag = (g1[1 * c0 + i] + g2[1 * c1 + j]) + g3[1 * c2 + k];
-    out[x] = omega * t / diag + (1.0f - omega) * rel[x];
+__global__ __launch_bounds__(256) void k_jor(Geo g, Coef c, Range3 r, int nbx, int nby, int nbz, double omega,
+                                             const double *rel, const double *bb, double *out) {
+    jor_column<double>(g, c, r, nbx, nby, nbz, omega, rel, bb, out, c.dg[0], c.dg[1], c.dg[2]);
+}
+
+__global__ __launch_bounds__(256) void k_jor_f32(Geo g, Coef c, Range3 r, int nbx, int nby, int nbz, float omega,
+                                                 const float *rel, const float *bb, float *out) {
+    jor_column<float>(g, c, r, nbx, nby, nbz, omega, rel, bb, out, c.dgf[0], c.dgf[1], c.dgf[2]);
 }
 
 // ---------------------------------------------------------------------------
@@ -444,44 +448,6 @@ __global__ void k_jor_f32(Geo g, Coef c, Range3 r, float omega, const float *rel
 // four wave sums as (w0+w1)+(w2+w3).  The result does not depend on the
 // launch or on scheduling, so repeated solves are bit-reproducible.
 // ---------------------------------------------------------------------------
-__device__ inline double wave_sum(double v) {
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) v = v + __shfl_xor(v, off, 64);
-    return v;
-}
-
-__device__ inline double wave_max(double v) {
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
-        const double o = __shfl_xor(v, off, 64);
-        v = (o > v) ? o : v;
-    }
-    return v;
-}
-
-__device__ inline double block_sum(double v, double *sm) {
-    v = wave_sum(v);
-    const int tid = threadIdx.x + threadIdx.y * blockDim.x;
-    if ((tid & 63) == 0) sm[tid >> 6] = v;
-    __syncthreads();
-    double r = 0.0;
-    if (tid == 0) r = (sm[0] + sm[1]) + (sm[2] + sm[3]);
-    return r;
-}
-
-__device__ inline double block_max(double v, double *sm) {
-    v = wave_max(v);
-    const int tid = threadIdx.x + threadIdx.y * blockDim.x;
-    if ((tid & 63) == 0) sm[tid >> 6] = v;
-    __syncthreads();
-    double r = 0.0;
-    if (tid == 0) {
-        r = sm[0];
-        for (int q = 1; q < 4; ++q) r = (sm[q] > r) ? sm[q] : r;
-    }
-    return r;
-}
-
 template <typename F>
 __device__ inline double tile_sum(const Geo &g, Range3 r, F f) {
     const int i = r.lo0 + (int)(blockIdx.x * RED_TX + threadIdx.x);
@@ -494,10 +460,6 @@ __device__ inline double tile_sum(const Geo &g, Range3 r, F f) {
         }
     }
     return acc;
-}
-
-__device__ inline int64_t part_index() {
-    return (int64_t)blockIdx.x + (int64_t)gridDim.x * ((int64_t)blockIdx.y + (int64_t)gridDim.y * blockIdx.z);
 }
 
 __global__ __launch_bounds__(256) void k_dot_partial(Geo g, Range3 r, const double *a, const double *b,
@@ -716,6 +678,7 @@ __global__ void k_apply_mask(Geo g, Range3 r, int q, const uint8_t *mask, double
 // ===========================================================================
 // launchers
 // ===========================================================================
+using namespace dev;
 static inline Range3 R(const Box &b) { return make_range(b); }
 
 void launch_exchange(hipStream_t s, const Geo &g, int dir, int nf, double *const *f, const int *vel_dir) {
@@ -864,25 +827,24 @@ void launch_jor(hipStream_t s, const Geo &g, const Coef &c, const Box &b, double
                 const double *bb, double *out) {
     Range3 r = R(b);
     if (empty(r)) return;
-    hipLaunchKernelGGL(k_jor, grid_for(r), dim3(BX, BY, 1), 0, s, g, c, r, omega, rel, bb, out);
+    const int nbx = (r.n0 + BX - 1) / BX, nby = (r.n1 + BY - 1) / BY, nbz = (r.n2 + JZ - 1) / JZ;
+    hipLaunchKernelGGL(k_jor, dim3(xcd_grid((int64_t)nbx * nby * nbz)), dim3(BX, BY, 1), 0, s, g, c, r, nbx, nby, nbz,
+                       omega, rel, bb, out);
 }
 
 void launch_jor_f32(hipStream_t s, const Geo &g, const Coef &c, const Box &b, float omega, const float *rel,
                     const float *bb, float *out) {
     Range3 r = R(b);
     if (empty(r)) return;
-    hipLaunchKernelGGL(k_jor_f32, grid_for(r), dim3(BX, BY, 1), 0, s, g, c, r, omega, rel, bb, out);
+    const int nbx = (r.n0 + BX - 1) / BX, nby = (r.n1 + BY - 1) / BY, nbz = (r.n2 + JZ - 1) / JZ;
+    hipLaunchKernelGGL(k_jor_f32, dim3(xcd_grid((int64_t)nbx * nby * nbz)), dim3(BX, BY, 1), 0, s, g, c, r, nbx, nby,
+                       nbz, omega, rel, bb, out);
 }
 
 int64_t red_blocks(const Box &b) {
     const int64_t nI = b.hi[0] - b.lo[0] + 1, nJ = b.hi[1] - b.lo[1] + 1, nK = b.hi[2] - b.lo[2] + 1;
     if (nI <= 0 || nJ <= 0 || nK <= 0) return 0;
     return ((nI + RED_TX - 1) / RED_TX) * ((nJ + RED_TY - 1) / RED_TY) * ((nK + RED_KC - 1) / RED_KC);
-}
-
-static dim3 red_grid(const Range3 &r) {
-    return dim3((unsigned)((r.n0 + RED_TX - 1) / RED_TX), (unsigned)((r.n1 + RED_TY - 1) / RED_TY),
-                (unsigned)((r.n2 + RED_KC - 1) / RED_KC));
 }
 
 void launch_residual_cdg(hipStream_t s, const Geo &g, const Coef &c, const Box &b, const double *rel,

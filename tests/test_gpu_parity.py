@@ -18,6 +18,11 @@ GRIDS = [
     pytest.param(((17, 13, 11), (1, 1, 1, 1, 1, 1), (2.0, 1.0, 1.0), (0.0, 1.5, 1.5)), id="walls-stretched"),
     pytest.param(((16, 12, 10), (-1, -1, 1, 1, -1, -1), (1.0, 1.0, 1.0), (0.0, 1.2, 0.0)), id="periodic-xz"),
 ]
+# several operator tiles in x and y and two z chunks of the fused kernels
+MULTI = [
+    pytest.param(((70, 9, 40), (1, 1, -1, -1, 1, 1), (2.0, 1.0, 1.0), (0.0, 0.0, 1.5)), id="multi-walls-x"),
+    pytest.param(((67, 10, 37), (-1, -1, 1, 1, -1, -1), (1.0, 1.0, 1.0), (0.0, 1.3, 0.0)), id="multi-periodic-xz"),
+]
 
 
 def pair(spec):
@@ -172,7 +177,7 @@ def test_nonlinear_and_helmholtz_against_oracle(spec):
         same(gc.download(ih.NL1 + q), nl[q], f"nl{q + 1}")
 
 
-@pytest.mark.parametrize("spec", GRIDS)
+@pytest.mark.parametrize("spec", GRIDS + MULTI)
 def test_product_div_grad(spec):
     oc, gc = pair(spec)
     p = rnd(oc.shape, 60)
@@ -184,7 +189,7 @@ def test_product_div_grad(spec):
     same(gc.download(ih.TMP2), lap, "div grad")
 
 
-@pytest.mark.parametrize("spec", GRIDS)
+@pytest.mark.parametrize("spec", GRIDS + MULTI)
 @pytest.mark.parametrize("sweeps", [1, 2, 7])
 def test_jor_and_residual(spec, sweeps):
     oc, gc = pair(spec)
@@ -201,7 +206,7 @@ def test_jor_and_residual(spec, sweeps):
     assert r_o == r_g
 
 
-@pytest.mark.parametrize("spec", GRIDS)
+@pytest.mark.parametrize("spec", GRIDS + MULTI)
 def test_dot_fixed_tree(spec):
     oc, gc = pair(spec)
     a = rnd(oc.shape, 80)
@@ -212,7 +217,7 @@ def test_dot_fixed_tree(spec):
     assert oc.dot(oc.Sp, oc.Np, a, b) == gc.dot(oc.Sp, oc.Np, ih.TMP1, ih.TMP2)
 
 
-@pytest.mark.parametrize("spec", GRIDS)
+@pytest.mark.parametrize("spec", GRIDS + MULTI)
 def test_bicgstab(spec):
     oc, gc = pair(spec)
     xt = rnd(oc.shape, 90)

@@ -87,7 +87,7 @@ def setup_case(ih, jor_fp32=False):
 
 def cpu_baseline(iters_per_step):
     """Oracle (single-threaded C restatement) on a bounded sample at the full size:
-    3 JOR sweeps, 2 BiCGstab iterations and one step with the Poisson solve cut to
+    20 JOR sweeps, 10 BiCGstab iterations and one step with the Poisson solve cut to
     one iteration; the per-step time is composed with the GPU run's iteration counts."""
     sys.path.insert(0, str(ROOT / "tests"))
     import ctypes
@@ -107,11 +107,11 @@ def cpu_baseline(iters_per_step):
     bb = np.ascontiguousarray(rng.standard_normal(oc.shape))
     rel = oc.zeros()
     t0 = time.perf_counter()
-    oc.jor(3, JOR_OMEGA, oc.Sp, oc.Np, rel, bb, oc.zeros())
-    t_jor = (time.perf_counter() - t0) / 3
+    oc.jor(20, JOR_OMEGA, oc.Sp, oc.Np, rel, bb, oc.zeros())
+    t_jor = (time.perf_counter() - t0) / 20
     log(f"cpu: {t_jor * 1e3:.1f} ms per JOR sweep")
     t0 = time.perf_counter()
-    st = oc.bicgstab(0.0, 2, oc.Sp, oc.Np, bb, rel)
+    st = oc.bicgstab(0.0, 10, oc.Sp, oc.Np, bb, rel)
     t_bi = (time.perf_counter() - t0) / max(st.iterations, 1)
     log(f"cpu: {t_bi * 1e3:.1f} ms per BiCGstab iteration (incl. start-up)")
     po = O.StepParams(Re=RE, dtime=DT, jor_sweeps=0, jor_omega=JOR_OMEGA, bicg_rel_tol=TOL, bicg_max_it=1,
@@ -122,17 +122,43 @@ def cpu_baseline(iters_per_step):
     t_rest = time.perf_counter() - t0
     log(f"cpu: {t_rest:.2f} s for a step with one BiCGstab iteration per substep")
     step = t_rest + 3 * JOR_SWEEPS * t_jor + max(iters_per_step - 3, 0) * t_bi
-    sample = t_rest + 3 * t_jor + 2 * t_bi
+    sample = t_rest + 20 * t_jor + 10 * t_bi
     return {
         "value": 1.0 / step,
         "unit": "time steps/s",
         "cores": 1,
         "kind": "port",
-        "sample": (f"oracle at 257x129x129 on 1 core: 3 JOR sweeps ({t_jor * 1e3:.0f} ms each), 2 BiCGstab "
+        "sample": (f"oracle at 257x129x129 on 1 core: 20 JOR sweeps ({t_jor * 1e3:.0f} ms each), 10 BiCGstab "
                    f"iterations ({t_bi:.2f} s each), one step with 1 BiCGstab iteration per substep "
                    f"({t_rest:.1f} s); step = that + 600 sweeps + {iters_per_step - 3} iterations "
                    f"(the GPU run's count); {sample:.0f} s of CPU work"),
     }
+
+
+def kernel_only(ih, which, launches):
+    """Run `launches` launches of one kernel family on the bench grid and one
+    device-to-device copy of a whole field (fe doubles read and written), so
+    that per-dispatch hardware counters of the kernel can be scaled by those of
+    a copy of known size."""
+    c = ih.Case(N, BC, LEN, STRETCH, device=0)
+    rng = np.random.default_rng(1)
+    for f in (ih.PRE, ih.RES, ih.TMP1):
+        c.upload(f, np.ascontiguousarray(rng.standard_normal(c.shape)))
+    c.sync()
+    c.timer_start()
+    if which == "jor":
+        c.jor(launches, JOR_OMEGA, c.Sp, c.Np)
+    else:
+        for _ in range(launches):
+            c.product_div_grad(ih.TMP1, ih.TMP2)
+    ms = c.timer_stop()
+    check = c.lib.ih_field_copy(c.h, ih.TMP3, ih.TMP1)
+    ih.check(check, "field copy")
+    c.sync()
+    fe = int(np.prod(c.shape))
+    print(json.dumps({"kernel_only": which, "launches": launches, "ms_per_launch": ms / launches,
+                      "points": N[0] * N[1] * N[2], "copy_bytes_each_way": 8 * fe}), flush=True)
+    c.close()
 
 
 def main():
@@ -142,6 +168,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--mixed", action="store_true", help="FP32 JOR sweeps with FP64 residual and update")
+    ap.add_argument("--only", choices=("jor", "operator"), default=None,
+                    help="launch one kernel family on the bench grid (for counter profiles): JOR sweeps or the "
+                         "fused Poisson operator, followed by one plain device copy of a field for calibration")
+    ap.add_argument("--launches", type=int, default=20)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs one GPU; --gpus {args.gpus} is not supported "
@@ -151,6 +181,8 @@ def main():
 
     import impact_hip as ih
 
+    if args.only:
+        return kernel_only(ih, args.only, args.launches)
     log(f"bench: case {N}, walls, tanh {STRETCH[1]}, stenosed tube, Re {RE}, dt {DT}")
     c, p = setup_case(ih, jor_fp32=args.mixed)
     log(f"bench: {c.bytes() / 2**30:.2f} GiB resident")

@@ -28,6 +28,12 @@ inline dim3 red_grid(const Range3 &r) {
                 (unsigned)((r.n2 + RED_KC - 1) / RED_KC));
 }
 
+// Planes kc0..kc1 of z chunk bz out of nbz over the range r (balanced chunks).
+__device__ inline void red_chunk(const Range3 &r, int bz, int nbz, int &kc0, int &kc1) {
+    kc0 = r.lo2 + (int)(((int64_t)bz * r.n2) / nbz);
+    kc1 = r.lo2 + (int)(((int64_t)(bz + 1) * r.n2) / nbz) - 1;
+}
+
 #define IH_POINT(r)                                                  \
     const int i = (r).lo0 + (int)(blockIdx.x * BX + threadIdx.x);    \
     const int j = (r).lo1 + (int)(blockIdx.y * BY + threadIdx.y);    \
@@ -39,7 +45,8 @@ __device__ inline int64_t sh(const Geo &g, int d) { return d == 0 ? 1 : (d == 1 
 __device__ inline int pick(int d, int i, int j, int k) { return d == 0 ? i : (d == 1 ? j : k); }
 
 // Reduction tree: xor butterfly within the 64-lane wavefront (offsets 32..1),
-// then the four wave sums of a 256-thread block as (w0+w1)+(w2+w3).
+// then the four wave sums of a 256-thread block as (w0+w1)+(w2+w3); lanes are
+// numbered threadIdx.x + blockDim.x * threadIdx.y.
 __device__ inline double wave_sum(double v) {
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) v = v + __shfl_xor(v, off, 64);

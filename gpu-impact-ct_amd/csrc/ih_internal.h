@@ -47,10 +47,13 @@ struct Coef {
     const double *xu[3];
 };
 
-// Reduction tree constants, shared with the oracle's restatement.
-constexpr int RED_TX = 64;
-constexpr int RED_TY = 4;
-constexpr int RED_KC = 8;
+// Reduction tree constants, shared with the oracle's restatement: blocks of
+// RED_TX x RED_TY threads over (i, j); along k the range is cut into
+// ceil(n / RED_KC) chunks of balanced length (chunk b holds the planes
+// [lo + b*n/nb, lo + (b+1)*n/nb) in integer arithmetic).
+constexpr int RED_TX = 16;
+constexpr int RED_TY = 16;
+constexpr int RED_KC = 32;
 constexpr int RED_THREADS = 256;
 
 // Launch-side helpers implemented in kernels.hip ------------------------------
@@ -105,12 +108,6 @@ void launch_absdiff_partial(hipStream_t s, const Geo &g, const Box &b, const dou
 void launch_final_max(hipStream_t s, const double *part, int64_t nb, double *out);
 
 // BiCGstab vector updates
-void launch_bicg_init(hipStream_t s, const Geo &g, const Box &b, const double *bb, double *rr, double *rh);
-void launch_bicg_p(hipStream_t s, const Geo &g, const Box &b, const double *rr, double *pk, const double *Ap,
-                   const double *scal /* beta at [0], omega2 at [1] on device */);
-void launch_bicg_s(hipStream_t s, const Geo &g, const Box &b, double *rr, const double *Ap, const double *alpha);
-void launch_bicg_x(hipStream_t s, const Geo &g, const Box &b, double *phi, double *rr, const double *pk,
-                   const double *Ar, const double *scal /* omega [0], alpha [1] */, double *part_max);
 void launch_copy_field_box(hipStream_t s, const Geo &g, const Box &b, const double *src, double *dst);
 
 void launch_fill_corners(hipStream_t s, const Geo &g, double *f);

@@ -443,10 +443,11 @@ __global__ __launch_bounds__(256) void k_jor_f32(Geo g, Coef c, Range3 r, int nb
 }
 
 // ---------------------------------------------------------------------------
-// Reductions.  Block tile 64 x 4 threads, each thread adds RED_KC planes in
-// order; the wavefront combines by an xor butterfly (offsets 32..1) and the
-// four wave sums as (w0+w1)+(w2+w3).  The result does not depend on the
-// launch or on scheduling, so repeated solves are bit-reproducible.
+// Reductions.  Block tile RED_TX x RED_TY threads, each thread adds the planes
+// of its z chunk in order; the wavefront combines by an xor butterfly (offsets
+// 32..1) and the four wave sums as (w0+w1)+(w2+w3).  The result does not
+// depend on the launch or on scheduling, so repeated solves are
+// bit-reproducible.
 // ---------------------------------------------------------------------------
 template <typename F>
 __device__ inline double tile_sum(const Geo &g, Range3 r, F f) {
@@ -454,10 +455,9 @@ __device__ inline double tile_sum(const Geo &g, Range3 r, F f) {
     const int j = r.lo1 + (int)(blockIdx.y * RED_TY + threadIdx.y);
     double acc = 0.0;
     if (i < r.lo0 + r.n0 && j < r.lo1 + r.n1) {
-        for (int t = 0; t < RED_KC; ++t) {
-            const int k = r.lo2 + (int)blockIdx.z * RED_KC + t;
-            if (k < r.lo2 + r.n2) acc = acc + f(gidx(g, i, j, k));
-        }
+        int kc0, kc1;
+        red_chunk(r, (int)blockIdx.z, (int)gridDim.z, kc0, kc1);
+        for (int k = kc0; k <= kc1; ++k) acc = acc + f(gidx(g, i, j, k));
     }
     return acc;
 }
@@ -491,12 +491,11 @@ __global__ __launch_bounds__(256) void k_max_partial(Geo g, Range3 r, const doub
     const int j = r.lo1 + (int)(blockIdx.y * RED_TY + threadIdx.y);
     double m = 0.0;
     if (i < r.lo0 + r.n0 && j < r.lo1 + r.n1) {
-        for (int t = 0; t < RED_KC; ++t) {
-            const int k = r.lo2 + (int)blockIdx.z * RED_KC + t;
-            if (k < r.lo2 + r.n2) {
-                const double v = fabs(a[gidx(g, i, j, k)]);
-                m = (v > m) ? v : m;
-            }
+        int kc0, kc1;
+        red_chunk(r, (int)blockIdx.z, (int)gridDim.z, kc0, kc1);
+        for (int k = kc0; k <= kc1; ++k) {
+            const double v = fabs(a[gidx(g, i, j, k)]);
+            m = (v > m) ? v : m;
         }
     }
     const double s = block_max(m, sm);
@@ -531,9 +530,9 @@ __global__ __launch_bounds__(256) void k_residual_cdg(Geo g, Coef c, Range3 r, c
     if (i < r.lo0 + r.n0 && j < r.lo1 + r.n1) {
         const double *g1 = c.dg[0], *g2 = c.dg[1], *g3 = c.dg[2];
         const int c0 = g.cst[0], c1 = g.cst[1], c2 = g.cst[2];
-        for (int tt = 0; tt < RED_KC; ++tt) {
-            const int k = r.lo2 + (int)blockIdx.z * RED_KC + tt;
-            if (k >= r.lo2 + r.n2) break;
+        int kc0, kc1;
+        red_chunk(r, (int)blockIdx.z, (int)gridDim.z, kc0, kc1);
+        for (int k = kc0; k <= kc1; ++k) {
             const int64_t x = gidx(g, i, j, k);
             int64_t xm1, xp1, xm2, xp2, xm3, xp3;
             jor_neighbours<double>(g, i, j, k, x, xm1, xp1, xm2, xp2, xm3, xp3);
@@ -554,52 +553,6 @@ __global__ __launch_bounds__(256) void k_residual_cdg(Geo g, Coef c, Range3 r, c
     if (threadIdx.x == 0 && threadIdx.y == 0) part[part_index()] = s;
 }
 
-// ---------------------------------------------------------------------------
-// BiCGstab vector updates (src/mod_solvers.f90:1576-1930)
-// ---------------------------------------------------------------------------
-__global__ void k_bicg_init(Geo g, Range3 r, const double *bb, double *rr, double *rh) {
-    IH_POINT(r);
-    const int64_t x = gidx(g, i, j, k);
-    const double v = bb[x] - rr[x];
-    rr[x] = v;
-    rh[x] = v;
-}
-
-__global__ void k_bicg_p(Geo g, Range3 r, const double *rr, double *pk, const double *Ap, const double *scal) {
-    IH_POINT(r);
-    const int64_t x = gidx(g, i, j, k);
-    const double beta = scal[0], om = scal[1];
-    pk[x] = rr[x] + beta * pk[x] + om * Ap[x];
-}
-
-__global__ void k_bicg_s(Geo g, Range3 r, double *rr, const double *Ap, const double *alpha) {
-    IH_POINT(r);
-    const int64_t x = gidx(g, i, j, k);
-    rr[x] = rr[x] - (*alpha) * Ap[x];
-}
-
-__global__ __launch_bounds__(256) void k_bicg_x(Geo g, Range3 r, double *phi, double *rr, const double *pk,
-                                                const double *Ar, const double *scal, double *part) {
-    __shared__ double sm[4];
-    const double om = scal[0], alpha = scal[1];
-    const int i = r.lo0 + (int)(blockIdx.x * RED_TX + threadIdx.x);
-    const int j = r.lo1 + (int)(blockIdx.y * RED_TY + threadIdx.y);
-    double m = 0.0;
-    if (i < r.lo0 + r.n0 && j < r.lo1 + r.n1) {
-        for (int t = 0; t < RED_KC; ++t) {
-            const int k = r.lo2 + (int)blockIdx.z * RED_KC + t;
-            if (k >= r.lo2 + r.n2) break;
-            const int64_t x = gidx(g, i, j, k);
-            phi[x] = phi[x] + om * rr[x] + alpha * pk[x];
-            const double v = rr[x] - om * Ar[x];
-            rr[x] = v;
-            const double a = fabs(v);
-            m = (a > m) ? a : m;
-        }
-    }
-    const double s = block_max(m, sm);
-    if (threadIdx.x == 0 && threadIdx.y == 0) part[part_index()] = s;
-}
 
 __global__ void k_copy_box(Geo g, Range3 r, const double *src, double *dst) {
     IH_POINT(r);
@@ -888,31 +841,6 @@ void launch_final_max(hipStream_t s, const double *part, int64_t nb, double *out
     hipLaunchKernelGGL(k_final_max, dim3(1), dim3(RED_TX, RED_TY, 1), 0, s, part, nb, out);
 }
 
-void launch_bicg_init(hipStream_t s, const Geo &g, const Box &b, const double *bb, double *rr, double *rh) {
-    Range3 r = R(b);
-    if (empty(r)) return;
-    hipLaunchKernelGGL(k_bicg_init, grid_for(r), dim3(BX, BY, 1), 0, s, g, r, bb, rr, rh);
-}
-
-void launch_bicg_p(hipStream_t s, const Geo &g, const Box &b, const double *rr, double *pk, const double *Ap,
-                   const double *scal) {
-    Range3 r = R(b);
-    if (empty(r)) return;
-    hipLaunchKernelGGL(k_bicg_p, grid_for(r), dim3(BX, BY, 1), 0, s, g, r, rr, pk, Ap, scal);
-}
-
-void launch_bicg_s(hipStream_t s, const Geo &g, const Box &b, double *rr, const double *Ap, const double *alpha) {
-    Range3 r = R(b);
-    if (empty(r)) return;
-    hipLaunchKernelGGL(k_bicg_s, grid_for(r), dim3(BX, BY, 1), 0, s, g, r, rr, Ap, alpha);
-}
-
-void launch_bicg_x(hipStream_t s, const Geo &g, const Box &b, double *phi, double *rr, const double *pk,
-                   const double *Ar, const double *scal, double *part_max) {
-    Range3 r = R(b);
-    if (empty(r)) return;
-    hipLaunchKernelGGL(k_bicg_x, red_grid(r), dim3(RED_TX, RED_TY, 1), 0, s, g, r, phi, rr, pk, Ar, scal, part_max);
-}
 
 void launch_copy_box(hipStream_t s, const Geo &g, const Box &b, const double *src, double *dst) {
     Range3 r = R(b);

@@ -1,16 +1,17 @@
 // Fused Poisson operator and device-controlled BiCGstab (gfx950).
 //
-// k_apply_A evaluates product_div_grad (src/mod_laplace.f90:38-197): for each
+// k_apply evaluates product_div_grad (src/mod_laplace.f90:38-197): for each
 // direction m the gradient G_m p (src/mod_diff.f90:309-414, zero on the wall
 // planes of its component, exchanged with the vel_dir = m rules) followed by
 // the divergence D_m (src/mod_diff.f90:52-118), accumulated in the reference's
-// order.  Instead of writing the three gradient fields to HBM it keeps them in
-// LDS: a 64 x 4 column tile marches in z; per plane the pressure cross
-// (x and y halos of 5) is staged in LDS, G_1 and G_2 are formed there once per
-// position, and G_3 lives in a per-thread register queue along z.  Every
-// gradient value is computed with the reference's operation order, so Ap is
-// bit-identical to the three-pass evaluation, while HBM traffic drops to one
-// read of p and one write of Ap.
+// order.  Instead of writing the three gradient fields to HBM it keeps them on
+// chip: a 16 x 16 column tile marches through a z chunk of up to 32 planes;
+// per plane the pressure cross (the tile plus x and y halos of 5) is staged in
+// LDS, every G_1 and G_2 value of the tile and its halo is formed once, and G_3
+// lives in a per-thread register queue along z.  Every gradient value is
+// computed with the reference's operation order, so Ap is bit-identical to the
+// three-pass evaluation, while HBM traffic drops to one read of p and one
+// write of Ap.
 //
 // The BiCGstab recurrences (src/mod_solvers.f90:1509-1935, preconditioner 0)
 // run without host round trips: one-block control kernels reduce the partial
@@ -26,10 +27,6 @@ using namespace dev;
 
 namespace {
 
-constexpr int AX = 64, AY = 4, AZ = 8;   // operator tile (x, y) and z chunk (= the reduction chunk)
-constexpr int HX = AX + 10, HY = AY + 10;  // pressure halo of 5 each side
-constexpr int G1W = AX + 5;               // G_1 at x0-3 .. x0+AX+1
-constexpr int G2H = AY + 5;               // G_2 at y0-3 .. y0+AY+1
 
 __device__ inline int wrap_p(const Geo &g, int d, int idx) {
     const int P = g.Np[d] - g.Sp[d] + 1;
@@ -60,172 +57,267 @@ __device__ inline bool g_src(const Geo &g, int d, int &idx) {
 
 __device__ inline bool wall_plane(const Geo &g, int e, int idx) { return !g.per[e] && (idx == 1 || idx == g.N[e]); }
 
-__device__ inline double grad6(const double *cg, int cst, int ci, const double *p, int stride) {
-    // G p = c(-2)*p(-2) + c(-1)*p(-1) + ... + c(3)*p(3), p[0] is offset -2
-    double v = cg[0 * cst + ci] * p[0];
-#pragma unroll
-    for (int s = 1; s < 6; ++s) v = v + cg[s * cst + ci] * p[s * stride];
-    return v;
-}
-
-// dot modes of k_apply_A
+// dot modes of k_apply
 enum { DOT_NONE = 0, DOT_W = 1, DOT_SELF = 2 };
 
-// One thread per (x, y) column of a 64 x 4 tile, AZ planes.  Per plane the
-// thread reads p(x-5..x+5, y) and p(x, y-5..y+5) into registers and forms its
-// six G_1 and six G_2 values itself (a value shared by neighbouring threads is
-// evaluated identically by each), G_3 comes from the z register queue.  The
-// wrap/zero rules and coefficients of the tile's x and y positions are resolved
-// once per block into LDS; there is no barrier inside the plane loop.
-__global__ __launch_bounds__(256) void k_apply_A(Geo g, Coef c, Range3 r, int nbx, int nby, int nbz,
-                                                 const double *__restrict__ src, double *__restrict__ dst,
-                                                 const double *__restrict__ w, int dot_mode, double *part1,
-                                                 double *part2, const int *active) {
+// One thread per (x, y) column of a TX x TY tile, the planes of one z chunk
+// (balanced chunks of at most AZ planes, the reduction chunks).  Pressure cross
+// and gradient tiles are double buffered, and the divergence of plane k-1 is
+// formed after the gradients of plane k, so each plane costs one barrier.  The
+// halo of plane k+2, p(k+7) of the column queue and w(k+1) are loaded while
+// planes k and k+1 are computed (the plane loop is unrolled twice so that the
+// two register sets alternate).  The gradient coefficients of a thread's
+// positions stay in registers for the whole chunk; every thread forms one G_1,
+// one G_2 and at most one halo gradient per plane without divergent branches.
+// dot_mode DOT_W accumulates <Ap, w>, DOT_SELF <p, Ap> and <Ap, Ap>, per
+// thread in plane order, then over the block with the fixed tree.
+template <int TX, int TY, int AZ>
+__global__ __launch_bounds__(TX *TY) void k_apply(
+    Geo g, Coef c, Range3 r, int nbx, int nby, int nbz, const double *__restrict__ src, double *__restrict__ dst,
+    const double *__restrict__ w, int dot_mode, double *part1, double *part2, const int *active) {
+    constexpr int NT = TX * TY;
+    constexpr int PW = TX + 10, PH = TY + 10, PS = PW * PH;
+    constexpr int NHX = 10 * TY;
+    constexpr int NH = NHX + 10 * TX;
+    constexpr int HL = (NH + NT - 1) / NT;
+    constexpr int NG1X = 5 * TY, NG2X = 5 * TX;
+    static_assert(NG1X + NG2X <= NT, "one extra gradient job per thread at most");
+    constexpr int E1 = NT - NG1X;
+    constexpr int G1S = TX + 5;                 // row stride of the G_1 tile
+    constexpr int OG2 = TY * G1S;               // offset of the G_2 tile
+    constexpr int GS = OG2 + (TY + 5) * TX + 1; // + one dummy slot
+    constexpr int ZW = AZ + 7;                  // G_3 positions kc0-4 .. kc0+AZ+2
     if (active && !*active) return;
     int bx, by, bz;
     if (!xcd_tile(nbx, nby, nbz, bx, by, bz)) return;
-    __shared__ double sG1[6][G1W];  // cG_1 at the tile's G_1 positions, 0 where G_1 is zero
-    __shared__ double sD1[6][AX];
-    __shared__ double sG2[6][G2H];
-    __shared__ double sD2[6][AY];
-    __shared__ int sPx[HX];         // source x of p at x0-5+u, -1 = zero
-    __shared__ int sPy[HY];
-    __shared__ int sV1[G1W];        // 1 where G_1 (G_2) at the position is a computed value
-    __shared__ int sV2[G2H];
+    __shared__ double sP[2][PS];
+    __shared__ double sG[2][GS];
+    __shared__ double sCG3[6][ZW];
+    __shared__ double sCD3[6][AZ];
+    __shared__ int sVZ[ZW];
     __shared__ double sm[8];
 
     const int tx = threadIdx.x, ty = threadIdx.y;
-    const int tid = tx + AX * ty;
-    const int x0 = r.lo0 + bx * AX;
-    const int y0 = r.lo1 + by * AY;
+    const int tid = tx + TX * ty;
+    const int x0 = r.lo0 + bx * TX, y0 = r.lo1 + by * TY;
     const int x = x0 + tx, y = y0 + ty;
-    const int xe = r.lo0 + r.n0 - 1, ye = r.lo1 + r.n1 - 1, ze = r.lo2 + r.n2 - 1;
+    const int xe = r.lo0 + r.n0 - 1, ye = r.lo1 + r.n1 - 1;
     const bool own = (x <= xe) && (y <= ye);
-    const int kc0 = r.lo2 + bz * AZ;
-    const int kc1 = min(kc0 + AZ - 1, ze);
+    const int kc0 = r.lo2 + (int)(((int64_t)bz * r.n2) / nbz);
+    const int kc1 = r.lo2 + (int)(((int64_t)(bz + 1) * r.n2) / nbz) - 1;
     const int c0 = g.cst[0], c1 = g.cst[1], c2 = g.cst[2];
-    const double *cg3 = c.G[2], *cd3 = c.D[2];
+    const int64_t pxy = g.pxy;
 
-    // ---- per-block tables (positions past the box halo are never read) ----
-    for (int e = tid; e < 6 * G1W; e += 256) {
-        const int s = e / G1W, u = e % G1W;
-        int xs = x0 - 3 + u;
-        const bool v = xs <= xe + 2 && g_src(g, 0, xs);
-        sG1[s][u] = v ? c.G[0][s * c0 + xs] : 0.0;
-        if (s == 0) sV1[u] = v ? 1 : 0;
+    for (int e = tid; e < 6 * ZW; e += NT) {
+        const int s = e / ZW, u = e % ZW;
+        int kk = kc0 - 4 + u;
+        const bool v = kk <= kc1 + 2 && g_src(g, 2, kk);
+        sCG3[s][u] = v ? c.G[2][s * c2 + kk] : 0.0;
+        if (s == 0) sVZ[u] = v ? 1 : 0;
     }
-    for (int e = tid; e < 6 * G2H; e += 256) {
-        const int s = e / G2H, u = e % G2H;
-        int ys = y0 - 3 + u;
-        const bool v = ys <= ye + 2 && g_src(g, 1, ys);
-        sG2[s][u] = v ? c.G[1][s * c1 + ys] : 0.0;
-        if (s == 0) sV2[u] = v ? 1 : 0;
+    for (int e = tid; e < 6 * AZ; e += NT) {
+        const int t = e / AZ, u = e % AZ;
+        sCD3[t][u] = (kc0 + u <= kc1) ? c.D[2][t * c2 + kc0 + u] : 0.0;
     }
-    for (int e = tid; e < 6 * AX; e += 256) {
-        const int t = e / AX, u = e % AX;
-        sD1[t][u] = (x0 + u <= xe) ? c.D[0][t * c0 + x0 + u] : 0.0;
+
+    const int64_t hbase = gidx(g, -3, -3, 0);  // a valid element; offsets below are >= 0 from it
+    int xs = x, ys = y;
+    const bool vcol = x <= xe + 5 && y <= ye + 5 && p_src(g, 0, xs) && p_src(g, 1, ys);
+    const int64_t colb = vcol ? gidx(g, xs, ys, 0) : hbase;
+    const int64_t selfb = own ? gidx(g, x, y, 0) : hbase;
+
+    int hoff[HL], hslot[HL];
+#pragma unroll
+    for (int h = 0; h < HL; ++h) {
+        const int e = tid + h * NT;
+        hoff[h] = -1;
+        hslot[h] = PS;  // past the tile: not written
+        if (e < NHX) {
+            const int row = e / 10, cc = e % 10;
+            const int col = cc < 5 ? cc : TX + cc;
+            int hx = x0 - 5 + col, hy = y0 + row;
+            hslot[h] = (row + 5) * PW + col;
+            if (hx <= xe + 5 && hy <= ye + 5 && p_src(g, 0, hx) && p_src(g, 1, hy))
+                hoff[h] = (int)(gidx(g, hx, hy, 0) - hbase);
+        } else if (e < NH) {
+            const int e2 = e - NHX;
+            const int rr = e2 / TX, col = e2 % TX;
+            const int row = rr < 5 ? rr : TY + rr;
+            int hx = x0 + col, hy = y0 - 5 + row;
+            hslot[h] = row * PW + col + 5;
+            if (hx <= xe + 5 && hy <= ye + 5 && p_src(g, 0, hx) && p_src(g, 1, hy))
+                hoff[h] = (int)(gidx(g, hx, hy, 0) - hbase);
+        }
     }
-    if (tid < 6 * AY) {
-        const int t = tid / AY, u = tid % AY;
-        sD2[t][u] = (y0 + u <= ye) ? c.D[1][t * c1 + y0 + u] : 0.0;
+
+    // gradient jobs (LDS base, stride, destination) and coefficients
+    double cg1[6], cg2[6], cgx[6], cd1[6], cd2[6];
+    bool v1, v2, vx;
+    const int b1 = (ty + 5) * PW + tx, w1 = ty * G1S + tx;           // G_1 at x0-3+tx, row ty
+    const int b2 = ty * PW + tx + 5, w2 = OG2 + ty * TX + tx;         // G_2 at y0-3+ty, column tx
+    int bxj = 0, sxj = 1, wxj = GS - 1;
+    {
+        int p1 = x0 - 3 + tx;
+        v1 = p1 <= xe + 2 && g_src(g, 0, p1) && !wall_plane(g, 1, y);
+        int p2 = y0 - 3 + ty;
+        v2 = p2 <= ye + 2 && g_src(g, 1, p2) && !wall_plane(g, 0, x);
+        const double *cx = c.G[0];
+        int cix = 0, cstx = c0;
+        vx = false;
+        const int e1 = tid - E1;
+        if (e1 >= 0) {
+            const int r1x = e1 / 5, u1x = TX + e1 % 5;
+            int px_ = x0 - 3 + u1x;
+            vx = px_ <= xe + 2 && g_src(g, 0, px_) && !wall_plane(g, 1, y0 + r1x);
+            cix = px_;
+            bxj = (r1x + 5) * PW + u1x;
+            sxj = 1;
+            wxj = r1x * G1S + u1x;
+        } else if (tid < NG2X) {
+            const int c2x = tid % TX, v2r = TY + tid / TX;
+            int py_ = y0 - 3 + v2r;
+            vx = py_ <= ye + 2 && g_src(g, 1, py_) && !wall_plane(g, 0, x0 + c2x);
+            cx = c.G[1];
+            cix = py_;
+            cstx = c1;
+            bxj = v2r * PW + c2x + 5;
+            sxj = PW;
+            wxj = OG2 + v2r * TX + c2x;
+        }
+#pragma unroll
+        for (int s = 0; s < 6; ++s) {
+            cg1[s] = v1 ? c.G[0][s * c0 + p1] : 0.0;
+            cg2[s] = v2 ? c.G[1][s * c1 + p2] : 0.0;
+            cgx[s] = vx ? cx[s * cstx + cix] : 0.0;
+            cd1[s] = own ? c.D[0][s * c0 + x] : 0.0;
+            cd2[s] = own ? c.D[1][s * c1 + y] : 0.0;
+        }
     }
-    if (tid < HX) {
-        int xx = x0 - 5 + tid;
-        sPx[tid] = (xx <= xe + 5 && p_src(g, 0, xx)) ? xx : -1;
-    }
-    if (tid >= 128 && tid < 128 + HY) {
-        int yy = y0 - 5 + (tid - 128);
-        sPy[tid - 128] = (yy <= ye + 5 && p_src(g, 1, yy)) ? yy : -1;
-    }
+    const bool g3on = own && !(wall_plane(g, 0, x) || wall_plane(g, 1, y));
     __syncthreads();
 
-    // G_1 positions x-3..x+2 and G_2 positions y-3..y+2 that are identically zero
-    // because of the wall planes of their component in the transverse directions
-    const bool zy1 = wall_plane(g, 1, y);   // G_1 on a y wall plane
-    const bool zx2 = wall_plane(g, 0, x);   // G_2 on an x wall plane
-    const bool z3 = zx2 || zy1;             // G_3 on an x or y wall plane
-
+    // queue at iteration k: q = p(k-1 .. k+4), g3 = G_3(k-4 .. k+1)
     double q[6], g3[6];
     {
         double pz[11];
 #pragma unroll
         for (int s = 0; s < 11; ++s) {
-            int kk = kc0 - 5 + s;
-            pz[s] = (own && p_src(g, 2, kk)) ? src[gidx(g, x, y, kk)] : 0.0;
+            int kk = kc0 - 6 + s;
+            const bool v = vcol && p_src(g, 2, kk);
+            const double t = src[v ? colb + (int64_t)kk * pxy : hbase];
+            pz[s] = v ? t : 0.0;
         }
 #pragma unroll
         for (int t = 0; t < 6; ++t) {
-            int kk = kc0 - 3 + t;
-            g3[t] = (!z3 && g_src(g, 2, kk)) ? grad6(cg3, c2, kk, &pz[t], 1) : 0.0;
+            double v = sCG3[0][t] * pz[t];
+#pragma unroll
+            for (int s = 1; s < 6; ++s) v = v + sCG3[s][t] * pz[t + s];
+            g3[t] = (g3on && sVZ[t]) ? v : 0.0;
         }
 #pragma unroll
         for (int s = 0; s < 6; ++s) q[s] = pz[5 + s];
     }
-
+    auto load_halo = [&](int kk, double (&hv)[HL]) {
+        const bool vk = kk <= kc1 && p_src(g, 2, kk);
+#pragma unroll
+        for (int h = 0; h < HL; ++h) {
+            const bool v = vk && hoff[h] >= 0;
+            const double t = src[v ? hbase + hoff[h] + (int64_t)kk * pxy : hbase];
+            hv[h] = v ? t : 0.0;
+        }
+    };
+    auto load_col = [&](int kk) -> double {
+        const bool v = vcol && kk <= kc1 + 5 && p_src(g, 2, kk);
+        const double t = src[v ? colb + (int64_t)kk * pxy : hbase];
+        return v ? t : 0.0;
+    };
+    auto load_w = [&](int kk) -> double {
+        if (dot_mode != DOT_W) return 0.0;  // w is only a valid pointer in this mode
+        const bool v = own && kk <= kc1;
+        const double t = w[v ? selfb + (int64_t)kk * pxy : hbase];
+        return v ? t : 0.0;
+    };
+    double hvA[HL], hvB[HL];
+    load_halo(kc0, hvA);
+    load_halo(kc0 + 1, hvB);
+    double pA = load_col(kc0 + 5), pB = load_col(kc0 + 6);
+    double wA = load_w(kc0), wB = load_w(kc0 + 1);
     double acc1 = 0.0, acc2 = 0.0;
-    for (int k = kc0; k <= kc1; ++k) {
-        double lap = 0.0;
-        if (own) {
-            const bool zk = wall_plane(g, 2, k);
-            const int64_t xk = gidx(g, 0, y, k);  // row base of (., y, k) at i = 0
-            double px[11], py[11];
+
+    auto divergence = [&](int k, const double *G, double wv) {
+        // plane k from the gradient tile G, g3 = G_3(k-3 .. k+2), q[0] = p(k)
+        double lap = cd1[0] * G[ty * G1S + tx];
 #pragma unroll
-            for (int u = 0; u < 11; ++u) {
-                const int xs = sPx[tx + u];
-                px[u] = (xs >= 0) ? src[xk + xs] : 0.0;
-            }
-            const int64_t yk = gidx(g, x, 0, k);
+        for (int t = 1; t < 6; ++t) lap = lap + cd1[t] * G[ty * G1S + tx + t];
 #pragma unroll
-            for (int u = 0; u < 11; ++u) {
-                const int ys = sPy[ty + u];
-                py[u] = (ys >= 0) ? src[yk + (int64_t)ys * g.px] : 0.0;
-            }
-            // direction 1: lap = D_1 G_1
+        for (int t = 0; t < 6; ++t) lap = lap + cd2[t] * G[OG2 + (ty + t) * TX + tx];
+        const int kz = k - kc0;
 #pragma unroll
-            for (int t = 0; t < 6; ++t) {
-                double g1 = 0.0;
-                if (!zy1 && !zk && sV1[tx + t]) {
-                    g1 = sG1[0][tx + t] * px[t];
-#pragma unroll
-                    for (int s = 1; s < 6; ++s) g1 = g1 + sG1[s][tx + t] * px[t + s];
-                }
-                lap = (t == 0) ? sD1[0][tx] * g1 : lap + sD1[t][tx] * g1;
-            }
-#pragma unroll
-            for (int t = 0; t < 6; ++t) {
-                double g2 = 0.0;
-                if (!zx2 && !zk && sV2[ty + t]) {
-                    g2 = sG2[0][ty + t] * py[t];
-#pragma unroll
-                    for (int s = 1; s < 6; ++s) g2 = g2 + sG2[s][ty + t] * py[t + s];
-                }
-                lap = lap + sD2[t][ty] * g2;
-            }
-#pragma unroll
-            for (int t = 0; t < 6; ++t) lap = lap + cd3[t * c2 + k] * g3[t];
-            const int64_t xi = gidx(g, x, y, k);
-            dst[xi] = lap;
-            if (dot_mode == DOT_W) {
-                acc1 = acc1 + lap * w[xi];
-            } else if (dot_mode == DOT_SELF) {
-                acc1 = acc1 + src[xi] * lap;
-                acc2 = acc2 + lap * lap;
-            }
+        for (int t = 0; t < 6; ++t) lap = lap + sCD3[t][kz] * g3[t];
+        if (own) dst[selfb + (int64_t)k * pxy] = lap;
+        if (dot_mode == DOT_W) {
+            acc1 = acc1 + (own ? lap * wv : 0.0);
+        } else if (dot_mode == DOT_SELF) {
+            acc1 = acc1 + (own ? q[0] * lap : 0.0);
+            acc2 = acc2 + (own ? lap * lap : 0.0);
         }
-        // ---- advance the z queues: G_3(k+3) from p(k+1 .. k+6) ----
+    };
+    auto body = [&](int k, int buf, double (&hv)[HL], double &pn, double &wprev, double &wcur) {
+        // stage plane k
+        double *P = sP[buf];
+        P[(ty + 5) * PW + tx + 5] = q[1];
+#pragma unroll
+        for (int h = 0; h < HL; ++h)
+            if (hslot[h] < PS) P[hslot[h]] = hv[h];
+        __syncthreads();
+        load_halo(k + 2, hv);
+        // gradients of plane k
+        const bool zk = wall_plane(g, 2, k);
+        double *G = sG[buf];
         {
-            int kn = k + 6;
-            const double pn = (own && p_src(g, 2, kn)) ? src[gidx(g, x, y, kn)] : 0.0;
+            double a = cg1[0] * P[b1];
 #pragma unroll
-            for (int s = 0; s < 5; ++s) q[s] = q[s + 1];
-            q[5] = pn;
-            int kk = k + 3;
+            for (int s = 1; s < 6; ++s) a = a + cg1[s] * P[b1 + s];
+            G[w1] = (v1 && !zk) ? a : 0.0;
+            double b = cg2[0] * P[b2];
 #pragma unroll
-            for (int t = 0; t < 5; ++t) g3[t] = g3[t + 1];
-            g3[5] = (!z3 && g_src(g, 2, kk)) ? grad6(cg3, c2, kk, q, 1) : 0.0;
+            for (int s = 1; s < 6; ++s) b = b + cg2[s] * P[b2 + s * PW];
+            G[w2] = (v2 && !zk) ? b : 0.0;
+            double e = cgx[0] * P[bxj];
+#pragma unroll
+            for (int s = 1; s < 6; ++s) e = e + cgx[s] * P[bxj + s * sxj];
+            G[wxj] = (vx && !zk) ? e : 0.0;
         }
+        // divergence of plane k-1 (its gradient tile is the other buffer)
+        if (k > kc0) {
+            divergence(k - 1, sG[buf ^ 1], wprev);
+            wprev = load_w(k + 1);
+        }
+        // advance: q = p(k .. k+5), g3 = G_3(k-3 .. k+2)
+#pragma unroll
+        for (int s = 0; s < 5; ++s) q[s] = q[s + 1];
+        q[5] = pn;
+        pn = load_col(k + 7);
+#pragma unroll
+        for (int t = 0; t < 5; ++t) g3[t] = g3[t + 1];
+        const int u = k - kc0 + 6;  // G_3 position k+2
+        double v = sCG3[0][u] * q[0];
+#pragma unroll
+        for (int s = 1; s < 6; ++s) v = v + sCG3[s][u] * q[s];
+        g3[5] = (g3on && sVZ[u]) ? v : 0.0;
+        (void)wcur;
+    };
+    // w(k-1) is consumed in iteration k: the A set serves even offsets k-kc0, the B set odd ones
+    for (int k = kc0; k <= kc1; k += 2) {
+        body(k, 0, hvA, pA, wB, wA);
+        if (k + 1 <= kc1) body(k + 1, 1, hvB, pB, wA, wB);
     }
-    // ---- fixed-tree partials (the z chunk is exactly one reduction chunk) ----
+    __syncthreads();
+    {
+        const int n = kc1 - kc0 + 1;
+        const int buf = (n - 1) & 1;
+        divergence(kc1, sG[buf], buf == 0 ? wA : wB);
+    }
     if (dot_mode != DOT_NONE) {
         const int64_t pi = (int64_t)bx + (int64_t)nbx * ((int64_t)by + (int64_t)nby * bz);
         const double s1 = block_sum(acc1, sm);
@@ -239,7 +331,7 @@ __global__ __launch_bounds__(256) void k_apply_A(Geo g, Coef c, Range3 r, int nb
 }
 
 // ---------------------------------------------------------------------------
-// BiCGstab vector kernels (tile 64 x 4 x RED_KC so that fused partials follow
+// BiCGstab vector kernels (reduction tiles, so that fused partials follow
 // the fixed tree) and the one-block control kernels.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_init_rr(Geo g, Range3 r, const double *bb, double *rr, double *rh,
@@ -249,9 +341,9 @@ __global__ __launch_bounds__(256) void k_init_rr(Geo g, Range3 r, const double *
     const int j = r.lo1 + (int)(blockIdx.y * RED_TY + threadIdx.y);
     double m = 0.0, d = 0.0;
     if (i < r.lo0 + r.n0 && j < r.lo1 + r.n1) {
-        for (int t = 0; t < RED_KC; ++t) {
-            const int k = r.lo2 + (int)blockIdx.z * RED_KC + t;
-            if (k >= r.lo2 + r.n2) break;
+        int kc0, kc1;
+        red_chunk(r, (int)blockIdx.z, (int)gridDim.z, kc0, kc1);
+        for (int k = kc0; k <= kc1; ++k) {
             const int64_t x = gidx(g, i, j, k);
             const double v = bb[x] - rr[x];
             rr[x] = v;
@@ -298,9 +390,9 @@ __global__ __launch_bounds__(256) void k_x_update(Geo g, Range3 r, double *phi, 
     const int j = r.lo1 + (int)(blockIdx.y * RED_TY + threadIdx.y);
     double m = 0.0, d = 0.0;
     if (i < r.lo0 + r.n0 && j < r.lo1 + r.n1) {
-        for (int t = 0; t < RED_KC; ++t) {
-            const int k = r.lo2 + (int)blockIdx.z * RED_KC + t;
-            if (k >= r.lo2 + r.n2) break;
+        int kc0, kc1;
+        red_chunk(r, (int)blockIdx.z, (int)gridDim.z, kc0, kc1);
+        for (int k = kc0; k <= kc1; ++k) {
             const int64_t x = gidx(g, i, j, k);
             phi[x] = phi[x] + om * rr[x] + alpha * pk[x];
             const double v = rr[x] - om * Ar[x];
@@ -473,9 +565,11 @@ void launch_apply_A(hipStream_t s, const Geo &g, const Coef &c, const Box &pb, c
                     const double *w, int dot_mode, double *part1, double *part2, const int *active) {
     Range3 r = Rk(pb);
     if (empty(r)) return;
-    const int nbx = (r.n0 + AX - 1) / AX, nby = (r.n1 + AY - 1) / AY, nbz = (r.n2 + AZ - 1) / AZ;
-    hipLaunchKernelGGL(k_apply_A, dim3(xcd_grid((int64_t)nbx * nby * nbz)), dim3(AX, AY, 1), 0, s, g, c, r, nbx, nby,
-                       nbz, src, dst, w, dot_mode, part1, part2, active);
+    // the operator tile is the reduction tile, so the fused dot partials follow the fixed tree
+    const int nbx = (r.n0 + RED_TX - 1) / RED_TX, nby = (r.n1 + RED_TY - 1) / RED_TY, nbz = (r.n2 + RED_KC - 1) / RED_KC;
+    hipLaunchKernelGGL((k_apply<RED_TX, RED_TY, RED_KC>), dim3(xcd_grid((int64_t)nbx * nby * nbz)),
+                       dim3(RED_TX, RED_TY, 1), 0, s, g, c, r, nbx, nby, nbz, src, dst, w, dot_mode, part1, part2,
+                       active);
 }
 
 void launch_krylov_init(hipStream_t s, const Geo &g, const Box &b, const double *bb, double *rr, double *rh,

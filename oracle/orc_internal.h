@@ -41,9 +41,9 @@ struct orc_case {
 #define CDG(a, ii, i) ((a)[((ii) + 1) + 3 * ((i) - 1)])
 
 /* canonical reduction tree shared with the device kernels */
-#define ORC_RED_TX 64
-#define ORC_RED_TY 4
-#define ORC_RED_KC 8
+#define ORC_RED_TX 16
+#define ORC_RED_TY 16
+#define ORC_RED_KC 32
 #define ORC_RED_THREADS 256
 
 typedef double (*orc_point_fn)(const void *ctx, int64_t idx);

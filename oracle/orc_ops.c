@@ -312,10 +312,13 @@ void orc_product_div_grad(const orc_case *c, double *phi, double *lap, double *d
 /* ------------------------------------------------------------------------ */
 /* reductions                                                                */
 /* ------------------------------------------------------------------------ */
-/* Fixed tree of the device kernels: threads of a 64x4 tile each add ORC_RED_KC
- * planes sequentially; each wavefront of 64 combines by an xor butterfly
- * (offsets 32..1), the four wave sums as (w0+w1)+(w2+w3); the block partials
- * are then added by 256 threads striding over them and the same block tree. */
+/* Fixed tree of the device kernels: threads of a 16x16 tile (lane tx + 16 ty)
+ * each add the planes of their z chunk sequentially, the k range being cut
+ * into ceil(nK / ORC_RED_KC) chunks of balanced length (chunk b holds planes
+ * ss + b*nK/GZ .. ss + (b+1)*nK/GZ - 1); each wavefront of 64 lanes combines by
+ * an xor butterfly (offsets 32..1), the four wave sums as (w0+w1)+(w2+w3); the
+ * block partials are then added by 256 threads striding over them and the same
+ * block tree. */
 static double block_tree(double *a)
 {
     double w[4];
@@ -358,11 +361,9 @@ double orc_reduce_sum(const orc_case *c, const int ss[3], const int nn[3], orc_p
                         const int j = ss[1] + by * ORC_RED_TY + ty;
                         double acc = 0.0;
                         if (i <= nn[0] && j <= nn[1]) {
-                            for (int t = 0; t < ORC_RED_KC; ++t) {
-                                const int k = ss[2] + bz * ORC_RED_KC + t;
-                                if (k <= nn[2])
-                                    acc = acc + f(ctx, ORC_IDX(c, i, j, k));
-                            }
+                            const int k0 = ss[2] + (int)(((int64_t)bz * nK) / GZ);
+                            const int k1 = ss[2] + (int)(((int64_t)(bz + 1) * nK) / GZ) - 1;
+                            for (int k = k0; k <= k1; ++k) acc = acc + f(ctx, ORC_IDX(c, i, j, k));
                         }
                         tile[tx + ORC_RED_TX * ty] = acc;
                     }

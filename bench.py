@@ -10,6 +10,12 @@ reduction at mid-length) as an immersed-boundary mask, pulsatile inflow,
 Re = 500, dt = 5e-4.  Inputs are synthetic and deterministic; all fields are
 resident in HBM when the timed region starts.
 
+The headline value is the plain FP64 path.  The same number of steps is then
+run in the mixed-precision form (FP32 JOR sweeps as defect correction and as
+the right preconditioner of BiCGstab; residuals, inner products and updates in
+FP64) from the same initial state; its rate, speedup and distance from the FP64
+fields are reported under "mixed_precision".
+
 Prints one JSON line (rank 0).  Progress goes to stderr.
 """
 from __future__ import annotations
@@ -67,7 +73,10 @@ def read_traffic():
         return None
 
 
-def setup_case(ih, jor_fp32=False):
+def setup_case(ih, mixed=False, precond_sweeps=4):
+    """The bench case; mixed=True runs the Poisson solve in the mixed-precision
+    form (FP32 JOR defect correction and an FP32-JOR-preconditioned BiCGstab,
+    residuals, inner products and updates in FP64)."""
     c = ih.Case(N, BC, LEN, STRETCH, device=0)
     r_u = tube_radius(c.coords(0, 1))
     r_p = tube_radius(c.coords(0, 0))
@@ -81,18 +90,162 @@ def setup_case(ih, jor_fp32=False):
     c.fill(ih.PRE, 0.0)
     p = ih.StepParams(Re=RE, dtime=DT, jor_sweeps=JOR_SWEEPS, jor_omega=JOR_OMEGA, bicg_rel_tol=TOL,
                       bicg_max_it=MAXIT, level_pressure=1, inflow_kind=2, inflow_amp=1.0, use_mask=1,
-                      jor_fp32=1 if jor_fp32 else 0)
+                      jor_fp32=1 if mixed else 0, precond_sweeps=precond_sweeps if mixed else 0)
     return c, p
 
 
+def run_steps(ih, c, p, steps, warmup, label):
+    """warmup untimed steps, then `steps` timed ones; returns (ms per step,
+    kernel-family timings, BiCGstab iterations of one more step, time)."""
+    import ctypes
+
+    t = 0.0
+    for w in range(warmup):
+        st = c.timestep(p, t, stats=True)
+        t += DT
+        log(f"{label} warmup step {w}: BiCGstab iterations {list(st.bicg_iterations)}, "
+            f"max|div u| {st.div_max_after:.3e}")
+    c.lib.ih_kernel_timing_enable(c.h, 1)
+    c.sync()
+    c.timer_start()
+    t_wall0 = time.perf_counter()
+    for s in range(steps):
+        c.timestep(p, t, stats=False)
+        t += DT
+        log(f"{label} timed step {s} done")
+    ms = c.timer_stop()
+    wall = time.perf_counter() - t_wall0
+
+    def fam(name):
+        tot, n = ctypes.c_double(0), ctypes.c_int64(0)
+        c.lib.ih_kernel_timing_get(c.h, name.encode(), ctypes.byref(tot), ctypes.byref(n))
+        return tot.value, n.value
+
+    fams = {f: fam(f) for f in ("jor", "jor_f32", "bicgstab")}
+    c.lib.ih_kernel_timing_enable(c.h, 0)
+    st = c.timestep(p, t, stats=True)
+    t += DT
+    return ms / steps, fams, int(sum(st.bicg_iterations)), wall
+
+
+def rel_errors(ih, a, b):
+    """relative L2 / Linf distance of the velocity and pressure of case a from case b"""
+    out = {}
+    for name, fields in (("vel", (ih.VEL1, ih.VEL2, ih.VEL3)), ("pre", (ih.PRE,))):
+        num2 = den2 = numi = deni = 0.0
+        for f in fields:
+            x, y = a.download(f)[3:-3, 3:-3, 3:-3], b.download(f)[3:-3, 3:-3, 3:-3]
+            num2 += float(np.sum((x - y) ** 2))
+            den2 += float(np.sum(y ** 2))
+            numi = max(numi, float(np.abs(x - y).max()))
+            deni = max(deni, float(np.abs(y).max()))
+        out[f"{name}_rel_l2"] = math.sqrt(num2 / den2) if den2 > 0 else 0.0
+        out[f"{name}_rel_linf"] = numi / deni if deni > 0 else 0.0
+    return out
+
+
+BATCH_SIZES = [(65, 65, 65), (97, 97, 97), (129, 65, 65), (193, 97, 97)]  # 64^3, 96^3, 128x64x64, 192x96x96 cells
+
+
+def batched_relaxation(ih, n_cases=32, sweeps=200, seed=4):
+    """Config-4 shape: n_cases independent grids drawn from BATCH_SIZES, all
+    relaxed together (one launch per sweep, replayed from a hipGraph, tile
+    shape autotuned), against relaxing them one after another."""
+    rng = np.random.default_rng(seed)
+    cases = []
+    for q in range(n_cases):
+        N = BATCH_SIZES[int(rng.integers(len(BATCH_SIZES)))]
+        c = ih.Case(N, BC, LEN, STRETCH, device=0)
+        c.upload(ih.PRE, np.zeros(c.shape))
+        c.upload(ih.RES, np.ascontiguousarray(rng.standard_normal(c.shape)))
+        cases.append(c)
+    pts = sum(c.N[0] * c.N[1] * c.N[2] for c in cases)
+    bt = ih.Batch(cases)
+    tpb, zc, _ = bt.autotune(5)
+    bt.jor(sweeps, JOR_OMEGA, graph=True)  # capture + warm
+    cases[0].sync()
+    t0 = time.perf_counter()
+    bt.jor(sweeps, JOR_OMEGA, graph=True)
+    t_batch = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for c in cases:
+        c.jor(sweeps, JOR_OMEGA, c.Sp, c.Np)
+    for c in cases:
+        c.sync()
+    t_seq = time.perf_counter() - t0
+    bt.close()
+    for c in cases:
+        c.close()
+    return {
+        "cases": n_cases,
+        "grids": "random from 65^3, 97^3, 129x65x65, 193x97x97 points",
+        "points": pts,
+        "sweeps": sweeps,
+        "tile": {"threads": tpb, "z_chunk": zc},
+        "ms_per_sweep": t_batch / sweeps * 1e3,
+        "hbm_gbs": JOR_BYTES_PER_POINT * pts * sweeps / t_batch / 1e9,
+        "sequential_ms_per_sweep": t_seq / sweeps * 1e3,
+        "speedup_vs_sequential": t_seq / t_batch,
+    }
+
+
+SERVE_SIZES = [(97, 97, 97), (129, 129, 129), (257, 129, 129)]  # 96^3, 128^3, 256x128x128 cells
+
+
+def serving(ih, n_requests=16, rate=4.0, seed=5, budget_gb=64, max_active=4):
+    """Config-5 shape, scaled to a bench run: Poisson arrivals at `rate`/s of
+    stenosed tubes (radius U[0.2, 0.35], stenosis U[0.3, 0.7]) of sizes from
+    SERVE_SIZES with 2-5 steps each, admitted by an HBM budget."""
+    rng = np.random.default_rng(seed)
+    t, reqs = 0.0, []
+    for q in range(n_requests):
+        t += float(rng.exponential(1.0 / rate))
+        reqs.append(dict(arrival_s=t, N=SERVE_SIZES[int(rng.integers(len(SERVE_SIZES)))],
+                         steps=int(rng.integers(2, 6)), radius=float(rng.uniform(0.2, 0.35)),
+                         stenosis=float(rng.uniform(0.3, 0.7))))
+    step = ih.StepParams(Re=RE, dtime=DT, jor_sweeps=JOR_SWEEPS, jor_omega=JOR_OMEGA, bicg_rel_tol=TOL,
+                         bicg_max_it=MAXIT, level_pressure=1, inflow_kind=2, inflow_amp=1.0, use_mask=1,
+                         jor_fp32=0, precond_sweeps=0)
+    res, summ, rc = ih.serve(reqs, step, hbm_budget=budget_gb << 30, max_active=max_active)
+    if rc != 0:
+        raise RuntimeError(f"serving failed: {rc}")
+    lat = np.array([r.latency_s for r in res])
+    steps = sum(r["steps"] for r in reqs)
+    pts = sum(r["steps"] * r["N"][0] * r["N"][1] * r["N"][2] for r in reqs)
+    return {
+        "requests": n_requests,
+        "arrival_rate_per_s": rate,
+        "sizes": "96^3, 128^3, 256x128x128 cells, 2-5 steps each",
+        "hbm_budget_gb": budget_gb,
+        "max_active": max_active,
+        "p50_latency_s": float(np.percentile(lat, 50)),
+        "p99_latency_s": float(np.percentile(lat, 99)),
+        "makespan_s": summ.makespan_s,
+        "steps_per_s": steps / summ.makespan_s,
+        "point_steps_per_s": pts / summ.makespan_s,
+        "peak_active": summ.peak_active,
+        "peak_hbm_gb": summ.peak_bytes / 2**30,
+    }
+
+
 def cpu_baseline(iters_per_step):
-    """Oracle (single-threaded C restatement) on a bounded sample at the full size:
-    20 JOR sweeps, 10 BiCGstab iterations and one step with the Poisson solve cut to
-    one iteration; the per-step time is composed with the GPU run's iteration counts."""
+    """The oracle (the C restatement of the reference loops, point loops on
+    OpenMP threads, dot products serial) on a bounded sample at the full size:
+    20 JOR sweeps, 10 BiCGstab iterations and one step with the Poisson solve
+    cut to one iteration; the per-step time is composed with the GPU run's
+    iteration counts."""
     sys.path.insert(0, str(ROOT / "tests"))
     import ctypes
 
     import oracle_lib as O
+
+    O.use_parallel_build()
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    if os.environ.get("OMP_NUM_THREADS", "").isdigit():
+        cores = min(cores, int(os.environ["OMP_NUM_THREADS"]))
 
     oc = O.OracleCase(N, BC, LEN, STRETCH)
     oc.set_reduction_mode(0)  # the reference's sequential loop order
@@ -126,9 +279,9 @@ def cpu_baseline(iters_per_step):
     return {
         "value": 1.0 / step,
         "unit": "time steps/s",
-        "cores": 1,
+        "cores": cores,
         "kind": "port",
-        "sample": (f"oracle at 257x129x129 on 1 core: 20 JOR sweeps ({t_jor * 1e3:.0f} ms each), 10 BiCGstab "
+        "sample": (f"oracle at 257x129x129 on {cores} threads: 20 JOR sweeps ({t_jor * 1e3:.0f} ms each), 10 BiCGstab "
                    f"iterations ({t_bi:.2f} s each), one step with 1 BiCGstab iteration per substep "
                    f"({t_rest:.1f} s); step = that + 600 sweeps + {iters_per_step - 3} iterations "
                    f"(the GPU run's count); {sample:.0f} s of CPU work"),
@@ -167,7 +320,11 @@ def main():
     ap.add_argument("--steps", type=int, default=4)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--mixed", action="store_true", help="FP32 JOR sweeps with FP64 residual and update")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision run")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-relaxation run")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving trace")
+    ap.add_argument("--precond-sweeps", type=int, default=4,
+                    help="FP32 JOR sweeps of the mixed-precision BiCGstab preconditioner")
     ap.add_argument("--only", choices=("jor", "operator"), default=None,
                     help="launch one kernel family on the bench grid (for counter profiles): JOR sweeps or the "
                          "fused Poisson operator, followed by one plain device copy of a field for calibration")
@@ -184,42 +341,14 @@ def main():
     if args.only:
         return kernel_only(ih, args.only, args.launches)
     log(f"bench: case {N}, walls, tanh {STRETCH[1]}, stenosed tube, Re {RE}, dt {DT}")
-    c, p = setup_case(ih, jor_fp32=args.mixed)
+    c, p = setup_case(ih)
     log(f"bench: {c.bytes() / 2**30:.2f} GiB resident")
-    t = 0.0
-    for w in range(args.warmup):
-        st = c.timestep(p, t, stats=True)
-        t += DT
-        log(f"warmup step {w}: BiCGstab iterations {list(st.bicg_iterations)}, max|div u| {st.div_max_after:.3e}")
-    c.lib.ih_kernel_timing_enable(c.h, 1)
-    iters = []
-    c.sync()
-    c.timer_start()
-    t_wall0 = time.perf_counter()
-    for s in range(args.steps):
-        st = c.timestep(p, t, stats=False)
-        t += DT
-        log(f"timed step {s} done")
-    ms = c.timer_stop()
-    wall = time.perf_counter() - t_wall0
-    import ctypes
-
-    def fam(name):
-        tot, n = ctypes.c_double(0), ctypes.c_int64(0)
-        c.lib.ih_kernel_timing_get(c.h, name.encode(), ctypes.byref(tot), ctypes.byref(n))
-        return tot.value, n.value
-
-    jor_name = "jor_f32" if args.mixed else "jor"
-    jor_ms, jor_n = fam(jor_name)
-    bi_ms, bi_n = fam("bicgstab")
-    c.lib.ih_kernel_timing_enable(c.h, 0)
-    # one more step with statistics for the iteration count of this workload
-    st = c.timestep(p, t, stats=True)
-    iters_per_step = int(sum(st.bicg_iterations))
-    ms_per_step = ms / args.steps
+    ms_per_step, fams, iters_per_step, wall = run_steps(ih, c, p, args.steps, args.warmup, "fp64")
     value = 1000.0 / ms_per_step
+    jor_ms, jor_n = fams["jor"]
+    bi_ms, bi_n = fams["bicgstab"]
     P = N[0] * N[1] * N[2]
-    jor_bytes = (12 if args.mixed else JOR_BYTES_PER_POINT) * P
+    jor_bytes = JOR_BYTES_PER_POINT * P
     jor_avg_ms = jor_ms / max(jor_n, 1)
     achieved = jor_bytes / (jor_avg_ms * 1e-3) / 1e9
     traffic = read_traffic()
@@ -234,7 +363,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32+f64" if args.mixed else "f64",
+        "dtype": "f64",
         "data": "synthetic",
         "config": {
             "workload": "stenosed-tube projection step, 257x129x129 points (256x128x128 cells), tanh 1.5 in y,z, "
@@ -245,7 +374,7 @@ def main():
             "parallelism": "single GPU",
         },
         "roofline": {
-            "kernel": "k_jor_f32" if args.mixed else "k_jor",
+            "kernel": "k_jor",
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
@@ -263,6 +392,29 @@ def main():
         "bicgstab_ms_per_iteration": bi_ms / max(bi_n, 1),
         "wall_s_timed": wall,
     }
+    if not args.no_mixed:
+        cm, pm = setup_case(ih, mixed=True, precond_sweeps=args.precond_sweeps)
+        ms_m, fams_m, iters_m, _ = run_steps(ih, cm, pm, args.steps, args.warmup, "mixed")
+        err = rel_errors(ih, cm, c)  # both cases advanced warmup + steps + 1 steps from the same state
+        f32_ms, f32_n = fams_m["jor_f32"]
+        result["mixed_precision"] = {
+            "form": (f"{JOR_SWEEPS} FP32 JOR sweeps as defect correction, then BiCGstab right-preconditioned by "
+                     f"{args.precond_sweeps} FP32 JOR sweeps; residuals, inner products and updates FP64"),
+            "steps_per_s": 1000.0 / ms_m,
+            "ms_per_step": ms_m,
+            "speedup_vs_fp64": ms_per_step / ms_m,
+            "bicgstab_iterations_per_step": iters_m,
+            "jor_f32_avg_launch_ms": f32_ms / max(f32_n, 1),
+            "steps_compared": args.warmup + args.steps + 1,
+            **{f"{k}_vs_fp64": v for k, v in err.items()},
+        }
+        cm.close()
+    if not args.no_batch:
+        log("batched relaxation ...")
+        result["batched_relaxation"] = batched_relaxation(ih)
+    if not args.no_serve:
+        log("serving trace ...")
+        result["serving"] = serving(ih)
     if not args.no_cpu_baseline and rank == 0:
         log("cpu baseline: timing the oracle on a bounded sample ...")
         result["cpu_baseline"] = cpu_baseline(iters_per_step)

@@ -253,6 +253,8 @@ int upload_coeffs(ih_case *c) {
         off_xu[d] = (int64_t)h.size();
         h.insert(h.end(), c->hg.xu[d].begin(), c->hg.xu[d].end());
     }
+    const int64_t off_zero = (int64_t)h.size();
+    h.resize(h.size() + 64, 0.0);
     if (c->dcoef) (void)hipFree(c->dcoef);
     if (c->dcoef32) (void)hipFree(c->dcoef32);
     c->dcoef = nullptr;
@@ -267,6 +269,7 @@ int upload_coeffs(ih_case *c) {
     }
     HIPCHK(hipMalloc(&c->dcoef32, sizeof(float) * h32.size()));
     HIPCHK(hipMemcpy(c->dcoef32, h32.data(), sizeof(float) * h32.size(), hipMemcpyHostToDevice));
+    c->c.zero = c->dcoef + off_zero;
     for (int d = 0; d < 3; ++d) {
         c->c.G[d] = c->dcoef + off_G[d];
         c->c.Ip[d] = c->dcoef + off_Ip[d];
@@ -503,7 +506,9 @@ static const std::vector<double> *coeff_by_name(const HostCoeffs &hc, const char
 
 static int grid_from_desc(const ih_grid_desc *gd, HostGrid &hg) {
     if (!gd) return fail(IH_EINVAL, "null grid");
-    if (gd->version != IH_ABI_VERSION) return fail(IH_EINVAL, "grid desc version %d, library %d", gd->version, IH_ABI_VERSION);
+    // the descriptor's layout is the same in every ABI version so far
+    if (gd->version < 1 || gd->version > IH_ABI_VERSION)
+        return fail(IH_EINVAL, "grid desc version %d, library %d", gd->version, IH_ABI_VERSION);
     for (int d = 0; d < 3; ++d) {
         if (gd->N[d] < 8 || gd->N[d] > 8192) return fail(IH_EINVAL, "N[%d] = %d out of range [8, 8192]", d, gd->N[d]);
         for (int s = 0; s < 2; ++s)
@@ -587,6 +592,20 @@ int64_t ih_field_elems(const ih_case *c) {
 }
 
 int64_t ih_case_bytes(const ih_case *c) { return c ? c->bytes : 0; }
+
+int64_t ih_grid_bytes(const ih_grid_desc *gd) {
+    if (!gd) return 0;
+    for (int d = 0; d < 3; ++d)
+        if (gd->N[d] < 1) return 0;
+    const int64_t px = ((gd->N[0] + 7) + 15) / 16 * 16;
+    const int64_t pxy = px * (gd->N[1] + 7);
+    const int64_t fe = ((12 + pxy * (gd->N[2] + 7)) + 63) / 64 * 64;
+    int64_t tables = 0;
+    for (int d = 0; d < 3; ++d) tables += (int64_t)(((gd->N[d] + 1) + 7) / 8 * 8) * 80 * 12 + 2 * (gd->N[d] + 7) * 8;
+    const int64_t parts = 2 * 8 * std::max<int64_t>(4096, ((gd->N[0] + 15) / 16) * ((gd->N[1] + 15) / 16) *
+                                                             ((gd->N[2] + 31) / 32));
+    return IH_NFIELDS * 8 * fe + 3 * 4 * fe + 3 * fe + tables + parts + (1 << 20);
+}
 
 int ih_field_upload(ih_case *c, ih_field f, const double *host) {
     if (!c || !host || !valid_field(f)) return fail(IH_EINVAL, "bad argument");
@@ -789,12 +808,13 @@ int ih_residual_cdg(ih_case *c, const int ss[3], const int nn[3], ih_field rel, 
 // init_yes = .FALSE.: the iteration runs on the device (scalars, stop test and
 // breakdown exits in one-block control kernels); the host enqueues batches of
 // iterations and reads the flags once per batch.
-int ih_bicgstab(ih_case *c, double eps, int n_it_max, const int ss[3], const int nn[3], ih_field bbf, ih_field phif,
-                ih_bicgstab_stats *st) {
+static int bicgstab_run(ih_case *c, double eps, int n_it_max, const int ss[3], const int nn[3], ih_field bbf,
+                        ih_field phif, int sweeps, double omega, ih_bicgstab_stats *st) {
     int rc = need_coeffs(c);
     if (rc) return rc;
-    if (!ss || !nn || !valid_field(bbf) || !valid_field(phif) || n_it_max < 0) return fail(IH_EINVAL, "bad argument");
-    for (int f : {IH_RR, IH_RH, IH_PK, IH_AP, IH_AR, IH_DIG})
+    if (!ss || !nn || !valid_field(bbf) || !valid_field(phif) || n_it_max < 0 || sweeps < 0)
+        return fail(IH_EINVAL, "bad argument");
+    for (int f : {IH_RR, IH_RH, IH_PK, IH_AP, IH_AR, IH_DIG, IH_Z1, IH_Z2})
         if (bbf == f || phif == f) return fail(IH_EINVAL, "bb/phi may not be a BiCGstab work field");
     Box b;
     if ((rc = to_box(c, ss, nn, b))) return rc;
@@ -807,6 +827,17 @@ int ih_bicgstab(ih_case *c, double eps, int n_it_max, const int ss[3], const int
     hipStream_t s = c->stream;
     KrylovPtrs k{F(c, IH_RR), F(c, IH_RH), F(c, IH_PK), F(c, IH_AP), F(c, IH_AR), F(c, phif), c->part, c->part2,
                  c->ksc, c->kfl};
+    if (sweeps > 0) {
+        k.z1 = F(c, IH_Z1);
+        k.z2 = F(c, IH_Z2);
+        k.e0 = c->f32[0];
+        k.e1 = c->f32[2];
+        k.b32 = c->f32[1];
+        k.sweeps = sweeps;
+        k.omega = (float)omega;
+        // ghost layers of the ping-pong buffer must hold finite values (zero)
+        launch_fill_f32(s, k.e1, 0.0f, c->g.fe);
+    }
     // rr = bb - A phi, rh = rr, norm and <rr, rh>
     if ((rc = apply_div_grad(c, F(c, phif), k.rr))) return rc;
     launch_krylov_init(s, c->g, b, F(c, bbf), k.rr, k.rh, k.part1, k.part2, k.sc, k.fl, eps, n_it_max);
@@ -834,6 +865,17 @@ int ih_bicgstab(ih_case *c, double eps, int n_it_max, const int ss[3], const int
         st->exit_reason = c->hkfl[KF_EXIT];
     }
     return IH_OK;
+}
+
+int ih_bicgstab(ih_case *c, double eps, int n_it_max, const int ss[3], const int nn[3], ih_field bbf, ih_field phif,
+                ih_bicgstab_stats *st) {
+    return bicgstab_run(c, eps, n_it_max, ss, nn, bbf, phif, 0, 0.0, st);
+}
+
+int ih_bicgstab_mixed(ih_case *c, double eps, int n_it_max, const int ss[3], const int nn[3], ih_field bbf,
+                      ih_field phif, int sweeps, double omega, ih_bicgstab_stats *st) {
+    if (sweeps < 1) return fail(IH_EINVAL, "the mixed-precision preconditioner needs at least one sweep");
+    return bicgstab_run(c, eps, n_it_max, ss, nn, bbf, phif, sweeps, omega, st);
 }
 
 int ih_fill_corners(ih_case *c, ih_field f) {
@@ -946,7 +988,10 @@ static int poisson_solve(ih_case *c, const ih_step_params *p, const Box &b, ih_b
     if (rc2) return rc2;
     const double eps = p->bicg_rel_tol * bnorm;
     PhaseTimer pt(c, "bicgstab");
-    rc = ih_bicgstab(c, eps, p->bicg_max_it, ss, nn, IH_RES, IH_PRE, bst);
+    if (p->precond_sweeps > 0)
+        rc = ih_bicgstab_mixed(c, eps, p->bicg_max_it, ss, nn, IH_RES, IH_PRE, p->precond_sweeps, p->jor_omega, bst);
+    else
+        rc = ih_bicgstab(c, eps, p->bicg_max_it, ss, nn, IH_RES, IH_PRE, bst);
     pt.done(bst ? bst->iterations : 0);
     return rc;
 }
@@ -1176,6 +1221,231 @@ int ih_kernel_timing_get(ih_case *c, const char *family, double *total_ms, int64
     }
     *total_ms = it->second.total_ms;
     *launches = it->second.launches;
+    return IH_OK;
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// Batched relaxation of many cases (uneven grids) in single launches.
+// ===========================================================================
+struct ih_batch {
+    std::vector<ih_case *> cases;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int tpb = 256, zchunk = 16;
+    int ntiles = 0;
+    JorBatchItem *items[2][2] = {};  // [fp32][direction]
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int g_sweeps = -1, g_fp32 = -1, g_tpb = -1, g_zchunk = -1;
+    double g_omega = 0.0;
+};
+
+namespace {
+
+void batch_drop_graph(ih_batch *b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    b->g_sweeps = -1;
+}
+
+// item tables for the current tile shape
+int batch_build(ih_batch *b) {
+    std::vector<JorBatchItem> h[2][2];
+    int t0 = 0;
+    for (ih_case *c : b->cases) {
+        const Box pb = pbox(c->g);
+        int ntxy, nbz;
+        const int nt = jor_batch_tiles(pb, b->tpb, b->zchunk, ntxy, nbz);
+        JorBatchItem it{};
+        it.g = c->g;
+        it.r = Range3{pb.lo[0], pb.lo[1], pb.lo[2], pb.hi[0] - pb.lo[0] + 1, pb.hi[1] - pb.lo[1] + 1,
+                      pb.hi[2] - pb.lo[2] + 1};
+        for (int d = 0; d < 3; ++d) {
+            it.dg[d] = c->c.dg[d];
+            it.dgf[d] = c->c.dgf[d];
+        }
+        it.ntxy = ntxy;
+        it.nbz = nbz;
+        it.tile0 = t0;
+        t0 += nt;
+        for (int dir = 0; dir < 2; ++dir) {
+            JorBatchItem a = it, f = it;
+            a.rel = dir == 0 ? F(c, IH_PRE) : F(c, IH_COMP);
+            a.out = dir == 0 ? F(c, IH_COMP) : F(c, IH_PRE);
+            a.bb = F(c, IH_RES);
+            f.rel = dir == 0 ? (void *)c->f32[0] : (void *)c->f32[2];
+            f.out = dir == 0 ? (void *)c->f32[2] : (void *)c->f32[0];
+            f.bb = c->f32[1];
+            h[0][dir].push_back(a);
+            h[1][dir].push_back(f);
+        }
+    }
+    b->ntiles = t0;
+    for (int p = 0; p < 2; ++p)
+        for (int dir = 0; dir < 2; ++dir) {
+            if (!b->items[p][dir]) HIPCHK(hipMalloc(&b->items[p][dir], sizeof(JorBatchItem) * b->cases.size()));
+            HIPCHK(hipMemcpy(b->items[p][dir], h[p][dir].data(), sizeof(JorBatchItem) * b->cases.size(),
+                             hipMemcpyHostToDevice));
+        }
+    batch_drop_graph(b);
+    return IH_OK;
+}
+
+int batch_sweeps(ih_batch *b, int n, double omega, bool fp32) {
+    const int nitems = (int)b->cases.size();
+    for (int q = 0; q < n; ++q)
+        if (launch_jor_batch(b->stream, b->items[fp32 ? 1 : 0][q & 1], nitems, b->ntiles, b->tpb, fp32, omega))
+            return fail(IH_EINVAL, "unsupported tile shape");
+    LAUNCHCHK();
+    return IH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ih_batch_create(ih_case *const *cases, int n, ih_batch **out) {
+    if (!cases || n <= 0 || !out) return fail(IH_EINVAL, "bad argument");
+    *out = nullptr;
+    for (int q = 0; q < n; ++q) {
+        if (!cases[q]) return fail(IH_EINVAL, "case %d is null", q);
+        if (!cases[q]->has_coeffs) return fail(IH_ESTATE, "case %d has no coefficients", q);
+        if (cases[q]->device != cases[0]->device) return fail(IH_EINVAL, "cases of one batch share a device");
+        for (int r = 0; r < q; ++r)
+            if (cases[r] == cases[q]) return fail(IH_EINVAL, "case %d appears twice", q);
+    }
+    ih_batch *b = new ih_batch();
+    b->cases.assign(cases, cases + n);
+    b->device = cases[0]->device;
+    int rc;
+    if (hipSetDevice(b->device) != hipSuccess || hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess) {
+        ih_batch_destroy(b);
+        return fail(IH_EHIP, "stream/event creation failed");
+    }
+    if ((rc = batch_build(b))) {
+        ih_batch_destroy(b);
+        return rc;
+    }
+    *out = b;
+    return IH_OK;
+}
+
+void ih_batch_destroy(ih_batch *b) {
+    if (!b) return;
+    (void)hipSetDevice(b->device);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    batch_drop_graph(b);
+    for (auto &pp : b->items)
+        for (auto *q : pp)
+            if (q) (void)hipFree(q);
+    if (b->ev0) (void)hipEventDestroy(b->ev0);
+    if (b->ev1) (void)hipEventDestroy(b->ev1);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+int ih_batch_autotune(ih_batch *b, int sweeps, int *tpb, int *zchunk, double *ms_best) {
+    if (!b || sweeps < 1) return fail(IH_EINVAL, "bad argument");
+    HIPCHK(hipSetDevice(b->device));
+    for (ih_case *c : b->cases) HIPCHK(hipStreamSynchronize(c->stream));
+    static const int tpbs[3] = {128, 256, 512}, zcs[3] = {8, 16, 32};
+    double best = 1e300;
+    int bt = b->tpb, bz = b->zchunk, rc;
+    for (int a : tpbs)
+        for (int z : zcs) {
+            b->tpb = a;
+            b->zchunk = z;
+            if ((rc = batch_build(b))) return rc;
+            // direction 0 only (PRE -> COMP): the relaxed field is left untouched
+            for (int q = 0; q < 2; ++q)
+                if (launch_jor_batch(b->stream, b->items[0][0], (int)b->cases.size(), b->ntiles, a, false, 0.8))
+                    return fail(IH_EINVAL, "tile shape");
+            HIPCHK(hipEventRecord(b->ev0, b->stream));
+            for (int q = 0; q < sweeps; ++q)
+                launch_jor_batch(b->stream, b->items[0][0], (int)b->cases.size(), b->ntiles, a, false, 0.8);
+            HIPCHK(hipEventRecord(b->ev1, b->stream));
+            HIPCHK(hipEventSynchronize(b->ev1));
+            LAUNCHCHK();
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+            if (ms < best) {
+                best = ms;
+                bt = a;
+                bz = z;
+            }
+        }
+    b->tpb = bt;
+    b->zchunk = bz;
+    if ((rc = batch_build(b))) return rc;
+    if (tpb) *tpb = bt;
+    if (zchunk) *zchunk = bz;
+    if (ms_best) *ms_best = best / sweeps;
+    return IH_OK;
+}
+
+int ih_batch_jor(ih_batch *b, int n_sweeps, double omega, int fp32, int use_graph) {
+    if (!b || n_sweeps < 0) return fail(IH_EINVAL, "bad argument");
+    HIPCHK(hipSetDevice(b->device));
+    int rc;
+    for (ih_case *c : b->cases) HIPCHK(hipStreamSynchronize(c->stream));
+    hipStream_t s = b->stream;
+    if (fp32) {
+        // defect correction per case: r = RES - A_cdg PRE in FP64, A e = r in FP32 from e = 0
+        for (ih_case *c : b->cases) {
+            const Box pb = pbox(c->g);
+            if ((rc = ensure_part(c, red_blocks(pb)))) return rc;
+            launch_exchange_relax(s, c->g, F(c, IH_PRE));
+            launch_residual_cdg(s, c->g, c->c, pb, F(c, IH_PRE), F(c, IH_RES), F(c, IH_TMP1), c->part);
+            launch_to_f32(s, c->g, pb, F(c, IH_TMP1), c->f32[1]);
+            launch_fill_f32(s, c->f32[0], 0.0f, c->g.fe);
+            launch_fill_f32(s, c->f32[2], 0.0f, c->g.fe);
+        }
+        LAUNCHCHK();
+    }
+    if (n_sweeps > 0) {
+        if (use_graph) {
+            if (!b->exec || b->g_sweeps != n_sweeps || b->g_fp32 != fp32 || b->g_omega != omega ||
+                b->g_tpb != b->tpb || b->g_zchunk != b->zchunk) {
+                batch_drop_graph(b);
+                HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+                rc = batch_sweeps(b, n_sweeps, omega, fp32 != 0);
+                hipGraph_t gr = nullptr;
+                const hipError_t e = hipStreamEndCapture(s, &gr);
+                if (rc) {
+                    if (gr) (void)hipGraphDestroy(gr);
+                    return rc;
+                }
+                if (e != hipSuccess) return fail(IH_EHIP, "graph capture: %s", hipGetErrorString(e));
+                b->graph = gr;
+                HIPCHK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+                b->g_sweeps = n_sweeps;
+                b->g_fp32 = fp32;
+                b->g_omega = omega;
+                b->g_tpb = b->tpb;
+                b->g_zchunk = b->zchunk;
+            }
+            HIPCHK(hipGraphLaunch(b->exec, s));
+        } else if ((rc = batch_sweeps(b, n_sweeps, omega, fp32 != 0))) {
+            return rc;
+        }
+    }
+    for (ih_case *c : b->cases) {
+        const Box pb = pbox(c->g);
+        if (fp32) {
+            launch_correct_f32(s, c->g, pb, (n_sweeps & 1) ? c->f32[2] : c->f32[0], F(c, IH_PRE));
+        } else if (n_sweeps & 1) {
+            launch_copy_box(s, c->g, pb, F(c, IH_COMP), F(c, IH_PRE));
+        }
+        launch_exchange_relax(s, c->g, F(c, IH_PRE));
+    }
+    LAUNCHCHK();
+    HIPCHK(hipStreamSynchronize(s));
     return IH_OK;
 }
 

@@ -9,9 +9,7 @@ namespace dev {
 constexpr int BX = 64;
 constexpr int BY = 4;
 
-struct Range3 {
-    int lo0, lo1, lo2, n0, n1, n2;
-};
+using Range3 = ih::Range3;
 
 inline Range3 make_range(const Box &b) {
     return Range3{b.lo[0], b.lo[1], b.lo[2], b.hi[0] - b.lo[0] + 1, b.hi[1] - b.lo[1] + 1, b.hi[2] - b.lo[2] + 1};
@@ -112,4 +110,5 @@ __device__ inline bool xcd_tile(int nbx, int nby, int nbz, int &bx, int &by, int
 }
 
 }  // namespace dev
+
 }  // namespace ih

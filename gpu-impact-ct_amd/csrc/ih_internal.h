@@ -45,6 +45,7 @@ struct Coef {
     const float *dgf[3];   // cdg in FP32 for the mixed-precision sweep
     const double *xp[3];   // coordinates (-3:N+3) at [i+3]
     const double *xu[3];
+    const double *zero;    // 64 doubles of 0.0, never written: the load source for positions outside the field
 };
 
 // Reduction tree constants, shared with the oracle's restatement: blocks of
@@ -59,6 +60,11 @@ constexpr int RED_THREADS = 256;
 // Launch-side helpers implemented in kernels.hip ------------------------------
 struct Box {
     int lo[3], hi[3];
+};
+
+// An index range as the kernels take it: lo and extent per direction.
+struct Range3 {
+    int lo0, lo1, lo2, n0, n1, n2;
 };
 
 void launch_exchange(hipStream_t s, const Geo &g, int dir, int nf, double *const *f, const int *vel_dir);
@@ -86,19 +92,37 @@ void launch_div_dir(hipStream_t s, const Geo &g, const Coef &c, int q, const dou
 void launch_jor(hipStream_t s, const Geo &g, const Coef &c, const Box &b, double omega, const double *rel,
                 const double *bb, double *out);
 void launch_jor_f32(hipStream_t s, const Geo &g, const Coef &c, const Box &b, float omega, const float *rel,
-                    const float *bb, float *out);
+                    const float *bb, float *out, const int *active = nullptr);
 void launch_residual_cdg(hipStream_t s, const Geo &g, const Coef &c, const Box &b, const double *rel,
                          const double *bb, double *out_res /* may be null */, double *partial_max);
 void launch_copy_box(hipStream_t s, const Geo &g, const Box &b, const double *src, double *dst);
 void launch_axpby(hipStream_t s, const Geo &g, double a, double b, const double *x, const double *y, double *z,
                   int64_t n);
 void launch_fill(hipStream_t s, double *f, double v, int64_t n);
-void launch_fill_f32(hipStream_t s, float *f, float v, int64_t n);
-void launch_to_f32(hipStream_t s, const Geo &g, const Box &b, const double *src, float *dst);
+void launch_fill_f32(hipStream_t s, float *f, float v, int64_t n, const int *active = nullptr);
+void launch_to_f32(hipStream_t s, const Geo &g, const Box &b, const double *src, float *dst,
+                   const int *active = nullptr);
+void launch_from_f32(hipStream_t s, const Geo &g, const Box &b, const float *e, double *z, const int *active = nullptr);
 void launch_correct_f32(hipStream_t s, const Geo &g, const Box &b, const float *e, double *x);
 
 // canonical reductions: returns number of partial blocks written
 int64_t red_blocks(const Box &b);
+
+// One case of a batched relaxation (k_jor_batch): geometry, relaxation box, cdg
+// in FP64 and FP32, the arrays of the sweep direction and the case's first
+// global tile.
+struct JorBatchItem {
+    Geo g;
+    Range3 r;
+    const double *dg[3];
+    const float *dgf[3];
+    const void *rel, *bb;
+    void *out;
+    int ntxy, nbz, tile0, pad;
+};
+int jor_batch_tiles(const Box &b, int tpb, int zchunk, int &ntxy, int &nbz);
+int launch_jor_batch(hipStream_t s, const JorBatchItem *items, int nitems, int ntiles, int tpb, bool fp32,
+                     double omega);
 void launch_dot_partial(hipStream_t s, const Geo &g, const Box &b, const double *a, const double *c, double *part);
 void launch_sum_partial(hipStream_t s, const Geo &g, const Box &b, const double *a, double *part);
 void launch_final_sum(hipStream_t s, const double *part, int64_t nb, double *out);

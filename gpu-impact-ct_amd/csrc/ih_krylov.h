@@ -28,10 +28,16 @@ struct KrylovPtrs {
     double *part1, *part2;
     double *sc;
     int *fl;
+    // right preconditioner M = `sweeps` FP32 JOR sweeps on cdg from zero (0: none):
+    // z1 = M pk, z2 = M rr; e0/e1 ping-pong, b32 the FP32 right-hand side
+    double *z1 = nullptr, *z2 = nullptr;
+    float *e0 = nullptr, *e1 = nullptr, *b32 = nullptr;
+    int sweeps = 0;
+    float omega = 0.8f;
 };
 
 // dot modes of the operator kernel
-enum { KDOT_NONE = 0, KDOT_W = 1, KDOT_SELF = 2 };
+enum { KDOT_NONE = 0, KDOT_W = 1, KDOT_SELF = 2, KDOT_PAIR = 3 };
 
 void launch_apply_A(hipStream_t s, const Geo &g, const Coef &c, const Box &pb, const double *src, double *dst,
                     const double *w, int dot_mode, double *part1, double *part2, const int *active);

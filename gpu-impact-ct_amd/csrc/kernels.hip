@@ -432,13 +432,120 @@ __device__ inline void jor_column(const Geo &g, const Coef &c, const Range3 &r, 
     }
 }
 
+// Flat form: the (i, j) plane of the range is numbered i-fastest and cut into
+// tiles of TPB consecutive points (no idle lanes at the ragged x end), and z
+// into balanced chunks.  The loop is unrolled twice with two register sets so
+// that the loads of plane k+1 are in flight while plane k is relaxed.
+template <typename T, int TPB>
+__device__ inline void jor_flat(const Geo &g, const Range3 &r, int nbz, int txy, int bz, T omega,
+                                const T *__restrict__ rel, const T *__restrict__ bb, T *__restrict__ out,
+                                const T *g1, const T *g2, const T *g3) {
+    const int64_t f = (int64_t)txy * TPB + threadIdx.x;
+    if (f >= (int64_t)r.n0 * r.n1) return;
+    const int i = r.lo0 + (int)(f % r.n0);
+    const int j = r.lo1 + (int)(f / r.n0);
+    int kc0, kc1;
+    red_chunk(r, bz, nbz, kc0, kc1);
+    const int c0 = g.cst[0], c1 = g.cst[1], c2 = g.cst[2];
+    int64_t dxm = -1, dxp = 1, dym = -(int64_t)g.px, dyp = g.px;
+    if (g.per[0]) {
+        if (i == g.Sp[0]) dxm = g.Np[0] - g.Sp[0];
+        if (i == g.Np[0]) dxp = -(g.Np[0] - g.Sp[0]);
+    }
+    if (g.per[1]) {
+        if (j == g.Sp[1]) dym = (int64_t)(g.Np[1] - g.Sp[1]) * g.px;
+        if (j == g.Np[1]) dyp = -(int64_t)(g.Np[1] - g.Sp[1]) * g.px;
+    }
+    const T a1m = g1[0 * c0 + i], a1p = g1[2 * c0 + i], a1d = g1[1 * c0 + i];
+    const T a2m = g2[0 * c1 + j], a2p = g2[2 * c1 + j], a2d = g2[1 * c1 + j];
+    auto zidx = [&](int k) -> int64_t {
+        if (g.per[2]) {
+            if (k < g.Sp[2]) k += g.Np[2] - g.Sp[2] + 1;
+            else if (k > g.Np[2]) k -= g.Np[2] - g.Sp[2] + 1;
+        }
+        return gidx(g, i, j, k);
+    };
+    struct Plane { T b, xm, xp, ym, yp, zp; };
+    auto load = [&](int k, Plane &P) {
+        const int kk = k <= kc1 ? k : kc1;  // past the chunk: reload its last plane (never used)
+        const int64_t x = gidx(g, i, j, kk);
+        P.b = bb[x];
+        P.xm = rel[x + dxm];
+        P.xp = rel[x + dxp];
+        P.ym = rel[x + dym];
+        P.yp = rel[x + dyp];
+        P.zp = rel[zidx(kk + 1)];
+    };
+    T zm = rel[zidx(kc0 - 1)], zc = rel[zidx(kc0)];
+    auto relax = [&](int k, const Plane &P) {
+        T t = P.b;
+        t = t - a1m * P.xm;
+        t = t - a1p * P.xp;
+        t = t - a2m * P.ym;
+        t = t - a2p * P.yp;
+        t = t - g3[0 * c2 + k] * zm;
+        t = t - g3[2 * c2 + k] * P.zp;
+        const T diag = (a1d + a2d) + g3[1 * c2 + k];
+        out[gidx(g, i, j, k)] = omega * t / diag + (T(1) - omega) * zc;
+        zm = zc;
+        zc = P.zp;
+    };
+    Plane A, B;
+    load(kc0, A);
+    load(kc0 + 1, B);
+    for (int k = kc0; k <= kc1; k += 2) {
+        relax(k, A);
+        load(k + 2, A);
+        if (k + 1 <= kc1) {
+            relax(k + 1, B);
+            load(k + 3, B);
+        }
+    }
+}
+
+template <int TPB>
+__global__ __launch_bounds__(TPB) void k_jor_flat(Geo g, Coef c, Range3 r, int ntxy, int nbz, double omega,
+                                                  const double *rel, const double *bb, double *out) {
+    int txy, dummy, bz;
+    if (!xcd_tile(ntxy, 1, nbz, txy, dummy, bz)) return;
+    jor_flat<double, TPB>(g, r, nbz, txy, bz, omega, rel, bb, out, c.dg[0], c.dg[1], c.dg[2]);
+}
+
+// Batched relaxation: the tiles of many cases (grids of any size) in one
+// launch.  Item q owns the global tiles tile0[q] .. tile0[q+1]-1 (sorted); a
+// block finds its item by binary search, then relaxes that tile exactly as the
+// single-case kernel would, so every case's result is bit-identical to its own
+// sweep.
+template <typename T, int TPB>
+__global__ __launch_bounds__(TPB) void k_jor_batch(const JorBatchItem *__restrict__ items, int nitems, int ntiles,
+                                                   T omega) {
+    int t, dummy, dummy2;
+    if (!xcd_tile(ntiles, 1, 1, t, dummy, dummy2)) return;
+    int lo = 0, hi = nitems - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (items[mid].tile0 <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    const JorBatchItem &it = items[lo];
+    const int local = t - it.tile0;
+    const int txy = local % it.ntxy, bz = local / it.ntxy;
+    if (sizeof(T) == 8)
+        jor_flat<T, TPB>(it.g, it.r, it.nbz, txy, bz, omega, (const T *)it.rel, (const T *)it.bb, (T *)it.out,
+                         (const T *)it.dg[0], (const T *)it.dg[1], (const T *)it.dg[2]);
+    else
+        jor_flat<T, TPB>(it.g, it.r, it.nbz, txy, bz, omega, (const T *)it.rel, (const T *)it.bb, (T *)it.out,
+                         (const T *)it.dgf[0], (const T *)it.dgf[1], (const T *)it.dgf[2]);
+}
+
 __global__ __launch_bounds__(256) void k_jor(Geo g, Coef c, Range3 r, int nbx, int nby, int nbz, double omega,
                                              const double *rel, const double *bb, double *out) {
     jor_column<double>(g, c, r, nbx, nby, nbz, omega, rel, bb, out, c.dg[0], c.dg[1], c.dg[2]);
 }
 
 __global__ __launch_bounds__(256) void k_jor_f32(Geo g, Coef c, Range3 r, int nbx, int nby, int nbz, float omega,
-                                                 const float *rel, const float *bb, float *out) {
+                                                 const float *rel, const float *bb, float *out, const int *active) {
+    if (active && !*active) return;
     jor_column<float>(g, c, r, nbx, nby, nbz, omega, rel, bb, out, c.dgf[0], c.dgf[1], c.dgf[2]);
 }
 
@@ -570,15 +677,24 @@ __global__ void k_fill(double *f, double v, int64_t n) {
     if (t < n) f[t] = v;
 }
 
-__global__ void k_fill_f32(float *f, float v, int64_t n) {
+__global__ void k_fill_f32(float *f, float v, int64_t n, const int *active) {
+    if (active && !*active) return;
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t < n) f[t] = v;
 }
 
-__global__ void k_to_f32(Geo g, Range3 r, const double *src, float *dst) {
+__global__ void k_to_f32(Geo g, Range3 r, const double *src, float *dst, const int *active) {
+    if (active && !*active) return;
     IH_POINT(r);
     const int64_t x = gidx(g, i, j, k);
     dst[x] = (float)src[x];
+}
+
+__global__ void k_from_f32(Geo g, Range3 r, const float *e, double *z, const int *active) {
+    if (active && !*active) return;
+    IH_POINT(r);
+    const int64_t x = gidx(g, i, j, k);
+    z[x] = (double)e[x];
 }
 
 __global__ void k_correct_f32(Geo g, Range3 r, const float *e, double *xx) {
@@ -786,12 +902,39 @@ void launch_jor(hipStream_t s, const Geo &g, const Coef &c, const Box &b, double
 }
 
 void launch_jor_f32(hipStream_t s, const Geo &g, const Coef &c, const Box &b, float omega, const float *rel,
-                    const float *bb, float *out) {
+                    const float *bb, float *out, const int *active) {
     Range3 r = R(b);
     if (empty(r)) return;
     const int nbx = (r.n0 + BX - 1) / BX, nby = (r.n1 + BY - 1) / BY, nbz = (r.n2 + JZ - 1) / JZ;
     hipLaunchKernelGGL(k_jor_f32, dim3(xcd_grid((int64_t)nbx * nby * nbz)), dim3(BX, BY, 1), 0, s, g, c, r, nbx, nby,
-                       nbz, omega, rel, bb, out);
+                       nbz, omega, rel, bb, out, active);
+}
+
+int jor_batch_tiles(const Box &b, int tpb, int zchunk, int &ntxy, int &nbz) {
+    Range3 r = R(b);
+    ntxy = (int)(((int64_t)r.n0 * r.n1 + tpb - 1) / tpb);
+    nbz = (r.n2 + zchunk - 1) / zchunk;
+    return ntxy * nbz;
+}
+
+int launch_jor_batch(hipStream_t s, const JorBatchItem *items, int nitems, int ntiles, int tpb, bool fp32,
+                     double omega) {
+    const dim3 grid(xcd_grid(ntiles));
+#define IH_JB(TPB)                                                                                              \
+    do {                                                                                                        \
+        if (fp32)                                                                                               \
+            hipLaunchKernelGGL((k_jor_batch<float, TPB>), grid, dim3(TPB), 0, s, items, nitems, ntiles, (float)omega); \
+        else                                                                                                    \
+            hipLaunchKernelGGL((k_jor_batch<double, TPB>), grid, dim3(TPB), 0, s, items, nitems, ntiles, omega);   \
+    } while (0)
+    switch (tpb) {
+    case 128: IH_JB(128); break;
+    case 256: IH_JB(256); break;
+    case 512: IH_JB(512); break;
+    default: return -1;
+    }
+#undef IH_JB
+    return 0;
 }
 
 int64_t red_blocks(const Box &b) {
@@ -862,14 +1005,20 @@ void launch_fill(hipStream_t s, double *f, double v, int64_t n) {
     hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, f, v, n);
 }
 
-void launch_fill_f32(hipStream_t s, float *f, float v, int64_t n) {
-    hipLaunchKernelGGL(k_fill_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, f, v, n);
+void launch_fill_f32(hipStream_t s, float *f, float v, int64_t n, const int *active) {
+    hipLaunchKernelGGL(k_fill_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, f, v, n, active);
 }
 
-void launch_to_f32(hipStream_t s, const Geo &g, const Box &b, const double *src, float *dst) {
+void launch_to_f32(hipStream_t s, const Geo &g, const Box &b, const double *src, float *dst, const int *active) {
     Range3 r = R(b);
     if (empty(r)) return;
-    hipLaunchKernelGGL(k_to_f32, grid_for(r), dim3(BX, BY, 1), 0, s, g, r, src, dst);
+    hipLaunchKernelGGL(k_to_f32, grid_for(r), dim3(BX, BY, 1), 0, s, g, r, src, dst, active);
+}
+
+void launch_from_f32(hipStream_t s, const Geo &g, const Box &b, const float *e, double *z, const int *active) {
+    Range3 r = R(b);
+    if (empty(r)) return;
+    hipLaunchKernelGGL(k_from_f32, grid_for(r), dim3(BX, BY, 1), 0, s, g, r, e, z, active);
 }
 
 void launch_correct_f32(hipStream_t s, const Geo &g, const Box &b, const float *e, double *x) {

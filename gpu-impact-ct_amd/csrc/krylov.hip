@@ -21,6 +21,8 @@
 #include "ih_device.h"
 #include "ih_krylov.h"
 
+#include <utility>
+
 namespace ih {
 
 using namespace dev;
@@ -58,7 +60,7 @@ __device__ inline bool g_src(const Geo &g, int d, int &idx) {
 __device__ inline bool wall_plane(const Geo &g, int e, int idx) { return !g.per[e] && (idx == 1 || idx == g.N[e]); }
 
 // dot modes of k_apply
-enum { DOT_NONE = 0, DOT_W = 1, DOT_SELF = 2 };
+enum { DOT_NONE = 0, DOT_W = 1, DOT_SELF = 2, DOT_PAIR = 3 };
 
 // One thread per (x, y) column of a TX x TY tile, the planes of one z chunk
 // (balanced chunks of at most AZ planes, the reduction chunks).  Pressure cross
@@ -69,12 +71,13 @@ enum { DOT_NONE = 0, DOT_W = 1, DOT_SELF = 2 };
 // two register sets alternate).  The gradient coefficients of a thread's
 // positions stay in registers for the whole chunk; every thread forms one G_1,
 // one G_2 and at most one halo gradient per plane without divergent branches.
-// dot_mode DOT_W accumulates <Ap, w>, DOT_SELF <p, Ap> and <Ap, Ap>, per
+// DM = DOT_W accumulates <Ap, w>, DOT_SELF <p, Ap> and <Ap, Ap>, DOT_PAIR
+// <w, Ap> and <Ap, Ap>, per
 // thread in plane order, then over the block with the fixed tree.
-template <int TX, int TY, int AZ>
+template <int TX, int TY, int AZ, int DM>
 __global__ __launch_bounds__(TX *TY) void k_apply(
     Geo g, Coef c, Range3 r, int nbx, int nby, int nbz, const double *__restrict__ src, double *__restrict__ dst,
-    const double *__restrict__ w, int dot_mode, double *part1, double *part2, const int *active) {
+    const double *__restrict__ w, double *part1, double *part2, const int *active) {
     constexpr int NT = TX * TY;
     constexpr int PW = TX + 10, PH = TY + 10, PS = PW * PH;
     constexpr int NHX = 10 * TY;
@@ -107,6 +110,9 @@ __global__ __launch_bounds__(TX *TY) void k_apply(
     const int kc1 = r.lo2 + (int)(((int64_t)(bz + 1) * r.n2) / nbz) - 1;
     const int c0 = g.cst[0], c1 = g.cst[1], c2 = g.cst[2];
     const int64_t pxy = g.pxy;
+    // loads outside the field read c.zero, so no value is masked after its load
+    // (a select on the loaded value would make the wave wait for it at once)
+    const double *zero = c.zero;
 
     for (int e = tid; e < 6 * ZW; e += NT) {
         const int s = e / ZW, u = e % ZW;
@@ -204,8 +210,7 @@ __global__ __launch_bounds__(TX *TY) void k_apply(
         for (int s = 0; s < 11; ++s) {
             int kk = kc0 - 6 + s;
             const bool v = vcol && p_src(g, 2, kk);
-            const double t = src[v ? colb + (int64_t)kk * pxy : hbase];
-            pz[s] = v ? t : 0.0;
+            pz[s] = *(v ? src + colb + (int64_t)kk * pxy : zero);
         }
 #pragma unroll
         for (int t = 0; t < 6; ++t) {
@@ -222,20 +227,17 @@ __global__ __launch_bounds__(TX *TY) void k_apply(
 #pragma unroll
         for (int h = 0; h < HL; ++h) {
             const bool v = vk && hoff[h] >= 0;
-            const double t = src[v ? hbase + hoff[h] + (int64_t)kk * pxy : hbase];
-            hv[h] = v ? t : 0.0;
+            hv[h] = *(v ? src + hbase + hoff[h] + (int64_t)kk * pxy : zero);
         }
     };
     auto load_col = [&](int kk) -> double {
         const bool v = vcol && kk <= kc1 + 5 && p_src(g, 2, kk);
-        const double t = src[v ? colb + (int64_t)kk * pxy : hbase];
-        return v ? t : 0.0;
+        return *(v ? src + colb + (int64_t)kk * pxy : zero);
     };
     auto load_w = [&](int kk) -> double {
-        if (dot_mode != DOT_W) return 0.0;  // w is only a valid pointer in this mode
+        if (DM != DOT_W && DM != DOT_PAIR) return 0.0;  // w is only a valid pointer in these modes
         const bool v = own && kk <= kc1;
-        const double t = w[v ? selfb + (int64_t)kk * pxy : hbase];
-        return v ? t : 0.0;
+        return *(v ? w + selfb + (int64_t)kk * pxy : zero);
     };
     double hvA[HL], hvB[HL];
     load_halo(kc0, hvA);
@@ -255,10 +257,13 @@ __global__ __launch_bounds__(TX *TY) void k_apply(
 #pragma unroll
         for (int t = 0; t < 6; ++t) lap = lap + sCD3[t][kz] * g3[t];
         if (own) dst[selfb + (int64_t)k * pxy] = lap;
-        if (dot_mode == DOT_W) {
+        if (DM == DOT_W) {
             acc1 = acc1 + (own ? lap * wv : 0.0);
-        } else if (dot_mode == DOT_SELF) {
+        } else if (DM == DOT_SELF) {
             acc1 = acc1 + (own ? q[0] * lap : 0.0);
+            acc2 = acc2 + (own ? lap * lap : 0.0);
+        } else if (DM == DOT_PAIR) {
+            acc1 = acc1 + (own ? wv * lap : 0.0);
             acc2 = acc2 + (own ? lap * lap : 0.0);
         }
     };
@@ -318,12 +323,12 @@ __global__ __launch_bounds__(TX *TY) void k_apply(
         const int buf = (n - 1) & 1;
         divergence(kc1, sG[buf], buf == 0 ? wA : wB);
     }
-    if (dot_mode != DOT_NONE) {
+    if (DM != DOT_NONE) {
         const int64_t pi = (int64_t)bx + (int64_t)nbx * ((int64_t)by + (int64_t)nby * bz);
         const double s1 = block_sum(acc1, sm);
         __syncthreads();
         if (tid == 0) part1[pi] = s1;
-        if (dot_mode == DOT_SELF) {
+        if (DM == DOT_SELF || DM == DOT_PAIR) {
             const double s2 = block_sum(acc2, sm + 4);
             if (tid == 0) part2[pi] = s2;
         }
@@ -380,9 +385,11 @@ __global__ void k_s_update(Geo g, Range3 r, double *rr, const double *Ap, const 
     rr[x] = rr[x] - sc[KS_ALPHA] * Ap[x];
 }
 
-__global__ __launch_bounds__(256) void k_x_update(Geo g, Range3 r, double *phi, double *rr, const double *pk,
-                                                  const double *Ar, const double *rh, const double *sc, const int *fl,
-                                                  double *pmax, double *pdot) {
+// phi = phi + omega*ra + alpha*pa; rr = rr - omega*Ar (ra = rr, pa = pk without a
+// preconditioner; ra = z2, pa = z1 with one, src/mod_solvers.f90:1873-1899)
+__global__ __launch_bounds__(256) void k_x_update(Geo g, Range3 r, double *phi, double *rr, const double *ra,
+                                                  const double *pa, const double *Ar, const double *rh,
+                                                  const double *sc, const int *fl, double *pmax, double *pdot) {
     if (!fl[KF_ACTIVE]) return;
     __shared__ double sm[8];
     const double om = sc[KS_OMEGA], alpha = sc[KS_ALPHA];
@@ -394,7 +401,7 @@ __global__ __launch_bounds__(256) void k_x_update(Geo g, Range3 r, double *phi, 
         red_chunk(r, (int)blockIdx.z, (int)gridDim.z, kc0, kc1);
         for (int k = kc0; k <= kc1; ++k) {
             const int64_t x = gidx(g, i, j, k);
-            phi[x] = phi[x] + om * rr[x] + alpha * pk[x];
+            phi[x] = phi[x] + om * ra[x] + alpha * pa[x];
             const double v = rr[x] - om * Ar[x];
             rr[x] = v;
             const double a = fabs(v);
@@ -567,9 +574,19 @@ void launch_apply_A(hipStream_t s, const Geo &g, const Coef &c, const Box &pb, c
     if (empty(r)) return;
     // the operator tile is the reduction tile, so the fused dot partials follow the fixed tree
     const int nbx = (r.n0 + RED_TX - 1) / RED_TX, nby = (r.n1 + RED_TY - 1) / RED_TY, nbz = (r.n2 + RED_KC - 1) / RED_KC;
-    hipLaunchKernelGGL((k_apply<RED_TX, RED_TY, RED_KC>), dim3(xcd_grid((int64_t)nbx * nby * nbz)),
-                       dim3(RED_TX, RED_TY, 1), 0, s, g, c, r, nbx, nby, nbz, src, dst, w, dot_mode, part1, part2,
-                       active);
+    const dim3 grid(xcd_grid((int64_t)nbx * nby * nbz)), block(RED_TX, RED_TY, 1);
+    if (dot_mode == DOT_W)
+        hipLaunchKernelGGL((k_apply<RED_TX, RED_TY, RED_KC, DOT_W>), grid, block, 0, s, g, c, r, nbx, nby, nbz, src,
+                           dst, w, part1, part2, active);
+    else if (dot_mode == DOT_SELF)
+        hipLaunchKernelGGL((k_apply<RED_TX, RED_TY, RED_KC, DOT_SELF>), grid, block, 0, s, g, c, r, nbx, nby, nbz,
+                           src, dst, w, part1, part2, active);
+    else if (dot_mode == DOT_PAIR)
+        hipLaunchKernelGGL((k_apply<RED_TX, RED_TY, RED_KC, DOT_PAIR>), grid, block, 0, s, g, c, r, nbx, nby, nbz,
+                           src, dst, w, part1, part2, active);
+    else
+        hipLaunchKernelGGL((k_apply<RED_TX, RED_TY, RED_KC, DOT_NONE>), grid, block, 0, s, g, c, r, nbx, nby, nbz,
+                           src, dst, w, part1, part2, active);
 }
 
 void launch_krylov_init(hipStream_t s, const Geo &g, const Box &b, const double *bb, double *rr, double *rh,
@@ -580,19 +597,40 @@ void launch_krylov_init(hipStream_t s, const Geo &g, const Box &b, const double 
                        nmax);
 }
 
+// z = M v: `sweeps` FP32 JOR sweeps on A_cdg e = (float)v from e = 0, z = (double)e
+static void launch_precond(hipStream_t s, const Geo &g, const Coef &c, const Box &b, const double *v, double *z,
+                           const KrylovPtrs &k) {
+    const int *act = k.fl + KF_ACTIVE;
+    launch_fill_f32(s, k.e0, 0.0f, g.fe, act);
+    launch_to_f32(s, g, b, v, k.b32, act);
+    float *x = k.e0, *y = k.e1;
+    for (int q = 0; q < k.sweeps; ++q) {
+        launch_jor_f32(s, g, c, b, k.omega, x, k.b32, y, act);
+        std::swap(x, y);
+    }
+    launch_from_f32(s, g, b, x, z, act);
+}
+
 void launch_krylov_iteration(hipStream_t s, const Geo &g, const Coef &c, const Box &b, const KrylovPtrs &k,
                              int from_stage) {
     Range3 r = Rk(b);
     const int64_t nb = red_blocks(b);
+    const bool pre = k.sweeps > 0;
     if (from_stage <= 0) hipLaunchKernelGGL(k_ctl_begin, dim3(1), dim3(1), 0, s, k.sc, k.fl);
     hipLaunchKernelGGL(k_p_update, grid_for(r), dim3(BX, BY, 1), 0, s, g, r, k.rr, k.pk, k.Ap, k.sc, k.fl);
-    launch_apply_A(s, g, c, b, k.pk, k.Ap, k.rh, DOT_W, k.part1, nullptr, k.fl + KF_ACTIVE);
+    if (pre) launch_precond(s, g, c, b, k.pk, k.z1, k);
+    launch_apply_A(s, g, c, b, pre ? k.z1 : k.pk, k.Ap, k.rh, DOT_W, k.part1, nullptr, k.fl + KF_ACTIVE);
     hipLaunchKernelGGL(k_ctl_alpha, dim3(1), dim3(RED_TX, RED_TY, 1), 0, s, k.part1, nb, k.sc, k.fl);
     hipLaunchKernelGGL(k_s_update, grid_for(r), dim3(BX, BY, 1), 0, s, g, r, k.rr, k.Ap, k.sc, k.fl);
-    launch_apply_A(s, g, c, b, k.rr, k.Ar, nullptr, DOT_SELF, k.part1, k.part2, k.fl + KF_ACTIVE);
+    if (pre) {
+        launch_precond(s, g, c, b, k.rr, k.z2, k);
+        launch_apply_A(s, g, c, b, k.z2, k.Ar, k.rr, DOT_PAIR, k.part1, k.part2, k.fl + KF_ACTIVE);
+    } else {
+        launch_apply_A(s, g, c, b, k.rr, k.Ar, nullptr, DOT_SELF, k.part1, k.part2, k.fl + KF_ACTIVE);
+    }
     hipLaunchKernelGGL(k_ctl_omega, dim3(1), dim3(RED_TX, RED_TY, 1), 0, s, k.part1, k.part2, nb, k.sc, k.fl);
-    hipLaunchKernelGGL(k_x_update, red_grid(r), dim3(RED_TX, RED_TY, 1), 0, s, g, r, k.phi, k.rr, k.pk, k.Ar, k.rh,
-                       k.sc, k.fl, k.part1, k.part2);
+    hipLaunchKernelGGL(k_x_update, red_grid(r), dim3(RED_TX, RED_TY, 1), 0, s, g, r, k.phi, k.rr, pre ? k.z2 : k.rr,
+                       pre ? k.z1 : k.pk, k.Ar, k.rh, k.sc, k.fl, k.part1, k.part2);
     hipLaunchKernelGGL(k_ctl_end, dim3(1), dim3(RED_TX, RED_TY, 1), 0, s, k.part1, k.part2, nb, k.sc, k.fl);
 }
 

@@ -19,11 +19,11 @@ import numpy as np
 PKG_ROOT = Path(__file__).resolve().parent.parent
 LIB_PATH = PKG_ROOT / "lib" / "libimpact_hip.so"
 
-IH_ABI_VERSION = 1
+IH_ABI_VERSION = 2
 
 # field ids (include/impact_hip.h)
 (VEL1, VEL2, VEL3, NL1, NL2, NL3, RHS1, RHS2, RHS3, WORK1, WORK2, WORK3, PRE, RES, GPRE, PP, COMP, DIG,
- RR, RH, PK, AP, AR, TMP1, TMP2, TMP3, TMP4, NFIELDS) = range(28)
+ RR, RH, PK, AP, AR, TMP1, TMP2, TMP3, TMP4, Z1, Z2, NFIELDS) = range(30)
 
 
 class IHError(RuntimeError):
@@ -48,11 +48,32 @@ class BicgStats(C.Structure):
 class StepParams(C.Structure):
     _fields_ = [("Re", C.c_double), ("dtime", C.c_double), ("jor_sweeps", C.c_int), ("jor_omega", C.c_double),
                 ("bicg_rel_tol", C.c_double), ("bicg_max_it", C.c_int), ("level_pressure", C.c_int),
-                ("inflow_kind", C.c_int), ("inflow_amp", C.c_double), ("use_mask", C.c_int), ("jor_fp32", C.c_int)]
+                ("inflow_kind", C.c_int), ("inflow_amp", C.c_double), ("use_mask", C.c_int), ("jor_fp32", C.c_int),
+                ("precond_sweeps", C.c_int)]
+
+
+class Request(C.Structure):
+    _fields_ = [("arrival_s", C.c_double), ("N", C.c_int * 3), ("steps", C.c_int), ("radius", C.c_double),
+                ("stenosis", C.c_double)]
+
+
+class RequestResult(C.Structure):
+    _fields_ = [("admit_s", C.c_double), ("start_s", C.c_double), ("finish_s", C.c_double), ("latency_s", C.c_double),
+                ("bicg_iterations", C.c_int), ("checksum_vel", C.c_double), ("checksum_pre", C.c_double),
+                ("status", C.c_int)]
 
 
 class StepStats(C.Structure):
     _fields_ = [("bicg_iterations", C.c_int * 3), ("bicg_final", C.c_double * 3), ("div_max_after", C.c_double)]
+
+
+class ServeConfig(C.Structure):
+    _fields_ = [("device", C.c_int), ("hbm_budget", C.c_int64), ("max_active", C.c_int), ("time_scale", C.c_double),
+                ("step", StepParams)]
+
+
+class ServeSummary(C.Structure):
+    _fields_ = [("makespan_s", C.c_double), ("peak_bytes", C.c_int64), ("peak_active", C.c_int)]
 
 
 _lib = None
@@ -103,6 +124,8 @@ def load(path: os.PathLike | None = None):
         "ih_jor": (C.c_int, [vp, C.c_int, C.c_double, ip, ip, C.c_int, C.c_int]),
         "ih_residual_cdg": (C.c_int, [vp, ip, ip, C.c_int, C.c_int, dp]),
         "ih_bicgstab": (C.c_int, [vp, C.c_double, C.c_int, ip, ip, C.c_int, C.c_int, C.POINTER(BicgStats)]),
+        "ih_bicgstab_mixed": (C.c_int, [vp, C.c_double, C.c_int, ip, ip, C.c_int, C.c_int, C.c_int, C.c_double,
+                                        C.POINTER(BicgStats)]),
         "ih_fill_corners": (C.c_int, [vp, C.c_int]),
         "ih_level_pressure": (C.c_int, [vp, C.c_int]),
         "ih_set_tube_mask": (C.c_int, [vp, C.c_double, C.c_double, dp, dp]),
@@ -117,6 +140,13 @@ def load(path: os.PathLike | None = None):
         "ih_device_axpby": (C.c_int, [vp, C.c_double, C.c_double, dp, dp, dp]),
         "ih_device_interpolate": (C.c_int, [vp, C.c_int, dp, dp]),
         "ih_device_interpolate2": (C.c_int, [vp, C.c_int, dp, dp]),
+        "ih_batch_create": (C.c_int, [C.POINTER(vp), C.c_int, C.POINTER(vp)]),
+        "ih_batch_destroy": (None, [vp]),
+        "ih_batch_autotune": (C.c_int, [vp, C.c_int, ip, ip, dp]),
+        "ih_batch_jor": (C.c_int, [vp, C.c_int, C.c_double, C.c_int, C.c_int]),
+        "ih_grid_bytes": (C.c_int64, [C.POINTER(GridDesc)]),
+        "ih_serve": (C.c_int, [C.POINTER(ServeConfig), C.POINTER(Request), C.c_int, C.POINTER(RequestResult),
+                               C.POINTER(ServeSummary)]),
         "ih_timer_start": (C.c_int, [vp]),
         "ih_timer_stop": (C.c_int, [vp, dp]),
         "ih_kernel_timing_enable": (C.c_int, [vp, C.c_int]),
@@ -310,6 +340,12 @@ class Case:
               "bicgstab")
         return st
 
+    def bicgstab_mixed(self, eps, nmax, ss, nn, sweeps, omega=0.8, bb=RES, phi=PRE) -> BicgStats:
+        st = BicgStats()
+        check(self.lib.ih_bicgstab_mixed(self.h, float(eps), int(nmax), _ints(ss), _ints(nn), bb, phi, int(sweeps),
+                                         float(omega), C.byref(st)), "bicgstab_mixed")
+        return st
+
     def fill_corners(self, f=PRE):
         check(self.lib.ih_fill_corners(self.h, f), "fill_corners")
 
@@ -367,3 +403,55 @@ class Case:
         ms = C.c_double(0.0)
         check(self.lib.ih_timer_stop(self.h, C.byref(ms)), "timer_stop")
         return ms.value
+
+
+class Batch:
+    """Many cases relaxed together (ih_batch_*); the cases must outlive the batch."""
+
+    def __init__(self, cases):
+        self.lib = load()
+        self.cases = list(cases)
+        arr = (C.c_void_p * len(self.cases))(*[c.h for c in self.cases])
+        h = C.c_void_p()
+        check(self.lib.ih_batch_create(arr, len(self.cases), C.byref(h)), "batch_create")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.lib.ih_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def autotune(self, sweeps=10):
+        tpb, zc, ms = C.c_int(), C.c_int(), C.c_double()
+        check(self.lib.ih_batch_autotune(self.h, int(sweeps), C.byref(tpb), C.byref(zc), C.byref(ms)), "autotune")
+        return tpb.value, zc.value, ms.value
+
+    def jor(self, n, omega=0.8, fp32=False, graph=False):
+        check(self.lib.ih_batch_jor(self.h, int(n), float(omega), 1 if fp32 else 0, 1 if graph else 0), "batch_jor")
+
+
+def serve(requests, step: StepParams, hbm_budget: int, max_active: int, time_scale: float = 1.0, device: int = 0):
+    """Run a request trace (list of dicts with arrival_s, N, steps, radius,
+    stenosis) through ih_serve; returns (results, summary, status)."""
+    L = load()
+    n = len(requests)
+    req = (Request * max(n, 1))()
+    for q, r in enumerate(requests):
+        req[q].arrival_s = float(r["arrival_s"])
+        req[q].N = (C.c_int * 3)(*r["N"])
+        req[q].steps = int(r["steps"])
+        req[q].radius = float(r["radius"])
+        req[q].stenosis = float(r["stenosis"])
+    out = (RequestResult * max(n, 1))()
+    cfg = ServeConfig(device=device, hbm_budget=int(hbm_budget), max_active=int(max_active),
+                      time_scale=float(time_scale), step=step)
+    summ = ServeSummary()
+    rc = L.ih_serve(C.byref(cfg), req, n, out, C.byref(summ))
+    return list(out[:n]), summ, rc
+

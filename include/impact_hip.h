@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define IH_ABI_VERSION 1
+#define IH_ABI_VERSION 2
 
 enum {
     IH_OK = 0,
@@ -53,6 +53,7 @@ typedef enum {
     IH_PRE, IH_RES, IH_GPRE, IH_PP, IH_COMP, IH_DIG,
     IH_RR, IH_RH, IH_PK, IH_AP, IH_AR,
     IH_TMP1, IH_TMP2, IH_TMP3, IH_TMP4,
+    IH_Z1, IH_Z2,   /* preconditioned search directions of the mixed-precision BiCGstab */
     IH_NFIELDS
 } ih_field;
 
@@ -152,6 +153,14 @@ int ih_residual_cdg(ih_case *c, const int ss[3], const int nn[3], ih_field rel, 
 /* BiCGstab_orig (preconditioner 0) on product_div_grad; stop at ||r||_inf < eps. */
 int ih_bicgstab(ih_case *c, double eps, int n_it_max, const int ss[3], const int nn[3], ih_field bb, ih_field phi,
                 ih_bicgstab_stats *st);
+/* BiCGstab_orig in its right-preconditioned form (z1 = M pp, Ap = A z1, ...,
+ * z2 = M rr, Ar = A z2, phi += omega z2 + alpha z1; src/mod_solvers.f90:1684-1693,
+ * 1750-1752, 1895-1899) with M = `sweeps` JOR sweeps (omega) on the cdg operator
+ * in FP32 from zero.  Residuals, inner products and the solution update stay
+ * FP64; only the smoothing inside M is FP32.  The reference preconditions with
+ * a multigrid V cycle instead. */
+int ih_bicgstab_mixed(ih_case *c, double eps, int n_it_max, const int ss[3], const int nn[3], ih_field bb,
+                      ih_field phi, int sweeps, double omega, ih_bicgstab_stats *st);
 
 int ih_fill_corners(ih_case *c, ih_field f);
 int ih_level_pressure(ih_case *c, ih_field pre);
@@ -176,6 +185,7 @@ typedef struct {
     double inflow_amp;
     int use_mask;
     int jor_fp32;   /* 1: JOR sweeps in FP32 (residual and update kept FP64) */
+    int precond_sweeps; /* > 0: BiCGstab right-preconditioned by this many FP32 JOR sweeps */
 } ih_step_params;
 
 typedef struct {
@@ -200,6 +210,72 @@ int ih_device_helmholtz(ih_case *c, int component_id, double multL, double *vel_
 int ih_device_axpby(ih_case *c, double a, double b, double *x, double *y, double *z);
 int ih_device_interpolate(ih_case *c, int dimension_id, double *phi, double *inter);
 int ih_device_interpolate2(ih_case *c, int dimension_id, double *phi, double *inter);
+
+/* ---- batched relaxation of many cases ------------------------------------ */
+/* Many cases (uneven grids, one device) relaxed together: each sweep is one
+ * launch over the tiles of all cases.  No reference counterpart: the reference
+ * relaxes one block per call (device_jacobi_solver); this serves callers that
+ * advance many patient geometries at once. */
+typedef struct ih_batch ih_batch;
+int  ih_batch_create(ih_case *const *cases, int n, ih_batch **out);
+void ih_batch_destroy(ih_batch *b);
+/* Times every tile shape (threads per tile 128/256/512 x z chunk 8/16/32) on
+ * `sweeps` sweeps without changing PRE, keeps the fastest, reports it and its
+ * time per sweep. */
+int  ih_batch_autotune(ih_batch *b, int sweeps, int *tpb, int *zchunk, double *ms_per_sweep);
+/* n_sweeps JOR sweeps (omega) of PRE on RES in every case, each bit-identical to
+ * ih_jor on that case.  fp32 != 0: FP32 defect correction instead (r = RES -
+ * A_cdg PRE in FP64, n_sweeps FP32 sweeps on A e = r from e = 0, PRE += e).
+ * use_graph != 0: the sweeps are captured once into a hipGraph and replayed. */
+int  ih_batch_jor(ih_batch *b, int n_sweeps, double omega, int fp32, int use_graph);
+
+/* ---- serving: a trace of simulation requests ------------------------------ */
+/* No reference counterpart (the reference runs one simulation per MPI job).
+ * Requests arrive over time; each is a stenosed tube (walls, L = 2 x 1 x 1,
+ * tanh 1.5 in y and z, axis at (0.5, 0.5), cosine constriction of half-length
+ * 0.4 at x = 1) started from plug flow at rest pressure and advanced `steps`
+ * time steps with `step`.  Requests are admitted first come first served while
+ * the resident cases fit in hbm_budget and fewer than max_active run; each runs
+ * on its own case and stream from its own host thread, so the kernels of the
+ * admitted requests overlap on the device.  Every result is independent of the
+ * mix it ran in: checksum_* are <u,u> and <p,p> over the pressure range, in the
+ * fixed reduction order. */
+typedef struct {
+    double arrival_s;   /* after the start of the trace (scaled by time_scale) */
+    int N[3];           /* grid points */
+    int steps;
+    double radius;      /* tube radius */
+    double stenosis;    /* fraction of the area removed at the throat, 0..1 */
+} ih_request;
+
+typedef struct {
+    double admit_s, start_s, finish_s; /* after the start of the trace */
+    double latency_s;                  /* finish - arrival */
+    int bicg_iterations;               /* over all steps */
+    double checksum_vel, checksum_pre;
+    int status;                        /* IH_OK, IH_ENOMEM if it can never fit, or the failing call's status */
+} ih_request_result;
+
+typedef struct {
+    int device;
+    int64_t hbm_budget;  /* bytes the resident cases may hold together */
+    int max_active;      /* simulations running at once */
+    double time_scale;   /* arrival times are multiplied by this (1 = as given) */
+    ih_step_params step;
+} ih_serve_config;
+
+typedef struct {
+    double makespan_s;
+    int64_t peak_bytes;  /* largest sum of admitted cases' HBM */
+    int peak_active;
+} ih_serve_summary;
+
+/* HBM a case of grid g holds (fields, FP32 buffers, mask, tables). */
+int64_t ih_grid_bytes(const ih_grid_desc *g);
+/* Runs the trace to completion; out[q] belongs to req[q].  Returns IH_OK or the
+ * status of a failed request. */
+int ih_serve(const ih_serve_config *cfg, const ih_request *req, int n, ih_request_result *out,
+             ih_serve_summary *summary);
 
 /* ---- timing of the case's stream (for benchmarks) ------------------------ */
 int ih_timer_start(ih_case *c);

@@ -97,6 +97,13 @@ typedef struct {
 } orc_bicgstab_stats;
 void orc_bicgstab(const orc_case *c, double eps, int n_it_max, const int ss[3], const int nn[3],
                   const double *bb, double *phi, double *work /* 7 fields */, orc_bicgstab_stats *st);
+/* Mixed precision (the device's FP32 smoothing; no reference counterpart):
+ * BiCGstab in its right-preconditioned form with M = `sweeps` FP32 JOR sweeps
+ * (omega) on cdg from zero, and FP32 JOR as defect correction of rel. */
+void orc_bicgstab_mixed(const orc_case *c, double eps, int n_it_max, const int ss[3], const int nn[3],
+                        const double *bb, double *phi, int sweeps, double omega, orc_bicgstab_stats *st);
+void orc_jor_mixed(const orc_case *c, int n_sweeps, double omega, const int ss[3], const int nn[3], double *rel,
+                   const double *bb);
 
 /* ---------------- projection step ---------------- */
 typedef struct {
@@ -109,6 +116,8 @@ typedef struct {
     int inflow_kind;       /* 0: none, 1: parabola(y,z), 2: pulsatile plug inside mask */
     double inflow_amp;
     const uint8_t *mask3;  /* optional: 3 masks (u,v,w grids), 1 = fluid; NULL = none */
+    int jor_fp32;          /* 1: the JOR sweeps as FP32 defect correction */
+    int precond_sweeps;    /* > 0: BiCGstab preconditioned by this many FP32 JOR sweeps */
 } orc_step_params;
 
 typedef struct {

@@ -34,6 +34,17 @@ struct orc_case {
     int red_mode;
 };
 
+/* ORC_OMP builds (bench baseline only) run the point loops in parallel over k;
+ * every point is written by one thread and no sum is split, so results are
+ * identical to the serial build. */
+#ifdef ORC_OMP
+#define ORC_PAR _Pragma("omp parallel for schedule(static)")
+#define ORC_PAR_MAX_NORM _Pragma("omp parallel for schedule(static) reduction(max : norm_inf)")
+#else
+#define ORC_PAR
+#define ORC_PAR_MAX_NORM
+#endif
+
 #define ORC_IDX(c, i, j, k) ((int64_t)((i) + 3) + (int64_t)((j) + 3) * (c)->nx + (int64_t)((k) + 3) * (c)->nxy)
 #define C7(a, ii, i) ((a)[((ii) + 3) + 7 * (i)])
 #define CG6(a, ii, i) ((a)[((ii) + 2) + 6 * (i)])

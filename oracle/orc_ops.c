@@ -113,6 +113,7 @@ void orc_divergence2(const orc_case *c, double *vel, double *div)
     orc_exchange(c, 2, 2, v);
     orc_exchange(c, 3, 3, w);
     const double *c1 = c->cD[0], *c2 = c->cD[1], *c3 = c->cD[2];
+    ORC_PAR
     for (int k = c->Sp[2]; k <= c->Np[2]; ++k)
         for (int j = c->Sp[1]; j <= c->Np[1]; ++j)
             for (int i = c->Sp[0]; i <= c->Np[0]; ++i) {
@@ -130,6 +131,7 @@ void orc_gradient(const orc_case *c, int m, double *phi, double *grad)
     const int q = m - 1;
     orc_exchange(c, m, 0, phi);
     const double *cg = c->cG[q];
+    ORC_PAR
     for (int k = c->SI[q][2]; k <= c->NI[q][2]; ++k)
         for (int j = c->SI[q][1]; j <= c->NI[q][1]; ++j)
             for (int i = c->SI[q][0]; i <= c->NI[q][0]; ++i) {
@@ -170,6 +172,7 @@ void orc_interpolate2_vel_pre(const orc_case *c, int m, double *phi, double *int
 {
     const int q = m - 1;
     const double *ci = c->cIup[q];
+    ORC_PAR
     for (int k = c->Sp[2]; k <= c->Np[2]; ++k)
         for (int j = c->Sp[1]; j <= c->Np[1]; ++j)
             for (int i = c->Sp[0]; i <= c->Np[0]; ++i) {
@@ -193,6 +196,7 @@ void orc_interpolate2_pre_vel(const orc_case *c, int m, double *phi, double *int
 {
     const int q = m - 1;
     const double *ci = c->cIpu[q];
+    ORC_PAR
     for (int k = c->SI[q][2]; k <= c->NI[q][2]; ++k)
         for (int j = c->SI[q][1]; j <= c->NI[q][1]; ++j)
             for (int i = c->SI[q][0]; i <= c->NI[q][0]; ++i) {
@@ -234,6 +238,7 @@ void orc_nonlinear(const orc_case *c, double *vel, double *work, double *nl, dou
             orc_interpolate2_pre_vel(c, q + 1, work + (int64_t)e * c->ntot, pp);
             const double *cU = (e == q) ? c->cNu[e][1] : c->cNp[e][1];
             const double *cDn = (e == q) ? c->cNu[e][0] : c->cNp[e][0];
+            ORC_PAR
             for (int k = c->SI[q][2]; k <= c->NI[q][2]; ++k)
                 for (int j = c->SI[q][1]; j <= c->NI[q][1]; ++j)
                     for (int i = c->SI[q][0]; i <= c->NI[q][0]; ++i) {
@@ -269,6 +274,7 @@ void orc_helmholtz_explicit(const orc_case *c, double multL, double *vel, double
         const double *c1 = (q == 0) ? c->cuu[0] : c->cpp[0];
         const double *c2 = (q == 1) ? c->cuu[1] : c->cpp[1];
         const double *c3 = (q == 2) ? c->cuu[2] : c->cpp[2];
+        ORC_PAR
         for (int k = c->SI[q][2]; k <= c->NI[q][2]; ++k)
             for (int j = c->SI[q][1]; j <= c->NI[q][1]; ++j)
                 for (int i = c->SI[q][0]; i <= c->NI[q][0]; ++i) {
@@ -289,6 +295,7 @@ void orc_product_div_grad(const orc_case *c, double *phi, double *lap, double *d
         orc_gradient(c, m, phi, dig);
         orc_exchange(c, m, m, dig);
         const double *cd = c->cD[m - 1];
+        ORC_PAR
         for (int k = c->Sp[2]; k <= c->Np[2]; ++k)
             for (int j = c->Sp[1]; j <= c->Np[1]; ++j)
                 for (int i = c->Sp[0]; i <= c->Np[0]; ++i) {

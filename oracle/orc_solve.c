@@ -26,6 +26,7 @@ void orc_jor(const orc_case *c, int n_sweeps, double omega, const int ss[3], con
     const double *g1 = c->cdg[0], *g2 = c->cdg[1], *g3 = c->cdg[2];
     for (int r = 0; r < n_sweeps; ++r) {
         orc_exchange_relax(c, rel);
+        ORC_PAR
         for (int k = ss[2]; k <= nn[2]; ++k)
             for (int j = ss[1]; j <= nn[1]; ++j)
                 for (int i = ss[0]; i <= nn[0]; ++i) {
@@ -39,6 +40,7 @@ void orc_jor(const orc_case *c, int n_sweeps, double omega, const int ss[3], con
                     const double diag = (CDG(g1, 0, i) + CDG(g2, 0, j)) + CDG(g3, 0, k);
                     A(comp, i, j, k) = omega * t / diag + (1.0 - omega) * A(rel, i, j, k);
                 }
+        ORC_PAR
         for (int k = ss[2]; k <= nn[2]; ++k)
             for (int j = ss[1]; j <= nn[1]; ++j)
                 for (int i = ss[0]; i <= nn[0]; ++i) A(rel, i, j, k) = A(comp, i, j, k);
@@ -69,6 +71,94 @@ double orc_residual_cdg(const orc_case *c, const int ss[3], const int nn[3], dou
     return m;
 }
 
+/* ------------------------------------------------------------------------ */
+/* Mixed precision (no reference counterpart; the device's FP32 smoothing):  */
+/* the JOR loop above in binary32, with cdg, omega and the right-hand side   */
+/* rounded to float, out of place, periodic neighbours read across the       */
+/* range (the values exchange_relax would place in the ghost layer).  e and  */
+/* tmp are float fields whose ghost cells hold 0; returns the buffer that    */
+/* holds the result.                                                         */
+/* ------------------------------------------------------------------------ */
+static float *jor_f32(const orc_case *c, int n_sweeps, float omega, const int ss[3], const int nn[3], float *e,
+                      float *tmp, const float *b)
+{
+    const double *g1 = c->cdg[0], *g2 = c->cdg[1], *g3 = c->cdg[2];
+    const int p0 = c->bc[0][0] == -1, p1 = c->bc[1][0] == -1, p2 = c->bc[2][0] == -1;
+    for (int r = 0; r < n_sweeps; ++r) {
+        ORC_PAR
+        for (int k = ss[2]; k <= nn[2]; ++k)
+            for (int j = ss[1]; j <= nn[1]; ++j)
+                for (int i = ss[0]; i <= nn[0]; ++i) {
+                    const int im = (p0 && i == c->Sp[0]) ? c->Np[0] : i - 1;
+                    const int ip = (p0 && i == c->Np[0]) ? c->Sp[0] : i + 1;
+                    const int jm = (p1 && j == c->Sp[1]) ? c->Np[1] : j - 1;
+                    const int jp = (p1 && j == c->Np[1]) ? c->Sp[1] : j + 1;
+                    const int km = (p2 && k == c->Sp[2]) ? c->Np[2] : k - 1;
+                    const int kp = (p2 && k == c->Np[2]) ? c->Sp[2] : k + 1;
+                    float t = A(b, i, j, k);
+                    t = t - (float)CDG(g1, -1, i) * A(e, im, j, k);
+                    t = t - (float)CDG(g1, 1, i) * A(e, ip, j, k);
+                    t = t - (float)CDG(g2, -1, j) * A(e, i, jm, k);
+                    t = t - (float)CDG(g2, 1, j) * A(e, i, jp, k);
+                    t = t - (float)CDG(g3, -1, k) * A(e, i, j, km);
+                    t = t - (float)CDG(g3, 1, k) * A(e, i, j, kp);
+                    const float diag = ((float)CDG(g1, 0, i) + (float)CDG(g2, 0, j)) + (float)CDG(g3, 0, k);
+                    A(tmp, i, j, k) = omega * t / diag + (1.0f - omega) * A(e, i, j, k);
+                }
+        float *sw = e;
+        e = tmp;
+        tmp = sw;
+    }
+    return e;
+}
+
+/* defect correction: r = bb - A_cdg rel (FP64), A e = r by FP32 sweeps from
+ * e = 0, rel += e (FP64); ghost layer of rel refreshed before and after */
+void orc_jor_mixed(const orc_case *c, int n_sweeps, double omega, const int ss[3], const int nn[3], double *rel,
+                   const double *bb)
+{
+    const double *g1 = c->cdg[0], *g2 = c->cdg[1], *g3 = c->cdg[2];
+    float *b = calloc((size_t)c->ntot, sizeof(float));
+    float *e = calloc((size_t)c->ntot, sizeof(float));
+    float *tmp = calloc((size_t)c->ntot, sizeof(float));
+    orc_exchange_relax(c, rel);
+    for (int k = ss[2]; k <= nn[2]; ++k)
+        for (int j = ss[1]; j <= nn[1]; ++j)
+            for (int i = ss[0]; i <= nn[0]; ++i) {
+                double t = A(bb, i, j, k);
+                t = t - CDG(g1, -1, i) * A(rel, i - 1, j, k);
+                t = t - CDG(g1, 1, i) * A(rel, i + 1, j, k);
+                t = t - CDG(g2, -1, j) * A(rel, i, j - 1, k);
+                t = t - CDG(g2, 1, j) * A(rel, i, j + 1, k);
+                t = t - CDG(g3, -1, k) * A(rel, i, j, k - 1);
+                t = t - CDG(g3, 1, k) * A(rel, i, j, k + 1);
+                t = t - ((CDG(g1, 0, i) + CDG(g2, 0, j)) + CDG(g3, 0, k)) * A(rel, i, j, k);
+                A(b, i, j, k) = (float)t;
+            }
+    const float *res = jor_f32(c, n_sweeps, (float)omega, ss, nn, e, tmp, b);
+    for (int k = ss[2]; k <= nn[2]; ++k)
+        for (int j = ss[1]; j <= nn[1]; ++j)
+            for (int i = ss[0]; i <= nn[0]; ++i) A(rel, i, j, k) = A(rel, i, j, k) + (double)A(res, i, j, k);
+    orc_exchange_relax(c, rel);
+    free(b);
+    free(e);
+    free(tmp);
+}
+
+/* z = M v, M = n_sweeps FP32 JOR sweeps on A_cdg e = (float)v from e = 0 */
+static void precond(const orc_case *c, int n_sweeps, float omega, const int ss[3], const int nn[3], const double *v,
+                    double *z, float *b, float *e, float *tmp)
+{
+    for (int k = ss[2]; k <= nn[2]; ++k)
+        for (int j = ss[1]; j <= nn[1]; ++j)
+            for (int i = ss[0]; i <= nn[0]; ++i) A(b, i, j, k) = (float)A(v, i, j, k);
+    memset(e, 0, sizeof(float) * (size_t)c->ntot);
+    const float *res = jor_f32(c, n_sweeps, omega, ss, nn, e, tmp, b);
+    for (int k = ss[2]; k <= nn[2]; ++k)
+        for (int j = ss[1]; j <= nn[1]; ++j)
+            for (int i = ss[0]; i <= nn[0]; ++i) A(z, i, j, k) = (double)A(res, i, j, k);
+}
+
 static double absmax(const orc_case *c, const int ss[3], const int nn[3], const double *a)
 {
     double m = 0.0;
@@ -86,8 +176,9 @@ static double absmax(const orc_case *c, const int ss[3], const int nn[3], const 
 /* src/mod_solvers.f90:1509-1935 with product_scalar (:2319-2354) and       */
 /* status_iteration (:2482-2515).                                            */
 /* ------------------------------------------------------------------------ */
-void orc_bicgstab(const orc_case *c, double eps, int n_it_max, const int ss[3], const int nn[3], const double *bb,
-                  double *phi, double *work, orc_bicgstab_stats *st)
+static void bicgstab_run(const orc_case *c, double eps, int n_it_max, const int ss[3], const int nn[3],
+                         const double *bb, double *phi, double *work, int sweeps, double pre_omega,
+                         orc_bicgstab_stats *st)
 {
     const int64_t n = c->ntot;
     double *own = NULL;
@@ -97,6 +188,17 @@ void orc_bicgstab(const orc_case *c, double eps, int n_it_max, const int ss[3], 
     }
     double *rr = work, *rh = work + n, *pk = work + 2 * n, *Ap = work + 3 * n, *Ar = work + 4 * n,
            *dig = work + 5 * n;
+    /* right preconditioner (src/mod_solvers.f90:1684-1693, 1750-1752, 1895-1899) */
+    double *z1 = NULL, *z2 = NULL;
+    float *fb = NULL, *fe = NULL, *ft = NULL;
+    const float omf = (float)pre_omega;
+    if (sweeps > 0) {
+        z1 = calloc((size_t)n, sizeof(double));
+        z2 = calloc((size_t)n, sizeof(double));
+        fb = calloc((size_t)n, sizeof(float));
+        fe = calloc((size_t)n, sizeof(float));
+        ft = calloc((size_t)n, sizeof(float));
+    }
 
     orc_product_div_grad(c, phi, rr, dig);
     for (int k = ss[2]; k <= nn[2]; ++k)
@@ -142,6 +244,7 @@ void orc_bicgstab(const orc_case *c, double eps, int n_it_max, const int ss[3], 
             }
             beta = (alpha / omega) * (rhr / rhr_prev);
             omega = -beta * omega;
+            ORC_PAR
             for (int k = ss[2]; k <= nn[2]; ++k)
                 for (int j = ss[1]; j <= nn[1]; ++j)
                     for (int i = ss[0]; i <= nn[0]; ++i)
@@ -151,17 +254,28 @@ void orc_bicgstab(const orc_case *c, double eps, int n_it_max, const int ss[3], 
                 for (int j = ss[1]; j <= nn[1]; ++j)
                     for (int i = ss[0]; i <= nn[0]; ++i) A(pk, i, j, k) = A(rr, i, j, k);
         }
-        orc_product_div_grad(c, pk, Ap, dig);
+        if (sweeps > 0) {
+            precond(c, sweeps, omf, ss, nn, pk, z1, fb, fe, ft);
+            orc_product_div_grad(c, z1, Ap, dig);
+        } else {
+            orc_product_div_grad(c, pk, Ap, dig);
+        }
         const double rhAp = orc_dot(c, ss, nn, Ap, rh);
         if (fabs(rhAp) == 0.0) {
             if (st) st->exit_reason = 2;
             break;
         }
         alpha = rhr / rhAp;
+        ORC_PAR
         for (int k = ss[2]; k <= nn[2]; ++k)
             for (int j = ss[1]; j <= nn[1]; ++j)
                 for (int i = ss[0]; i <= nn[0]; ++i) A(rr, i, j, k) = A(rr, i, j, k) - alpha * A(Ap, i, j, k);
-        orc_product_div_grad(c, rr, Ar, dig);
+        if (sweeps > 0) {
+            precond(c, sweeps, omf, ss, nn, rr, z2, fb, fe, ft);
+            orc_product_div_grad(c, z2, Ar, dig);
+        } else {
+            orc_product_div_grad(c, rr, Ar, dig);
+        }
         const double rAr = orc_dot(c, ss, nn, rr, Ar);
         const double ArAr = orc_dot(c, ss, nn, Ar, Ar);
         if (fabs(ArAr) == 0.0) {
@@ -170,10 +284,14 @@ void orc_bicgstab(const orc_case *c, double eps, int n_it_max, const int ss[3], 
         }
         omega = rAr / ArAr;
         norm_inf = 0.0;
+        ORC_PAR_MAX_NORM
         for (int k = ss[2]; k <= nn[2]; ++k)
             for (int j = ss[1]; j <= nn[1]; ++j)
                 for (int i = ss[0]; i <= nn[0]; ++i) {
-                    A(phi, i, j, k) = A(phi, i, j, k) + omega * A(rr, i, j, k) + alpha * A(pk, i, j, k);
+                    if (sweeps > 0)
+                        A(phi, i, j, k) = A(phi, i, j, k) + omega * A(z2, i, j, k) + alpha * A(z1, i, j, k);
+                    else
+                        A(phi, i, j, k) = A(phi, i, j, k) + omega * A(rr, i, j, k) + alpha * A(pk, i, j, k);
                     A(rr, i, j, k) = A(rr, i, j, k) - omega * A(Ar, i, j, k);
                     const double a = fabs(A(rr, i, j, k));
                     norm_inf = (a > norm_inf) ? a : norm_inf;
@@ -184,6 +302,23 @@ void orc_bicgstab(const orc_case *c, double eps, int n_it_max, const int ss[3], 
         st->norm_inf_final = norm_inf;
     }
     free(own);
+    free(z1);
+    free(z2);
+    free(fb);
+    free(fe);
+    free(ft);
+}
+
+void orc_bicgstab(const orc_case *c, double eps, int n_it_max, const int ss[3], const int nn[3], const double *bb,
+                  double *phi, double *work, orc_bicgstab_stats *st)
+{
+    bicgstab_run(c, eps, n_it_max, ss, nn, bb, phi, work, 0, 0.0, st);
+}
+
+void orc_bicgstab_mixed(const orc_case *c, double eps, int n_it_max, const int ss[3], const int nn[3],
+                        const double *bb, double *phi, int sweeps, double omega, orc_bicgstab_stats *st)
+{
+    bicgstab_run(c, eps, n_it_max, ss, nn, bb, phi, NULL, sweeps, omega, st);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -398,11 +533,15 @@ void orc_timestep(orc_case *c, const orc_step_params *p, int step_index, double 
         /* explicit */
         for (int q = 0; q < 3; ++q) range_op(c, q, vel + q * n, rhs + q * n, 0.0, 0);
         orc_divergence2(c, vel, res);
-        if (p->jor_sweeps > 0)
-            orc_jor(c, p->jor_sweeps, p->jor_omega, ss, nn, pre, res, comp);
+        if (p->jor_sweeps > 0) {
+            if (p->jor_fp32)
+                orc_jor_mixed(c, p->jor_sweeps, p->jor_omega, ss, nn, pre, res);
+            else
+                orc_jor(c, p->jor_sweeps, p->jor_omega, ss, nn, pre, res, comp);
+        }
         orc_bicgstab_stats bst;
         const double eps = p->bicg_rel_tol * absmax(c, ss, nn, res);
-        orc_bicgstab(c, eps, p->bicg_max_it, ss, nn, res, pre, bw, &bst);
+        bicgstab_run(c, eps, p->bicg_max_it, ss, nn, res, pre, bw, p->precond_sweeps, p->jor_omega, &bst);
         if (st) {
             st->bicg_iterations[s] = bst.iterations;
             st->bicg_final[s] = bst.norm_inf_final;

@@ -15,22 +15,33 @@ import numpy as np
 ROOT = Path(__file__).resolve().parent.parent
 ORACLE_DIR = ROOT / "oracle"
 LIB_PATH = ORACLE_DIR / "_build" / "liboracle.so"
+OMP_LIB_PATH = ORACLE_DIR / "_build" / "liboracle_omp.so"  # same code, point loops on OpenMP threads
 
 _lib = None
+_path = LIB_PATH
+
+
+def use_parallel_build():
+    """Load the OpenMP build instead (identical results, for timing the CPU
+    baseline); must be called before the first OracleCase."""
+    global _path
+    if _lib is not None and _path != OMP_LIB_PATH:
+        raise RuntimeError("the serial oracle is already loaded")
+    _path = OMP_LIB_PATH
 
 
 def build(force: bool = False) -> Path:
-    if force or not LIB_PATH.exists():
+    if force or not _path.exists():
         subprocess.run(["make", "-s", "-C", str(ORACLE_DIR)], check=True)
-    return LIB_PATH
+    return _path
 
 
 def lib():
     global _lib
     if _lib is None:
-        if not LIB_PATH.exists():
+        if not _path.exists():
             build()
-        L = C.CDLL(str(LIB_PATH))
+        L = C.CDLL(str(_path))
         dp = C.POINTER(C.c_double)
         ip = C.POINTER(C.c_int)
         vp = C.c_void_p
@@ -63,6 +74,9 @@ def lib():
         L.orc_residual_cdg.restype = C.c_double
         L.orc_residual_cdg.argtypes = [vp, ip, ip, dp, dp]
         L.orc_bicgstab.argtypes = [vp, C.c_double, C.c_int, ip, ip, dp, dp, dp, C.POINTER(BicgStats)]
+        L.orc_bicgstab_mixed.argtypes = [vp, C.c_double, C.c_int, ip, ip, dp, dp, C.c_int, C.c_double,
+                                         C.POINTER(BicgStats)]
+        L.orc_jor_mixed.argtypes = [vp, C.c_int, C.c_double, ip, ip, dp, dp]
         L.orc_timestep.argtypes = [vp, C.POINTER(StepParams), C.c_int, C.c_double, dp, dp, dp, C.POINTER(StepStats)]
         L.orc_tube_mask.argtypes = [vp, C.c_double, C.c_double, dp, dp, C.POINTER(C.c_uint8)]
         L.orc_apply_mask.argtypes = [vp, C.POINTER(C.c_uint8), dp]
@@ -83,7 +97,7 @@ class StepParams(C.Structure):
     _fields_ = [("Re", C.c_double), ("dtime", C.c_double), ("jor_sweeps", C.c_int),
                 ("jor_omega", C.c_double), ("bicg_rel_tol", C.c_double), ("bicg_max_it", C.c_int),
                 ("level_pressure", C.c_int), ("inflow_kind", C.c_int), ("inflow_amp", C.c_double),
-                ("mask3", C.POINTER(C.c_uint8))]
+                ("mask3", C.POINTER(C.c_uint8)), ("jor_fp32", C.c_int), ("precond_sweeps", C.c_int)]
 
 
 class StepStats(C.Structure):
@@ -206,6 +220,15 @@ class OracleCase:
         st = BicgStats()
         self.L.orc_bicgstab(self.h, eps, nmax, _ip(ss), _ip(nn), _dp(bb), _dp(phi), None, C.byref(st))
         return st
+
+    def bicgstab_mixed(self, eps, nmax, ss, nn, bb, phi, sweeps, omega=0.8):
+        st = BicgStats()
+        self.L.orc_bicgstab_mixed(self.h, eps, nmax, _ip(ss), _ip(nn), _dp(bb), _dp(phi), int(sweeps),
+                                  float(omega), C.byref(st))
+        return st
+
+    def jor_mixed(self, n, omega, ss, nn, rel, bb):
+        self.L.orc_jor_mixed(self.h, n, omega, _ip(ss), _ip(nn), _dp(rel), _dp(bb))
 
     def timestep(self, params: StepParams, step, time, vel3, pre, nl3):
         st = StepStats()

@@ -153,3 +153,48 @@ def test_rk3_amplification_of_a_shear_mode():
     c.timestep(p, 0, 0.0, vel, pre, nl)
     np.testing.assert_allclose(vel[0][s], R * u0[s], rtol=0, atol=1e-12 * A)
     assert np.abs(vel[1][s]).max() < 1e-12 and np.abs(vel[2][s]).max() < 1e-12
+
+
+def _random_rhs(c, seed):
+    """A right-hand side in the range of product_div_grad (so the solve is consistent)."""
+    rng = np.random.default_rng(seed)
+    xt = np.ascontiguousarray(rng.standard_normal(c.shape))
+    bb = c.zeros()
+    c.product_div_grad(xt, bb, c.zeros())
+    return bb
+
+
+@pytest.mark.parametrize("bc", [(1, 1, 1, 1, 1, 1), (-1, -1, 1, 1, -1, -1)])
+def test_mixed_bicgstab_reaches_the_fp64_solution(bc):
+    """Right-preconditioned BiCGstab with FP32 JOR sweeps as M stops on the
+    FP64 residual, so it reaches the same tolerance as the plain solver, in
+    fewer iterations.  (The two solutions need not agree: product_div_grad has
+    a null space beyond the constants, and they differ within it.)"""
+    c = O.OracleCase((21, 17, 13), bc, (1.5, 1.0, 1.0), (0.0, 1.3, 0.0))
+    bb = _random_rhs(c, 21)
+    s = _box(c.Sp, c.Np)
+    eps = 1e-10 * np.abs(bb[s]).max()
+    x64, x32 = c.zeros(), c.zeros()
+    st64 = c.bicgstab(eps, 3000, c.Sp, c.Np, bb, x64)
+    st32 = c.bicgstab_mixed(eps, 3000, c.Sp, c.Np, bb, x32, 4, 0.8)
+    assert st64.exit_reason == 0 and st32.exit_reason == 0
+    assert st32.iterations < st64.iterations
+    for x in (x64, x32):
+        lap = c.zeros()
+        c.product_div_grad(x.copy(), lap, c.zeros())
+        assert np.abs(lap[s] - bb[s]).max() <= eps * 1.0001
+
+
+def test_fp32_jor_correction_tracks_fp64_jor():
+    """FP32 defect correction from e = 0 equals the FP64 sweeps up to binary32
+    rounding: the first FP32 sweep is the FP64 sweep rounded."""
+    c = O.OracleCase((19, 15, 13), (1, 1, 1, 1, 1, 1), (1.5, 1.0, 1.0), (0.0, 1.3, 1.3))
+    rng = np.random.default_rng(22)
+    bb = np.ascontiguousarray(rng.standard_normal(c.shape))
+    s = _box(c.Sp, c.Np)
+    for n in (1, 30):
+        x64, x32 = c.zeros(), c.zeros()
+        c.jor(n, 0.8, c.Sp, c.Np, x64, bb, c.zeros())
+        c.jor_mixed(n, 0.8, c.Sp, c.Np, x32, bb)
+        err = np.linalg.norm(x32[s] - x64[s]) / np.linalg.norm(x64[s])
+        assert 0 < err < 5e-6 * n, (n, err)

@@ -73,7 +73,7 @@ def read_traffic():
         return None
 
 
-def setup_case(ih, mixed=False, precond_sweeps=4):
+def setup_case(ih, mixed=False, precond_sweeps=8):
     """The bench case; mixed=True runs the Poisson solve in the mixed-precision
     form (FP32 JOR defect correction and an FP32-JOR-preconditioned BiCGstab,
     residuals, inner products and updates in FP64)."""
@@ -161,7 +161,7 @@ def batched_relaxation(ih, n_cases=32, sweeps=200, seed=4):
         cases.append(c)
     pts = sum(c.N[0] * c.N[1] * c.N[2] for c in cases)
     bt = ih.Batch(cases)
-    tpb, zc, _ = bt.autotune(5)
+    tpb, zc, layout, _ = bt.autotune(5)
     bt.jor(sweeps, JOR_OMEGA, graph=True)  # capture + warm
     cases[0].sync()
     t0 = time.perf_counter()
@@ -181,7 +181,7 @@ def batched_relaxation(ih, n_cases=32, sweeps=200, seed=4):
         "grids": "random from 65^3, 97^3, 129x65x65, 193x97x97 points",
         "points": pts,
         "sweeps": sweeps,
-        "tile": {"threads": tpb, "z_chunk": zc},
+        "tile": {"threads": tpb, "z_chunk": zc, "layout": "rows" if layout else "flat"},
         "ms_per_sweep": t_batch / sweeps * 1e3,
         "hbm_gbs": JOR_BYTES_PER_POINT * pts * sweeps / t_batch / 1e9,
         "sequential_ms_per_sweep": t_seq / sweeps * 1e3,
@@ -323,7 +323,7 @@ def main():
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision run")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-relaxation run")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving trace")
-    ap.add_argument("--precond-sweeps", type=int, default=4,
+    ap.add_argument("--precond-sweeps", type=int, default=8,
                     help="FP32 JOR sweeps of the mixed-precision BiCGstab preconditioner")
     ap.add_argument("--only", choices=("jor", "operator"), default=None,
                     help="launch one kernel family on the bench grid (for counter profiles): JOR sweeps or the "

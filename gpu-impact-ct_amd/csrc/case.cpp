@@ -835,7 +835,8 @@ static int bicgstab_run(ih_case *c, double eps, int n_it_max, const int ss[3], c
         k.b32 = c->f32[1];
         k.sweeps = sweeps;
         k.omega = (float)omega;
-        // ghost layers of the ping-pong buffer must hold finite values (zero)
+        // ghost layers of the ping-pong buffers must hold finite values (zero)
+        launch_fill_f32(s, k.e0, 0.0f, c->g.fe);
         launch_fill_f32(s, k.e1, 0.0f, c->g.fe);
     }
     // rr = bb - A phi, rh = rr, norm and <rr, rh>
@@ -988,10 +989,24 @@ static int poisson_solve(ih_case *c, const ih_step_params *p, const Box &b, ih_b
     if (rc2) return rc2;
     const double eps = p->bicg_rel_tol * bnorm;
     PhaseTimer pt(c, "bicgstab");
-    if (p->precond_sweeps > 0)
-        rc = ih_bicgstab_mixed(c, eps, p->bicg_max_it, ss, nn, IH_RES, IH_PRE, p->precond_sweeps, p->jor_omega, bst);
-    else
+    if (p->precond_sweeps > 0) {
+        // keep the initial guess: if the preconditioned iteration breaks down or
+        // does not converge, the plain FP64 solver restarts from it
+        HIPCHK(hipMemcpyAsync(F(c, IH_COMP), F(c, IH_PRE), sizeof(double) * (size_t)c->g.fe, hipMemcpyDeviceToDevice,
+                              c->stream));
+        ih_bicgstab_stats mst{};
+        rc = ih_bicgstab_mixed(c, eps, p->bicg_max_it, ss, nn, IH_RES, IH_PRE, p->precond_sweeps, p->jor_omega, &mst);
+        if (!rc && mst.exit_reason != 0) {
+            HIPCHK(hipMemcpyAsync(F(c, IH_PRE), F(c, IH_COMP), sizeof(double) * (size_t)c->g.fe,
+                                  hipMemcpyDeviceToDevice, c->stream));
+            rc = ih_bicgstab(c, eps, p->bicg_max_it, ss, nn, IH_RES, IH_PRE, bst);
+            if (bst) bst->iterations += mst.iterations;
+        } else if (bst) {
+            *bst = mst;
+        }
+    } else {
         rc = ih_bicgstab(c, eps, p->bicg_max_it, ss, nn, IH_RES, IH_PRE, bst);
+    }
     pt.done(bst ? bst->iterations : 0);
     return rc;
 }
@@ -1234,12 +1249,14 @@ struct ih_batch {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    int tpb = 256, zchunk = 16;
-    int ntiles = 0;
+    int tpb = 256, zchunk = 16, layout = 0;
+    int ntiles = 0, row_tiles = 0;
     JorBatchItem *items[2][2] = {};  // [fp32][direction]
+    int *tile_item = nullptr;        // item of every global tile
+    int tile_cap = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    int g_sweeps = -1, g_fp32 = -1, g_tpb = -1, g_zchunk = -1;
+    int g_sweeps = -1, g_fp32 = -1, g_tpb = -1, g_zchunk = -1, g_layout = -1;
     double g_omega = 0.0;
 };
 
@@ -1257,10 +1274,12 @@ void batch_drop_graph(ih_batch *b) {
 int batch_build(ih_batch *b) {
     std::vector<JorBatchItem> h[2][2];
     int t0 = 0;
+    b->row_tiles = 0;
     for (ih_case *c : b->cases) {
         const Box pb = pbox(c->g);
         int ntxy, nbz;
         const int nt = jor_batch_tiles(pb, b->tpb, b->zchunk, ntxy, nbz);
+        b->row_tiles = std::max(b->row_tiles, nt);
         JorBatchItem it{};
         it.g = c->g;
         it.r = Range3{pb.lo[0], pb.lo[1], pb.lo[2], pb.hi[0] - pb.lo[0] + 1, pb.hi[1] - pb.lo[1] + 1,
@@ -1286,6 +1305,18 @@ int batch_build(ih_batch *b) {
         }
     }
     b->ntiles = t0;
+    std::vector<int> ti((size_t)t0);
+    for (size_t q = 0; q < b->cases.size(); ++q) {
+        const int a = h[0][0][q].tile0, e = q + 1 < b->cases.size() ? h[0][0][q + 1].tile0 : t0;
+        for (int t = a; t < e; ++t) ti[(size_t)t] = (int)q;
+    }
+    if (t0 > b->tile_cap) {
+        if (b->tile_item) (void)hipFree(b->tile_item);
+        b->tile_item = nullptr;
+        HIPCHK(hipMalloc(&b->tile_item, sizeof(int) * (size_t)t0));
+        b->tile_cap = t0;
+    }
+    HIPCHK(hipMemcpy(b->tile_item, ti.data(), sizeof(int) * (size_t)t0, hipMemcpyHostToDevice));
     for (int p = 0; p < 2; ++p)
         for (int dir = 0; dir < 2; ++dir) {
             if (!b->items[p][dir]) HIPCHK(hipMalloc(&b->items[p][dir], sizeof(JorBatchItem) * b->cases.size()));
@@ -1297,9 +1328,9 @@ int batch_build(ih_batch *b) {
 }
 
 int batch_sweeps(ih_batch *b, int n, double omega, bool fp32) {
-    const int nitems = (int)b->cases.size();
     for (int q = 0; q < n; ++q)
-        if (launch_jor_batch(b->stream, b->items[fp32 ? 1 : 0][q & 1], nitems, b->ntiles, b->tpb, fp32, omega))
+        if (launch_jor_batch(b->stream, b->items[fp32 ? 1 : 0][q & 1], b->tile_item, b->ntiles, (int)b->cases.size(),
+                             b->row_tiles, b->layout, b->tpb, fp32, omega))
             return fail(IH_EINVAL, "unsupported tile shape");
     LAUNCHCHK();
     return IH_OK;
@@ -1344,47 +1375,66 @@ void ih_batch_destroy(ih_batch *b) {
     for (auto &pp : b->items)
         for (auto *q : pp)
             if (q) (void)hipFree(q);
+    if (b->tile_item) (void)hipFree(b->tile_item);
     if (b->ev0) (void)hipEventDestroy(b->ev0);
     if (b->ev1) (void)hipEventDestroy(b->ev1);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
 }
 
-int ih_batch_autotune(ih_batch *b, int sweeps, int *tpb, int *zchunk, double *ms_best) {
+int ih_batch_set_shape(ih_batch *b, int tpb, int zchunk, int layout) {
+    if (!b || (tpb != 128 && tpb != 256 && tpb != 512) || zchunk < 1 || zchunk > 64 || (layout != 0 && layout != 1))
+        return fail(IH_EINVAL, "bad launch shape");
+    HIPCHK(hipSetDevice(b->device));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    b->tpb = tpb;
+    b->zchunk = zchunk;
+    b->layout = layout;
+    return batch_build(b);
+}
+
+int ih_batch_autotune(ih_batch *b, int sweeps, int *tpb, int *zchunk, int *layout, double *ms_best) {
     if (!b || sweeps < 1) return fail(IH_EINVAL, "bad argument");
     HIPCHK(hipSetDevice(b->device));
     for (ih_case *c : b->cases) HIPCHK(hipStreamSynchronize(c->stream));
     static const int tpbs[3] = {128, 256, 512}, zcs[3] = {8, 16, 32};
     double best = 1e300;
-    int bt = b->tpb, bz = b->zchunk, rc;
-    for (int a : tpbs)
-        for (int z : zcs) {
-            b->tpb = a;
-            b->zchunk = z;
-            if ((rc = batch_build(b))) return rc;
-            // direction 0 only (PRE -> COMP): the relaxed field is left untouched
-            for (int q = 0; q < 2; ++q)
-                if (launch_jor_batch(b->stream, b->items[0][0], (int)b->cases.size(), b->ntiles, a, false, 0.8))
-                    return fail(IH_EINVAL, "tile shape");
-            HIPCHK(hipEventRecord(b->ev0, b->stream));
-            for (int q = 0; q < sweeps; ++q)
-                launch_jor_batch(b->stream, b->items[0][0], (int)b->cases.size(), b->ntiles, a, false, 0.8);
-            HIPCHK(hipEventRecord(b->ev1, b->stream));
-            HIPCHK(hipEventSynchronize(b->ev1));
-            LAUNCHCHK();
-            float ms = 0.f;
-            HIPCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
-            if (ms < best) {
-                best = ms;
-                bt = a;
-                bz = z;
+    int bt = b->tpb, bz = b->zchunk, bl = b->layout, rc;
+    const int n = (int)b->cases.size();
+    for (int lay = 0; lay < 2; ++lay)
+        for (int a : tpbs)
+            for (int z : zcs) {
+                b->tpb = a;
+                b->zchunk = z;
+                if ((rc = batch_build(b))) return rc;
+                // direction 0 only (PRE -> COMP): the relaxed field is left untouched
+                for (int q = 0; q < 2; ++q)
+                    if (launch_jor_batch(b->stream, b->items[0][0], b->tile_item, b->ntiles, n, b->row_tiles, lay, a,
+                                         false, 0.8))
+                        return fail(IH_EINVAL, "tile shape");
+                HIPCHK(hipEventRecord(b->ev0, b->stream));
+                for (int q = 0; q < sweeps; ++q)
+                    launch_jor_batch(b->stream, b->items[0][0], b->tile_item, b->ntiles, n, b->row_tiles, lay, a, false,
+                                     0.8);
+                HIPCHK(hipEventRecord(b->ev1, b->stream));
+                HIPCHK(hipEventSynchronize(b->ev1));
+                LAUNCHCHK();
+                float ms = 0.f;
+                HIPCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+                if (ms < best) {
+                    best = ms;
+                    bt = a;
+                    bz = z;
+                    bl = lay;
+                }
             }
-        }
     b->tpb = bt;
     b->zchunk = bz;
+    b->layout = bl;
     if ((rc = batch_build(b))) return rc;
     if (tpb) *tpb = bt;
     if (zchunk) *zchunk = bz;
+    if (layout) *layout = bl;
     if (ms_best) *ms_best = best / sweeps;
     return IH_OK;
 }
@@ -1411,7 +1461,7 @@ int ih_batch_jor(ih_batch *b, int n_sweeps, double omega, int fp32, int use_grap
     if (n_sweeps > 0) {
         if (use_graph) {
             if (!b->exec || b->g_sweeps != n_sweeps || b->g_fp32 != fp32 || b->g_omega != omega ||
-                b->g_tpb != b->tpb || b->g_zchunk != b->zchunk) {
+                b->g_tpb != b->tpb || b->g_zchunk != b->zchunk || b->g_layout != b->layout) {
                 batch_drop_graph(b);
                 HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
                 rc = batch_sweeps(b, n_sweeps, omega, fp32 != 0);
@@ -1429,6 +1479,7 @@ int ih_batch_jor(ih_batch *b, int n_sweeps, double omega, int fp32, int use_grap
                 b->g_omega = omega;
                 b->g_tpb = b->tpb;
                 b->g_zchunk = b->zchunk;
+                b->g_layout = b->layout;
             }
             HIPCHK(hipGraphLaunch(b->exec, s));
         } else if ((rc = batch_sweeps(b, n_sweeps, omega, fp32 != 0))) {

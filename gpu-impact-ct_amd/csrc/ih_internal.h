@@ -121,8 +121,16 @@ struct JorBatchItem {
     int ntxy, nbz, tile0, pad;
 };
 int jor_batch_tiles(const Box &b, int tpb, int zchunk, int &ntxy, int &nbz);
-int launch_jor_batch(hipStream_t s, const JorBatchItem *items, int nitems, int ntiles, int tpb, bool fp32,
-                     double omega);
+// layout 0: one flat grid over all tiles with a tile -> item table; 1: a row
+// of row_tiles blocks per item
+int launch_jor_batch(hipStream_t s, const JorBatchItem *items, const int *tile_item, int ntiles, int nitems,
+                     int row_tiles, int layout, int tpb, bool fp32, double omega);
+// the last FP32 sweep written widened to double, and the first one from e = 0
+// on A e = (float)v (stores b32; writes e, or e64 widened when e64 != nullptr)
+void launch_jor_f32_to64(hipStream_t s, const Geo &g, const Coef &c, const Box &b, float omega, const float *rel,
+                         const float *bb, double *out, const int *active);
+void launch_jor_f32_first(hipStream_t s, const Geo &g, const Coef &c, const Box &b, float omega, const double *v,
+                          float *b32, float *e, double *e64, const int *active);
 void launch_dot_partial(hipStream_t s, const Geo &g, const Box &b, const double *a, const double *c, double *part);
 void launch_sum_partial(hipStream_t s, const Geo &g, const Box &b, const double *a, double *part);
 void launch_final_sum(hipStream_t s, const double *part, int64_t nb, double *out);

@@ -377,69 +377,13 @@ __device__ inline void jor_neighbours(const Geo &g, int i, int j, int k, int64_t
     }
 }
 
-// One thread per (x, y) column walks JZ planes, keeping rel(k-1), rel(k),
-// rel(k+1) in registers, so each plane of rel is fetched once per tile column
-// instead of three times; tiles are dealt to the XCDs in contiguous ranges.
-constexpr int JZ = 16;
-
-template <typename T>
-__device__ inline void jor_column(const Geo &g, const Coef &c, const Range3 &r, int nbx, int nby, int nbz, T omega,
-                                  const T *__restrict__ rel, const T *__restrict__ bb, T *__restrict__ out,
-                                  const T *g1, const T *g2, const T *g3) {
-    int bx, by, bz;
-    if (!xcd_tile(nbx, nby, nbz, bx, by, bz)) return;
-    const int i = r.lo0 + bx * BX + (int)threadIdx.x;
-    const int j = r.lo1 + by * BY + (int)threadIdx.y;
-    if (i >= r.lo0 + r.n0 || j >= r.lo1 + r.n1) return;
-    const int k0 = r.lo2 + bz * JZ;
-    const int k1 = min(k0 + JZ - 1, r.lo2 + r.n2 - 1);
-    const int c0 = g.cst[0], c1 = g.cst[1], c2 = g.cst[2];
-    // x and y neighbours: periodic columns read the opposite side of the block
-    int64_t dxm = -1, dxp = 1, dym = -(int64_t)g.px, dyp = g.px;
-    if (g.per[0]) {
-        if (i == g.Sp[0]) dxm = g.Np[0] - g.Sp[0];
-        if (i == g.Np[0]) dxp = -(g.Np[0] - g.Sp[0]);
-    }
-    if (g.per[1]) {
-        if (j == g.Sp[1]) dym = (int64_t)(g.Np[1] - g.Sp[1]) * g.px;
-        if (j == g.Np[1]) dyp = -(int64_t)(g.Np[1] - g.Sp[1]) * g.px;
-    }
-    const T a1m = g1[0 * c0 + i], a1p = g1[2 * c0 + i], a1d = g1[1 * c0 + i];
-    const T a2m = g2[0 * c1 + j], a2p = g2[2 * c1 + j], a2d = g2[1 * c1 + j];
-    auto zidx = [&](int k) -> int64_t {
-        if (g.per[2]) {
-            if (k < g.Sp[2]) k += g.Np[2] - g.Sp[2] + 1;
-            else if (k > g.Np[2]) k -= g.Np[2] - g.Sp[2] + 1;
-        }
-        return gidx(g, i, j, k);
-    };
-    T zm = rel[zidx(k0 - 1)];
-    T zc = rel[zidx(k0)];
-    for (int k = k0; k <= k1; ++k) {
-        const T zp = rel[zidx(k + 1)];
-        const int64_t x = gidx(g, i, j, k);
-        T t = bb[x];
-        t = t - a1m * rel[x + dxm];
-        t = t - a1p * rel[x + dxp];
-        t = t - a2m * rel[x + dym];
-        t = t - a2p * rel[x + dyp];
-        t = t - g3[0 * c2 + k] * zm;
-        t = t - g3[2 * c2 + k] * zp;
-        const T diag = (a1d + a2d) + g3[1 * c2 + k];
-        out[x] = omega * t / diag + (T(1) - omega) * zc;
-        zm = zc;
-        zc = zp;
-    }
-}
-
 // Flat form: the (i, j) plane of the range is numbered i-fastest and cut into
 // tiles of TPB consecutive points (no idle lanes at the ragged x end), and z
 // into balanced chunks.  The loop is unrolled twice with two register sets so
 // that the loads of plane k+1 are in flight while plane k is relaxed.
-template <typename T, int TPB>
-__device__ inline void jor_flat(const Geo &g, const Range3 &r, int nbz, int txy, int bz, T omega,
-                                const T *__restrict__ rel, const T *__restrict__ bb, T *__restrict__ out,
-                                const T *g1, const T *g2, const T *g3) {
+template <typename T, int TPB, typename TO = T, typename PC = const T *, typename PO = TO *>
+__device__ inline void jor_flat(const Geo &g, const Range3 &r, int nbz, int txy, int bz, T omega, PC __restrict__ rel,
+                                PC __restrict__ bb, PO __restrict__ out, PC g1, PC g2, PC g3) {
     const int64_t f = (int64_t)txy * TPB + threadIdx.x;
     if (f >= (int64_t)r.n0 * r.n1) return;
     const int i = r.lo0 + (int)(f % r.n0);
@@ -486,7 +430,7 @@ __device__ inline void jor_flat(const Geo &g, const Range3 &r, int nbz, int txy,
         t = t - g3[0 * c2 + k] * zm;
         t = t - g3[2 * c2 + k] * P.zp;
         const T diag = (a1d + a2d) + g3[1 * c2 + k];
-        out[gidx(g, i, j, k)] = omega * t / diag + (T(1) - omega) * zc;
+        out[gidx(g, i, j, k)] = (TO)(omega * t / diag + (T(1) - omega) * zc);
         zm = zc;
         zc = P.zp;
     };
@@ -503,50 +447,95 @@ __device__ inline void jor_flat(const Geo &g, const Range3 &r, int nbz, int txy,
     }
 }
 
-template <int TPB>
-__global__ __launch_bounds__(TPB) void k_jor_flat(Geo g, Coef c, Range3 r, int ntxy, int nbz, double omega,
-                                                  const double *rel, const double *bb, double *out) {
+// One sweep over the box: tiles of JOR_TPB points of the (i, j) plane, z in
+// balanced chunks of at most JOR_KC planes.  T is the arithmetic type (double,
+// or float with the FP32 copy of cdg), TO the type written (a float sweep may
+// write its result widened to double).
+constexpr int JOR_TPB = 256, JOR_KC = 32;
+
+template <typename T, typename TO>
+__global__ __launch_bounds__(JOR_TPB) void k_jor(Geo g, Coef c, Range3 r, int ntxy, int nbz, T omega, const T *rel,
+                                                 const T *bb, TO *out, const int *active) {
+    if (active && !*active) return;
     int txy, dummy, bz;
     if (!xcd_tile(ntxy, 1, nbz, txy, dummy, bz)) return;
-    jor_flat<double, TPB>(g, r, nbz, txy, bz, omega, rel, bb, out, c.dg[0], c.dg[1], c.dg[2]);
+    const T *g1 = (const T *)(sizeof(T) == 8 ? (const void *)c.dg[0] : (const void *)c.dgf[0]);
+    const T *g2 = (const T *)(sizeof(T) == 8 ? (const void *)c.dg[1] : (const void *)c.dgf[1]);
+    const T *g3 = (const T *)(sizeof(T) == 8 ? (const void *)c.dg[2] : (const void *)c.dgf[2]);
+    jor_flat<T, JOR_TPB, TO>(g, r, nbz, txy, bz, omega, rel, bb, out, g1, g2, g3);
 }
+
+// First FP32 sweep from e = 0 on A e = (float)v: b32 = (float)v is stored for
+// the later sweeps; every neighbour term is the product with 0 that the
+// generic sweep would form, so the result is that sweep's bit for bit.
+template <typename TO>
+__global__ void k_jor_f32_first(Geo g, Coef c, Range3 r, float omega, const double *v, float *b32, TO *e,
+                                const int *active) {
+    if (active && !*active) return;
+    IH_POINT(r);
+    const int64_t x = gidx(g, i, j, k);
+    const float b = (float)v[x];
+    b32[x] = b;
+    const float *g1 = c.dgf[0], *g2 = c.dgf[1], *g3 = c.dgf[2];
+    const int c0 = g.cst[0], c1 = g.cst[1], c2 = g.cst[2];
+    const float z = 0.0f;
+    float t = b;
+    t = t - g1[0 * c0 + i] * z;
+    t = t - g1[2 * c0 + i] * z;
+    t = t - g2[0 * c1 + j] * z;
+    t = t - g2[2 * c1 + j] * z;
+    t = t - g3[0 * c2 + k] * z;
+    t = t - g3[2 * c2 + k] * z;
+    const float diag = (g1[1 * c0 + i] + g2[1 * c1 + j]) + g3[1 * c2 + k];
+    e[x] = (TO)(omega * t / diag + (1.0f - omega) * z);
+}
+
+// A pointer read from a table is a generic pointer, so loads through it would be
+// flat instructions; declaring it global (where every field lives) lets the
+// compiler emit global loads as for kernel arguments.
+template <typename T>
+using gconst = const __attribute__((address_space(1))) T *;
+template <typename T>
+using gmut = __attribute__((address_space(1))) T *;
 
 // Batched relaxation: the tiles of many cases (grids of any size) in one
-// launch.  Item q owns the global tiles tile0[q] .. tile0[q+1]-1 (sorted); a
-// block finds its item by binary search, then relaxes that tile exactly as the
-// single-case kernel would, so every case's result is bit-identical to its own
-// sweep.
+// launch.  Item q owns the global tiles tile0[q] .. tile0[q+1]-1 and
+// tile_item maps a tile to its item; the block relaxes that tile exactly as
+// the single-case kernel would, so every case's result is bit-identical to its
+// own sweep.
 template <typename T, int TPB>
-__global__ __launch_bounds__(TPB) void k_jor_batch(const JorBatchItem *__restrict__ items, int nitems, int ntiles,
-                                                   T omega) {
+__global__ __launch_bounds__(TPB) void k_jor_batch(const JorBatchItem *__restrict__ items,
+                                                   const int *__restrict__ tile_item, int ntiles, T omega) {
     int t, dummy, dummy2;
     if (!xcd_tile(ntiles, 1, 1, t, dummy, dummy2)) return;
-    int lo = 0, hi = nitems - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (items[mid].tile0 <= t) lo = mid;
-        else hi = mid - 1;
-    }
-    const JorBatchItem &it = items[lo];
+    const JorBatchItem &it = items[tile_item[t]];
     const int local = t - it.tile0;
     const int txy = local % it.ntxy, bz = local / it.ntxy;
-    if (sizeof(T) == 8)
-        jor_flat<T, TPB>(it.g, it.r, it.nbz, txy, bz, omega, (const T *)it.rel, (const T *)it.bb, (T *)it.out,
-                         (const T *)it.dg[0], (const T *)it.dg[1], (const T *)it.dg[2]);
-    else
-        jor_flat<T, TPB>(it.g, it.r, it.nbz, txy, bz, omega, (const T *)it.rel, (const T *)it.bb, (T *)it.out,
-                         (const T *)it.dgf[0], (const T *)it.dgf[1], (const T *)it.dgf[2]);
+    const void *g1 = sizeof(T) == 8 ? (const void *)it.dg[0] : (const void *)it.dgf[0];
+    const void *g2 = sizeof(T) == 8 ? (const void *)it.dg[1] : (const void *)it.dgf[1];
+    const void *g3 = sizeof(T) == 8 ? (const void *)it.dg[2] : (const void *)it.dgf[2];
+    jor_flat<T, TPB, T, gconst<T>, gmut<T>>(it.g, it.r, it.nbz, txy, bz, omega, (gconst<T>)it.rel,
+                                             (gconst<T>)it.bb, (gmut<T>)it.out, (gconst<T>)g1, (gconst<T>)g2,
+                                             (gconst<T>)g3);
 }
 
-__global__ __launch_bounds__(256) void k_jor(Geo g, Coef c, Range3 r, int nbx, int nby, int nbz, double omega,
-                                             const double *rel, const double *bb, double *out) {
-    jor_column<double>(g, c, r, nbx, nby, nbz, omega, rel, bb, out, c.dg[0], c.dg[1], c.dg[2]);
-}
-
-__global__ __launch_bounds__(256) void k_jor_f32(Geo g, Coef c, Range3 r, int nbx, int nby, int nbz, float omega,
-                                                 const float *rel, const float *bb, float *out, const int *active) {
-    if (active && !*active) return;
-    jor_column<float>(g, c, r, nbx, nby, nbz, omega, rel, bb, out, c.dgf[0], c.dgf[1], c.dgf[2]);
+// Row layout of the same batch: grid (row_tiles, cases); a block's case is its
+// row, so its table entry is read without a dependent lookup; rows shorter
+// than row_tiles leave idle blocks that return at once.
+template <typename T, int TPB>
+__global__ __launch_bounds__(TPB) void k_jor_batch_rows(const JorBatchItem *__restrict__ items, int row_tiles,
+                                                        T omega) {
+    const JorBatchItem &it = items[blockIdx.y];
+    const int per = row_tiles / NXCD, id = (int)blockIdx.x;
+    const int t = (id % NXCD) * per + id / NXCD;  // tiles of a row dealt to the XCDs in contiguous ranges
+    if (t >= it.ntxy * it.nbz) return;
+    const int txy = t % it.ntxy, bz = t / it.ntxy;
+    const void *g1 = sizeof(T) == 8 ? (const void *)it.dg[0] : (const void *)it.dgf[0];
+    const void *g2 = sizeof(T) == 8 ? (const void *)it.dg[1] : (const void *)it.dgf[1];
+    const void *g3 = sizeof(T) == 8 ? (const void *)it.dg[2] : (const void *)it.dgf[2];
+    jor_flat<T, TPB, T, gconst<T>, gmut<T>>(it.g, it.r, it.nbz, txy, bz, omega, (gconst<T>)it.rel,
+                                             (gconst<T>)it.bb, (gmut<T>)it.out, (gconst<T>)g1, (gconst<T>)g2,
+                                             (gconst<T>)g3);
 }
 
 // ---------------------------------------------------------------------------
@@ -892,22 +881,51 @@ void launch_div_dir(hipStream_t s, const Geo &g, const Coef &c, int q, const dou
     hipLaunchKernelGGL(k_div_dir, grid_for(r), dim3(BX, BY, 1), 0, s, g, c, r, q, dig, lap, assign);
 }
 
+static void jor_grid(const Range3 &r, int &ntxy, int &nbz) {
+    ntxy = (int)(((int64_t)r.n0 * r.n1 + JOR_TPB - 1) / JOR_TPB);
+    nbz = (r.n2 + JOR_KC - 1) / JOR_KC;
+}
+
 void launch_jor(hipStream_t s, const Geo &g, const Coef &c, const Box &b, double omega, const double *rel,
                 const double *bb, double *out) {
     Range3 r = R(b);
     if (empty(r)) return;
-    const int nbx = (r.n0 + BX - 1) / BX, nby = (r.n1 + BY - 1) / BY, nbz = (r.n2 + JZ - 1) / JZ;
-    hipLaunchKernelGGL(k_jor, dim3(xcd_grid((int64_t)nbx * nby * nbz)), dim3(BX, BY, 1), 0, s, g, c, r, nbx, nby, nbz,
-                       omega, rel, bb, out);
+    int ntxy, nbz;
+    jor_grid(r, ntxy, nbz);
+    hipLaunchKernelGGL((k_jor<double, double>), dim3(xcd_grid((int64_t)ntxy * nbz)), dim3(JOR_TPB), 0, s, g, c, r,
+                       ntxy, nbz, omega, rel, bb, out, (const int *)nullptr);
 }
 
 void launch_jor_f32(hipStream_t s, const Geo &g, const Coef &c, const Box &b, float omega, const float *rel,
                     const float *bb, float *out, const int *active) {
     Range3 r = R(b);
     if (empty(r)) return;
-    const int nbx = (r.n0 + BX - 1) / BX, nby = (r.n1 + BY - 1) / BY, nbz = (r.n2 + JZ - 1) / JZ;
-    hipLaunchKernelGGL(k_jor_f32, dim3(xcd_grid((int64_t)nbx * nby * nbz)), dim3(BX, BY, 1), 0, s, g, c, r, nbx, nby,
-                       nbz, omega, rel, bb, out, active);
+    int ntxy, nbz;
+    jor_grid(r, ntxy, nbz);
+    hipLaunchKernelGGL((k_jor<float, float>), dim3(xcd_grid((int64_t)ntxy * nbz)), dim3(JOR_TPB), 0, s, g, c, r,
+                       ntxy, nbz, omega, rel, bb, out, active);
+}
+
+void launch_jor_f32_to64(hipStream_t s, const Geo &g, const Coef &c, const Box &b, float omega, const float *rel,
+                         const float *bb, double *out, const int *active) {
+    Range3 r = R(b);
+    if (empty(r)) return;
+    int ntxy, nbz;
+    jor_grid(r, ntxy, nbz);
+    hipLaunchKernelGGL((k_jor<float, double>), dim3(xcd_grid((int64_t)ntxy * nbz)), dim3(JOR_TPB), 0, s, g, c, r,
+                       ntxy, nbz, omega, rel, bb, out, active);
+}
+
+void launch_jor_f32_first(hipStream_t s, const Geo &g, const Coef &c, const Box &b, float omega, const double *v,
+                          float *b32, float *e, double *e64, const int *active) {
+    Range3 r = R(b);
+    if (empty(r)) return;
+    if (e64)
+        hipLaunchKernelGGL((k_jor_f32_first<double>), grid_for(r), dim3(BX, BY, 1), 0, s, g, c, r, omega, v, b32, e64,
+                           active);
+    else
+        hipLaunchKernelGGL((k_jor_f32_first<float>), grid_for(r), dim3(BX, BY, 1), 0, s, g, c, r, omega, v, b32, e,
+                           active);
 }
 
 int jor_batch_tiles(const Box &b, int tpb, int zchunk, int &ntxy, int &nbz) {
@@ -917,15 +935,21 @@ int jor_batch_tiles(const Box &b, int tpb, int zchunk, int &ntxy, int &nbz) {
     return ntxy * nbz;
 }
 
-int launch_jor_batch(hipStream_t s, const JorBatchItem *items, int nitems, int ntiles, int tpb, bool fp32,
-                     double omega) {
-    const dim3 grid(xcd_grid(ntiles));
-#define IH_JB(TPB)                                                                                              \
-    do {                                                                                                        \
-        if (fp32)                                                                                               \
-            hipLaunchKernelGGL((k_jor_batch<float, TPB>), grid, dim3(TPB), 0, s, items, nitems, ntiles, (float)omega); \
-        else                                                                                                    \
-            hipLaunchKernelGGL((k_jor_batch<double, TPB>), grid, dim3(TPB), 0, s, items, nitems, ntiles, omega);   \
+int launch_jor_batch(hipStream_t s, const JorBatchItem *items, const int *tile_item, int ntiles, int nitems,
+                     int row_tiles, int layout, int tpb, bool fp32, double omega) {
+    const dim3 flat(xcd_grid(ntiles)), rows(xcd_grid(row_tiles), (unsigned)nitems);
+    const int rt = (int)xcd_grid(row_tiles);
+#define IH_JB(TPB)                                                                                                \
+    do {                                                                                                          \
+        if (layout == 0 && fp32)                                                                                  \
+            hipLaunchKernelGGL((k_jor_batch<float, TPB>), flat, dim3(TPB), 0, s, items, tile_item, ntiles,        \
+                               (float)omega);                                                                     \
+        else if (layout == 0)                                                                                     \
+            hipLaunchKernelGGL((k_jor_batch<double, TPB>), flat, dim3(TPB), 0, s, items, tile_item, ntiles, omega); \
+        else if (fp32)                                                                                            \
+            hipLaunchKernelGGL((k_jor_batch_rows<float, TPB>), rows, dim3(TPB), 0, s, items, rt, (float)omega);    \
+        else                                                                                                      \
+            hipLaunchKernelGGL((k_jor_batch_rows<double, TPB>), rows, dim3(TPB), 0, s, items, rt, omega);          \
     } while (0)
     switch (tpb) {
     case 128: IH_JB(128); break;

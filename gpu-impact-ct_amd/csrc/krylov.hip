@@ -487,6 +487,10 @@ __global__ void k_ctl_begin(double *sc, int *fl) {
         fl[KF_EXIT] = 1;
     }
     if (counter < 1 && norm != 0.0) ex = false;
+    if (!(norm == norm) || norm > 1.0e300) {  // a non-finite residual: stop as a breakdown
+        ex = true;
+        fl[KF_EXIT] = 2;
+    }
     if (ex) {
         fl[KF_ACTIVE] = 0;
         fl[KF_DONE] = 1;
@@ -597,18 +601,23 @@ void launch_krylov_init(hipStream_t s, const Geo &g, const Box &b, const double 
                        nmax);
 }
 
-// z = M v: `sweeps` FP32 JOR sweeps on A_cdg e = (float)v from e = 0, z = (double)e
+// z = M v: `sweeps` FP32 JOR sweeps on A_cdg e = (float)v from e = 0, z = (double)e.
+// The first sweep reads v itself (and stores b = (float)v), the last writes z
+// widened to double, so M costs sweeps passes over the box and no more.
 static void launch_precond(hipStream_t s, const Geo &g, const Coef &c, const Box &b, const double *v, double *z,
                            const KrylovPtrs &k) {
     const int *act = k.fl + KF_ACTIVE;
-    launch_fill_f32(s, k.e0, 0.0f, g.fe, act);
-    launch_to_f32(s, g, b, v, k.b32, act);
+    if (k.sweeps == 1) {
+        launch_jor_f32_first(s, g, c, b, k.omega, v, k.b32, nullptr, z, act);
+        return;
+    }
+    launch_jor_f32_first(s, g, c, b, k.omega, v, k.b32, k.e0, nullptr, act);
     float *x = k.e0, *y = k.e1;
-    for (int q = 0; q < k.sweeps; ++q) {
+    for (int q = 1; q < k.sweeps - 1; ++q) {
         launch_jor_f32(s, g, c, b, k.omega, x, k.b32, y, act);
         std::swap(x, y);
     }
-    launch_from_f32(s, g, b, x, z, act);
+    launch_jor_f32_to64(s, g, c, b, k.omega, x, k.b32, z, act);
 }
 
 void launch_krylov_iteration(hipStream_t s, const Geo &g, const Coef &c, const Box &b, const KrylovPtrs &k,

@@ -142,7 +142,8 @@ def load(path: os.PathLike | None = None):
         "ih_device_interpolate2": (C.c_int, [vp, C.c_int, dp, dp]),
         "ih_batch_create": (C.c_int, [C.POINTER(vp), C.c_int, C.POINTER(vp)]),
         "ih_batch_destroy": (None, [vp]),
-        "ih_batch_autotune": (C.c_int, [vp, C.c_int, ip, ip, dp]),
+        "ih_batch_autotune": (C.c_int, [vp, C.c_int, ip, ip, ip, dp]),
+        "ih_batch_set_shape": (C.c_int, [vp, C.c_int, C.c_int, C.c_int]),
         "ih_batch_jor": (C.c_int, [vp, C.c_int, C.c_double, C.c_int, C.c_int]),
         "ih_grid_bytes": (C.c_int64, [C.POINTER(GridDesc)]),
         "ih_serve": (C.c_int, [C.POINTER(ServeConfig), C.POINTER(Request), C.c_int, C.POINTER(RequestResult),
@@ -428,9 +429,14 @@ class Batch:
             pass
 
     def autotune(self, sweeps=10):
-        tpb, zc, ms = C.c_int(), C.c_int(), C.c_double()
-        check(self.lib.ih_batch_autotune(self.h, int(sweeps), C.byref(tpb), C.byref(zc), C.byref(ms)), "autotune")
-        return tpb.value, zc.value, ms.value
+        """returns (threads per tile, z chunk, layout, ms per sweep) of the fastest launch shape"""
+        tpb, zc, lay, ms = C.c_int(), C.c_int(), C.c_int(), C.c_double()
+        check(self.lib.ih_batch_autotune(self.h, int(sweeps), C.byref(tpb), C.byref(zc), C.byref(lay), C.byref(ms)),
+              "autotune")
+        return tpb.value, zc.value, lay.value, ms.value
+
+    def set_shape(self, tpb, zchunk, layout):
+        check(self.lib.ih_batch_set_shape(self.h, int(tpb), int(zchunk), int(layout)), "batch_set_shape")
 
     def jor(self, n, omega=0.8, fp32=False, graph=False):
         check(self.lib.ih_batch_jor(self.h, int(n), float(omega), 1 if fp32 else 0, 1 if graph else 0), "batch_jor")

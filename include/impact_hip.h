@@ -219,10 +219,13 @@ int ih_device_interpolate2(ih_case *c, int dimension_id, double *phi, double *in
 typedef struct ih_batch ih_batch;
 int  ih_batch_create(ih_case *const *cases, int n, ih_batch **out);
 void ih_batch_destroy(ih_batch *b);
-/* Times every tile shape (threads per tile 128/256/512 x z chunk 8/16/32) on
- * `sweeps` sweeps without changing PRE, keeps the fastest, reports it and its
- * time per sweep. */
-int  ih_batch_autotune(ih_batch *b, int sweeps, int *tpb, int *zchunk, double *ms_per_sweep);
+/* Times every launch shape -- threads per tile 128/256/512, z chunk 8/16/32,
+ * layout 0 (one flat grid over all tiles) or 1 (one row of blocks per case) --
+ * on `sweeps` sweeps without changing PRE, keeps the fastest and reports it
+ * with its time per sweep. */
+int  ih_batch_autotune(ih_batch *b, int sweeps, int *tpb, int *zchunk, int *layout, double *ms_per_sweep);
+/* Sets the launch shape directly (tpb 128/256/512, zchunk 1..64, layout 0/1). */
+int  ih_batch_set_shape(ih_batch *b, int tpb, int zchunk, int layout);
 /* n_sweeps JOR sweeps (omega) of PRE on RES in every case, each bit-identical to
  * ih_jor on that case.  fp32 != 0: FP32 defect correction instead (r = RES -
  * A_cdg PRE in FP64, n_sweeps FP32 sweeps on A e = r from e = 0, PRE += e).

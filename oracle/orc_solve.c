@@ -225,6 +225,11 @@ static void bicgstab_run(const orc_case *c, double eps, int n_it_max, const int 
         }
         if (counter < 1 && norm_inf != 0.0)
             exit_yes = 0; /* number_poisson == 1: at least one iteration */
+        if (!(norm_inf == norm_inf) || norm_inf > 1.0e300) {
+            /* non-finite residual: stop as a breakdown (the reference would run to n_it_max) */
+            exit_yes = 1;
+            if (st) st->exit_reason = 2;
+        }
         if (exit_yes)
             break;
         counter = counter + 1;
@@ -541,7 +546,20 @@ void orc_timestep(orc_case *c, const orc_step_params *p, int step_index, double 
         }
         orc_bicgstab_stats bst;
         const double eps = p->bicg_rel_tol * absmax(c, ss, nn, res);
-        bicgstab_run(c, eps, p->bicg_max_it, ss, nn, res, pre, bw, p->precond_sweeps, p->jor_omega, &bst);
+        if (p->precond_sweeps > 0) {
+            /* the device keeps the initial guess and restarts plain FP64 BiCGstab
+             * from it when the preconditioned iteration does not converge */
+            memcpy(comp, pre, sizeof(double) * (size_t)n);
+            bicgstab_run(c, eps, p->bicg_max_it, ss, nn, res, pre, bw, p->precond_sweeps, p->jor_omega, &bst);
+            if (bst.exit_reason != 0) {
+                const int mixed_its = bst.iterations;
+                memcpy(pre, comp, sizeof(double) * (size_t)n);
+                bicgstab_run(c, eps, p->bicg_max_it, ss, nn, res, pre, bw, 0, 0.0, &bst);
+                bst.iterations += mixed_its;
+            }
+        } else {
+            bicgstab_run(c, eps, p->bicg_max_it, ss, nn, res, pre, bw, 0, 0.0, &bst);
+        }
         if (st) {
             st->bicg_iterations[s] = bst.iterations;
             st->bicg_final[s] = bst.norm_inf_final;

@@ -28,9 +28,10 @@ def _cases(seed):
     return cs, data
 
 
+@pytest.mark.parametrize("shape", [(256, 32, 0), (128, 8, 1), (512, 16, 1)])
 @pytest.mark.parametrize("graph", [False, True])
 @pytest.mark.parametrize("sweeps", [1, 6])
-def test_batch_jor_equals_single_case(sweeps, graph):
+def test_batch_jor_equals_single_case(sweeps, graph, shape):
     cs, data = _cases(300)
     ref = []
     for c, (p, b) in zip(cs, data):
@@ -41,6 +42,7 @@ def test_batch_jor_equals_single_case(sweeps, graph):
         ref.append(s.download(ih.PRE))
         s.close()
     bt = ih.Batch(cs)
+    bt.set_shape(*shape)
     bt.jor(sweeps, 0.8, graph=graph)
     if graph:  # a second replay of the same graph continues from the new state
         bt.jor(sweeps, 0.8, graph=True)
@@ -61,8 +63,8 @@ def test_batch_jor_equals_single_case(sweeps, graph):
 def test_batch_fp32_equals_oracle():
     cs, data = _cases(320)
     bt = ih.Batch(cs)
-    tpb, zc, ms = bt.autotune(3)
-    assert tpb in (128, 256, 512) and zc in (8, 16, 32) and ms > 0
+    tpb, zc, lay, ms = bt.autotune(3)
+    assert tpb in (128, 256, 512) and zc in (8, 16, 32) and lay in (0, 1) and ms > 0
     for c, (p, b) in zip(cs, data):   # autotuning leaves PRE untouched
         same(c.download(ih.PRE)[box(c.Sp, c.Np)], p[box(c.Sp, c.Np)], "autotune")
     bt.jor(5, 0.8, fp32=True, graph=True)

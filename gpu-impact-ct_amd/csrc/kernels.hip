@@ -447,22 +447,186 @@ __device__ inline void jor_flat(const Geo &g, const Range3 &r, int nbz, int txy,
     }
 }
 
-// One sweep over the box: tiles of JOR_TPB points of the (i, j) plane, z in
-// balanced chunks of at most JOR_KC planes.  T is the arithmetic type (double,
-// or float with the FP32 copy of cdg), TO the type written (a float sweep may
-// write its result widened to double).
-constexpr int JOR_TPB = 256, JOR_KC = 32;
+// Vector form of the sweep, used for single-case launches.  A lane relaxes
+// V = 16 / sizeof(T) consecutive points of one row, so every load and store is
+// one 16-byte access per lane: the L1 address path handles a quad of lanes per
+// clock whatever the width, and 8-byte accesses left it, not HBM, as the limit.
+// The x neighbours come from the adjacent lanes by DPP.  Each row is numbered
+// as nv vectors (aligned, so the first and last may reach past the range) plus
+// one halo vector at either end that only supplies a neighbour, and the rows
+// follow each other in one flat sequence.  A wavefront covers 64 consecutive
+// entries of it and advances by 62: its first and last lane only supply the
+// neighbours of lanes 1 and 62, so no lane has to load an x neighbour.
+// Periodic x reads the wrapped neighbour with one more load per plane.
+struct JorVecShape {
+    int i0;      // first point of the first vector of each row (aligned)
+    int nv;      // vectors per row covering the range
+    int nent;    // entries per row: nv + 2
+    int nent_all;  // nent * rows
+    int waves;   // wavefronts over one plane
+};
+
+template <typename T>
+struct VecOf;
+template <>
+struct VecOf<double> {
+    typedef double type __attribute__((ext_vector_type(2)));
+};
+template <>
+struct VecOf<float> {
+    typedef float type __attribute__((ext_vector_type(4)));
+};
+
+// Value of the previous (next) lane of the wavefront; lane 0 (63) gets 0.
+__device__ inline float lane_prev(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xf, 0xf, false));
+}
+__device__ inline float lane_next(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xf, 0xf, false));
+}
+__device__ inline double lane_prev(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x138, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x138, 0xf, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+__device__ inline double lane_next(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x130, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x130, 0xf, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+
+// default z chunk (planes per block) of the double and the float sweep
+constexpr int JOR_VEC_TPB = 256, JOR_VEC_KC64 = 12, JOR_VEC_KC32 = 10;
 
 template <typename T, typename TO>
-__global__ __launch_bounds__(JOR_TPB) void k_jor(Geo g, Coef c, Range3 r, int ntxy, int nbz, T omega, const T *rel,
-                                                 const T *bb, TO *out, const int *active) {
+__global__ __launch_bounds__(JOR_VEC_TPB) void k_jor_vec(Geo g, Coef c, Range3 r, JorVecShape sh, int nbw, int nbz,
+                                                         T omega, const T *__restrict__ rel, const T *__restrict__ bb,
+                                                         TO *__restrict__ out, const int *active) {
+    constexpr int V = 16 / sizeof(T);
+    typedef typename VecOf<T>::type VT;
     if (active && !*active) return;
-    int txy, dummy, bz;
-    if (!xcd_tile(ntxy, 1, nbz, txy, dummy, bz)) return;
+    int tw, dummy, bz;
+    if (!xcd_tile(nbw, 1, nbz, tw, dummy, bz)) return;
+    const int lane = threadIdx.x & 63;
+    const int w = tw * (JOR_VEC_TPB / 64) + (int)(threadIdx.x >> 6);
+    if (w >= sh.waves) return;  // whole wavefronts only
+    int q = w * 62 + lane - 1;
+    const bool out_lane = lane >= 1 && lane <= 62 && q < sh.nent_all;
+    q = q < 0 ? 0 : (q >= sh.nent_all ? sh.nent_all - 1 : q);
+    const int jj = q / sh.nent, e = q - jj * sh.nent;
+    const int j = r.lo1 + jj;
+    const int is = sh.i0 + V * (e - 1);  // e = 0 and e = nv + 1 are the halo vectors
+    const int hi0 = r.lo0 + r.n0 - 1;
+    const bool real = out_lane && e >= 1 && e <= sh.nv;
+    const bool full = real && is >= r.lo0 && is + V - 1 <= hi0;
     const T *g1 = (const T *)(sizeof(T) == 8 ? (const void *)c.dg[0] : (const void *)c.dgf[0]);
     const T *g2 = (const T *)(sizeof(T) == 8 ? (const void *)c.dg[1] : (const void *)c.dgf[1]);
     const T *g3 = (const T *)(sizeof(T) == 8 ? (const void *)c.dg[2] : (const void *)c.dgf[2]);
-    jor_flat<T, JOR_TPB, TO>(g, r, nbz, txy, bz, omega, rel, bb, out, g1, g2, g3);
+    const int c0 = g.cst[0], c1 = g.cst[1], c2 = g.cst[2];
+    T a1m[V], a1p[V], a1d[V];
+#pragma unroll
+    for (int t = 0; t < V; ++t) {
+        int ic = is + t;  // coefficient rows exist for the range only
+        ic = ic < r.lo0 ? r.lo0 : (ic > hi0 ? hi0 : ic);
+        a1m[t] = g1[0 * c0 + ic];
+        a1p[t] = g1[2 * c0 + ic];
+        a1d[t] = g1[1 * c0 + ic];
+    }
+    const T a2m = g2[0 * c1 + j], a2p = g2[2 * c1 + j], a2d = g2[1 * c1 + j];
+    int64_t dym = -(int64_t)g.px, dyp = g.px;
+    if (g.per[1]) {
+        if (j == g.Sp[1]) dym = (int64_t)(g.Np[1] - g.Sp[1]) * g.px;
+        if (j == g.Np[1]) dyp = -(int64_t)(g.Np[1] - g.Sp[1]) * g.px;
+    }
+    // periodic x: the element at Sp takes its left neighbour from Np and the
+    // element at Np its right neighbour from Sp
+    const bool px_per = g.per[0] != 0;
+    const int tsp = g.Sp[0] - is, tnp = g.Np[0] - is;  // positions in this vector (may be outside 0..V-1)
+    const int64_t dwm = (int64_t)(g.Np[0] - is), dwp = (int64_t)(g.Sp[0] - is);
+    int kc0, kc1;
+    red_chunk(r, bz, nbz, kc0, kc1);
+    const int64_t xy = gidx(g, is, j, 0) - 3 * g.pxy;  // element (is, j, k) at xy + (k + 3) * pxy
+    auto zidx = [&](int k) -> int64_t {
+        if (g.per[2]) {
+            if (k < g.Sp[2]) k += g.Np[2] - g.Sp[2] + 1;
+            else if (k > g.Np[2]) k -= g.Np[2] - g.Sp[2] + 1;
+        }
+        return xy + (int64_t)(k + 3) * g.pxy;
+    };
+    struct Plane {
+        VT b, ym, yp, zp;
+        T wm, wp;
+    };
+    auto load = [&](int k, Plane &P) {
+        const int kk = k <= kc1 ? k : kc1;  // past the chunk: reload its last plane (never used)
+        const int64_t x = xy + (int64_t)(kk + 3) * g.pxy;
+        P.b = *(const VT *)(bb + x);
+        P.ym = *(const VT *)(rel + x + dym);
+        P.yp = *(const VT *)(rel + x + dyp);
+        P.zp = *(const VT *)(rel + zidx(kk + 1));
+        P.wm = T(0);
+        P.wp = T(0);
+        if (px_per) {
+            P.wm = rel[x + dwm];
+            P.wp = rel[x + dwp];
+        }
+    };
+    VT zm = *(const VT *)(rel + zidx(kc0 - 1)), zc = *(const VT *)(rel + zidx(kc0));
+    auto relax = [&](int k, const Plane &P) {
+        T xm[V], xp[V];
+        xm[0] = lane_prev(zc[V - 1]);
+        xp[V - 1] = lane_next(zc[0]);
+#pragma unroll
+        for (int t = 1; t < V; ++t) xm[t] = zc[t - 1];
+#pragma unroll
+        for (int t = 0; t < V - 1; ++t) xp[t] = zc[t + 1];
+        if (px_per) {
+#pragma unroll
+            for (int t = 0; t < V; ++t) {
+                if (t == tsp) xm[t] = P.wm;
+                if (t == tnp) xp[t] = P.wp;
+            }
+        }
+        const T g3m = g3[0 * c2 + k], g3p = g3[2 * c2 + k], g3d = g3[1 * c2 + k];
+        TO res[V];
+#pragma unroll
+        for (int t = 0; t < V; ++t) {
+            T s = P.b[t];
+            s = s - a1m[t] * xm[t];
+            s = s - a1p[t] * xp[t];
+            s = s - a2m * P.ym[t];
+            s = s - a2p * P.yp[t];
+            s = s - g3m * zm[t];
+            s = s - g3p * P.zp[t];
+            const T diag = (a1d[t] + a2d) + g3d;
+            res[t] = (TO)(omega * s / diag + (T(1) - omega) * zc[t]);
+        }
+        TO *o = out + xy + (int64_t)(k + 3) * g.pxy;
+        if (full) {
+            typedef TO VO __attribute__((ext_vector_type(V)));
+            VO v;
+#pragma unroll
+            for (int t = 0; t < V; ++t) v[t] = res[t];
+            *(VO *)o = v;
+        } else if (real) {
+#pragma unroll
+            for (int t = 0; t < V; ++t)
+                if (is + t >= r.lo0 && is + t <= hi0) o[t] = res[t];
+        }
+        zm = zc;
+        zc = P.zp;
+    };
+    Plane A, B;
+    load(kc0, A);
+    load(kc0 + 1, B);
+    for (int k = kc0; k <= kc1; k += 2) {
+        relax(k, A);
+        load(k + 2, A);
+        if (k + 1 <= kc1) {
+            relax(k + 1, B);
+            load(k + 3, B);
+        }
+    }
 }
 
 // First FP32 sweep from e = 0 on A e = (float)v: b32 = (float)v is stored for
@@ -881,39 +1045,48 @@ void launch_div_dir(hipStream_t s, const Geo &g, const Coef &c, int q, const dou
     hipLaunchKernelGGL(k_div_dir, grid_for(r), dim3(BX, BY, 1), 0, s, g, c, r, q, dig, lap, assign);
 }
 
-static void jor_grid(const Range3 &r, int &ntxy, int &nbz) {
-    ntxy = (int)(((int64_t)r.n0 * r.n1 + JOR_TPB - 1) / JOR_TPB);
-    nbz = (r.n2 + JOR_KC - 1) / JOR_KC;
+
+// Vector numbering of a range for k_jor_vec: V-point vectors aligned in
+// memory (px and pxy are multiples of 16 elements and every field starts on a
+// 256-byte boundary, so alignment depends on i only).
+static JorVecShape jor_vec_shape(const Geo &g, const Range3 &r, int V, int kc, int &nbw, int &nbz) {
+    JorVecShape sh;
+    const int a = (int)(((int64_t)g.lead + r.lo0 + 3) % V);
+    sh.i0 = r.lo0 - a;
+    sh.nv = (r.lo0 + r.n0 - 1 - sh.i0) / V + 1;
+    sh.nent = sh.nv + 2;
+    sh.nent_all = sh.nent * r.n1;
+    sh.waves = (sh.nent_all + 61) / 62;
+    nbw = (sh.waves + JOR_VEC_TPB / 64 - 1) / (JOR_VEC_TPB / 64);
+    if (kc <= 0) kc = V == 2 ? JOR_VEC_KC64 : JOR_VEC_KC32;
+    nbz = (r.n2 + kc - 1) / kc;
+    return sh;
+}
+
+template <typename T, typename TO>
+static void launch_jor_vec(hipStream_t s, const Geo &g, const Coef &c, const Box &b, T omega, const T *rel,
+                           const T *bb, TO *out, const int *active, int zchunk) {
+    Range3 r = R(b);
+    if (empty(r)) return;
+    int nbw, nbz;
+    const JorVecShape sh = jor_vec_shape(g, r, 16 / (int)sizeof(T), zchunk, nbw, nbz);
+    hipLaunchKernelGGL((k_jor_vec<T, TO>), dim3(xcd_grid((int64_t)nbw * nbz)), dim3(JOR_VEC_TPB), 0, s, g, c, r, sh,
+                       nbw, nbz, omega, rel, bb, out, active);
 }
 
 void launch_jor(hipStream_t s, const Geo &g, const Coef &c, const Box &b, double omega, const double *rel,
-                const double *bb, double *out) {
-    Range3 r = R(b);
-    if (empty(r)) return;
-    int ntxy, nbz;
-    jor_grid(r, ntxy, nbz);
-    hipLaunchKernelGGL((k_jor<double, double>), dim3(xcd_grid((int64_t)ntxy * nbz)), dim3(JOR_TPB), 0, s, g, c, r,
-                       ntxy, nbz, omega, rel, bb, out, (const int *)nullptr);
+                const double *bb, double *out, int zchunk) {
+    launch_jor_vec<double, double>(s, g, c, b, omega, rel, bb, out, nullptr, zchunk);
 }
 
 void launch_jor_f32(hipStream_t s, const Geo &g, const Coef &c, const Box &b, float omega, const float *rel,
-                    const float *bb, float *out, const int *active) {
-    Range3 r = R(b);
-    if (empty(r)) return;
-    int ntxy, nbz;
-    jor_grid(r, ntxy, nbz);
-    hipLaunchKernelGGL((k_jor<float, float>), dim3(xcd_grid((int64_t)ntxy * nbz)), dim3(JOR_TPB), 0, s, g, c, r,
-                       ntxy, nbz, omega, rel, bb, out, active);
+                    const float *bb, float *out, const int *active, int zchunk) {
+    launch_jor_vec<float, float>(s, g, c, b, omega, rel, bb, out, active, zchunk);
 }
 
 void launch_jor_f32_to64(hipStream_t s, const Geo &g, const Coef &c, const Box &b, float omega, const float *rel,
-                         const float *bb, double *out, const int *active) {
-    Range3 r = R(b);
-    if (empty(r)) return;
-    int ntxy, nbz;
-    jor_grid(r, ntxy, nbz);
-    hipLaunchKernelGGL((k_jor<float, double>), dim3(xcd_grid((int64_t)ntxy * nbz)), dim3(JOR_TPB), 0, s, g, c, r,
-                       ntxy, nbz, omega, rel, bb, out, active);
+                         const float *bb, double *out, const int *active, int zchunk) {
+    launch_jor_vec<float, double>(s, g, c, b, omega, rel, bb, out, active, zchunk);
 }
 
 void launch_jor_f32_first(hipStream_t s, const Geo &g, const Coef &c, const Box &b, float omega, const double *v,

@@ -85,7 +85,6 @@ __global__ __launch_bounds__(TX *TY) void k_apply(
     constexpr int HL = (NH + NT - 1) / NT;
     constexpr int NG1X = 5 * TY, NG2X = 5 * TX;
     static_assert(NG1X + NG2X <= NT, "one extra gradient job per thread at most");
-    constexpr int E1 = NT - NG1X;
     constexpr int G1S = TX + 5;                 // row stride of the G_1 tile
     constexpr int OG2 = TY * G1S;               // offset of the G_2 tile
     constexpr int GS = OG2 + (TY + 5) * TX + 1; // + one dummy slot
@@ -170,8 +169,8 @@ __global__ __launch_bounds__(TX *TY) void k_apply(
         const double *cx = c.G[0];
         int cix = 0, cstx = c0;
         vx = false;
-        const int e1 = tid - E1;
-        if (e1 >= 0) {
+        const int e1 = tid - NG2X;
+        if (e1 >= 0 && e1 < NG1X) {
             const int r1x = e1 / 5, u1x = TX + e1 % 5;
             int px_ = x0 - 3 + u1x;
             vx = px_ <= xe + 2 && g_src(g, 0, px_) && !wall_plane(g, 1, y0 + r1x);
@@ -288,10 +287,12 @@ __global__ __launch_bounds__(TX *TY) void k_apply(
 #pragma unroll
             for (int s = 1; s < 6; ++s) b = b + cg2[s] * P[b2 + s * PW];
             G[w2] = (v2 && !zk) ? b : 0.0;
-            double e = cgx[0] * P[bxj];
+            if (tid < NG1X + NG2X) {  // halo gradient jobs; the last wavefront has none
+                double e = cgx[0] * P[bxj];
 #pragma unroll
-            for (int s = 1; s < 6; ++s) e = e + cgx[s] * P[bxj + s * sxj];
-            G[wxj] = (vx && !zk) ? e : 0.0;
+                for (int s = 1; s < 6; ++s) e = e + cgx[s] * P[bxj + s * sxj];
+                G[wxj] = (vx && !zk) ? e : 0.0;
+            }
         }
         // divergence of plane k-1 (its gradient tile is the other buffer)
         if (k > kc0) {

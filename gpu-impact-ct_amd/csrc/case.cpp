@@ -1307,10 +1307,10 @@ struct ih_batch {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int tpb = 256, zchunk = 16, layout = 0;
-    int ntiles = 0, row_tiles = 0;
-    JorBatchItem *items[2][2] = {};  // [fp32][direction]
-    int *tile_item = nullptr;        // item of every global tile
-    int tile_cap = 0;
+    int ntiles[2] = {}, row_tiles[2] = {};  // [fp32]: the two types tile differently
+    JorBatchItem *items[2][2] = {};        // [fp32][direction]
+    int *tile_item[2] = {};                // [fp32]: item of every global tile
+    int tile_cap[2] = {};
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     int g_sweeps = -1, g_fp32 = -1, g_tpb = -1, g_zchunk = -1, g_layout = -1;
@@ -1327,67 +1327,70 @@ void batch_drop_graph(ih_batch *b) {
     b->g_sweeps = -1;
 }
 
-// item tables for the current tile shape
+// item tables for the current tile shape, one numbering per type (2 doubles
+// or 4 floats per lane)
 int batch_build(ih_batch *b) {
-    std::vector<JorBatchItem> h[2][2];
-    int t0 = 0;
-    b->row_tiles = 0;
-    for (ih_case *c : b->cases) {
-        const Box pb = pbox(c->g);
-        int ntxy, nbz;
-        const int nt = jor_batch_tiles(pb, b->tpb, b->zchunk, ntxy, nbz);
-        b->row_tiles = std::max(b->row_tiles, nt);
-        JorBatchItem it{};
-        it.g = c->g;
-        it.r = Range3{pb.lo[0], pb.lo[1], pb.lo[2], pb.hi[0] - pb.lo[0] + 1, pb.hi[1] - pb.lo[1] + 1,
-                      pb.hi[2] - pb.lo[2] + 1};
-        for (int d = 0; d < 3; ++d) {
-            it.dg[d] = c->c.dg[d];
-            it.dgf[d] = c->c.dgf[d];
+    for (int p = 0; p < 2; ++p) {
+        std::vector<JorBatchItem> h[2];
+        int t0 = 0;
+        b->row_tiles[p] = 0;
+        for (ih_case *c : b->cases) {
+            const Box pb = pbox(c->g);
+            JorBatchItem it{};
+            const int nt = jor_batch_tiles(pb, c->g, p ? 4 : 2, b->tpb, b->zchunk, it.sh, it.ntxy, it.nbz);
+            b->row_tiles[p] = std::max(b->row_tiles[p], nt);
+            it.g = c->g;
+            it.r = Range3{pb.lo[0], pb.lo[1], pb.lo[2], pb.hi[0] - pb.lo[0] + 1, pb.hi[1] - pb.lo[1] + 1,
+                          pb.hi[2] - pb.lo[2] + 1};
+            for (int d = 0; d < 3; ++d) {
+                it.dg[d] = c->c.dg[d];
+                it.dgf[d] = c->c.dgf[d];
+            }
+            it.tile0 = t0;
+            t0 += nt;
+            for (int dir = 0; dir < 2; ++dir) {
+                JorBatchItem a = it;
+                if (p == 0) {
+                    a.rel = dir == 0 ? F(c, IH_PRE) : F(c, IH_COMP);
+                    a.out = dir == 0 ? F(c, IH_COMP) : F(c, IH_PRE);
+                    a.bb = F(c, IH_RES);
+                } else {
+                    a.rel = dir == 0 ? (void *)c->f32[0] : (void *)c->f32[2];
+                    a.out = dir == 0 ? (void *)c->f32[2] : (void *)c->f32[0];
+                    a.bb = c->f32[1];
+                }
+                h[dir].push_back(a);
+            }
         }
-        it.ntxy = ntxy;
-        it.nbz = nbz;
-        it.tile0 = t0;
-        t0 += nt;
-        for (int dir = 0; dir < 2; ++dir) {
-            JorBatchItem a = it, f = it;
-            a.rel = dir == 0 ? F(c, IH_PRE) : F(c, IH_COMP);
-            a.out = dir == 0 ? F(c, IH_COMP) : F(c, IH_PRE);
-            a.bb = F(c, IH_RES);
-            f.rel = dir == 0 ? (void *)c->f32[0] : (void *)c->f32[2];
-            f.out = dir == 0 ? (void *)c->f32[2] : (void *)c->f32[0];
-            f.bb = c->f32[1];
-            h[0][dir].push_back(a);
-            h[1][dir].push_back(f);
+        b->ntiles[p] = t0;
+        std::vector<int> ti((size_t)t0);
+        for (size_t q = 0; q < b->cases.size(); ++q) {
+            const int a = h[0][q].tile0, e = q + 1 < b->cases.size() ? h[0][q + 1].tile0 : t0;
+            for (int t = a; t < e; ++t) ti[(size_t)t] = (int)q;
         }
-    }
-    b->ntiles = t0;
-    std::vector<int> ti((size_t)t0);
-    for (size_t q = 0; q < b->cases.size(); ++q) {
-        const int a = h[0][0][q].tile0, e = q + 1 < b->cases.size() ? h[0][0][q + 1].tile0 : t0;
-        for (int t = a; t < e; ++t) ti[(size_t)t] = (int)q;
-    }
-    if (t0 > b->tile_cap) {
-        if (b->tile_item) (void)hipFree(b->tile_item);
-        b->tile_item = nullptr;
-        HIPCHK(hipMalloc(&b->tile_item, sizeof(int) * (size_t)t0));
-        b->tile_cap = t0;
-    }
-    HIPCHK(hipMemcpy(b->tile_item, ti.data(), sizeof(int) * (size_t)t0, hipMemcpyHostToDevice));
-    for (int p = 0; p < 2; ++p)
+        if (t0 > b->tile_cap[p]) {
+            if (b->tile_item[p]) (void)hipFree(b->tile_item[p]);
+            b->tile_item[p] = nullptr;
+            HIPCHK(hipMalloc(&b->tile_item[p], sizeof(int) * (size_t)t0));
+            b->tile_cap[p] = t0;
+        }
+        if (t0 > 0)
+            HIPCHK(hipMemcpy(b->tile_item[p], ti.data(), sizeof(int) * (size_t)t0, hipMemcpyHostToDevice));
         for (int dir = 0; dir < 2; ++dir) {
             if (!b->items[p][dir]) HIPCHK(hipMalloc(&b->items[p][dir], sizeof(JorBatchItem) * b->cases.size()));
-            HIPCHK(hipMemcpy(b->items[p][dir], h[p][dir].data(), sizeof(JorBatchItem) * b->cases.size(),
+            HIPCHK(hipMemcpy(b->items[p][dir], h[dir].data(), sizeof(JorBatchItem) * b->cases.size(),
                              hipMemcpyHostToDevice));
         }
+    }
     batch_drop_graph(b);
     return IH_OK;
 }
 
 int batch_sweeps(ih_batch *b, int n, double omega, bool fp32) {
     for (int q = 0; q < n; ++q)
-        if (launch_jor_batch(b->stream, b->items[fp32 ? 1 : 0][q & 1], b->tile_item, b->ntiles, (int)b->cases.size(),
-                             b->row_tiles, b->layout, b->tpb, fp32, omega))
+        if (launch_jor_batch(b->stream, b->items[fp32 ? 1 : 0][q & 1], b->tile_item[fp32 ? 1 : 0],
+                             b->ntiles[fp32 ? 1 : 0], (int)b->cases.size(), b->row_tiles[fp32 ? 1 : 0], b->layout,
+                             b->tpb, fp32, omega))
             return fail(IH_EINVAL, "unsupported tile shape");
     LAUNCHCHK();
     return IH_OK;
@@ -1432,7 +1435,8 @@ void ih_batch_destroy(ih_batch *b) {
     for (auto &pp : b->items)
         for (auto *q : pp)
             if (q) (void)hipFree(q);
-    if (b->tile_item) (void)hipFree(b->tile_item);
+    for (int *t : b->tile_item)
+        if (t) (void)hipFree(t);
     if (b->ev0) (void)hipEventDestroy(b->ev0);
     if (b->ev1) (void)hipEventDestroy(b->ev1);
     if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -1454,7 +1458,7 @@ int ih_batch_autotune(ih_batch *b, int sweeps, int *tpb, int *zchunk, int *layou
     if (!b || sweeps < 1) return fail(IH_EINVAL, "bad argument");
     HIPCHK(hipSetDevice(b->device));
     for (ih_case *c : b->cases) HIPCHK(hipStreamSynchronize(c->stream));
-    static const int tpbs[3] = {128, 256, 512}, zcs[3] = {8, 16, 32};
+    static const int tpbs[3] = {128, 256, 512}, zcs[5] = {8, 12, 16, 24, 32};
     double best = 1e300;
     int bt = b->tpb, bz = b->zchunk, bl = b->layout, rc;
     const int n = (int)b->cases.size();
@@ -1466,13 +1470,13 @@ int ih_batch_autotune(ih_batch *b, int sweeps, int *tpb, int *zchunk, int *layou
                 if ((rc = batch_build(b))) return rc;
                 // direction 0 only (PRE -> COMP): the relaxed field is left untouched
                 for (int q = 0; q < 2; ++q)
-                    if (launch_jor_batch(b->stream, b->items[0][0], b->tile_item, b->ntiles, n, b->row_tiles, lay, a,
-                                         false, 0.8))
+                    if (launch_jor_batch(b->stream, b->items[0][0], b->tile_item[0], b->ntiles[0], n, b->row_tiles[0],
+                                         lay, a, false, 0.8))
                         return fail(IH_EINVAL, "tile shape");
                 HIPCHK(hipEventRecord(b->ev0, b->stream));
                 for (int q = 0; q < sweeps; ++q)
-                    launch_jor_batch(b->stream, b->items[0][0], b->tile_item, b->ntiles, n, b->row_tiles, lay, a, false,
-                                     0.8);
+                    launch_jor_batch(b->stream, b->items[0][0], b->tile_item[0], b->ntiles[0], n, b->row_tiles[0], lay,
+                                     a, false, 0.8);
                 HIPCHK(hipEventRecord(b->ev1, b->stream));
                 HIPCHK(hipEventSynchronize(b->ev1));
                 LAUNCHCHK();

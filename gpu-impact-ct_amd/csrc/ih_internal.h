@@ -109,19 +109,34 @@ void launch_correct_f32(hipStream_t s, const Geo &g, const Box &b, const float *
 // canonical reductions: returns number of partial blocks written
 int64_t red_blocks(const Box &b);
 
-// One case of a batched relaxation (k_jor_batch): geometry, relaxation box, cdg
-// in FP64 and FP32, the arrays of the sweep direction and the case's first
-// global tile.
+// Vector numbering of a relaxation range for the JOR sweep: every row is nv
+// aligned V-point vectors plus a halo vector at either end (nent entries), the
+// rows one flat sequence of nent_all entries, covered by `waves` wavefronts
+// that advance by 62 entries.
+struct JorVecShape {
+    int i0;        // first point of the first vector of each row (aligned)
+    int nv;        // vectors per row covering the range
+    int nent;      // entries per row: nv + 2
+    int nent_all;  // nent * rows
+    int waves;     // wavefronts over one plane
+};
+
+// One case of a batched relaxation (k_jor_batch): geometry, relaxation box, its
+// vector numbering, cdg in FP64 and FP32, the arrays of the sweep direction,
+// tiles per plane chunk (ntxy), chunks (nbz) and the case's first global tile.
 struct JorBatchItem {
     Geo g;
     Range3 r;
+    JorVecShape sh;
     const double *dg[3];
     const float *dgf[3];
     const void *rel, *bb;
     void *out;
     int ntxy, nbz, tile0, pad;
 };
-int jor_batch_tiles(const Box &b, int tpb, int zchunk, int &ntxy, int &nbz);
+// tiles of box b for the batch kernels: V points per lane (2 FP64, 4 FP32),
+// tpb threads per tile, z chunks of at most zchunk planes
+int jor_batch_tiles(const Box &b, const Geo &g, int V, int tpb, int zchunk, JorVecShape &sh, int &ntxy, int &nbz);
 // layout 0: one flat grid over all tiles with a tile -> item table; 1: a row
 // of row_tiles blocks per item
 int launch_jor_batch(hipStream_t s, const JorBatchItem *items, const int *tile_item, int ntiles, int nitems,

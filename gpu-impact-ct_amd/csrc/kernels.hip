@@ -377,95 +377,18 @@ __device__ inline void jor_neighbours(const Geo &g, int i, int j, int k, int64_t
     }
 }
 
-// Flat form: the (i, j) plane of the range is numbered i-fastest and cut into
-// tiles of TPB consecutive points (no idle lanes at the ragged x end), and z
-// into balanced chunks.  The loop is unrolled twice with two register sets so
-// that the loads of plane k+1 are in flight while plane k is relaxed.
-template <typename T, int TPB, typename TO = T, typename PC = const T *, typename PO = TO *>
-__device__ inline void jor_flat(const Geo &g, const Range3 &r, int nbz, int txy, int bz, T omega, PC __restrict__ rel,
-                                PC __restrict__ bb, PO __restrict__ out, PC g1, PC g2, PC g3) {
-    const int64_t f = (int64_t)txy * TPB + threadIdx.x;
-    if (f >= (int64_t)r.n0 * r.n1) return;
-    const int i = r.lo0 + (int)(f % r.n0);
-    const int j = r.lo1 + (int)(f / r.n0);
-    int kc0, kc1;
-    red_chunk(r, bz, nbz, kc0, kc1);
-    const int c0 = g.cst[0], c1 = g.cst[1], c2 = g.cst[2];
-    int64_t dxm = -1, dxp = 1, dym = -(int64_t)g.px, dyp = g.px;
-    if (g.per[0]) {
-        if (i == g.Sp[0]) dxm = g.Np[0] - g.Sp[0];
-        if (i == g.Np[0]) dxp = -(g.Np[0] - g.Sp[0]);
-    }
-    if (g.per[1]) {
-        if (j == g.Sp[1]) dym = (int64_t)(g.Np[1] - g.Sp[1]) * g.px;
-        if (j == g.Np[1]) dyp = -(int64_t)(g.Np[1] - g.Sp[1]) * g.px;
-    }
-    const T a1m = g1[0 * c0 + i], a1p = g1[2 * c0 + i], a1d = g1[1 * c0 + i];
-    const T a2m = g2[0 * c1 + j], a2p = g2[2 * c1 + j], a2d = g2[1 * c1 + j];
-    auto zidx = [&](int k) -> int64_t {
-        if (g.per[2]) {
-            if (k < g.Sp[2]) k += g.Np[2] - g.Sp[2] + 1;
-            else if (k > g.Np[2]) k -= g.Np[2] - g.Sp[2] + 1;
-        }
-        return gidx(g, i, j, k);
-    };
-    struct Plane { T b, xm, xp, ym, yp, zp; };
-    auto load = [&](int k, Plane &P) {
-        const int kk = k <= kc1 ? k : kc1;  // past the chunk: reload its last plane (never used)
-        const int64_t x = gidx(g, i, j, kk);
-        P.b = bb[x];
-        P.xm = rel[x + dxm];
-        P.xp = rel[x + dxp];
-        P.ym = rel[x + dym];
-        P.yp = rel[x + dyp];
-        P.zp = rel[zidx(kk + 1)];
-    };
-    T zm = rel[zidx(kc0 - 1)], zc = rel[zidx(kc0)];
-    auto relax = [&](int k, const Plane &P) {
-        T t = P.b;
-        t = t - a1m * P.xm;
-        t = t - a1p * P.xp;
-        t = t - a2m * P.ym;
-        t = t - a2p * P.yp;
-        t = t - g3[0 * c2 + k] * zm;
-        t = t - g3[2 * c2 + k] * P.zp;
-        const T diag = (a1d + a2d) + g3[1 * c2 + k];
-        out[gidx(g, i, j, k)] = (TO)(omega * t / diag + (T(1) - omega) * zc);
-        zm = zc;
-        zc = P.zp;
-    };
-    Plane A, B;
-    load(kc0, A);
-    load(kc0 + 1, B);
-    for (int k = kc0; k <= kc1; k += 2) {
-        relax(k, A);
-        load(k + 2, A);
-        if (k + 1 <= kc1) {
-            relax(k + 1, B);
-            load(k + 3, B);
-        }
-    }
-}
-
-// Vector form of the sweep, used for single-case launches.  A lane relaxes
-// V = 16 / sizeof(T) consecutive points of one row, so every load and store is
-// one 16-byte access per lane: the L1 address path handles a quad of lanes per
-// clock whatever the width, and 8-byte accesses left it, not HBM, as the limit.
-// The x neighbours come from the adjacent lanes by DPP.  Each row is numbered
-// as nv vectors (aligned, so the first and last may reach past the range) plus
-// one halo vector at either end that only supplies a neighbour, and the rows
-// follow each other in one flat sequence.  A wavefront covers 64 consecutive
-// entries of it and advances by 62: its first and last lane only supply the
-// neighbours of lanes 1 and 62, so no lane has to load an x neighbour.
-// Periodic x reads the wrapped neighbour with one more load per plane.
-struct JorVecShape {
-    int i0;      // first point of the first vector of each row (aligned)
-    int nv;      // vectors per row covering the range
-    int nent;    // entries per row: nv + 2
-    int nent_all;  // nent * rows
-    int waves;   // wavefronts over one plane
-};
-
+// Vector form of the sweep.  A lane relaxes V = 16 / sizeof(T) consecutive
+// points of one row, so every load and store is one 16-byte access per lane:
+// the L1 address path handles a quad of lanes per clock whatever the width, and
+// 8-byte accesses left it, not HBM, as the limit.  The x neighbours come from
+// the adjacent lanes by DPP.  Each row is numbered as nv vectors (aligned, so
+// the first and last may reach past the range) plus one halo vector at either
+// end that only supplies a neighbour, and the rows follow each other in one
+// flat sequence (JorVecShape).  A wavefront covers 64 consecutive entries of it
+// and advances by 62: its first and last lane only supply the neighbours of
+// lanes 1 and 62, so no lane has to load an x neighbour.  Periodic x reads the
+// wrapped neighbour with one more load per plane.  A block of TPB threads is
+// one tile of TPB/64 wavefronts; z is cut into balanced chunks.
 template <typename T>
 struct VecOf;
 template <>
@@ -476,6 +399,31 @@ template <>
 struct VecOf<float> {
     typedef float type __attribute__((ext_vector_type(4)));
 };
+
+// A pointer read from a table is a generic pointer, so loads through it would be
+// flat instructions; declaring it global (where every field lives) lets the
+// compiler emit global loads as for kernel arguments.
+template <typename T>
+using gconst = const __attribute__((address_space(1))) T *;
+template <typename T>
+using gmut = __attribute__((address_space(1))) T *;
+
+template <typename VT, typename T>
+__device__ inline VT vload(const T *p) {
+    return *(const VT *)p;
+}
+template <typename VT, typename T>
+__device__ inline VT vload(gconst<T> p) {
+    return *(gconst<VT>)p;
+}
+template <typename VT, typename T>
+__device__ inline void vstore(T *p, VT v) {
+    *(VT *)p = v;
+}
+template <typename VT, typename T>
+__device__ inline void vstore(gmut<T> p, VT v) {
+    *(gmut<VT>)p = v;
+}
 
 // Value of the previous (next) lane of the wavefront; lane 0 (63) gets 0.
 __device__ inline float lane_prev(float v) {
@@ -497,18 +445,23 @@ __device__ inline double lane_next(double v) {
 
 // default z chunk (planes per block) of the double and the float sweep
 constexpr int JOR_VEC_TPB = 256, JOR_VEC_KC64 = 12, JOR_VEC_KC32 = 10;
+// register sets of the plane pipeline (planes in flight + 1); deeper sets
+// measured no faster for either type
+constexpr int JOR_VEC_NB = 2;
 
-template <typename T, typename TO>
-__global__ __launch_bounds__(JOR_VEC_TPB) void k_jor_vec(Geo g, Coef c, Range3 r, JorVecShape sh, int nbw, int nbz,
-                                                         T omega, const T *__restrict__ rel, const T *__restrict__ bb,
-                                                         TO *__restrict__ out, const int *active) {
+// One block: tile tw of the vector numbering, z chunk bz of nbz.  T is the
+// arithmetic type (double, or float with the FP32 copy of cdg), TO the type
+// written (a float sweep may write its result widened to double).
+template <typename T, typename TO, int TPB, typename PC, typename PO>
+__device__ inline void jor_vec_block(const Geo &g, const Range3 &r, const JorVecShape &sh, int tw, int bz, int nbz,
+                                     T omega, PC __restrict__ rel, PC __restrict__ bb, PO __restrict__ out, PC g1,
+                                     PC g2, PC g3) {
     constexpr int V = 16 / sizeof(T);
+    constexpr int NB = JOR_VEC_NB;
     typedef typename VecOf<T>::type VT;
-    if (active && !*active) return;
-    int tw, dummy, bz;
-    if (!xcd_tile(nbw, 1, nbz, tw, dummy, bz)) return;
+    typedef TO VO __attribute__((ext_vector_type(V)));
     const int lane = threadIdx.x & 63;
-    const int w = tw * (JOR_VEC_TPB / 64) + (int)(threadIdx.x >> 6);
+    const int w = tw * (TPB / 64) + (int)(threadIdx.x >> 6);
     if (w >= sh.waves) return;  // whole wavefronts only
     int q = w * 62 + lane - 1;
     const bool out_lane = lane >= 1 && lane <= 62 && q < sh.nent_all;
@@ -519,9 +472,6 @@ __global__ __launch_bounds__(JOR_VEC_TPB) void k_jor_vec(Geo g, Coef c, Range3 r
     const int hi0 = r.lo0 + r.n0 - 1;
     const bool real = out_lane && e >= 1 && e <= sh.nv;
     const bool full = real && is >= r.lo0 && is + V - 1 <= hi0;
-    const T *g1 = (const T *)(sizeof(T) == 8 ? (const void *)c.dg[0] : (const void *)c.dgf[0]);
-    const T *g2 = (const T *)(sizeof(T) == 8 ? (const void *)c.dg[1] : (const void *)c.dgf[1]);
-    const T *g3 = (const T *)(sizeof(T) == 8 ? (const void *)c.dg[2] : (const void *)c.dgf[2]);
     const int c0 = g.cst[0], c1 = g.cst[1], c2 = g.cst[2];
     T a1m[V], a1p[V], a1d[V];
 #pragma unroll
@@ -560,10 +510,10 @@ __global__ __launch_bounds__(JOR_VEC_TPB) void k_jor_vec(Geo g, Coef c, Range3 r
     auto load = [&](int k, Plane &P) {
         const int kk = k <= kc1 ? k : kc1;  // past the chunk: reload its last plane (never used)
         const int64_t x = xy + (int64_t)(kk + 3) * g.pxy;
-        P.b = *(const VT *)(bb + x);
-        P.ym = *(const VT *)(rel + x + dym);
-        P.yp = *(const VT *)(rel + x + dyp);
-        P.zp = *(const VT *)(rel + zidx(kk + 1));
+        P.b = vload<VT>(bb + x);
+        P.ym = vload<VT>(rel + x + dym);
+        P.yp = vload<VT>(rel + x + dyp);
+        P.zp = vload<VT>(rel + zidx(kk + 1));
         P.wm = T(0);
         P.wp = T(0);
         if (px_per) {
@@ -571,7 +521,7 @@ __global__ __launch_bounds__(JOR_VEC_TPB) void k_jor_vec(Geo g, Coef c, Range3 r
             P.wp = rel[x + dwp];
         }
     };
-    VT zm = *(const VT *)(rel + zidx(kc0 - 1)), zc = *(const VT *)(rel + zidx(kc0));
+    VT zm = vload<VT>(rel + zidx(kc0 - 1)), zc = vload<VT>(rel + zidx(kc0));
     auto relax = [&](int k, const Plane &P) {
         T xm[V], xp[V];
         xm[0] = lane_prev(zc[V - 1]);
@@ -587,7 +537,8 @@ __global__ __launch_bounds__(JOR_VEC_TPB) void k_jor_vec(Geo g, Coef c, Range3 r
                 if (t == tnp) xp[t] = P.wp;
             }
         }
-        const T g3m = g3[0 * c2 + k], g3p = g3[2 * c2 + k], g3d = g3[1 * c2 + k];
+        const int kg = k <= kc1 ? k : kc1;
+        const T g3m = g3[0 * c2 + kg], g3p = g3[2 * c2 + kg], g3d = g3[1 * c2 + kg];
         TO res[V];
 #pragma unroll
         for (int t = 0; t < V; ++t) {
@@ -601,14 +552,14 @@ __global__ __launch_bounds__(JOR_VEC_TPB) void k_jor_vec(Geo g, Coef c, Range3 r
             const T diag = (a1d[t] + a2d) + g3d;
             res[t] = (TO)(omega * s / diag + (T(1) - omega) * zc[t]);
         }
-        TO *o = out + xy + (int64_t)(k + 3) * g.pxy;
-        if (full) {
-            typedef TO VO __attribute__((ext_vector_type(V)));
+        const PO o = out + xy + (int64_t)(k + 3) * g.pxy;
+        const bool in = k <= kc1;  // planes past the chunk are relaxed but not stored
+        if (full && in) {
             VO v;
 #pragma unroll
             for (int t = 0; t < V; ++t) v[t] = res[t];
-            *(VO *)o = v;
-        } else if (real) {
+            vstore<VO>(o, v);
+        } else if (real && in) {
 #pragma unroll
             for (int t = 0; t < V; ++t)
                 if (is + t >= r.lo0 && is + t <= hi0) o[t] = res[t];
@@ -616,17 +567,30 @@ __global__ __launch_bounds__(JOR_VEC_TPB) void k_jor_vec(Geo g, Coef c, Range3 r
         zm = zc;
         zc = P.zp;
     };
-    Plane A, B;
-    load(kc0, A);
-    load(kc0 + 1, B);
-    for (int k = kc0; k <= kc1; k += 2) {
-        relax(k, A);
-        load(k + 2, A);
-        if (k + 1 <= kc1) {
-            relax(k + 1, B);
-            load(k + 3, B);
+    // NB register sets: the loads of plane k + NB - 1 are in flight while plane k is relaxed
+    Plane S[NB];
+#pragma unroll
+    for (int u = 0; u < NB; ++u) load(kc0 + u, S[u]);
+    for (int k = kc0; k <= kc1; k += NB) {
+#pragma unroll
+        for (int u = 0; u < NB; ++u) {
+            relax(k + u, S[u]);
+            load(k + u + NB, S[u]);
         }
     }
+}
+
+template <typename T, typename TO>
+__global__ __launch_bounds__(JOR_VEC_TPB) void k_jor_vec(Geo g, Coef c, Range3 r, JorVecShape sh, int nbw, int nbz,
+                                                         T omega, const T *__restrict__ rel, const T *__restrict__ bb,
+                                                         TO *__restrict__ out, const int *active) {
+    if (active && !*active) return;
+    int tw, dummy, bz;
+    if (!xcd_tile(nbw, 1, nbz, tw, dummy, bz)) return;
+    const T *g1 = (const T *)(sizeof(T) == 8 ? (const void *)c.dg[0] : (const void *)c.dgf[0]);
+    const T *g2 = (const T *)(sizeof(T) == 8 ? (const void *)c.dg[1] : (const void *)c.dgf[1]);
+    const T *g3 = (const T *)(sizeof(T) == 8 ? (const void *)c.dg[2] : (const void *)c.dgf[2]);
+    jor_vec_block<T, TO, JOR_VEC_TPB, const T *, TO *>(g, r, sh, tw, bz, nbz, omega, rel, bb, out, g1, g2, g3);
 }
 
 // First FP32 sweep from e = 0 on A e = (float)v: b32 = (float)v is stored for
@@ -654,14 +618,6 @@ __global__ void k_jor_f32_first(Geo g, Coef c, Range3 r, float omega, const doub
     e[x] = (TO)(omega * t / diag + (1.0f - omega) * z);
 }
 
-// A pointer read from a table is a generic pointer, so loads through it would be
-// flat instructions; declaring it global (where every field lives) lets the
-// compiler emit global loads as for kernel arguments.
-template <typename T>
-using gconst = const __attribute__((address_space(1))) T *;
-template <typename T>
-using gmut = __attribute__((address_space(1))) T *;
-
 // Batched relaxation: the tiles of many cases (grids of any size) in one
 // launch.  Item q owns the global tiles tile0[q] .. tile0[q+1]-1 and
 // tile_item maps a tile to its item; the block relaxes that tile exactly as
@@ -674,13 +630,13 @@ __global__ __launch_bounds__(TPB) void k_jor_batch(const JorBatchItem *__restric
     if (!xcd_tile(ntiles, 1, 1, t, dummy, dummy2)) return;
     const JorBatchItem &it = items[tile_item[t]];
     const int local = t - it.tile0;
-    const int txy = local % it.ntxy, bz = local / it.ntxy;
+    const int tw = local % it.ntxy, bz = local / it.ntxy;
     const void *g1 = sizeof(T) == 8 ? (const void *)it.dg[0] : (const void *)it.dgf[0];
     const void *g2 = sizeof(T) == 8 ? (const void *)it.dg[1] : (const void *)it.dgf[1];
     const void *g3 = sizeof(T) == 8 ? (const void *)it.dg[2] : (const void *)it.dgf[2];
-    jor_flat<T, TPB, T, gconst<T>, gmut<T>>(it.g, it.r, it.nbz, txy, bz, omega, (gconst<T>)it.rel,
-                                             (gconst<T>)it.bb, (gmut<T>)it.out, (gconst<T>)g1, (gconst<T>)g2,
-                                             (gconst<T>)g3);
+    jor_vec_block<T, T, TPB, gconst<T>, gmut<T>>(it.g, it.r, it.sh, tw, bz, it.nbz, omega, (gconst<T>)it.rel,
+                                                 (gconst<T>)it.bb, (gmut<T>)it.out, (gconst<T>)g1, (gconst<T>)g2,
+                                                 (gconst<T>)g3);
 }
 
 // Row layout of the same batch: grid (row_tiles, cases); a block's case is its
@@ -693,13 +649,13 @@ __global__ __launch_bounds__(TPB) void k_jor_batch_rows(const JorBatchItem *__re
     const int per = row_tiles / NXCD, id = (int)blockIdx.x;
     const int t = (id % NXCD) * per + id / NXCD;  // tiles of a row dealt to the XCDs in contiguous ranges
     if (t >= it.ntxy * it.nbz) return;
-    const int txy = t % it.ntxy, bz = t / it.ntxy;
+    const int tw = t % it.ntxy, bz = t / it.ntxy;
     const void *g1 = sizeof(T) == 8 ? (const void *)it.dg[0] : (const void *)it.dgf[0];
     const void *g2 = sizeof(T) == 8 ? (const void *)it.dg[1] : (const void *)it.dgf[1];
     const void *g3 = sizeof(T) == 8 ? (const void *)it.dg[2] : (const void *)it.dgf[2];
-    jor_flat<T, TPB, T, gconst<T>, gmut<T>>(it.g, it.r, it.nbz, txy, bz, omega, (gconst<T>)it.rel,
-                                             (gconst<T>)it.bb, (gmut<T>)it.out, (gconst<T>)g1, (gconst<T>)g2,
-                                             (gconst<T>)g3);
+    jor_vec_block<T, T, TPB, gconst<T>, gmut<T>>(it.g, it.r, it.sh, tw, bz, it.nbz, omega, (gconst<T>)it.rel,
+                                                 (gconst<T>)it.bb, (gmut<T>)it.out, (gconst<T>)g1, (gconst<T>)g2,
+                                                 (gconst<T>)g3);
 }
 
 // ---------------------------------------------------------------------------
@@ -1046,10 +1002,10 @@ void launch_div_dir(hipStream_t s, const Geo &g, const Coef &c, int q, const dou
 }
 
 
-// Vector numbering of a range for k_jor_vec: V-point vectors aligned in
-// memory (px and pxy are multiples of 16 elements and every field starts on a
-// 256-byte boundary, so alignment depends on i only).
-static JorVecShape jor_vec_shape(const Geo &g, const Range3 &r, int V, int kc, int &nbw, int &nbz) {
+// Vector numbering of a range (px and pxy are multiples of 16 elements and
+// every field starts on a 256-byte boundary, so alignment depends on i only);
+// tiles of tpb threads, z chunks of at most kc planes (0: the type's default).
+static JorVecShape jor_vec_shape(const Geo &g, const Range3 &r, int V, int tpb, int kc, int &nbw, int &nbz) {
     JorVecShape sh;
     const int a = (int)(((int64_t)g.lead + r.lo0 + 3) % V);
     sh.i0 = r.lo0 - a;
@@ -1057,7 +1013,7 @@ static JorVecShape jor_vec_shape(const Geo &g, const Range3 &r, int V, int kc, i
     sh.nent = sh.nv + 2;
     sh.nent_all = sh.nent * r.n1;
     sh.waves = (sh.nent_all + 61) / 62;
-    nbw = (sh.waves + JOR_VEC_TPB / 64 - 1) / (JOR_VEC_TPB / 64);
+    nbw = (sh.waves + tpb / 64 - 1) / (tpb / 64);
     if (kc <= 0) kc = V == 2 ? JOR_VEC_KC64 : JOR_VEC_KC32;
     nbz = (r.n2 + kc - 1) / kc;
     return sh;
@@ -1069,9 +1025,9 @@ static void launch_jor_vec(hipStream_t s, const Geo &g, const Coef &c, const Box
     Range3 r = R(b);
     if (empty(r)) return;
     int nbw, nbz;
-    const JorVecShape sh = jor_vec_shape(g, r, 16 / (int)sizeof(T), zchunk, nbw, nbz);
-    hipLaunchKernelGGL((k_jor_vec<T, TO>), dim3(xcd_grid((int64_t)nbw * nbz)), dim3(JOR_VEC_TPB), 0, s, g, c, r, sh,
-                       nbw, nbz, omega, rel, bb, out, active);
+    const JorVecShape sh = jor_vec_shape(g, r, 16 / (int)sizeof(T), JOR_VEC_TPB, zchunk, nbw, nbz);
+    hipLaunchKernelGGL((k_jor_vec<T, TO>), dim3(xcd_grid((int64_t)nbw * nbz)), dim3(JOR_VEC_TPB), 0, s, g, c,
+                       r, sh, nbw, nbz, omega, rel, bb, out, active);
 }
 
 void launch_jor(hipStream_t s, const Geo &g, const Coef &c, const Box &b, double omega, const double *rel,
@@ -1101,10 +1057,14 @@ void launch_jor_f32_first(hipStream_t s, const Geo &g, const Coef &c, const Box 
                            active);
 }
 
-int jor_batch_tiles(const Box &b, int tpb, int zchunk, int &ntxy, int &nbz) {
+int jor_batch_tiles(const Box &b, const Geo &g, int V, int tpb, int zchunk, JorVecShape &sh, int &ntxy, int &nbz) {
     Range3 r = R(b);
-    ntxy = (int)(((int64_t)r.n0 * r.n1 + tpb - 1) / tpb);
-    nbz = (r.n2 + zchunk - 1) / zchunk;
+    if (empty(r)) {
+        sh = JorVecShape{};
+        ntxy = nbz = 0;
+        return 0;
+    }
+    sh = jor_vec_shape(g, r, V, tpb, zchunk, ntxy, nbz);
     return ntxy * nbz;
 }
 

@@ -219,7 +219,7 @@ int ih_device_interpolate2(ih_case *c, int dimension_id, double *phi, double *in
 typedef struct ih_batch ih_batch;
 int  ih_batch_create(ih_case *const *cases, int n, ih_batch **out);
 void ih_batch_destroy(ih_batch *b);
-/* Times every launch shape -- threads per tile 128/256/512, z chunk 8/16/32,
+/* Times every launch shape -- threads per tile 128/256/512, z chunk 8/12/16/24/32,
  * layout 0 (one flat grid over all tiles) or 1 (one row of blocks per case) --
  * on `sweeps` sweeps without changing PRE, keeps the fastest and reports it
  * with its time per sweep. */

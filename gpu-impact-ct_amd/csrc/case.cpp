@@ -899,7 +899,7 @@ static int bicgstab_run(ih_case *c, double eps, int n_it_max, const int ss[3], c
     LAUNCHCHK();
     const int batch = 8;
     for (;;) {
-        for (int q = 0; q < batch; ++q) launch_krylov_iteration(s, c->g, c->c, b, k, 0);
+        for (int q = 0; q < batch; ++q) launch_krylov_iteration(s, c->g, c->c, b, k);
         LAUNCHCHK();
         HIPCHK(hipMemcpyAsync(c->hkfl, c->kfl, sizeof(int) * KF_NSLOTS, hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
@@ -907,7 +907,7 @@ static int bicgstab_run(ih_case *c, double eps, int n_it_max, const int ss[3], c
         if (c->hkfl[KF_RESET]) {
             // rhr == 0: new reference vector rh = rr, then finish this iteration
             launch_krylov_resume(s, c->g, b, k);
-            launch_krylov_iteration(s, c->g, c->c, b, k, 1);
+            launch_krylov_iteration(s, c->g, c->c, b, k);
             LAUNCHCHK();
         }
     }

@@ -44,8 +44,9 @@ void launch_apply_A(hipStream_t s, const Geo &g, const Coef &c, const Box &pb, c
                     const double *w, int dot_mode, double *part1, double *part2, const int *active);
 void launch_krylov_init(hipStream_t s, const Geo &g, const Box &b, const double *bb, double *rr, double *rh,
                         double *pmax, double *pdot, double *sc, int *fl, double eps, int nmax);
-void launch_krylov_iteration(hipStream_t s, const Geo &g, const Coef &c, const Box &b, const KrylovPtrs &k,
-                             int from_stage);
+// one iteration from p_update to the next iteration's head (k_ctl_end); the head
+// of the first one runs in launch_krylov_init's last kernel
+void launch_krylov_iteration(hipStream_t s, const Geo &g, const Coef &c, const Box &b, const KrylovPtrs &k);
 void launch_krylov_resume(hipStream_t s, const Geo &g, const Box &b, const KrylovPtrs &k);
 
 }  // namespace ih

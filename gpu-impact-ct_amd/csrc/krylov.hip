@@ -433,31 +433,6 @@ __device__ inline double reduce_max_parts(const double *part, int64_t nb, double
     return block_max(m, sm);
 }
 
-// after the initial residual: norm, rhr_next; reset the iteration state
-__global__ __launch_bounds__(256) void k_ctl_setup(const double *pmax, const double *pdot, int64_t nb, double *sc,
-                                                   int *fl, double eps, int nmax) {
-    __shared__ double sm[8];
-    const double nm = reduce_max_parts(pmax, nb, sm);
-    __syncthreads();
-    const double dt = reduce_sum_parts(pdot, nb, sm + 4);
-    if (threadIdx.x == 0 && threadIdx.y == 0) {
-        sc[KS_NORM] = nm;
-        sc[KS_NORM0] = nm;
-        sc[KS_RHR_NEXT] = dt;
-        sc[KS_RHR] = 0.0;
-        sc[KS_ALPHA] = 0.0;
-        sc[KS_OMEGA] = 0.0;
-        sc[KS_EPS] = eps;
-        fl[KF_ACTIVE] = 1;
-        fl[KF_COUNTER] = 0;
-        fl[KF_NMAX] = nmax;
-        fl[KF_EXIT] = 0;
-        fl[KF_RESET] = 0;
-        fl[KF_MODE] = 0;
-        fl[KF_DONE] = 0;
-    }
-}
-
 // status_iteration + the scalar part of the iteration head (mod_solvers.f90:1589-1666)
 __device__ inline void ctl_head_tail(double *sc, int *fl, int counter) {
     if (counter >= 2) {
@@ -478,7 +453,11 @@ __device__ inline void ctl_head_tail(double *sc, int *fl, int counter) {
     }
 }
 
-__global__ void k_ctl_begin(double *sc, int *fl) {
+// The head of an iteration: stop test, counter, rho rotation and beta.  It runs
+// in the same thread right after the scalars it reads were written, at the end
+// of k_ctl_setup and of k_ctl_end (the head of the next iteration), so no
+// launch of its own is needed.
+__device__ inline void ctl_begin(double *sc, int *fl) {
     if (!fl[KF_ACTIVE]) return;
     const double norm = sc[KS_NORM];
     int counter = fl[KF_COUNTER];
@@ -508,6 +487,32 @@ __global__ void k_ctl_begin(double *sc, int *fl) {
         return;
     }
     ctl_head_tail(sc, fl, counter);
+}
+
+// after the initial residual: norm, rhr_next; reset the iteration state
+__global__ __launch_bounds__(256) void k_ctl_setup(const double *pmax, const double *pdot, int64_t nb, double *sc,
+                                                   int *fl, double eps, int nmax) {
+    __shared__ double sm[8];
+    const double nm = reduce_max_parts(pmax, nb, sm);
+    __syncthreads();
+    const double dt = reduce_sum_parts(pdot, nb, sm + 4);
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+        sc[KS_NORM] = nm;
+        sc[KS_NORM0] = nm;
+        sc[KS_RHR_NEXT] = dt;
+        sc[KS_RHR] = 0.0;
+        sc[KS_ALPHA] = 0.0;
+        sc[KS_OMEGA] = 0.0;
+        sc[KS_EPS] = eps;
+        fl[KF_ACTIVE] = 1;
+        fl[KF_COUNTER] = 0;
+        fl[KF_NMAX] = nmax;
+        fl[KF_EXIT] = 0;
+        fl[KF_RESET] = 0;
+        fl[KF_MODE] = 0;
+        fl[KF_DONE] = 0;
+        ctl_begin(sc, fl);  // the head of the first iteration
+    }
 }
 
 __global__ void k_ctl_resume(double *sc, int *fl) {
@@ -559,6 +564,7 @@ __global__ __launch_bounds__(256) void k_ctl_end(const double *pmax, const doubl
     if (threadIdx.x == 0 && threadIdx.y == 0) {
         sc[KS_NORM] = nm;
         sc[KS_RHR_NEXT] = dt;
+        ctl_begin(sc, fl);  // the head of the next iteration
     }
 }
 
@@ -621,12 +627,10 @@ static void launch_precond(hipStream_t s, const Geo &g, const Coef &c, const Box
     launch_jor_f32_to64(s, g, c, b, k.omega, x, k.b32, z, act, k.zchunk);
 }
 
-void launch_krylov_iteration(hipStream_t s, const Geo &g, const Coef &c, const Box &b, const KrylovPtrs &k,
-                             int from_stage) {
+void launch_krylov_iteration(hipStream_t s, const Geo &g, const Coef &c, const Box &b, const KrylovPtrs &k) {
     Range3 r = Rk(b);
     const int64_t nb = red_blocks(b);
     const bool pre = k.sweeps > 0;
-    if (from_stage <= 0) hipLaunchKernelGGL(k_ctl_begin, dim3(1), dim3(1), 0, s, k.sc, k.fl);
     hipLaunchKernelGGL(k_p_update, grid_for(r), dim3(BX, BY, 1), 0, s, g, r, k.rr, k.pk, k.Ap, k.sc, k.fl);
     if (pre) launch_precond(s, g, c, b, k.pk, k.z1, k);
     launch_apply_A(s, g, c, b, pre ? k.z1 : k.pk, k.Ap, k.rh, DOT_W, k.part1, nullptr, k.fl + KF_ACTIVE);

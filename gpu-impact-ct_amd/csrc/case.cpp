@@ -762,49 +762,60 @@ int ih_dot(ih_case *c, const int ss[3], const int nn[3], ih_field a, ih_field b,
     return IH_OK;
 }
 
-// Planes per block along z of the JOR sweep over box b, chosen once per case,
-// precision and box by timing each candidate: which is fastest depends on how
-// the blocks of a launch fill the compute units.  A sweep is a pure function
-// of its inputs, so `sweep` writes the same output however often it runs and
-// whatever chunk it is given.  While the stream is being captured no timing is
-// possible and the default is returned (not remembered).
+// Launch shape (threads per tile, planes per block along z) of the JOR sweep
+// over box b, chosen once per case, precision and box by timing each
+// candidate: which is fastest depends on how the blocks of a launch fill the
+// compute units.  A sweep is a pure function of its inputs, so `sweep` writes
+// the same output however often it runs and whatever shape it is given.
+// While the stream is being captured no timing is possible and the default
+// is returned (not remembered).
 extern "C++" {
 template <typename F>
-static int jor_zchunk(ih_case *c, const Box &b, bool fp32, F sweep) {
+static JorLaunch jor_shape(ih_case *c, const Box &b, bool fp32, F sweep) {
     const std::vector<int> key{fp32 ? 1 : 0, b.lo[0], b.lo[1], b.lo[2], b.hi[0], b.hi[1], b.hi[2]};
-    auto it = c->jor_zchunk.find(key);
-    if (it != c->jor_zchunk.end()) return it->second;
+    auto it = c->jor_shape.find(key);
+    if (it != c->jor_shape.end()) return it->second;
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-    if (hipStreamIsCapturing(c->stream, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return 0;
+    if (hipStreamIsCapturing(c->stream, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return JorLaunch();
     const int n2 = b.hi[2] - b.lo[2] + 1;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
         if (e0) (void)hipEventDestroy(e0);
-        return 0;
+        return JorLaunch();
     }
-    int best = 0, last_nbz = -1;
+    JorLaunch best;
     float best_ms = 0.0f;
-    for (int kc : {6, 8, 10, 12, 14, 16, 20, 24, 32}) {
-        const int nbz = (n2 + kc - 1) / kc;
-        if (nbz == last_nbz) continue;  // same launch as the previous candidate
-        last_nbz = nbz;
-        sweep(kc);  // warm
-        (void)hipEventRecord(e0, c->stream);
-        for (int q = 0; q < 3; ++q) sweep(kc);
-        (void)hipEventRecord(e1, c->stream);
-        float ms = 0.0f;
-        if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
-            best = 0;
-            break;
+    bool have = false, failed = false;
+    for (int tpb : {128, 256, 512}) {
+        int last_nbz = -1;
+        for (int kc : {6, 8, 10, 12, 14, 16, 20, 24, 32}) {
+            const int nbz = (n2 + kc - 1) / kc;
+            if (nbz == last_nbz) continue;  // same launch as the previous candidate
+            last_nbz = nbz;
+            JorLaunch l;
+            l.tpb = tpb;
+            l.kc = kc;
+            sweep(l);  // warm
+            (void)hipEventRecord(e0, c->stream);
+            for (int q = 0; q < 3; ++q) sweep(l);
+            (void)hipEventRecord(e1, c->stream);
+            float ms = 0.0f;
+            if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+                failed = true;
+                break;
+            }
+            if (!have || ms < best_ms) {
+                best = l;
+                best_ms = ms;
+                have = true;
+            }
         }
-        if (best == 0 || ms < best_ms) {
-            best = kc;
-            best_ms = ms;
-        }
+        if (failed) break;
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    c->jor_zchunk[key] = best;
+    if (failed) best = JorLaunch();
+    c->jor_shape[key] = best;
     return best;
 }
 }
@@ -818,12 +829,12 @@ int ih_jor(ih_case *c, int n_sweeps, double omega, const int ss[3], const int nn
     if ((rc = to_box(c, ss, nn, b))) return rc;
     double *x = F(c, rel), *y = F(c, IH_COMP);
     const double *B = F(c, bb);
-    const int kc = n_sweeps > 0 ? jor_zchunk(c, b, false, [&](int z) {
-        launch_jor(c->stream, c->g, c->c, b, omega, x, B, y, z);
-    }) : 0;
+    const JorLaunch sh = n_sweeps > 0 ? jor_shape(c, b, false, [&](JorLaunch l) {
+        launch_jor(c->stream, c->g, c->c, b, omega, x, B, y, l);
+    }) : JorLaunch();
     PhaseTimer pt(c, "jor");
     for (int s = 0; s < n_sweeps; ++s) {
-        launch_jor(c->stream, c->g, c->c, b, omega, x, B, y, kc);
+        launch_jor(c->stream, c->g, c->c, b, omega, x, B, y, sh);
         std::swap(x, y);
     }
     pt.done(n_sweeps);
@@ -886,8 +897,8 @@ static int bicgstab_run(ih_case *c, double eps, int n_it_max, const int ss[3], c
         k.sweeps = sweeps;
         k.omega = (float)omega;
         // the timing sweeps (first solve only) write e1, cleared below
-        k.zchunk = jor_zchunk(c, b, true, [&](int z) {
-            launch_jor_f32(s, c->g, c->c, b, k.omega, k.e0, k.b32, k.e1, nullptr, z);
+        k.shape = jor_shape(c, b, true, [&](JorLaunch l) {
+            launch_jor_f32(s, c->g, c->c, b, k.omega, k.e0, k.b32, k.e1, nullptr, l);
         });
         // ghost layers of the ping-pong buffers must hold finite values (zero)
         launch_fill_f32(s, k.e0, 0.0f, c->g.fe);
@@ -1024,12 +1035,12 @@ static int poisson_solve(ih_case *c, const ih_step_params *p, const Box &b, ih_b
             launch_fill_f32(c->stream, c->f32[0], 0.0f, g.fe);
             launch_fill_f32(c->stream, c->f32[2], 0.0f, g.fe);
             float *x = c->f32[0], *y = c->f32[2];
-            const int kc = jor_zchunk(c, b, true, [&](int z) {
-                launch_jor_f32(c->stream, g, c->c, b, (float)p->jor_omega, x, c->f32[1], y, nullptr, z);
+            const JorLaunch sh = jor_shape(c, b, true, [&](JorLaunch l) {
+                launch_jor_f32(c->stream, g, c->c, b, (float)p->jor_omega, x, c->f32[1], y, nullptr, l);
             });
             PhaseTimer pt(c, "jor_f32");
             for (int s = 0; s < p->jor_sweeps; ++s) {
-                launch_jor_f32(c->stream, g, c->c, b, (float)p->jor_omega, x, c->f32[1], y, nullptr, kc);
+                launch_jor_f32(c->stream, g, c->c, b, (float)p->jor_omega, x, c->f32[1], y, nullptr, sh);
                 std::swap(x, y);
             }
             pt.done(p->jor_sweeps);

@@ -68,5 +68,5 @@ struct ih_case {
     bool ktime = false;
     std::map<std::string, ih::KernelTiming> ktimes;
     std::map<std::string, std::vector<ih::PendingTiming>> kpending;
-    std::map<std::vector<int>, int> jor_zchunk;  // (fp32, box) -> planes per block of the JOR sweep
+    std::map<std::vector<int>, ih::JorLaunch> jor_shape;  // (fp32, box) -> launch shape of the JOR sweep
 };

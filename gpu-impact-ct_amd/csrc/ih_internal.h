@@ -89,11 +89,17 @@ void launch_gradient(hipStream_t s, const Geo &g, const Coef &c, int q, const do
 void launch_grad_walls(hipStream_t s, const Geo &g, int q, double *grad);
 void launch_grad_correct(hipStream_t s, const Geo &g, const Coef &c, int q, const double *phi, double *vel);
 void launch_div_dir(hipStream_t s, const Geo &g, const Coef &c, int q, const double *dig, double *lap, int assign);
-// JOR sweeps; zchunk = planes per block along z (0: the default of the type)
+// Launch shape of a JOR sweep: threads per tile (128, 256 or 512) and planes
+// per block along z; 0 selects the default of the type.
+struct JorLaunch {
+    int tpb = 0;
+    int kc = 0;
+};
+// JOR sweeps
 void launch_jor(hipStream_t s, const Geo &g, const Coef &c, const Box &b, double omega, const double *rel,
-                const double *bb, double *out, int zchunk = 0);
+                const double *bb, double *out, JorLaunch shape = JorLaunch());
 void launch_jor_f32(hipStream_t s, const Geo &g, const Coef &c, const Box &b, float omega, const float *rel,
-                    const float *bb, float *out, const int *active = nullptr, int zchunk = 0);
+                    const float *bb, float *out, const int *active = nullptr, JorLaunch shape = JorLaunch());
 void launch_residual_cdg(hipStream_t s, const Geo &g, const Coef &c, const Box &b, const double *rel,
                          const double *bb, double *out_res /* may be null */, double *partial_max);
 void launch_copy_box(hipStream_t s, const Geo &g, const Box &b, const double *src, double *dst);
@@ -144,7 +150,7 @@ int launch_jor_batch(hipStream_t s, const JorBatchItem *items, const int *tile_i
 // the last FP32 sweep written widened to double, and the first one from e = 0
 // on A e = (float)v (stores b32; writes e, or e64 widened when e64 != nullptr)
 void launch_jor_f32_to64(hipStream_t s, const Geo &g, const Coef &c, const Box &b, float omega, const float *rel,
-                         const float *bb, double *out, const int *active, int zchunk = 0);
+                         const float *bb, double *out, const int *active, JorLaunch shape = JorLaunch());
 void launch_jor_f32_first(hipStream_t s, const Geo &g, const Coef &c, const Box &b, float omega, const double *v,
                           float *b32, float *e, double *e64, const int *active);
 void launch_dot_partial(hipStream_t s, const Geo &g, const Box &b, const double *a, const double *c, double *part);

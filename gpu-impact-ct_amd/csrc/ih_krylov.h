@@ -34,7 +34,7 @@ struct KrylovPtrs {
     float *e0 = nullptr, *e1 = nullptr, *b32 = nullptr;
     int sweeps = 0;
     float omega = 0.8f;
-    int zchunk = 0;  // z chunk of the FP32 sweeps (0: default)
+    JorLaunch shape;  // launch shape of the FP32 sweeps
 };
 
 // dot modes of the operator kernel

@@ -443,7 +443,7 @@ __device__ inline double lane_next(double v) {
     return __hiloint2double(hi, lo);
 }
 
-// default z chunk (planes per block) of the double and the float sweep
+// default threads per tile and z chunk (planes per block) of the double and the float sweep
 constexpr int JOR_VEC_TPB = 256, JOR_VEC_KC64 = 12, JOR_VEC_KC32 = 10;
 // register sets of the plane pipeline (planes in flight + 1); deeper sets
 // measured no faster for either type
@@ -580,8 +580,8 @@ __device__ inline void jor_vec_block(const Geo &g, const Range3 &r, const JorVec
     }
 }
 
-template <typename T, typename TO>
-__global__ __launch_bounds__(JOR_VEC_TPB) void k_jor_vec(Geo g, Coef c, Range3 r, JorVecShape sh, int nbw, int nbz,
+template <typename T, typename TO, int TPB>
+__global__ __launch_bounds__(TPB) void k_jor_vec(Geo g, Coef c, Range3 r, JorVecShape sh, int nbw, int nbz,
                                                          T omega, const T *__restrict__ rel, const T *__restrict__ bb,
                                                          TO *__restrict__ out, const int *active) {
     if (active && !*active) return;
@@ -590,7 +590,7 @@ __global__ __launch_bounds__(JOR_VEC_TPB) void k_jor_vec(Geo g, Coef c, Range3 r
     const T *g1 = (const T *)(sizeof(T) == 8 ? (const void *)c.dg[0] : (const void *)c.dgf[0]);
     const T *g2 = (const T *)(sizeof(T) == 8 ? (const void *)c.dg[1] : (const void *)c.dgf[1]);
     const T *g3 = (const T *)(sizeof(T) == 8 ? (const void *)c.dg[2] : (const void *)c.dgf[2]);
-    jor_vec_block<T, TO, JOR_VEC_TPB, const T *, TO *>(g, r, sh, tw, bz, nbz, omega, rel, bb, out, g1, g2, g3);
+    jor_vec_block<T, TO, TPB, const T *, TO *>(g, r, sh, tw, bz, nbz, omega, rel, bb, out, g1, g2, g3);
 }
 
 // First FP32 sweep from e = 0 on A e = (float)v: b32 = (float)v is stored for
@@ -1021,28 +1021,37 @@ static JorVecShape jor_vec_shape(const Geo &g, const Range3 &r, int V, int tpb, 
 
 template <typename T, typename TO>
 static void launch_jor_vec(hipStream_t s, const Geo &g, const Coef &c, const Box &b, T omega, const T *rel,
-                           const T *bb, TO *out, const int *active, int zchunk) {
+                           const T *bb, TO *out, const int *active, JorLaunch shape) {
     Range3 r = R(b);
     if (empty(r)) return;
+    const int tpb = shape.tpb == 128 || shape.tpb == 512 ? shape.tpb : JOR_VEC_TPB;
     int nbw, nbz;
-    const JorVecShape sh = jor_vec_shape(g, r, 16 / (int)sizeof(T), JOR_VEC_TPB, zchunk, nbw, nbz);
-    hipLaunchKernelGGL((k_jor_vec<T, TO>), dim3(xcd_grid((int64_t)nbw * nbz)), dim3(JOR_VEC_TPB), 0, s, g, c,
-                       r, sh, nbw, nbz, omega, rel, bb, out, active);
+    const JorVecShape sh = jor_vec_shape(g, r, 16 / (int)sizeof(T), tpb, shape.kc, nbw, nbz);
+    const dim3 grid(xcd_grid((int64_t)nbw * nbz));
+    if (tpb == 128)
+        hipLaunchKernelGGL((k_jor_vec<T, TO, 128>), grid, dim3(128), 0, s, g, c, r, sh, nbw, nbz, omega, rel, bb, out,
+                           active);
+    else if (tpb == 512)
+        hipLaunchKernelGGL((k_jor_vec<T, TO, 512>), grid, dim3(512), 0, s, g, c, r, sh, nbw, nbz, omega, rel, bb, out,
+                           active);
+    else
+        hipLaunchKernelGGL((k_jor_vec<T, TO, 256>), grid, dim3(256), 0, s, g, c, r, sh, nbw, nbz, omega, rel, bb, out,
+                           active);
 }
 
 void launch_jor(hipStream_t s, const Geo &g, const Coef &c, const Box &b, double omega, const double *rel,
-                const double *bb, double *out, int zchunk) {
-    launch_jor_vec<double, double>(s, g, c, b, omega, rel, bb, out, nullptr, zchunk);
+                const double *bb, double *out, JorLaunch shape) {
+    launch_jor_vec<double, double>(s, g, c, b, omega, rel, bb, out, nullptr, shape);
 }
 
 void launch_jor_f32(hipStream_t s, const Geo &g, const Coef &c, const Box &b, float omega, const float *rel,
-                    const float *bb, float *out, const int *active, int zchunk) {
-    launch_jor_vec<float, float>(s, g, c, b, omega, rel, bb, out, active, zchunk);
+                    const float *bb, float *out, const int *active, JorLaunch shape) {
+    launch_jor_vec<float, float>(s, g, c, b, omega, rel, bb, out, active, shape);
 }
 
 void launch_jor_f32_to64(hipStream_t s, const Geo &g, const Coef &c, const Box &b, float omega, const float *rel,
-                         const float *bb, double *out, const int *active, int zchunk) {
-    launch_jor_vec<float, double>(s, g, c, b, omega, rel, bb, out, active, zchunk);
+                         const float *bb, double *out, const int *active, JorLaunch shape) {
+    launch_jor_vec<float, double>(s, g, c, b, omega, rel, bb, out, active, shape);
 }
 
 void launch_jor_f32_first(hipStream_t s, const Geo &g, const Coef &c, const Box &b, float omega, const double *v,

@@ -621,10 +621,10 @@ static void launch_precond(hipStream_t s, const Geo &g, const Coef &c, const Box
     launch_jor_f32_first(s, g, c, b, k.omega, v, k.b32, k.e0, nullptr, act);
     float *x = k.e0, *y = k.e1;
     for (int q = 1; q < k.sweeps - 1; ++q) {
-        launch_jor_f32(s, g, c, b, k.omega, x, k.b32, y, act, k.zchunk);
+        launch_jor_f32(s, g, c, b, k.omega, x, k.b32, y, act, k.shape);
         std::swap(x, y);
     }
-    launch_jor_f32_to64(s, g, c, b, k.omega, x, k.b32, z, act, k.zchunk);
+    launch_jor_f32_to64(s, g, c, b, k.omega, x, k.b32, z, act, k.shape);
 }
 
 void launch_krylov_iteration(hipStream_t s, const Geo &g, const Coef &c, const Box &b, const KrylovPtrs &k) {

@@ -1,6 +1,7 @@
 ! A Fortran host calling libimpact_hip the way IMPACT's host calls its device
-! wrappers: arrays with the reference's bounds (-3:N+3 per direction), REAL(8),
-! passed to the reference-signature entry points through iso_c_binding.
+! wrappers: the drop-in subroutines of include/impact_hip_host.f90 (the
+! reference's names and argument lists) on arrays with the reference's bounds
+! (-3:N+3 per direction), plus the resident calls of include/impact_hip_c.f90.
 !
 ! Usage: dropin_driver <output directory>
 ! Grid 17x13x11, walls, L = (2, 1, 1), tanh 1.5 in y and z.  Inputs are the
@@ -10,14 +11,14 @@
 program dropin_driver
   use iso_c_binding
   use impact_hip_c
+  use impact_hip_state
   implicit none
-  integer, parameter :: N1 = 17, N2 = 13, N3 = 11
   real(8), allocatable :: phi(:,:,:), grad(:,:,:), vel(:,:,:,:), div(:,:,:), inter(:,:,:)
-  real(8), allocatable :: x(:,:,:), y(:,:,:), z(:,:,:), rel(:,:,:), bb(:,:,:)
-  type(ih_grid_desc) :: d
-  type(c_ptr) :: ctx
+  real(8), allocatable :: x(:,:,:), y(:,:,:), z(:,:,:), rel(:,:,:), bb(:,:,:), work(:,:,:,:), nl(:,:,:,:)
   character(len=1024) :: outdir
-  integer :: m
+  integer :: m, q, d
+  external :: device_gradient, device_divergence2, device_interpolate, device_interpolate2, device_axpby
+  external :: device_Nonlinear_upwind, device_helmholtz, device_jacobi_solver
 
   call get_command_argument(1, outdir)
   if (len_trim(outdir) == 0) then
@@ -29,74 +30,80 @@ program dropin_driver
      stop 1
   end if
 
-  allocate(phi(-3:N1+3,-3:N2+3,-3:N3+3), grad(-3:N1+3,-3:N2+3,-3:N3+3), div(-3:N1+3,-3:N2+3,-3:N3+3))
-  allocate(inter(-3:N1+3,-3:N2+3,-3:N3+3), vel(-3:N1+3,-3:N2+3,-3:N3+3,1:3))
-  allocate(x(-3:N1+3,-3:N2+3,-3:N3+3), y(-3:N1+3,-3:N2+3,-3:N3+3), z(-3:N1+3,-3:N2+3,-3:N3+3))
-  allocate(rel(-3:N1+3,-3:N2+3,-3:N3+3), bb(-3:N1+3,-3:N2+3,-3:N3+3))
+  ! GPU_malloc
+  call impact_hip_init((/ 17, 13, 11 /), (/ 1, 1, 1, 1, 1, 1 /), (/ 2d0, 1d0, 1d0 /), (/ 0d0, 1.5d0, 1.5d0 /), 0)
 
-  d%version = IH_ABI_VERSION
-  d%N = (/ N1, N2, N3 /)
-  d%bc = (/ 1, 1, 1, 1, 1, 1 /)
-  d%L = (/ 2d0, 1d0, 1d0 /)
-  d%stretch = (/ 0d0, 1.5d0, 1.5d0 /)
-  call check(ih_case_create(d, 0_c_int, ctx))
-  call check(ih_case_generate_coeffs(ctx))
+  allocate(phi(b1L:N1+b1U, b2L:N2+b2U, b3L:N3+b3U), grad(b1L:N1+b1U, b2L:N2+b2U, b3L:N3+b3U))
+  allocate(div(b1L:N1+b1U, b2L:N2+b2U, b3L:N3+b3U), inter(b1L:N1+b1U, b2L:N2+b2U, b3L:N3+b3U))
+  allocate(vel(b1L:N1+b1U, b2L:N2+b2U, b3L:N3+b3U, 1:3), work(b1L:N1+b1U, b2L:N2+b2U, b3L:N3+b3U, 1:3))
+  allocate(nl(b1L:N1+b1U, b2L:N2+b2U, b3L:N3+b3U, 1:3))
+  allocate(x(b1L:N1+b1U, b2L:N2+b2U, b3L:N3+b3U), y(b1L:N1+b1U, b2L:N2+b2U, b3L:N3+b3U))
+  allocate(z(b1L:N1+b1U, b2L:N2+b2U, b3L:N3+b3U))
+  allocate(rel(b1L:N1+b1U, b2L:N2+b2U, b3L:N3+b3U), bb(b1L:N1+b1U, b2L:N2+b2U, b3L:N3+b3U))
 
-  ! device_gradient (src/grad_host.f90:39)
   call fill(phi, 1)
   call fill(grad, 2)
-  call check(ih_device_gradient(ctx, 2_c_int, phi, grad))
+  call device_gradient(2, phi, grad)
   call dump('gradient2', grad)
 
-  ! device_divergence2 (src/divergence2_host.f90:6)
   do m = 1, 3
      call fill(vel(:,:,:,m), 10 + m)
   end do
   call fill(div, 3)
-  call check(ih_device_divergence2(ctx, vel, div))
+  call device_divergence2(vel, div)
   call dump('divergence2', div)
 
-  ! device_interpolate2 (src/interpolate2_vel_pre_host.f90)
   call fill(phi, 4)
   call fill(inter, 5)
-  call check(ih_device_interpolate2(ctx, 3_c_int, phi, inter))
+  call device_interpolate2(3, phi, inter)
   call dump('interpolate2_3', inter)
 
-  ! device_axpby (src/axpby_host.f90:5): z = x - y
   call fill(x, 6)
   call fill(y, 7)
   z = 0d0
-  call check(ih_device_axpby(ctx, 1d0, -1d0, x, y, z))
+  call device_axpby(1d0, -1d0, x, y, z)
   call dump('axpby', z)
+
+  ! the upwind nonlinear term and the explicit Helmholtz step per component,
+  ! as rhs_vel drives them: pp = I_q(work_d), then the upwind term of component
+  ! (q, d), then the Helmholtz term of q
+  do m = 1, 3
+     call fill(vel(:,:,:,m), 49 + m)
+     call fill(work(:,:,:,m), 52 + m)
+     call fill(nl(:,:,:,m), 55 + m)
+  end do
+  multL = 1d0 / 137d0
+  do q = 1, 3
+     do d = 1, 3
+        call device_interpolate(q, work(:,:,:,d), pp)
+        call device_Nonlinear_upwind(3 * (q - 1) + d, vel(:,:,:,q), nl(:,:,:,q))
+     end do
+     call device_helmholtz(q, vel(:,:,:,q), nl(:,:,:,q))
+  end do
+  call dump('nl1', nl(:,:,:,1))
+  call dump('nl2', nl(:,:,:,2))
+  call dump('nl3', nl(:,:,:,3))
+
+  call fill(rel, 20)
+  call fill(bb, 21)
+  call device_jacobi_solver(1, 0.8d0, 1, 1, 1, N1, N2, N3, rel, bb)
+  call dump('jacobi', rel)
 
   ! resident mode: upload, 5 JOR sweeps on the pressure range, download
   call fill(rel, 8)
   call fill(bb, 9)
-  call check(ih_field_upload(ctx, IH_PRE, rel))
-  call check(ih_field_upload(ctx, IH_RES, bb))
-  call check(ih_jor(ctx, 5_c_int, 0.8d0, (/ 1_c_int, 1_c_int, 1_c_int /), (/ N1, N2, N3 /), IH_PRE, IH_RES))
-  call check(ih_field_download(ctx, IH_PRE, rel))
+  call ih_check(ih_field_upload(ih_ctx, IH_PRE, rel))
+  call ih_check(ih_field_upload(ih_ctx, IH_RES, bb))
+  call ih_check(ih_jor(ih_ctx, 5_c_int, 0.8d0, (/ 1_c_int, 1_c_int, 1_c_int /), (/ int(N1, c_int), &
+       int(N2, c_int), int(N3, c_int) /), IH_PRE, IH_RES))
+  call ih_check(ih_field_download(ih_ctx, IH_PRE, rel))
   call dump('jor5', rel)
 
-  call ih_case_destroy(ctx)
+  ! GPU_memfree
+  call impact_hip_finish()
   write(*,*) 'dropin_driver: done'
 
 contains
-
-  subroutine check(rc)
-    integer(c_int), intent(in) :: rc
-    character(kind=c_char) :: msg(512)
-    integer(c_int) :: n, q
-    if (rc /= IH_OK) then
-       n = ih_last_error(msg, int(512, c_size_t))
-       write(*,'(a)', advance='no') 'impact_hip: '
-       do q = 1, min(n, 512)
-          write(*,'(a)', advance='no') msg(q)
-       end do
-       write(*,*)
-       stop 1
-    end if
-  end subroutine check
 
   subroutine fill(a, s)
     real(8), intent(out) :: a(-3:, -3:, -3:)

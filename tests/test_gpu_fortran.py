@@ -1,7 +1,9 @@
 """A Fortran host on the C ABI: tests/fortran/dropin_driver.f90, built with
-amdflang against include/impact_hip_c.f90 (the iso_c_binding module a Fortran
-host such as IMPACT adds), calls the reference-signature entry points with
-arrays of the reference's bounds and a resident JOR, and writes the results.
+amdflang against include/impact_hip_c.f90 (the iso_c_binding module) and
+include/impact_hip_host.f90 (the drop-in device_* subroutines with the
+reference's names and argument lists), calls them on arrays of the
+reference's bounds as IMPACT's host does, runs a resident JOR, and writes the
+results.
 Every result is compared bit for bit with the oracle on the same inputs (an
 exactly representable pattern that numpy and Fortran produce identically)."""
 import subprocess
@@ -10,6 +12,7 @@ from pathlib import Path
 import numpy as np
 import pytest
 
+import impact_hip as ih
 import oracle_lib as O
 
 pytestmark = pytest.mark.gpu
@@ -64,7 +67,22 @@ def test_fortran_host_dropin(tmp_path):
     x, y = pattern(oc, 6), pattern(oc, 7)
     assert np.array_equal(got("axpby"), x - y), "device_axpby"
 
+    vel = np.ascontiguousarray(np.stack([pattern(oc, 49 + m) for m in (1, 2, 3)]))
+    work = np.ascontiguousarray(np.stack([pattern(oc, 52 + m) for m in (1, 2, 3)]))
+    nl = np.ascontiguousarray(np.stack([pattern(oc, 55 + m) for m in (1, 2, 3)]))
+    oc.nonlinear(vel, work, nl, oc.zeros())
+    oc.helmholtz(1.0 / 137.0, vel, nl)
+    for q in range(3):
+        assert np.array_equal(got(f"nl{q + 1}"), nl[q]), f"device_Nonlinear_upwind + device_helmholtz, component {q + 1}"
+
     rel, bb = pattern(oc, 8), pattern(oc, 9)
     oc.jor(5, 0.8, oc.Sp, oc.Np, rel, bb, oc.zeros())
     m = readable(oc)
     assert np.array_equal(got("jor5")[m], rel[m]), "ih_jor"
+
+    # device_jacobi_solver: the same entry point called from Python on the same inputs
+    gc = ih.Case(N, (1, 1, 1, 1, 1, 1), (2.0, 1.0, 1.0), (0.0, 1.5, 1.5))
+    rel, bb = pattern(oc, 20), pattern(oc, 21)
+    gc.device_jacobi_solver(0.8, oc.Sp, oc.Np, rel, bb)
+    assert np.array_equal(got("jacobi"), rel), "device_jacobi_solver"
+    gc.close()

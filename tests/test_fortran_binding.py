@@ -1,5 +1,6 @@
 """The Fortran binding (include/impact_hip_c.f90) against the C header: the
-field ids and the ABI version it declares are the C enum's, and it compiles."""
+field ids and the ABI version it declares are the C enum's, and it compiles,
+as do the drop-in subroutines on top of it (include/impact_hip_host.f90)."""
 import re
 import shutil
 import subprocess
@@ -46,6 +47,8 @@ def test_binding_compiles(tmp_path):
     fc = shutil.which("amdflang") or ("/opt/rocm/bin/amdflang" if Path("/opt/rocm/bin/amdflang").exists() else None)
     if not fc:
         pytest.skip("no Fortran compiler")
-    r = subprocess.run([fc, "-c", f"-J{tmp_path}", str(ROOT / "include" / "impact_hip_c.f90"), "-o",
-                        str(tmp_path / "impact_hip_c.o")], capture_output=True, text=True, timeout=300)
-    assert r.returncode == 0, r.stderr
+    for src in ("impact_hip_c.f90", "impact_hip_host.f90"):
+        r = subprocess.run([fc, "-c", f"-J{tmp_path}", str(ROOT / "include" / src), "-o",
+                            str(tmp_path / (src[:-4] + ".o"))], capture_output=True, text=True, timeout=300,
+                           cwd=tmp_path)
+        assert r.returncode == 0, r.stderr

@@ -832,6 +832,17 @@ int ih_jor(ih_case *c, int n_sweeps, double omega, const int ss[3], const int nn
     const JorLaunch sh = n_sweeps > 0 ? jor_shape(c, b, false, [&](JorLaunch l) {
         launch_jor(c->stream, c->g, c->c, b, omega, x, B, y, l);
     }) : JorLaunch();
+    // Sweeps after the first read the ping-pong field outside the box (the
+    // one-cell frame, and across a periodic direction the wrapped neighbour).
+    // Over the whole pressure range those reads are ghosts with zero wall
+    // coupling or wrapped points of the box itself; over a smaller box they are
+    // points the reference loop reads from rel, unchanged, so COMP starts as a
+    // copy of rel.
+    const Box pb = pbox(c->g);
+    bool whole = true;
+    for (int d = 0; d < 3; ++d) whole = whole && b.lo[d] == pb.lo[d] && b.hi[d] == pb.hi[d];
+    if (n_sweeps > 1 && !whole)
+        HIPCHK(hipMemcpyAsync(y, x, sizeof(double) * (size_t)c->g.fe, hipMemcpyDeviceToDevice, c->stream));
     PhaseTimer pt(c, "jor");
     for (int s = 0; s < n_sweeps; ++s) {
         launch_jor(c->stream, c->g, c->c, b, omega, x, B, y, sh);

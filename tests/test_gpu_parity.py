@@ -207,6 +207,25 @@ def test_jor_and_residual(spec, sweeps):
 
 
 @pytest.mark.parametrize("spec", GRIDS + MULTI)
+@pytest.mark.parametrize("lo,hi", [((2, 2, 2), (-1, -1, -1)), ((3, 1, 2), (0, -2, -1)), ((2, 3, 1), (-2, 0, 0)),
+                                   ((4, 2, 3), (-3, -1, -2))], ids=["even-start", "odd-start", "even-short", "inner"])
+def test_jor_sub_box(spec, lo, hi):
+    """JOR over boxes that start and end anywhere in a row: the vector sweep's
+    first and last vectors then reach past the box on either side (a start at
+    an even i is not aligned to a 2-point vector)."""
+    oc, gc = pair(spec)
+    ss = list(lo)
+    nn = [oc.N[d] + hi[d] for d in range(3)]
+    rel = rnd(oc.shape, 80)
+    bb = rnd(oc.shape, 81)
+    gc.upload(ih.PRE, rel)
+    gc.upload(ih.RES, bb)
+    oc.jor(3, 0.8, ss, nn, rel, bb, oc.zeros())
+    gc.jor(3, 0.8, ss, nn)
+    same(gc.download(ih.PRE)[readable(oc)], rel[readable(oc)], f"jor on {ss}..{nn}")
+
+
+@pytest.mark.parametrize("spec", GRIDS + MULTI)
 def test_dot_fixed_tree(spec):
     oc, gc = pair(spec)
     a = rnd(oc.shape, 80)

@@ -451,8 +451,12 @@ constexpr int JOR_VEC_NB = 2;
 
 // One block: tile tw of the vector numbering, z chunk bz of nbz.  T is the
 // arithmetic type (double, or float with the FP32 copy of cdg), TO the type
-// written (a float sweep may write its result widened to double).
-template <typename T, typename TO, int TPB, typename PC, typename PO>
+// written (a float sweep may write its result widened to double).  PXM: 0 x
+// not periodic, 1 periodic, 2 read from g at run time.  A run-time choice
+// leaves loads on one branch only inside the plane loop, and the compiler then
+// drains every load in flight once a trip; launches that know the periodicity
+// pass it as a constant.
+template <typename T, typename TO, int TPB, int PXM, typename PC, typename PO>
 __device__ inline void jor_vec_block(const Geo &g, const Range3 &r, const JorVecShape &sh, int tw, int bz, int nbz,
                                      T omega, PC __restrict__ rel, PC __restrict__ bb, PO __restrict__ out, PC g1,
                                      PC g2, PC g3) {
@@ -490,7 +494,7 @@ __device__ inline void jor_vec_block(const Geo &g, const Range3 &r, const JorVec
     }
     // periodic x: the element at Sp takes its left neighbour from Np and the
     // element at Np its right neighbour from Sp
-    const bool px_per = g.per[0] != 0;
+    const bool px_per = PXM == 2 ? g.per[0] != 0 : PXM == 1;
     const int tsp = g.Sp[0] - is, tnp = g.Np[0] - is;  // positions in this vector (may be outside 0..V-1)
     const int64_t dwm = (int64_t)(g.Np[0] - is), dwp = (int64_t)(g.Sp[0] - is);
     int kc0, kc1;
@@ -580,7 +584,7 @@ __device__ inline void jor_vec_block(const Geo &g, const Range3 &r, const JorVec
     }
 }
 
-template <typename T, typename TO, int TPB>
+template <typename T, typename TO, int TPB, int PXM>
 __global__ __launch_bounds__(TPB) void k_jor_vec(Geo g, Coef c, Range3 r, JorVecShape sh, int nbw, int nbz,
                                                          T omega, const T *__restrict__ rel, const T *__restrict__ bb,
                                                          TO *__restrict__ out, const int *active) {
@@ -590,7 +594,7 @@ __global__ __launch_bounds__(TPB) void k_jor_vec(Geo g, Coef c, Range3 r, JorVec
     const T *g1 = (const T *)(sizeof(T) == 8 ? (const void *)c.dg[0] : (const void *)c.dgf[0]);
     const T *g2 = (const T *)(sizeof(T) == 8 ? (const void *)c.dg[1] : (const void *)c.dgf[1]);
     const T *g3 = (const T *)(sizeof(T) == 8 ? (const void *)c.dg[2] : (const void *)c.dgf[2]);
-    jor_vec_block<T, TO, TPB, const T *, TO *>(g, r, sh, tw, bz, nbz, omega, rel, bb, out, g1, g2, g3);
+    jor_vec_block<T, TO, TPB, PXM, const T *, TO *>(g, r, sh, tw, bz, nbz, omega, rel, bb, out, g1, g2, g3);
 }
 
 // First FP32 sweep from e = 0 on A e = (float)v: b32 = (float)v is stored for
@@ -634,7 +638,7 @@ __global__ __launch_bounds__(TPB) void k_jor_batch(const JorBatchItem *__restric
     const void *g1 = sizeof(T) == 8 ? (const void *)it.dg[0] : (const void *)it.dgf[0];
     const void *g2 = sizeof(T) == 8 ? (const void *)it.dg[1] : (const void *)it.dgf[1];
     const void *g3 = sizeof(T) == 8 ? (const void *)it.dg[2] : (const void *)it.dgf[2];
-    jor_vec_block<T, T, TPB, gconst<T>, gmut<T>>(it.g, it.r, it.sh, tw, bz, it.nbz, omega, (gconst<T>)it.rel,
+    jor_vec_block<T, T, TPB, 2, gconst<T>, gmut<T>>(it.g, it.r, it.sh, tw, bz, it.nbz, omega, (gconst<T>)it.rel,
                                                  (gconst<T>)it.bb, (gmut<T>)it.out, (gconst<T>)g1, (gconst<T>)g2,
                                                  (gconst<T>)g3);
 }
@@ -653,7 +657,7 @@ __global__ __launch_bounds__(TPB) void k_jor_batch_rows(const JorBatchItem *__re
     const void *g1 = sizeof(T) == 8 ? (const void *)it.dg[0] : (const void *)it.dgf[0];
     const void *g2 = sizeof(T) == 8 ? (const void *)it.dg[1] : (const void *)it.dgf[1];
     const void *g3 = sizeof(T) == 8 ? (const void *)it.dg[2] : (const void *)it.dgf[2];
-    jor_vec_block<T, T, TPB, gconst<T>, gmut<T>>(it.g, it.r, it.sh, tw, bz, it.nbz, omega, (gconst<T>)it.rel,
+    jor_vec_block<T, T, TPB, 2, gconst<T>, gmut<T>>(it.g, it.r, it.sh, tw, bz, it.nbz, omega, (gconst<T>)it.rel,
                                                  (gconst<T>)it.bb, (gmut<T>)it.out, (gconst<T>)g1, (gconst<T>)g2,
                                                  (gconst<T>)g3);
 }
@@ -1028,15 +1032,18 @@ static void launch_jor_vec(hipStream_t s, const Geo &g, const Coef &c, const Box
     int nbw, nbz;
     const JorVecShape sh = jor_vec_shape(g, r, 16 / (int)sizeof(T), tpb, shape.kc, nbw, nbz);
     const dim3 grid(xcd_grid((int64_t)nbw * nbz));
-    if (tpb == 128)
-        hipLaunchKernelGGL((k_jor_vec<T, TO, 128>), grid, dim3(128), 0, s, g, c, r, sh, nbw, nbz, omega, rel, bb, out,
-                           active);
-    else if (tpb == 512)
-        hipLaunchKernelGGL((k_jor_vec<T, TO, 512>), grid, dim3(512), 0, s, g, c, r, sh, nbw, nbz, omega, rel, bb, out,
-                           active);
-    else
-        hipLaunchKernelGGL((k_jor_vec<T, TO, 256>), grid, dim3(256), 0, s, g, c, r, sh, nbw, nbz, omega, rel, bb, out,
-                           active);
+#define IH_JV(TPBV, PXV)                                                                                        \
+    hipLaunchKernelGGL((k_jor_vec<T, TO, TPBV, PXV>), grid, dim3(TPBV), 0, s, g, c, r, sh, nbw, nbz, omega, rel, bb, \
+                       out, active)
+    const bool px = g.per[0] != 0;
+    if (tpb == 128) {
+        if (px) IH_JV(128, 1); else IH_JV(128, 0);
+    } else if (tpb == 512) {
+        if (px) IH_JV(512, 1); else IH_JV(512, 0);
+    } else {
+        if (px) IH_JV(256, 1); else IH_JV(256, 0);
+    }
+#undef IH_JV
 }
 
 void launch_jor(hipStream_t s, const Geo &g, const Coef &c, const Box &b, double omega, const double *rel,

@@ -1333,6 +1333,7 @@ struct ih_batch {
     JorBatchItem *items[2][2] = {};        // [fp32][direction]
     int *tile_item[2] = {};                // [fp32]: item of every global tile
     int tile_cap[2] = {};
+    bool any_periodic_x = false;           // some case is periodic in x
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     int g_sweeps = -1, g_fp32 = -1, g_tpb = -1, g_zchunk = -1, g_layout = -1;
@@ -1352,6 +1353,8 @@ void batch_drop_graph(ih_batch *b) {
 // item tables for the current tile shape, one numbering per type (2 doubles
 // or 4 floats per lane)
 int batch_build(ih_batch *b) {
+    b->any_periodic_x = false;
+    for (ih_case *c : b->cases) b->any_periodic_x = b->any_periodic_x || c->g.per[0] != 0;
     for (int p = 0; p < 2; ++p) {
         std::vector<JorBatchItem> h[2];
         int t0 = 0;
@@ -1412,7 +1415,7 @@ int batch_sweeps(ih_batch *b, int n, double omega, bool fp32) {
     for (int q = 0; q < n; ++q)
         if (launch_jor_batch(b->stream, b->items[fp32 ? 1 : 0][q & 1], b->tile_item[fp32 ? 1 : 0],
                              b->ntiles[fp32 ? 1 : 0], (int)b->cases.size(), b->row_tiles[fp32 ? 1 : 0], b->layout,
-                             b->tpb, fp32, omega))
+                             b->tpb, fp32, omega, b->any_periodic_x))
             return fail(IH_EINVAL, "unsupported tile shape");
     LAUNCHCHK();
     return IH_OK;
@@ -1493,12 +1496,12 @@ int ih_batch_autotune(ih_batch *b, int sweeps, int *tpb, int *zchunk, int *layou
                 // direction 0 only (PRE -> COMP): the relaxed field is left untouched
                 for (int q = 0; q < 2; ++q)
                     if (launch_jor_batch(b->stream, b->items[0][0], b->tile_item[0], b->ntiles[0], n, b->row_tiles[0],
-                                         lay, a, false, 0.8))
+                                         lay, a, false, 0.8, b->any_periodic_x))
                         return fail(IH_EINVAL, "tile shape");
                 HIPCHK(hipEventRecord(b->ev0, b->stream));
                 for (int q = 0; q < sweeps; ++q)
                     launch_jor_batch(b->stream, b->items[0][0], b->tile_item[0], b->ntiles[0], n, b->row_tiles[0], lay,
-                                     a, false, 0.8);
+                                     a, false, 0.8, b->any_periodic_x);
                 HIPCHK(hipEventRecord(b->ev1, b->stream));
                 HIPCHK(hipEventSynchronize(b->ev1));
                 LAUNCHCHK();

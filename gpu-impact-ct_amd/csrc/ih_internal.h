@@ -146,7 +146,7 @@ int jor_batch_tiles(const Box &b, const Geo &g, int V, int tpb, int zchunk, JorV
 // layout 0: one flat grid over all tiles with a tile -> item table; 1: a row
 // of row_tiles blocks per item
 int launch_jor_batch(hipStream_t s, const JorBatchItem *items, const int *tile_item, int ntiles, int nitems,
-                     int row_tiles, int layout, int tpb, bool fp32, double omega);
+                     int row_tiles, int layout, int tpb, bool fp32, double omega, bool any_periodic_x);
 // the last FP32 sweep written widened to double, and the first one from e = 0
 // on A e = (float)v (stores b32; writes e, or e64 widened when e64 != nullptr)
 void launch_jor_f32_to64(hipStream_t s, const Geo &g, const Coef &c, const Box &b, float omega, const float *rel,

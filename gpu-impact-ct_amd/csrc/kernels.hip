@@ -627,7 +627,7 @@ __global__ void k_jor_f32_first(Geo g, Coef c, Range3 r, float omega, const doub
 // tile_item maps a tile to its item; the block relaxes that tile exactly as
 // the single-case kernel would, so every case's result is bit-identical to its
 // own sweep.
-template <typename T, int TPB>
+template <typename T, int TPB, int PXM>
 __global__ __launch_bounds__(TPB) void k_jor_batch(const JorBatchItem *__restrict__ items,
                                                    const int *__restrict__ tile_item, int ntiles, T omega) {
     int t, dummy, dummy2;
@@ -638,7 +638,7 @@ __global__ __launch_bounds__(TPB) void k_jor_batch(const JorBatchItem *__restric
     const void *g1 = sizeof(T) == 8 ? (const void *)it.dg[0] : (const void *)it.dgf[0];
     const void *g2 = sizeof(T) == 8 ? (const void *)it.dg[1] : (const void *)it.dgf[1];
     const void *g3 = sizeof(T) == 8 ? (const void *)it.dg[2] : (const void *)it.dgf[2];
-    jor_vec_block<T, T, TPB, 2, gconst<T>, gmut<T>>(it.g, it.r, it.sh, tw, bz, it.nbz, omega, (gconst<T>)it.rel,
+    jor_vec_block<T, T, TPB, PXM, gconst<T>, gmut<T>>(it.g, it.r, it.sh, tw, bz, it.nbz, omega, (gconst<T>)it.rel,
                                                  (gconst<T>)it.bb, (gmut<T>)it.out, (gconst<T>)g1, (gconst<T>)g2,
                                                  (gconst<T>)g3);
 }
@@ -646,7 +646,7 @@ __global__ __launch_bounds__(TPB) void k_jor_batch(const JorBatchItem *__restric
 // Row layout of the same batch: grid (row_tiles, cases); a block's case is its
 // row, so its table entry is read without a dependent lookup; rows shorter
 // than row_tiles leave idle blocks that return at once.
-template <typename T, int TPB>
+template <typename T, int TPB, int PXM>
 __global__ __launch_bounds__(TPB) void k_jor_batch_rows(const JorBatchItem *__restrict__ items, int row_tiles,
                                                         T omega) {
     const JorBatchItem &it = items[blockIdx.y];
@@ -657,7 +657,7 @@ __global__ __launch_bounds__(TPB) void k_jor_batch_rows(const JorBatchItem *__re
     const void *g1 = sizeof(T) == 8 ? (const void *)it.dg[0] : (const void *)it.dgf[0];
     const void *g2 = sizeof(T) == 8 ? (const void *)it.dg[1] : (const void *)it.dgf[1];
     const void *g3 = sizeof(T) == 8 ? (const void *)it.dg[2] : (const void *)it.dgf[2];
-    jor_vec_block<T, T, TPB, 2, gconst<T>, gmut<T>>(it.g, it.r, it.sh, tw, bz, it.nbz, omega, (gconst<T>)it.rel,
+    jor_vec_block<T, T, TPB, PXM, gconst<T>, gmut<T>>(it.g, it.r, it.sh, tw, bz, it.nbz, omega, (gconst<T>)it.rel,
                                                  (gconst<T>)it.bb, (gmut<T>)it.out, (gconst<T>)g1, (gconst<T>)g2,
                                                  (gconst<T>)g3);
 }
@@ -1085,25 +1085,28 @@ int jor_batch_tiles(const Box &b, const Geo &g, int V, int tpb, int zchunk, JorV
 }
 
 int launch_jor_batch(hipStream_t s, const JorBatchItem *items, const int *tile_item, int ntiles, int nitems,
-                     int row_tiles, int layout, int tpb, bool fp32, double omega) {
+                     int row_tiles, int layout, int tpb, bool fp32, double omega, bool any_periodic_x) {
     const dim3 flat(xcd_grid(ntiles)), rows(xcd_grid(row_tiles), (unsigned)nitems);
     const int rt = (int)xcd_grid(row_tiles);
-#define IH_JB(TPB)                                                                                                \
+#define IH_JB(TPB, PXM)                                                                                           \
     do {                                                                                                          \
         if (layout == 0 && fp32)                                                                                  \
-            hipLaunchKernelGGL((k_jor_batch<float, TPB>), flat, dim3(TPB), 0, s, items, tile_item, ntiles,        \
+            hipLaunchKernelGGL((k_jor_batch<float, TPB, PXM>), flat, dim3(TPB), 0, s, items, tile_item, ntiles,   \
                                (float)omega);                                                                     \
         else if (layout == 0)                                                                                     \
-            hipLaunchKernelGGL((k_jor_batch<double, TPB>), flat, dim3(TPB), 0, s, items, tile_item, ntiles, omega); \
+            hipLaunchKernelGGL((k_jor_batch<double, TPB, PXM>), flat, dim3(TPB), 0, s, items, tile_item, ntiles,  \
+                               omega);                                                                            \
         else if (fp32)                                                                                            \
-            hipLaunchKernelGGL((k_jor_batch_rows<float, TPB>), rows, dim3(TPB), 0, s, items, rt, (float)omega);    \
+            hipLaunchKernelGGL((k_jor_batch_rows<float, TPB, PXM>), rows, dim3(TPB), 0, s, items, rt,             \
+                               (float)omega);                                                                     \
         else                                                                                                      \
-            hipLaunchKernelGGL((k_jor_batch_rows<double, TPB>), rows, dim3(TPB), 0, s, items, rt, omega);          \
+            hipLaunchKernelGGL((k_jor_batch_rows<double, TPB, PXM>), rows, dim3(TPB), 0, s, items, rt, omega);     \
     } while (0)
+    // a batch without periodic x runs the loop without the run-time branch
     switch (tpb) {
-    case 128: IH_JB(128); break;
-    case 256: IH_JB(256); break;
-    case 512: IH_JB(512); break;
+    case 128: if (any_periodic_x) IH_JB(128, 2); else IH_JB(128, 0); break;
+    case 256: if (any_periodic_x) IH_JB(256, 2); else IH_JB(256, 0); break;
+    case 512: if (any_periodic_x) IH_JB(512, 2); else IH_JB(512, 0); break;
     default: return -1;
     }
 #undef IH_JB

@@ -163,6 +163,7 @@ def batched_relaxation(ih, n_cases=32, sweeps=200, seed=4):
     pts = sum(c.N[0] * c.N[1] * c.N[2] for c in cases)
     bt = ih.Batch(cases)
     tpb, zc, layout, _ = bt.autotune(5)
+    gbytes, ngroups = bt.groups()
     bt.jor(sweeps, JOR_OMEGA, graph=True)  # capture + warm
     cases[0].sync()
     t0 = time.perf_counter()
@@ -187,8 +188,9 @@ def batched_relaxation(ih, n_cases=32, sweeps=200, seed=4):
         "points": pts,
         "sweeps": sweeps,
         "tile": {"threads": tpb, "z_chunk": zc, "layout": "rows" if layout else "flat"},
+        "groups": {"count": ngroups, "budget_mb": gbytes / 2**20 if gbytes else None},
         "ms_per_sweep": t_batch / sweeps * 1e3,
-        "hbm_gbs": JOR_BYTES_PER_POINT * pts * sweeps / t_batch / 1e9,
+        "algorithmic_gbs": JOR_BYTES_PER_POINT * pts * sweeps / t_batch / 1e9,
         "sequential_ms_per_sweep": t_seq / sweeps * 1e3,
         "speedup_vs_sequential": t_seq / t_batch,
     }

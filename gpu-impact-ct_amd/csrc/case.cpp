@@ -1334,6 +1334,14 @@ struct ih_batch {
     int *tile_item[2] = {};                // [fp32]: item of every global tile
     int tile_cap[2] = {};
     bool any_periodic_x = false;           // some case is periodic in x
+    // Cases are relaxed in groups whose fields (rel, out, bb) fit group_bytes,
+    // all sweeps of a group before the next: a group small enough to stay in
+    // the device's last-level cache is swept from there.  0: one group.
+    int64_t group_bytes = 0;
+    std::vector<int> group;                // first case of every group, then the case count
+    std::vector<int> gtile[2];             // [fp32]: first global tile of every group, then the tile count
+    std::vector<int> grows[2];             // [fp32]: row tiles of every group (its largest case's)
+    int64_t g_group = -1;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     int g_sweeps = -1, g_fp32 = -1, g_tpb = -1, g_zchunk = -1, g_layout = -1;
@@ -1355,6 +1363,17 @@ void batch_drop_graph(ih_batch *b) {
 int batch_build(ih_batch *b) {
     b->any_periodic_x = false;
     for (ih_case *c : b->cases) b->any_periodic_x = b->any_periodic_x || c->g.per[0] != 0;
+    b->group.assign(1, 0);
+    int64_t used = 0;
+    for (size_t q = 0; q < b->cases.size(); ++q) {
+        const int64_t ws = 3 * (int64_t)sizeof(double) * b->cases[q]->g.fe;
+        if (b->group_bytes > 0 && used > 0 && used + ws > b->group_bytes) {
+            b->group.push_back((int)q);
+            used = 0;
+        }
+        used += ws;
+    }
+    b->group.push_back((int)b->cases.size());
     for (int p = 0; p < 2; ++p) {
         std::vector<JorBatchItem> h[2];
         int t0 = 0;
@@ -1388,6 +1407,18 @@ int batch_build(ih_batch *b) {
             }
         }
         b->ntiles[p] = t0;
+        b->gtile[p].clear();
+        b->grows[p].clear();
+        for (size_t gi = 0; gi + 1 < b->group.size(); ++gi) {
+            b->gtile[p].push_back(h[0][(size_t)b->group[gi]].tile0);
+            int rows = 0;
+            for (int q = b->group[gi]; q < b->group[gi + 1]; ++q) {
+                const int e = q + 1 < (int)b->cases.size() ? h[0][(size_t)q + 1].tile0 : t0;
+                rows = std::max(rows, e - h[0][(size_t)q].tile0);
+            }
+            b->grows[p].push_back(rows);
+        }
+        b->gtile[p].push_back(t0);
         std::vector<int> ti((size_t)t0);
         for (size_t q = 0; q < b->cases.size(); ++q) {
             const int a = h[0][q].tile0, e = q + 1 < b->cases.size() ? h[0][q + 1].tile0 : t0;
@@ -1411,15 +1442,26 @@ int batch_build(ih_batch *b) {
     return IH_OK;
 }
 
-int batch_sweeps(ih_batch *b, int n, double omega, bool fp32) {
-    for (int q = 0; q < n; ++q)
-        if (launch_jor_batch(b->stream, b->items[fp32 ? 1 : 0][q & 1], b->tile_item[fp32 ? 1 : 0],
-                             b->ntiles[fp32 ? 1 : 0], (int)b->cases.size(), b->row_tiles[fp32 ? 1 : 0], b->layout,
-                             b->tpb, fp32, omega, b->any_periodic_x))
-            return fail(IH_EINVAL, "unsupported tile shape");
+// n sweeps of every case, group after group; dir >= 0 keeps one direction
+// (PRE -> COMP, for timing without changing PRE), else the sweeps alternate
+int batch_run(ih_batch *b, int n, double omega, bool fp32, int layout, int dir) {
+    const int p = fp32 ? 1 : 0;
+    for (size_t gi = 0; gi + 1 < b->group.size(); ++gi) {
+        const int c0 = b->group[gi], c1 = b->group[gi + 1];
+        const int t0 = b->gtile[p][gi], nt = b->gtile[p][gi + 1] - t0;
+        if (nt == 0) continue;
+        for (int q = 0; q < n; ++q) {
+            const JorBatchItem *it = b->items[p][dir >= 0 ? dir : (q & 1)];
+            if (launch_jor_batch(b->stream, layout == 1 ? it + c0 : it, b->tile_item[p], t0, nt, c1 - c0,
+                                 b->grows[p][gi], layout, b->tpb, fp32, omega, b->any_periodic_x))
+                return fail(IH_EINVAL, "unsupported tile shape");
+        }
+    }
     LAUNCHCHK();
     return IH_OK;
 }
+
+int batch_sweeps(ih_batch *b, int n, double omega, bool fp32) { return batch_run(b, n, omega, fp32, b->layout, -1); }
 
 }  // namespace
 
@@ -1484,44 +1526,58 @@ int ih_batch_autotune(ih_batch *b, int sweeps, int *tpb, int *zchunk, int *layou
     HIPCHK(hipSetDevice(b->device));
     for (ih_case *c : b->cases) HIPCHK(hipStreamSynchronize(c->stream));
     static const int tpbs[3] = {128, 256, 512}, zcs[5] = {8, 12, 16, 24, 32};
+    static const int64_t gbs[4] = {0, (int64_t)192 << 20, (int64_t)96 << 20, (int64_t)48 << 20};
     double best = 1e300;
     int bt = b->tpb, bz = b->zchunk, bl = b->layout, rc;
-    const int n = (int)b->cases.size();
-    for (int lay = 0; lay < 2; ++lay)
-        for (int a : tpbs)
-            for (int z : zcs) {
-                b->tpb = a;
-                b->zchunk = z;
-                if ((rc = batch_build(b))) return rc;
-                // direction 0 only (PRE -> COMP): the relaxed field is left untouched
-                for (int q = 0; q < 2; ++q)
-                    if (launch_jor_batch(b->stream, b->items[0][0], b->tile_item[0], b->ntiles[0], n, b->row_tiles[0],
-                                         lay, a, false, 0.8, b->any_periodic_x))
-                        return fail(IH_EINVAL, "tile shape");
-                HIPCHK(hipEventRecord(b->ev0, b->stream));
-                for (int q = 0; q < sweeps; ++q)
-                    launch_jor_batch(b->stream, b->items[0][0], b->tile_item[0], b->ntiles[0], n, b->row_tiles[0], lay,
-                                     a, false, 0.8, b->any_periodic_x);
-                HIPCHK(hipEventRecord(b->ev1, b->stream));
-                HIPCHK(hipEventSynchronize(b->ev1));
-                LAUNCHCHK();
-                float ms = 0.f;
-                HIPCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
-                if (ms < best) {
-                    best = ms;
-                    bt = a;
-                    bz = z;
-                    bl = lay;
+    int64_t bg = b->group_bytes;
+    for (int64_t gb : gbs)
+        for (int lay = 0; lay < 2; ++lay)
+            for (int a : tpbs)
+                for (int z : zcs) {
+                    b->tpb = a;
+                    b->zchunk = z;
+                    b->group_bytes = gb;
+                    if ((rc = batch_build(b))) return rc;
+                    // direction 0 only (PRE -> COMP): the relaxed field is left untouched
+                    if ((rc = batch_run(b, 2, 0.8, false, lay, 0))) return rc;
+                    HIPCHK(hipEventRecord(b->ev0, b->stream));
+                    if ((rc = batch_run(b, sweeps, 0.8, false, lay, 0))) return rc;
+                    HIPCHK(hipEventRecord(b->ev1, b->stream));
+                    HIPCHK(hipEventSynchronize(b->ev1));
+                    float ms = 0.f;
+                    HIPCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+                    if (ms < best) {
+                        best = ms;
+                        bt = a;
+                        bz = z;
+                        bl = lay;
+                        bg = gb;
+                    }
                 }
-            }
     b->tpb = bt;
     b->zchunk = bz;
     b->layout = bl;
+    b->group_bytes = bg;
     if ((rc = batch_build(b))) return rc;
     if (tpb) *tpb = bt;
     if (zchunk) *zchunk = bz;
     if (layout) *layout = bl;
     if (ms_best) *ms_best = best / sweeps;
+    return IH_OK;
+}
+
+int ih_batch_set_groups(ih_batch *b, int64_t bytes) {
+    if (!b || bytes < 0) return fail(IH_EINVAL, "bad argument");
+    HIPCHK(hipSetDevice(b->device));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    b->group_bytes = bytes;
+    return batch_build(b);
+}
+
+int ih_batch_get_groups(ih_batch *b, int64_t *bytes, int *count) {
+    if (!b) return fail(IH_EINVAL, "bad argument");
+    if (bytes) *bytes = b->group_bytes;
+    if (count) *count = (int)b->group.size() - 1;
     return IH_OK;
 }
 
@@ -1547,7 +1603,8 @@ int ih_batch_jor(ih_batch *b, int n_sweeps, double omega, int fp32, int use_grap
     if (n_sweeps > 0) {
         if (use_graph) {
             if (!b->exec || b->g_sweeps != n_sweeps || b->g_fp32 != fp32 || b->g_omega != omega ||
-                b->g_tpb != b->tpb || b->g_zchunk != b->zchunk || b->g_layout != b->layout) {
+                b->g_tpb != b->tpb || b->g_zchunk != b->zchunk || b->g_layout != b->layout ||
+                b->g_group != b->group_bytes) {
                 batch_drop_graph(b);
                 HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
                 rc = batch_sweeps(b, n_sweeps, omega, fp32 != 0);
@@ -1566,6 +1623,7 @@ int ih_batch_jor(ih_batch *b, int n_sweeps, double omega, int fp32, int use_grap
                 b->g_tpb = b->tpb;
                 b->g_zchunk = b->zchunk;
                 b->g_layout = b->layout;
+                b->g_group = b->group_bytes;
             }
             HIPCHK(hipGraphLaunch(b->exec, s));
         } else if ((rc = batch_sweeps(b, n_sweeps, omega, fp32 != 0))) {

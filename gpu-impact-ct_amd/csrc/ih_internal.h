@@ -143,10 +143,10 @@ struct JorBatchItem {
 // tiles of box b for the batch kernels: V points per lane (2 FP64, 4 FP32),
 // tpb threads per tile, z chunks of at most zchunk planes
 int jor_batch_tiles(const Box &b, const Geo &g, int V, int tpb, int zchunk, JorVecShape &sh, int &ntxy, int &nbz);
-// layout 0: one flat grid over all tiles with a tile -> item table; 1: a row
-// of row_tiles blocks per item
-int launch_jor_batch(hipStream_t s, const JorBatchItem *items, const int *tile_item, int ntiles, int nitems,
-                     int row_tiles, int layout, int tpb, bool fp32, double omega, bool any_periodic_x);
+// layout 0: one flat grid over the global tiles tile_base .. tile_base + ntiles - 1
+// with a tile -> item table; 1: a row of row_tiles blocks for each of nitems items
+int launch_jor_batch(hipStream_t s, const JorBatchItem *items, const int *tile_item, int tile_base, int ntiles,
+                     int nitems, int row_tiles, int layout, int tpb, bool fp32, double omega, bool any_periodic_x);
 // the last FP32 sweep written widened to double, and the first one from e = 0
 // on A e = (float)v (stores b32; writes e, or e64 widened when e64 != nullptr)
 void launch_jor_f32_to64(hipStream_t s, const Geo &g, const Coef &c, const Box &b, float omega, const float *rel,

@@ -629,9 +629,11 @@ __global__ void k_jor_f32_first(Geo g, Coef c, Range3 r, float omega, const doub
 // own sweep.
 template <typename T, int TPB, int PXM>
 __global__ __launch_bounds__(TPB) void k_jor_batch(const JorBatchItem *__restrict__ items,
-                                                   const int *__restrict__ tile_item, int ntiles, T omega) {
+                                                   const int *__restrict__ tile_item, int tile_base, int ntiles,
+                                                   T omega) {
     int t, dummy, dummy2;
     if (!xcd_tile(ntiles, 1, 1, t, dummy, dummy2)) return;
+    t += tile_base;  // the launch covers the global tiles tile_base .. tile_base + ntiles - 1
     const JorBatchItem &it = items[tile_item[t]];
     const int local = t - it.tile0;
     const int tw = local % it.ntxy, bz = local / it.ntxy;
@@ -1084,18 +1086,18 @@ int jor_batch_tiles(const Box &b, const Geo &g, int V, int tpb, int zchunk, JorV
     return ntxy * nbz;
 }
 
-int launch_jor_batch(hipStream_t s, const JorBatchItem *items, const int *tile_item, int ntiles, int nitems,
-                     int row_tiles, int layout, int tpb, bool fp32, double omega, bool any_periodic_x) {
+int launch_jor_batch(hipStream_t s, const JorBatchItem *items, const int *tile_item, int tile_base, int ntiles,
+                     int nitems, int row_tiles, int layout, int tpb, bool fp32, double omega, bool any_periodic_x) {
     const dim3 flat(xcd_grid(ntiles)), rows(xcd_grid(row_tiles), (unsigned)nitems);
     const int rt = (int)xcd_grid(row_tiles);
 #define IH_JB(TPB, PXM)                                                                                           \
     do {                                                                                                          \
         if (layout == 0 && fp32)                                                                                  \
-            hipLaunchKernelGGL((k_jor_batch<float, TPB, PXM>), flat, dim3(TPB), 0, s, items, tile_item, ntiles,   \
-                               (float)omega);                                                                     \
+            hipLaunchKernelGGL((k_jor_batch<float, TPB, PXM>), flat, dim3(TPB), 0, s, items, tile_item, tile_base, \
+                               ntiles, (float)omega);                                                             \
         else if (layout == 0)                                                                                     \
-            hipLaunchKernelGGL((k_jor_batch<double, TPB, PXM>), flat, dim3(TPB), 0, s, items, tile_item, ntiles,  \
-                               omega);                                                                            \
+            hipLaunchKernelGGL((k_jor_batch<double, TPB, PXM>), flat, dim3(TPB), 0, s, items, tile_item, tile_base,\
+                               ntiles, omega);                                                                    \
         else if (fp32)                                                                                            \
             hipLaunchKernelGGL((k_jor_batch_rows<float, TPB, PXM>), rows, dim3(TPB), 0, s, items, rt,             \
                                (float)omega);                                                                     \

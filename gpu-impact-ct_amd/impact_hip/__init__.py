@@ -144,6 +144,8 @@ def load(path: os.PathLike | None = None):
         "ih_batch_destroy": (None, [vp]),
         "ih_batch_autotune": (C.c_int, [vp, C.c_int, ip, ip, ip, dp]),
         "ih_batch_set_shape": (C.c_int, [vp, C.c_int, C.c_int, C.c_int]),
+        "ih_batch_set_groups": (C.c_int, [vp, C.c_int64]),
+        "ih_batch_get_groups": (C.c_int, [vp, C.POINTER(C.c_int64), ip]),
         "ih_batch_jor": (C.c_int, [vp, C.c_int, C.c_double, C.c_int, C.c_int]),
         "ih_grid_bytes": (C.c_int64, [C.POINTER(GridDesc)]),
         "ih_serve": (C.c_int, [C.POINTER(ServeConfig), C.POINTER(Request), C.c_int, C.POINTER(RequestResult),
@@ -437,6 +439,16 @@ class Batch:
 
     def set_shape(self, tpb, zchunk, layout):
         check(self.lib.ih_batch_set_shape(self.h, int(tpb), int(zchunk), int(layout)), "batch_set_shape")
+
+    def set_groups(self, nbytes):
+        """relax the cases in groups whose fields fit nbytes (0: one group)"""
+        check(self.lib.ih_batch_set_groups(self.h, int(nbytes)), "batch_set_groups")
+
+    def groups(self):
+        """(group budget in bytes, number of groups)"""
+        nb, cnt = C.c_int64(), C.c_int()
+        check(self.lib.ih_batch_get_groups(self.h, C.byref(nb), C.byref(cnt)), "batch_get_groups")
+        return nb.value, cnt.value
 
     def jor(self, n, omega=0.8, fp32=False, graph=False):
         check(self.lib.ih_batch_jor(self.h, int(n), float(omega), 1 if fp32 else 0, 1 if graph else 0), "batch_jor")

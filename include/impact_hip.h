@@ -220,12 +220,19 @@ typedef struct ih_batch ih_batch;
 int  ih_batch_create(ih_case *const *cases, int n, ih_batch **out);
 void ih_batch_destroy(ih_batch *b);
 /* Times every launch shape -- threads per tile 128/256/512, z chunk 8/12/16/24/32,
- * layout 0 (one flat grid over all tiles) or 1 (one row of blocks per case) --
+ * layout 0 (one flat grid over all tiles) or 1 (one row of blocks per case),
+ * group budget all/192/96/48 MB (ih_batch_set_groups) --
  * on `sweeps` sweeps without changing PRE, keeps the fastest and reports it
  * with its time per sweep. */
 int  ih_batch_autotune(ih_batch *b, int sweeps, int *tpb, int *zchunk, int *layout, double *ms_per_sweep);
 /* Sets the launch shape directly (tpb 128/256/512, zchunk 1..64, layout 0/1). */
 int  ih_batch_set_shape(ih_batch *b, int tpb, int zchunk, int layout);
+/* Cases are relaxed in groups (consecutive cases whose rel, out and b fields
+ * together fit `bytes`; 0: one group), all sweeps of a group before the next,
+ * so that a group can be swept from the device's last-level cache.  Results
+ * do not depend on the grouping.  ih_batch_autotune also chooses it. */
+int  ih_batch_set_groups(ih_batch *b, int64_t bytes);
+int  ih_batch_get_groups(ih_batch *b, int64_t *bytes, int *count);
 /* n_sweeps JOR sweeps (omega) of PRE on RES in every case, each bit-identical to
  * ih_jor on that case.  fp32 != 0: FP32 defect correction instead (r = RES -
  * A_cdg PRE in FP64, n_sweeps FP32 sweeps on A e = r from e = 0, PRE += e).

@@ -106,3 +106,33 @@ def test_batch_without_periodic_x(graph, shape):
         oc.jor_mixed(3, 0.8, oc.Sp, oc.Np, p, b)
         same(c.download(ih.PRE)[box(c.Sp, c.Np)], p[box(c.Sp, c.Np)], f"fp32 case {q}")
     bt.close()
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_batch_groups(graph):
+    """Cases relaxed group after group (a budget of one case's fields: every
+    case its own group, and one of two cases) give the same results as one group."""
+    cs, data = _cases(360)
+    ref = []
+    for c, (p, b) in zip(cs, data):
+        s = ih.Case(c.N, c.bc, c.L, c.stretch)
+        s.upload(ih.PRE, p)
+        s.upload(ih.RES, b)
+        s.jor(5, 0.8, s.Sp, s.Np)
+        ref.append(s.download(ih.PRE))
+        s.close()
+    bt = ih.Batch(cs)
+    for layout in (0, 1):
+        for budget, expect in ((1, len(cs)), (2 << 20, None)):
+            for c, (p, b) in zip(cs, data):
+                c.upload(ih.PRE, p)
+            bt.set_shape(256, 12, layout)
+            bt.set_groups(budget)
+            nb, cnt = bt.groups()
+            assert nb == budget and cnt >= 1
+            if expect:
+                assert cnt == expect
+            bt.jor(5, 0.8, graph=graph)
+            for q, c in enumerate(cs):
+                same(c.download(ih.PRE)[box(c.Sp, c.Np)], ref[q][box(c.Sp, c.Np)], f"case {q}, budget {budget}")
+    bt.close()

@@ -59,6 +59,30 @@ __device__ inline bool g_src(const Geo &g, int d, int &idx) {
 
 __device__ inline bool wall_plane(const Geo &g, int e, int idx) { return !g.per[e] && (idx == 1 || idx == g.N[e]); }
 
+// p_src, g_src and wall_plane along z with the periodicity of z a compile-time
+// constant: the plane loop of k_apply then has no branches on it (they cost
+// scalar instructions and branch joins every plane)
+template <bool PZ>
+__device__ inline bool p_src_z(const Geo &g, int &idx) {
+    if (PZ) {
+        idx = wrap_p(g, 2, idx);
+        return true;
+    }
+    return idx >= 1 && idx <= g.N[2];
+}
+template <bool PZ>
+__device__ inline bool g_src_z(const Geo &g, int &idx) {
+    if (PZ) {
+        idx = wrap_p(g, 2, idx);
+        return true;
+    }
+    return idx >= 1 && idx <= g.N[2] - 1;
+}
+template <bool PZ>
+__device__ inline bool wall_z(const Geo &g, int idx) {
+    return !PZ && (idx == 1 || idx == g.N[2]);
+}
+
 // dot modes of k_apply
 enum { DOT_NONE = 0, DOT_W = 1, DOT_SELF = 2, DOT_PAIR = 3 };
 
@@ -74,7 +98,7 @@ enum { DOT_NONE = 0, DOT_W = 1, DOT_SELF = 2, DOT_PAIR = 3 };
 // DM = DOT_W accumulates <Ap, w>, DOT_SELF <p, Ap> and <Ap, Ap>, DOT_PAIR
 // <w, Ap> and <Ap, Ap>, per
 // thread in plane order, then over the block with the fixed tree.
-template <int TX, int TY, int AZ, int DM>
+template <int TX, int TY, int AZ, int DM, bool PZ>
 __global__ __launch_bounds__(TX *TY) void k_apply(
     Geo g, Coef c, Range3 r, int nbx, int nby, int nbz, const double *__restrict__ src, double *__restrict__ dst,
     const double *__restrict__ w, double *part1, double *part2, const int *active) {
@@ -116,7 +140,7 @@ __global__ __launch_bounds__(TX *TY) void k_apply(
     for (int e = tid; e < 6 * ZW; e += NT) {
         const int s = e / ZW, u = e % ZW;
         int kk = kc0 - 4 + u;
-        const bool v = kk <= kc1 + 2 && g_src(g, 2, kk);
+        const bool v = kk <= kc1 + 2 && g_src_z<PZ>(g, kk);
         sCG3[s][u] = v ? c.G[2][s * c2 + kk] : 0.0;
         if (s == 0) sVZ[u] = v ? 1 : 0;
     }
@@ -208,7 +232,7 @@ __global__ __launch_bounds__(TX *TY) void k_apply(
 #pragma unroll
         for (int s = 0; s < 11; ++s) {
             int kk = kc0 - 6 + s;
-            const bool v = vcol && p_src(g, 2, kk);
+            const bool v = vcol && p_src_z<PZ>(g, kk);
             pz[s] = *(v ? src + colb + (int64_t)kk * pxy : zero);
         }
 #pragma unroll
@@ -222,7 +246,7 @@ __global__ __launch_bounds__(TX *TY) void k_apply(
         for (int s = 0; s < 6; ++s) q[s] = pz[5 + s];
     }
     auto load_halo = [&](int kk, double (&hv)[HL]) {
-        const bool vk = kk <= kc1 && p_src(g, 2, kk);
+        const bool vk = kk <= kc1 && p_src_z<PZ>(g, kk);
 #pragma unroll
         for (int h = 0; h < HL; ++h) {
             const bool v = vk && hoff[h] >= 0;
@@ -230,7 +254,7 @@ __global__ __launch_bounds__(TX *TY) void k_apply(
         }
     };
     auto load_col = [&](int kk) -> double {
-        const bool v = vcol && kk <= kc1 + 5 && p_src(g, 2, kk);
+        const bool v = vcol && kk <= kc1 + 5 && p_src_z<PZ>(g, kk);
         return *(v ? src + colb + (int64_t)kk * pxy : zero);
     };
     auto load_w = [&](int kk) -> double {
@@ -276,7 +300,7 @@ __global__ __launch_bounds__(TX *TY) void k_apply(
         __syncthreads();
         load_halo(k + 2, hv);
         // gradients of plane k
-        const bool zk = wall_plane(g, 2, k);
+        const bool zk = wall_z<PZ>(g, k);
         double *G = sG[buf];
         {
             double a = cg1[0] * P[b1];
@@ -586,18 +610,24 @@ void launch_apply_A(hipStream_t s, const Geo &g, const Coef &c, const Box &pb, c
     // the operator tile is the reduction tile, so the fused dot partials follow the fixed tree
     const int nbx = (r.n0 + RED_TX - 1) / RED_TX, nby = (r.n1 + RED_TY - 1) / RED_TY, nbz = (r.n2 + RED_KC - 1) / RED_KC;
     const dim3 grid(xcd_grid((int64_t)nbx * nby * nbz)), block(RED_TX, RED_TY, 1);
+#define IH_APPLY(DMV)                                                                                           \
+    do {                                                                                                        \
+        if (g.per[2])                                                                                           \
+            hipLaunchKernelGGL((k_apply<RED_TX, RED_TY, RED_KC, DMV, true>), grid, block, 0, s, g, c, r, nbx, nby, \
+                               nbz, src, dst, w, part1, part2, active);                                         \
+        else                                                                                                    \
+            hipLaunchKernelGGL((k_apply<RED_TX, RED_TY, RED_KC, DMV, false>), grid, block, 0, s, g, c, r, nbx,     \
+                               nby, nbz, src, dst, w, part1, part2, active);                                    \
+    } while (0)
     if (dot_mode == DOT_W)
-        hipLaunchKernelGGL((k_apply<RED_TX, RED_TY, RED_KC, DOT_W>), grid, block, 0, s, g, c, r, nbx, nby, nbz, src,
-                           dst, w, part1, part2, active);
+        IH_APPLY(DOT_W);
     else if (dot_mode == DOT_SELF)
-        hipLaunchKernelGGL((k_apply<RED_TX, RED_TY, RED_KC, DOT_SELF>), grid, block, 0, s, g, c, r, nbx, nby, nbz,
-                           src, dst, w, part1, part2, active);
+        IH_APPLY(DOT_SELF);
     else if (dot_mode == DOT_PAIR)
-        hipLaunchKernelGGL((k_apply<RED_TX, RED_TY, RED_KC, DOT_PAIR>), grid, block, 0, s, g, c, r, nbx, nby, nbz,
-                           src, dst, w, part1, part2, active);
+        IH_APPLY(DOT_PAIR);
     else
-        hipLaunchKernelGGL((k_apply<RED_TX, RED_TY, RED_KC, DOT_NONE>), grid, block, 0, s, g, c, r, nbx, nby, nbz,
-                           src, dst, w, part1, part2, active);
+        IH_APPLY(DOT_NONE);
+#undef IH_APPLY
 }
 
 void launch_krylov_init(hipStream_t s, const Geo &g, const Box &b, const double *bb, double *rr, double *rh,

@@ -599,27 +599,100 @@ __global__ __launch_bounds__(TPB) void k_jor_vec(Geo g, Coef c, Range3 r, JorVec
 
 // First FP32 sweep from e = 0 on A e = (float)v: b32 = (float)v is stored for
 // the later sweeps; every neighbour term is the product with 0 that the
-// generic sweep would form, so the result is that sweep's bit for bit.
+// generic sweep would form, so the result is that sweep's bit for bit.  A
+// lane handles 4 consecutive points of a row (16-byte stores of b32 and e, two
+// 16-byte loads of v) through a z chunk, so the x and y coefficients are read
+// once per column.  Rows are numbered in aligned 4-point vectors from i0
+// (the first and last may reach past the range; those elements are not stored).
+// short z chunks: the sweep has no z neighbours, and it needs the blocks to fill the device
+constexpr int F32F_TPB = 256, F32F_KC = 4;
+
 template <typename TO>
-__global__ void k_jor_f32_first(Geo g, Coef c, Range3 r, float omega, const double *v, float *b32, TO *e,
-                                const int *active) {
+__global__ __launch_bounds__(F32F_TPB) void k_jor_f32_first(Geo g, Coef c, Range3 r, int i0, int nv, int ntile,
+                                                            int nbz, float omega, const double *__restrict__ v,
+                                                            float *__restrict__ b32, TO *__restrict__ e,
+                                                            const int *active) {
+    typedef double D4 __attribute__((ext_vector_type(2)));
+    typedef float F4 __attribute__((ext_vector_type(4)));
+    typedef TO O4 __attribute__((ext_vector_type(4)));
     if (active && !*active) return;
-    IH_POINT(r);
-    const int64_t x = gidx(g, i, j, k);
-    const float b = (float)v[x];
-    b32[x] = b;
+    int tw, dummy, bz;
+    if (!xcd_tile(ntile, 1, nbz, tw, dummy, bz)) return;
+    const int f = tw * F32F_TPB + (int)threadIdx.x;
+    if (f >= nv * r.n1) return;
+    const int jj = f / nv, q = f - jj * nv;
+    const int j = r.lo1 + jj, is = i0 + 4 * q;
+    const int hi0 = r.lo0 + r.n0 - 1;
+    const bool full = is >= r.lo0 && is + 3 <= hi0;
     const float *g1 = c.dgf[0], *g2 = c.dgf[1], *g3 = c.dgf[2];
     const int c0 = g.cst[0], c1 = g.cst[1], c2 = g.cst[2];
+    float a1m[4], a1p[4], a1d[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        int ic = is + t;  // coefficient rows exist for the range only
+        ic = ic < r.lo0 ? r.lo0 : (ic > hi0 ? hi0 : ic);
+        a1m[t] = g1[0 * c0 + ic];
+        a1p[t] = g1[2 * c0 + ic];
+        a1d[t] = g1[1 * c0 + ic];
+    }
+    const float a2m = g2[0 * c1 + j], a2p = g2[2 * c1 + j], a2d = g2[1 * c1 + j];
+    int kc0, kc1;
+    red_chunk(r, bz, nbz, kc0, kc1);
+    const int64_t xy = gidx(g, is, j, 0) - 3 * g.pxy;
     const float z = 0.0f;
-    float t = b;
-    t = t - g1[0 * c0 + i] * z;
-    t = t - g1[2 * c0 + i] * z;
-    t = t - g2[0 * c1 + j] * z;
-    t = t - g2[2 * c1 + j] * z;
-    t = t - g3[0 * c2 + k] * z;
-    t = t - g3[2 * c2 + k] * z;
-    const float diag = (g1[1 * c0 + i] + g2[1 * c1 + j]) + g3[1 * c2 + k];
-    e[x] = (TO)(omega * t / diag + (1.0f - omega) * z);
+    auto ldv = [&](int k, D4 &w0, D4 &w1) {
+        const int64_t x = xy + (int64_t)((k <= kc1 ? k : kc1) + 3) * g.pxy;  // past the chunk: its last plane again
+        w0 = *(const D4 *)(v + x);
+        w1 = *(const D4 *)(v + x + 2);
+    };
+    auto plane = [&](int k, const D4 &v0, const D4 &v1) {
+        const bool in = k <= kc1;  // planes past the chunk are computed but not stored
+        const int kg = in ? k : kc1;
+        const int64_t x = xy + (int64_t)(kg + 3) * g.pxy;
+        const float bv[4] = {(float)v0[0], (float)v0[1], (float)v1[0], (float)v1[1]};
+        const float a3m = g3[0 * c2 + kg], a3p = g3[2 * c2 + kg], a3d = g3[1 * c2 + kg];
+        TO res[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            float s = bv[t];
+            s = s - a1m[t] * z;
+            s = s - a1p[t] * z;
+            s = s - a2m * z;
+            s = s - a2p * z;
+            s = s - a3m * z;
+            s = s - a3p * z;
+            const float diag = (a1d[t] + a2d) + a3d;
+            res[t] = (TO)(omega * s / diag + (1.0f - omega) * z);
+        }
+        if (full && in) {
+            F4 bo;
+            O4 eo;
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                bo[t] = bv[t];
+                eo[t] = res[t];
+            }
+            *(F4 *)(b32 + x) = bo;
+            *(O4 *)(e + x) = eo;
+        } else if (in) {
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+                if (is + t >= r.lo0 && is + t <= hi0) {
+                    b32[x + t] = bv[t];
+                    e[x + t] = res[t];
+                }
+        }
+    };
+    // two register sets: the loads of the next plane are in flight while one is computed
+    D4 a0, a1, b0, b1;
+    ldv(kc0, a0, a1);
+    ldv(kc0 + 1, b0, b1);
+    for (int k = kc0; k <= kc1; k += 2) {
+        plane(k, a0, a1);
+        ldv(k + 2, a0, a1);
+        plane(k + 1, b0, b1);
+        ldv(k + 3, b0, b1);
+    }
 }
 
 // Batched relaxation: the tiles of many cases (grids of any size) in one
@@ -1067,12 +1140,17 @@ void launch_jor_f32_first(hipStream_t s, const Geo &g, const Coef &c, const Box 
                           float *b32, float *e, double *e64, const int *active) {
     Range3 r = R(b);
     if (empty(r)) return;
+    // aligned 4-point vectors: element index of i0 a multiple of 4 (px and pxy are)
+    const int a = (int)(((int64_t)g.lead + r.lo0 + 3) % 4);
+    const int i0 = r.lo0 - a, nv = (r.lo0 + r.n0 - 1 - i0) / 4 + 1;
+    const int ntile = (int)(((int64_t)nv * r.n1 + F32F_TPB - 1) / F32F_TPB), nbz = (r.n2 + F32F_KC - 1) / F32F_KC;
+    const dim3 grid(xcd_grid((int64_t)ntile * nbz));
     if (e64)
-        hipLaunchKernelGGL((k_jor_f32_first<double>), grid_for(r), dim3(BX, BY, 1), 0, s, g, c, r, omega, v, b32, e64,
-                           active);
+        hipLaunchKernelGGL((k_jor_f32_first<double>), grid, dim3(F32F_TPB), 0, s, g, c, r, i0, nv, ntile, nbz, omega, v,
+                           b32, e64, active);
     else
-        hipLaunchKernelGGL((k_jor_f32_first<float>), grid_for(r), dim3(BX, BY, 1), 0, s, g, c, r, omega, v, b32, e,
-                           active);
+        hipLaunchKernelGGL((k_jor_f32_first<float>), grid, dim3(F32F_TPB), 0, s, g, c, r, i0, nv, ntile, nbz, omega, v,
+                           b32, e, active);
 }
 
 int jor_batch_tiles(const Box &b, const Geo &g, int V, int tpb, int zchunk, JorVecShape &sh, int &ntxy, int &nbz) {

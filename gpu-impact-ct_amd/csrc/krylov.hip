@@ -98,8 +98,11 @@ enum { DOT_NONE = 0, DOT_W = 1, DOT_SELF = 2, DOT_PAIR = 3 };
 // DM = DOT_W accumulates <Ap, w>, DOT_SELF <p, Ap> and <Ap, Ap>, DOT_PAIR
 // <w, Ap> and <Ap, Ap>, per
 // thread in plane order, then over the block with the fixed tree.
+// The bound asks for 3 wavefronts per SIMD (at most 168 VGPRs): the pair-dot
+// form otherwise takes 170, which the 8-register allocation rounds to 176, and
+// runs at 2 wavefronts per SIMD, 40% slower.
 template <int TX, int TY, int AZ, int DM, bool PZ>
-__global__ __launch_bounds__(TX *TY) void k_apply(
+__global__ __launch_bounds__(TX *TY, 3) void k_apply(
     Geo g, Coef c, Range3 r, int nbx, int nby, int nbz, const double *__restrict__ src, double *__restrict__ dst,
     const double *__restrict__ w, double *part1, double *part2, const int *active) {
     constexpr int NT = TX * TY;

@@ -10,6 +10,7 @@
 #include <string>
 
 #include "ih_host.h"
+#include "ih_mg.h"
 
 using namespace ih;
 
@@ -216,8 +217,11 @@ int apply_div_grad(ih_case *c, double *phi, double *lap) {
     return IH_OK;
 }
 
+void drop_mg(ih_case *c);
+
 int upload_coeffs(ih_case *c) {
     const Geo &g = c->g;
+    drop_mg(c);  // level 1 of the multigrid cycle holds the old tables
     // SoA layout: per direction 10 arrays of up to 7 taps plus cdg (3 taps) and coordinates
     std::vector<double> h;
     std::vector<int64_t> off_G(3), off_Ip(3), off_D(3), off_Iu(3), off_NpD(3), off_NpU(3), off_NuD(3), off_NuU(3),
@@ -310,6 +314,16 @@ double absmax_host(ih_case *c, const Box &b, const double *a, int *rc) {
     if (*rc) return 0.0;
     *rc = read_scalars(c, S_MAX, 1);
     return c->hscal[S_MAX];
+}
+
+void drop_mg(ih_case *c) {
+    if (!c->mg && !c->mgmem) return;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    delete c->mg;
+    c->mg = nullptr;
+    if (c->mgmem) (void)hipFree(c->mgmem);
+    c->mgmem = nullptr;
+    c->mgh.clear();
 }
 
 }  // namespace
@@ -430,6 +444,8 @@ void ih_case_destroy(ih_case *c) {
     if (c->hscal) (void)hipHostFree(c->hscal);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
+    delete c->mg;
+    if (c->mgmem) (void)hipFree(c->mgmem);
     resolve_timings(c);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -771,8 +787,8 @@ int ih_dot(ih_case *c, const int ss[3], const int nn[3], ih_field a, ih_field b,
 // is returned (not remembered).
 extern "C++" {
 template <typename F>
-static JorLaunch jor_shape(ih_case *c, const Box &b, bool fp32, F sweep) {
-    const std::vector<int> key{fp32 ? 1 : 0, b.lo[0], b.lo[1], b.lo[2], b.hi[0], b.hi[1], b.hi[2]};
+static JorLaunch jor_shape(ih_case *c, const Box &b, bool fp32, F sweep, int level = 0) {
+    const std::vector<int> key{fp32 ? 1 : 0, b.lo[0], b.lo[1], b.lo[2], b.hi[0], b.hi[1], b.hi[2], level};
     auto it = c->jor_shape.find(key);
     if (it != c->jor_shape.end()) return it->second;
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
@@ -880,8 +896,10 @@ int ih_residual_cdg(ih_case *c, const int ss[3], const int nn[3], ih_field rel, 
 // init_yes = .FALSE.: the iteration runs on the device (scalars, stop test and
 // breakdown exits in one-block control kernels); the host enqueues batches of
 // iterations and reads the flags once per batch.
+static int mg_setup(ih_case *c);
+
 static int bicgstab_run(ih_case *c, double eps, int n_it_max, const int ss[3], const int nn[3], ih_field bbf,
-                        ih_field phif, int sweeps, double omega, ih_bicgstab_stats *st) {
+                        ih_field phif, int sweeps, double omega, bool mg, ih_bicgstab_stats *st) {
     int rc = need_coeffs(c);
     if (rc) return rc;
     if (!ss || !nn || !valid_field(bbf) || !valid_field(phif) || n_it_max < 0 || sweeps < 0)
@@ -915,11 +933,19 @@ static int bicgstab_run(ih_case *c, double eps, int n_it_max, const int ss[3], c
         launch_fill_f32(s, k.e0, 0.0f, c->g.fe);
         launch_fill_f32(s, k.e1, 0.0f, c->g.fe);
     }
+    if (mg) {
+        if ((rc = mg_setup(c))) return rc;
+        k.z1 = F(c, IH_Z1);
+        k.z2 = F(c, IH_Z2);
+        k.mg = c->mg;
+    }
     // rr = bb - A phi, rh = rr, norm and <rr, rh>
     if ((rc = apply_div_grad(c, F(c, phif), k.rr))) return rc;
     launch_krylov_init(s, c->g, b, F(c, bbf), k.rr, k.rh, k.part1, k.part2, k.sc, k.fl, eps, n_it_max);
     LAUNCHCHK();
-    const int batch = 8;
+    // iterations enqueued per convergence check: a preconditioned iteration is
+    // many launches, and those enqueued past the stop run as no-ops
+    const int batch = mg ? 2 : 8;
     for (;;) {
         for (int q = 0; q < batch; ++q) launch_krylov_iteration(s, c->g, c->c, b, k);
         LAUNCHCHK();
@@ -946,13 +972,197 @@ static int bicgstab_run(ih_case *c, double eps, int n_it_max, const int ss[3], c
 
 int ih_bicgstab(ih_case *c, double eps, int n_it_max, const int ss[3], const int nn[3], ih_field bbf, ih_field phif,
                 ih_bicgstab_stats *st) {
-    return bicgstab_run(c, eps, n_it_max, ss, nn, bbf, phif, 0, 0.0, st);
+    return bicgstab_run(c, eps, n_it_max, ss, nn, bbf, phif, 0, 0.0, false, st);
+}
+
+int ih_bicgstab_precond(ih_case *c, double eps, int n_it_max, const int ss[3], const int nn[3], ih_field bbf,
+                        ih_field phif, int preconditioner, ih_bicgstab_stats *st) {
+    if (preconditioner == 0) return bicgstab_run(c, eps, n_it_max, ss, nn, bbf, phif, 0, 0.0, false, st);
+    if (preconditioner == 1) return bicgstab_run(c, eps, n_it_max, ss, nn, bbf, phif, 0, 0.0, true, st);
+    return fail(IH_EINVAL, "preconditioner %d (0: none, 1: multigrid V cycle)", preconditioner);
 }
 
 int ih_bicgstab_mixed(ih_case *c, double eps, int n_it_max, const int ss[3], const int nn[3], ih_field bbf,
                       ih_field phif, int sweeps, double omega, ih_bicgstab_stats *st) {
     if (sweeps < 1) return fail(IH_EINVAL, "the mixed-precision preconditioner needs at least one sweep");
-    return bicgstab_run(c, eps, n_it_max, ss, nn, bbf, phif, sweeps, omega, st);
+    return bicgstab_run(c, eps, n_it_max, ss, nn, bbf, phif, sweeps, omega, false, st);
+}
+
+// ---------------------------------------------------------------------------
+// Multigrid V cycle (preconditioner 1): levels, arrays and tables on the device
+// ---------------------------------------------------------------------------
+static void mg_free(ih_case *c) { drop_mg(c); }
+
+static int mg_setup(ih_case *c) {
+    if (c->mg) return IH_OK;
+    if (c->hg.xp[0].empty())
+        return fail(IH_ESTATE, "the multigrid levels need generated coordinates (ih_case_generate_coeffs)");
+    int bc6[6];
+    for (int d = 0; d < 3; ++d) {
+        bc6[2 * d] = c->hg.bc[d][0];
+        bc6[2 * d + 1] = c->hg.bc[d][1];
+        if (!(c->g.per[d] || (bc6[2 * d] > 0 && bc6[2 * d + 1] > 0)))
+            return fail(IH_EINVAL, "the multigrid cycle takes walls or periodic pairs in every direction");
+    }
+    host_mg_levels(c->hg, c->mg_cfg[0], c->mgh);
+    const int G = (int)c->mgh.size();
+    MgDev *m = new MgDev();
+    m->G = G;
+    m->n_down = c->mg_cfg[1];
+    m->n_up = c->mg_cfg[2];
+    m->n_bottom = c->mg_cfg[3];
+    // level geometry, then one allocation: level arrays (each a multiple of 64
+    // elements, so every array starts 512-byte aligned), then the tables
+    int64_t arr = 0;
+    int64_t offA[MG_LEVELS_MAX + 1], offB[MG_LEVELS_MAX + 1], offC[MG_LEVELS_MAX + 1], offD[MG_LEVELS_MAX + 1];
+    for (int g = 1; g <= G; ++g) {
+        MgLevel &L = m->L[g];
+        if (g == 1) {
+            L.g = c->g;
+        } else {
+            Geo &q = L.g;
+            q = Geo{};
+            for (int d = 0; d < 3; ++d) q.N[d] = c->mgh[g - 1].N[d];
+            init_limits(q, bc6);
+            q.lead = 12;
+            q.px = ((q.N[0] + 7) + 15) / 16 * 16;
+            q.pxy = (int64_t)q.px * (q.N[1] + 7);
+            q.fe = ((q.lead + q.pxy * (q.N[2] + 7)) + 63) / 64 * 64;
+            for (int d = 0; d < 3; ++d) q.cst[d] = ((q.N[d] + 1) + 7) / 8 * 8;
+        }
+        L.pr = pbox(L.g);
+        for (int d = 0; d < 3; ++d) L.dn[d] = c->mgh[g - 1].dn[d];
+        offA[g] = offB[g] = -1;
+        if (g >= 2) {
+            offA[g] = arr;
+            arr += L.g.fe;
+            offB[g] = arr;
+            arr += L.g.fe;
+        }
+        offC[g] = arr;
+        arr += L.g.fe;
+        offD[g] = arr;
+        arr += L.g.fe;
+    }
+    std::vector<double> tab;
+    int64_t offdg[MG_LEVELS_MAX + 1][3], offR[MG_LEVELS_MAX + 1][3], offI[MG_LEVELS_MAX + 1][3];
+    for (int g = 1; g <= G; ++g)
+        for (int d = 0; d < 3; ++d) {
+            const HostMgLevel &h = c->mgh[g - 1];
+            const int N = h.N[d], cst = m->L[g].g.cst[d];
+            auto put = [&](const std::vector<double> &src, int taps) -> int64_t {
+                const int64_t o = (int64_t)tab.size();
+                tab.resize(tab.size() + (size_t)taps * cst, 0.0);
+                for (int i = 1; i <= N; ++i)
+                    for (int t = 0; t < taps; ++t) tab[(size_t)o + (size_t)t * cst + i] = src[(size_t)t + (size_t)taps * (i - 1)];
+                return o;
+            };
+            offdg[g][d] = put(h.dg[d], 3);
+            offR[g][d] = put(h.cR[d], 3);
+            offI[g][d] = put(h.cI[d], 2);
+        }
+    const int64_t offZ = (int64_t)tab.size();
+    tab.resize(tab.size() + 64, 0.0);
+    const int64_t total = arr + (int64_t)tab.size();
+    if (hipMalloc(&c->mgmem, sizeof(double) * (size_t)total) != hipSuccess) {
+        delete m;
+        c->mgmem = nullptr;
+        return fail(IH_ENOMEM, "out of device memory for the multigrid levels (%lld bytes)",
+                    (long long)(sizeof(double) * total));
+    }
+    HIPCHK(hipMemsetAsync(c->mgmem, 0, sizeof(double) * (size_t)arr, c->stream));
+    HIPCHK(hipMemcpyAsync(c->mgmem + arr, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice, c->stream));
+    double *base = c->mgmem, *tb = c->mgmem + arr;
+    for (int g = 1; g <= G; ++g) {
+        MgLevel &L = m->L[g];
+        if (g == 1) {
+            L.c = c->c;
+        } else {
+            L.c = Coef{};
+            for (int d = 0; d < 3; ++d) L.c.dg[d] = tb + offdg[g][d];
+            L.c.zero = tb + offZ;
+            L.A = base + offA[g];
+            L.B = base + offB[g];
+        }
+        L.C = base + offC[g];
+        L.D = base + offD[g];
+        for (int d = 0; d < 3; ++d) {
+            L.cR[d] = tb + offR[g][d];
+            L.cI[d] = tb + offI[g][d];
+        }
+    }
+    // launch shape of each level's sweep (a sweep is a pure function of its inputs)
+    for (int g = 1; g <= G; ++g) {
+        MgLevel &L = m->L[g];
+        const double *bb = g == 1 ? L.C : L.A;
+        const double *x0 = g == 1 ? L.C : L.B;
+        L.shape = jor_shape(c, L.pr, false, [&](JorLaunch l) {
+            launch_jor(c->stream, L.g, L.c, L.pr, m->omega, x0, bb, L.D, l);
+        }, g);
+    }
+    LAUNCHCHK();
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->mg = m;
+    return IH_OK;
+}
+
+int ih_mg_configure(ih_case *c, int n_grids_limit, int n_relax_down, int n_relax_up, int n_relax_bottom) {
+    if (!c) return fail(IH_EINVAL, "null case");
+    if (n_grids_limit < 1 || n_grids_limit > MG_LEVELS_MAX || n_relax_down < 1 || n_relax_up < 1 ||
+        n_relax_bottom < 1)
+        return fail(IH_EINVAL, "multigrid: 1 <= n_grids_limit <= %d and at least one relaxation per phase",
+                    MG_LEVELS_MAX);
+    HIPCHK(hipSetDevice(c->device));
+    mg_free(c);
+    c->mg_cfg[0] = n_grids_limit;
+    c->mg_cfg[1] = n_relax_down;
+    c->mg_cfg[2] = n_relax_up;
+    c->mg_cfg[3] = n_relax_bottom;
+    return IH_OK;
+}
+
+int ih_mg_levels(ih_case *c, int *n_grids, int *NN) {
+    int rc = need_coeffs(c);
+    if (rc) return rc;
+    if (!n_grids || !NN) return fail(IH_EINVAL, "null argument");
+    HIPCHK(hipSetDevice(c->device));
+    if ((rc = mg_setup(c))) return rc;
+    *n_grids = c->mg->G;
+    for (int g = 1; g <= c->mg->G; ++g)
+        for (int d = 0; d < 3; ++d) NN[3 * (g - 1) + d] = c->mgh[g - 1].N[d];
+    return IH_OK;
+}
+
+int ih_mg_get_coeff(ih_case *c, int level, const char *name, double *out, int64_t *n) {
+    int rc = need_coeffs(c);
+    if (rc) return rc;
+    if (!name || !n) return fail(IH_EINVAL, "null argument");
+    HIPCHK(hipSetDevice(c->device));
+    if ((rc = mg_setup(c))) return rc;
+    if (level < 1 || level > c->mg->G) return fail(IH_EINVAL, "level %d out of 1..%d", level, c->mg->G);
+    const HostMgLevel &h = c->mgh[level - 1];
+    const size_t len = strlen(name);
+    const int d = len >= 3 ? name[len - 1] - '1' : -1;
+    if (d < 0 || d > 2) return fail(IH_EINVAL, "unknown multigrid table '%s'", name);
+    const std::vector<double> *v = nullptr;
+    if (!strncmp(name, "cdg", 3) && len == 4) v = &h.dg[d];
+    else if (!strncmp(name, "cR", 2) && len == 3) v = &h.cR[d];
+    else if (!strncmp(name, "cI", 2) && len == 3) v = &h.cI[d];
+    if (!v) return fail(IH_EINVAL, "unknown multigrid table '%s'", name);
+    if (out) std::copy(v->begin(), v->end(), out);
+    *n = (int64_t)v->size();
+    return IH_OK;
+}
+
+int ih_multigrid_v(ih_case *c, ih_field bb, ih_field phi) {
+    int rc = need_coeffs(c);
+    if (rc) return rc;
+    if (!valid_field(bb) || !valid_field(phi) || bb == phi) return fail(IH_EINVAL, "bad argument");
+    HIPCHK(hipSetDevice(c->device));
+    if ((rc = mg_setup(c))) return rc;
+    launch_mg_vcycle(c->stream, *c->mg, F(c, bb), F(c, phi), nullptr);
+    LAUNCHCHK();
+    return IH_OK;
 }
 
 int ih_fill_corners(ih_case *c, ih_field f) {
@@ -1084,7 +1294,7 @@ static int poisson_solve(ih_case *c, const ih_step_params *p, const Box &b, ih_b
             *bst = mst;
         }
     } else {
-        rc = ih_bicgstab(c, eps, p->bicg_max_it, ss, nn, IH_RES, IH_PRE, bst);
+        rc = ih_bicgstab_precond(c, eps, p->bicg_max_it, ss, nn, IH_RES, IH_PRE, p->precond_poisson == 1 ? 1 : 0, bst);
     }
     pt.done(bst ? bst->iterations : 0);
     return rc;

@@ -8,9 +8,13 @@
 // computed here exactly, as rational Lagrange-derivative weights solved in
 // integer arithmetic, and rounded once to double; the reference reads the same
 // rationals from its tables (src/mod_coeffs.f90:598-772).  cdg follows
-// get_stencil (src/mod_coeffs.f90:1048-1258).
+// get_stencil (src/mod_coeffs.f90:1048-1258); the multigrid levels follow the
+// hierarchy of src/mod_setup.f90:96-143, the level coordinates of
+// src/mod_geometry.f90:104-166, get_stencil on every level, interp_coeffs
+// (src/mod_coeffs.f90:1683-1746) and restr_coeffs (:1879-2035).
 #include "ih_host.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -287,6 +291,136 @@ bool host_coeffs(const HostGrid &hg, HostCoeffs &hc, std::string &err) {
             for (int q = 0; q < 3; ++q) dg[q + 3 * (size_t)(N - 1)] = 2.0 * dg[q + 3 * (size_t)(N - 1)];
     }
     return true;
+}
+
+namespace {
+
+// cdg of one direction from the level's coordinates xp (0..N+1), xu (0..N) at
+// [i+3]; at walls the finest grid's first and last u-cell widths (yu0) enter
+// on every level, as get_stencil writes them (src/mod_coeffs.f90:1093, 1101)
+void level_dg(int N, int bl, int bu, const std::vector<double> &xp, const std::vector<double> &xu,
+              const std::vector<double> &yu0, int M, std::vector<double> &dg) {
+    std::vector<double> cDm(N + 1, 0.0), cD0(N + 1, 0.0), cG0(N + 1, 0.0), cG1(N + 1, 0.0);
+    for (int i = 1; i <= N; ++i) {
+        const double w = xu[i + 3] - xu[i + 2];
+        cDm[i] = -1.0 / w;
+        cD0[i] = 1.0 / w;
+    }
+    for (int i = 1; i <= N - 1; ++i) {
+        const double w = xp[i + 4] - xp[i + 3];
+        cG0[i] = -1.0 / w;
+        cG1[i] = 1.0 / w;
+    }
+    if (bl > 0) {
+        cG0[0] = cG1[0] = 0.0;
+        cDm[1] = 0.0;
+        cD0[1] = 1.0 / (yu0[1 + 3] - yu0[0 + 3]);
+    } else {
+        cG0[0] = -1.0 / (xp[1 + 3] - xp[0 + 3]);
+        cG1[0] = 1.0 / (xp[1 + 3] - xp[0 + 3]);
+    }
+    if (bu > 0) {
+        cG0[N] = cG1[N] = 0.0;
+        cD0[N] = 0.0;
+        cDm[N] = -1.0 / (yu0[M + 3] - yu0[M - 1 + 3]);
+    } else {
+        cG0[N] = -1.0 / (xp[N + 4] - xp[N + 3]);
+        cG1[N] = 1.0 / (xp[N + 4] - xp[N + 3]);
+    }
+    dg.assign(3 * (size_t)N, 0.0);
+    for (int i = 1; i <= N; ++i) {
+        double *r = &dg[3 * (size_t)(i - 1)];
+        r[0] = cDm[i] * cG0[i - 1];
+        r[1] = cDm[i] * cG1[i - 1] + cD0[i] * cG0[i];
+        r[2] = cD0[i] * cG1[i];
+    }
+    for (int q = 0; q < 3; ++q) {
+        if (bl > 0) dg[q] = 2.0 * dg[q];
+        if (bu > 0) dg[3 * (size_t)(N - 1) + q] = 2.0 * dg[3 * (size_t)(N - 1) + q];
+    }
+}
+
+}  // namespace
+
+void host_mg_levels(const HostGrid &hg, int n_grids_limit, std::vector<HostMgLevel> &out) {
+    // sizes: a direction halves while N - 1 is even and N >= 5; the number of
+    // grids is the last level on which any direction halved
+    int NN[MG_MAX_LEVELS + 1][3];
+    int G = 1;
+    for (int d = 0; d < 3; ++d) NN[1][d] = hg.N[d];
+    for (int g = 2; g <= MG_MAX_LEVELS; ++g)
+        for (int d = 0; d < 3; ++d) {
+            const int p = NN[g - 1][d];
+            const bool halve = (p - 1) % 2 == 0 && p >= 5;
+            NN[g][d] = halve ? (p - 1) / 2 + 1 : p;
+            if (halve) G = g;
+        }
+    G = std::min(G, std::max(1, n_grids_limit));
+    out.assign(G, HostMgLevel{});
+    for (int g = 1; g <= G; ++g) {
+        HostMgLevel &lv = out[g - 1];
+        for (int d = 0; d < 3; ++d) {
+            const int N = NN[g][d], M = hg.N[d];
+            const int bl = hg.bc[d][0], bu = hg.bc[d][1];
+            const double Lx = hg.L[d];
+            const std::vector<double> &yp0 = hg.xp[d];
+            lv.N[d] = N;
+            lv.dn[d] = g < G ? (N - 1) / (NN[g + 1][d] - 1) : 1;
+            std::vector<double> xp, xu;
+            if (g == 1) {
+                xp = hg.xp[d];
+                xu = hg.xu[d];
+            } else {
+                const int di = (M - 1) / (N - 1);
+                xp.assign((size_t)N + 7, 0.0);
+                xu.assign((size_t)N + 7, 0.0);
+                for (int i = 1; i <= N; ++i) xp[i + 3] = yp0[(1 + di * (i - 1)) + 3];
+                for (int i = 1; i <= N - 1; ++i) xu[i + 3] = yp0[(1 + di * i - di / 2) + 3];
+                if (bl == -1) {
+                    std::vector<double> p0(xp), u0(xu);  // array assignments read the old values
+                    for (int b = -3; b <= 0; ++b) {
+                        xp[b + 3] = p0[(N - 1 + b) + 3] - Lx;
+                        xu[b + 3] = u0[(N - 1 + b) + 3] - Lx;
+                    }
+                    std::vector<double> p1(xp), u1(xu);
+                    for (int t = 0; t <= 3; ++t) {
+                        xp[(N + t) + 3] = p1[(1 + t) + 3] + Lx;
+                        xu[(N + t) + 3] = u1[(1 + t) + 3] + Lx;
+                    }
+                }
+                if (bl > 0) {
+                    for (int b = -3; b <= 0; ++b) xp[b + 3] = 2.0 * xp[1 + 3] - xp[(2 - b) + 3];
+                    for (int b = -3; b <= 0; ++b) xu[b + 3] = 2.0 * xp[1 + 3] - xu[(1 - b) + 3];
+                }
+                if (bu > 0) {
+                    for (int t = 1; t <= 3; ++t) xp[(N + t) + 3] = 2.0 * xp[N + 3] - xp[(N - t) + 3];
+                    for (int t = 0; t <= 3; ++t) xu[(N + t) + 3] = 2.0 * xp[N + 3] - xu[(N - 1 - t) + 3];
+                }
+            }
+            level_dg(N, bl, bu, xp, xu, hg.xu[d], M, lv.dg[d]);
+            lv.cR[d].assign(3 * (size_t)N, 0.0);
+            if (g >= 2) {
+                for (int ii = 1; ii <= N; ++ii) {
+                    double *r = &lv.cR[d][3 * (size_t)(ii - 1)];
+                    const bool ident = (bl > 0 && ii <= 2) || (bu > 0 && ii >= N - 1);
+                    r[0] = ident ? 0.0 : 1.0 / 4.0;
+                    r[1] = ident ? 1.0 : 2.0 / 4.0;
+                    r[2] = ident ? 0.0 : 1.0 / 4.0;
+                }
+            }
+            lv.cI[d].assign(2 * (size_t)N, 0.0);
+            if (g < G) {
+                const int di = (M - 1) / (N - 1);
+                for (int i = 2; i <= N - 1; i += 2) {
+                    const int i0 = 1 + (i - 1) * di;
+                    const double d10 = yp0[i0 + 3] - yp0[(i0 - di) + 3];
+                    const double d12 = yp0[(i0 + di) + 3] - yp0[(i0 - di) + 3];
+                    lv.cI[d][2 * (size_t)(i - 1)] = 1.0 - d10 / d12;
+                    lv.cI[d][2 * (size_t)(i - 1) + 1] = d10 / d12;
+                }
+            }
+        }
+    }
 }
 
 }  // namespace ih

@@ -28,6 +28,19 @@ struct HostCoeffs {
 void host_coords(HostGrid &hg);
 bool host_coeffs(const HostGrid &hg, HostCoeffs &hc, std::string &err);
 
+// One level of the multigrid hierarchy (preconditioner 1): sizes, cdg (-1:1,
+// 1:N) at (ii+1) + 3(i-1), restriction weights cR (same layout, levels >= 2),
+// interpolation weights cI (1:2, 1:N) at (t-1) + 2(i-1) (levels <= G-1), and the
+// coarsening ratio dn to the next level.
+struct HostMgLevel {
+    int N[3];
+    int dn[3];
+    std::vector<double> dg[3], cR[3], cI[3];
+};
+constexpr int MG_MAX_LEVELS = 15;
+// levels 1..G of the grid (index 0 holds level 1), at most n_grids_limit
+void host_mg_levels(const HostGrid &hg, int n_grids_limit, std::vector<HostMgLevel> &out);
+
 struct KernelTiming {
     double total_ms = 0.0;
     int64_t launches = 0;
@@ -68,5 +81,10 @@ struct ih_case {
     bool ktime = false;
     std::map<std::string, ih::KernelTiming> ktimes;
     std::map<std::string, std::vector<ih::PendingTiming>> kpending;
-    std::map<std::vector<int>, ih::JorLaunch> jor_shape;  // (fp32, box) -> launch shape of the JOR sweep
+    std::map<std::vector<int>, ih::JorLaunch> jor_shape;  // (fp32, box, level) -> launch shape of the JOR sweep
+    // multigrid V cycle (preconditioner 1), built on first use
+    ih::MgDev *mg = nullptr;
+    double *mgmem = nullptr;          // level arrays and coefficient tables
+    int mg_cfg[4] = {15, 4, 4, 4};    // n_grids_limit, n_relax_down, n_relax_up, n_relax_bottom
+    std::vector<ih::HostMgLevel> mgh;
 };

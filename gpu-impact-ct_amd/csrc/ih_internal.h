@@ -98,6 +98,9 @@ struct JorLaunch {
 // JOR sweeps
 void launch_jor(hipStream_t s, const Geo &g, const Coef &c, const Box &b, double omega, const double *rel,
                 const double *bb, double *out, JorLaunch shape = JorLaunch());
+// the same, a no-op once *active is 0
+void launch_jor(hipStream_t s, const Geo &g, const Coef &c, const Box &b, double omega, const double *rel,
+                const double *bb, double *out, const int *active, JorLaunch shape);
 void launch_jor_f32(hipStream_t s, const Geo &g, const Coef &c, const Box &b, float omega, const float *rel,
                     const float *bb, float *out, const int *active = nullptr, JorLaunch shape = JorLaunch());
 void launch_residual_cdg(hipStream_t s, const Geo &g, const Coef &c, const Box &b, const double *rel,

@@ -6,6 +6,8 @@
 
 namespace ih {
 
+struct MgDev;
+
 // double slots
 enum {
     KS_RHR = 0,
@@ -35,6 +37,8 @@ struct KrylovPtrs {
     int sweeps = 0;
     float omega = 0.8f;
     JorLaunch shape;  // launch shape of the FP32 sweeps
+    // right preconditioner M = the multigrid V cycle (preconditioner 1) instead
+    const MgDev *mg = nullptr;
 };
 
 // dot modes of the operator kernel

@@ -1126,6 +1126,11 @@ void launch_jor(hipStream_t s, const Geo &g, const Coef &c, const Box &b, double
     launch_jor_vec<double, double>(s, g, c, b, omega, rel, bb, out, nullptr, shape);
 }
 
+void launch_jor(hipStream_t s, const Geo &g, const Coef &c, const Box &b, double omega, const double *rel,
+                const double *bb, double *out, const int *active, JorLaunch shape) {
+    launch_jor_vec<double, double>(s, g, c, b, omega, rel, bb, out, active, shape);
+}
+
 void launch_jor_f32(hipStream_t s, const Geo &g, const Coef &c, const Box &b, float omega, const float *rel,
                     const float *bb, float *out, const int *active, JorLaunch shape) {
     launch_jor_vec<float, float>(s, g, c, b, omega, rel, bb, out, active, shape);

@@ -13,13 +13,15 @@
 // three-pass evaluation, while HBM traffic drops to one read of p and one
 // write of Ap.
 //
-// The BiCGstab recurrences (src/mod_solvers.f90:1509-1935, preconditioner 0)
-// run without host round trips: one-block control kernels reduce the partial
+// The BiCGstab recurrences (src/mod_solvers.f90:1509-1935; preconditioner 0,
+// or the right-preconditioned form with FP32 JOR sweeps or the multigrid V
+// cycle as M) run without host round trips: one-block control kernels reduce the partial
 // dot products with the fixed tree and update the scalars in device memory; an
 // active flag turns the remaining launches of a batch into no-ops once the
 // iteration has stopped, so the host checks convergence once per batch.
 #include "ih_device.h"
 #include "ih_krylov.h"
+#include "ih_mg.h"
 
 #include <utility>
 
@@ -644,9 +646,14 @@ void launch_krylov_init(hipStream_t s, const Geo &g, const Box &b, const double 
 // z = M v: `sweeps` FP32 JOR sweeps on A_cdg e = (float)v from e = 0, z = (double)e.
 // The first sweep reads v itself (and stores b = (float)v), the last writes z
 // widened to double, so M costs sweeps passes over the box and no more.
-static void launch_precond(hipStream_t s, const Geo &g, const Coef &c, const Box &b, const double *v, double *z,
+static void launch_precond(hipStream_t s, const Geo &g, const Coef &c, const Box &b, double *v, double *z,
                            const KrylovPtrs &k) {
     const int *act = k.fl + KF_ACTIVE;
+    if (k.mg) {
+        // multigridV(.TRUE.,1,v,z) (src/mod_solvers.f90:1688, 1751)
+        launch_mg_vcycle(s, *k.mg, v, z, act);
+        return;
+    }
     if (k.sweeps == 1) {
         launch_jor_f32_first(s, g, c, b, k.omega, v, k.b32, nullptr, z, act);
         return;
@@ -663,7 +670,7 @@ static void launch_precond(hipStream_t s, const Geo &g, const Coef &c, const Box
 void launch_krylov_iteration(hipStream_t s, const Geo &g, const Coef &c, const Box &b, const KrylovPtrs &k) {
     Range3 r = Rk(b);
     const int64_t nb = red_blocks(b);
-    const bool pre = k.sweeps > 0;
+    const bool pre = k.sweeps > 0 || k.mg;
     hipLaunchKernelGGL(k_p_update, grid_for(r), dim3(BX, BY, 1), 0, s, g, r, k.rr, k.pk, k.Ap, k.sc, k.fl);
     if (pre) launch_precond(s, g, c, b, k.pk, k.z1, k);
     launch_apply_A(s, g, c, b, pre ? k.z1 : k.pk, k.Ap, k.rh, DOT_W, k.part1, nullptr, k.fl + KF_ACTIVE);

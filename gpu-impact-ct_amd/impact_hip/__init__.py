@@ -19,7 +19,7 @@ import numpy as np
 PKG_ROOT = Path(__file__).resolve().parent.parent
 LIB_PATH = PKG_ROOT / "lib" / "libimpact_hip.so"
 
-IH_ABI_VERSION = 2
+IH_ABI_VERSION = 3
 
 # field ids (include/impact_hip.h)
 (VEL1, VEL2, VEL3, NL1, NL2, NL3, RHS1, RHS2, RHS3, WORK1, WORK2, WORK3, PRE, RES, GPRE, PP, COMP, DIG,
@@ -49,7 +49,7 @@ class StepParams(C.Structure):
     _fields_ = [("Re", C.c_double), ("dtime", C.c_double), ("jor_sweeps", C.c_int), ("jor_omega", C.c_double),
                 ("bicg_rel_tol", C.c_double), ("bicg_max_it", C.c_int), ("level_pressure", C.c_int),
                 ("inflow_kind", C.c_int), ("inflow_amp", C.c_double), ("use_mask", C.c_int), ("jor_fp32", C.c_int),
-                ("precond_sweeps", C.c_int)]
+                ("precond_sweeps", C.c_int), ("precond_poisson", C.c_int)]
 
 
 class Request(C.Structure):
@@ -126,6 +126,12 @@ def load(path: os.PathLike | None = None):
         "ih_bicgstab": (C.c_int, [vp, C.c_double, C.c_int, ip, ip, C.c_int, C.c_int, C.POINTER(BicgStats)]),
         "ih_bicgstab_mixed": (C.c_int, [vp, C.c_double, C.c_int, ip, ip, C.c_int, C.c_int, C.c_int, C.c_double,
                                         C.POINTER(BicgStats)]),
+        "ih_bicgstab_precond": (C.c_int, [vp, C.c_double, C.c_int, ip, ip, C.c_int, C.c_int, C.c_int,
+                                          C.POINTER(BicgStats)]),
+        "ih_mg_configure": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int]),
+        "ih_mg_levels": (C.c_int, [vp, ip, ip]),
+        "ih_mg_get_coeff": (C.c_int, [vp, C.c_int, C.c_char_p, dp, C.POINTER(C.c_int64)]),
+        "ih_multigrid_v": (C.c_int, [vp, C.c_int, C.c_int]),
         "ih_fill_corners": (C.c_int, [vp, C.c_int]),
         "ih_level_pressure": (C.c_int, [vp, C.c_int]),
         "ih_set_tube_mask": (C.c_int, [vp, C.c_double, C.c_double, dp, dp]),
@@ -348,6 +354,34 @@ class Case:
         check(self.lib.ih_bicgstab_mixed(self.h, float(eps), int(nmax), _ints(ss), _ints(nn), bb, phi, int(sweeps),
                                          float(omega), C.byref(st)), "bicgstab_mixed")
         return st
+
+    def bicgstab_precond(self, eps, nmax, ss, nn, preconditioner, bb=RES, phi=PRE) -> BicgStats:
+        """BiCGstab_orig with the reference's preconditioner argument (0 none, 1 multigrid V cycle)."""
+        st = BicgStats()
+        check(self.lib.ih_bicgstab_precond(self.h, float(eps), int(nmax), _ints(ss), _ints(nn), bb, phi,
+                                           int(preconditioner), C.byref(st)), "bicgstab_precond")
+        return st
+
+    def mg_configure(self, n_grids_limit=15, n_relax_down=4, n_relax_up=4, n_relax_bottom=4):
+        check(self.lib.ih_mg_configure(self.h, int(n_grids_limit), int(n_relax_down), int(n_relax_up),
+                                       int(n_relax_bottom)), "mg_configure")
+
+    def mg_levels(self):
+        G = C.c_int(0)
+        nn = (C.c_int * 45)()
+        check(self.lib.ih_mg_levels(self.h, C.byref(G), nn), "mg_levels")
+        return [tuple(nn[3 * g:3 * g + 3]) for g in range(G.value)]
+
+    def mg_coeff(self, level, name) -> np.ndarray:
+        n = C.c_int64(0)
+        check(self.lib.ih_mg_get_coeff(self.h, int(level), name.encode(), None, C.byref(n)), "mg_get_coeff")
+        out = np.zeros(n.value)
+        check(self.lib.ih_mg_get_coeff(self.h, int(level), name.encode(), _dp(out), C.byref(n)), "mg_get_coeff")
+        return out
+
+    def multigrid_v(self, bb, phi):
+        """phi = M bb (one V cycle); bb's edges of wall pairs are averaged in place."""
+        check(self.lib.ih_multigrid_v(self.h, bb, phi), "multigrid_v")
 
     def fill_corners(self, f=PRE):
         check(self.lib.ih_fill_corners(self.h, f), "fill_corners")

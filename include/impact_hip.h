@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define IH_ABI_VERSION 2
+#define IH_ABI_VERSION 3
 
 enum {
     IH_OK = 0,
@@ -162,6 +162,28 @@ int ih_bicgstab(ih_case *c, double eps, int n_it_max, const int ss[3], const int
 int ih_bicgstab_mixed(ih_case *c, double eps, int n_it_max, const int ss[3], const int nn[3], ih_field bb,
                       ih_field phi, int sweeps, double omega, ih_bicgstab_stats *st);
 
+/* BiCGstab_orig with the reference's preconditioner argument: 0 none (as
+ * ih_bicgstab), 1 the multigrid V cycle multigridV (src/mod_solvers.f90:1684-1693,
+ * 1750-1752; z1 = M pp, z2 = M rr).  The cycle averages the edges of wall pairs
+ * of pp and rr in place, as the reference's restriction does. */
+int ih_bicgstab_precond(ih_case *c, double eps, int n_it_max, const int ss[3], const int nn[3], ih_field bb,
+                        ih_field phi, int preconditioner, ih_bicgstab_stats *st);
+
+/* ---- multigrid V cycle (preconditioner 1) --------------------------------- */
+/* Levels and relaxations per level (prog/config.txt:108-116: 15, 4, 4, 4, the
+ * defaults).  The cycle follows multigridV (src/mod_solvers.f90:456-585) with
+ * Jacobi smoothing (omega 0.98, src/mod_laplace.f90:470) on each level's
+ * pressure range; the hierarchy halves a direction while N-1 is even and N >= 5
+ * (src/mod_setup.f90:96-143).  Walls or periodic pairs in every direction. */
+int ih_mg_configure(ih_case *c, int n_grids_limit, int n_relax_down, int n_relax_up, int n_relax_bottom);
+/* number of levels and their sizes NN[3*(g-1)+d], g = 1..n_grids (NN holds 45 ints) */
+int ih_mg_levels(ih_case *c, int *n_grids, int *NN);
+/* level tables: "cdg1".."cdg3" (3N), "cR1".."cR3" (3N, levels >= 2),
+ * "cI1".."cI3" (2N, levels below the coarsest); out may be NULL to query n */
+int ih_mg_get_coeff(ih_case *c, int level, const char *name, double *out, int64_t *n);
+/* phi = M bb, one V cycle multigridV(.TRUE., 1, bb, phi); bb's edges of wall pairs are averaged */
+int ih_multigrid_v(ih_case *c, ih_field bb, ih_field phi);
+
 int ih_fill_corners(ih_case *c, ih_field f);
 int ih_level_pressure(ih_case *c, ih_field pre);
 
@@ -186,6 +208,7 @@ typedef struct {
     int use_mask;
     int jor_fp32;   /* 1: JOR sweeps in FP32 (residual and update kept FP64) */
     int precond_sweeps; /* > 0: BiCGstab right-preconditioned by this many FP32 JOR sweeps */
+    int precond_poisson; /* with precond_sweeps 0: BiCGstab's preconditioner, 0 none, 1 multigrid V cycle */
 } ih_step_params;
 
 typedef struct {

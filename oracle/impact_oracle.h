@@ -18,6 +18,8 @@
  *   - JOR relaxation      src/mod_laplace.f90:466-597, src/jacobi_solver_host_v4.f90:283-420
  *   - BiCGstab            src/mod_solvers.f90:1509-1935, 2319-2515;
  *                         operator src/mod_laplace.f90:38-197
+ *   - multigrid V cycle   src/mod_solvers.f90:353-1141, src/mod_laplace.f90:200-790,
+ *                         src/mod_coeffs.f90:1683-2035 (orc_mg.c)
  *   - projection / RK3    src/mod_solvers.f90:64-346, src/mod_rhs.f90:35-420,
  *                         src/mod_timeint.f90:275-377, src/mod_lib.f90:507-575
  *   - IBM volume mask     src/mod_sharp_interface_ibm.f90:585-640
@@ -105,6 +107,32 @@ void orc_bicgstab_mixed(const orc_case *c, double eps, int n_it_max, const int s
 void orc_jor_mixed(const orc_case *c, int n_sweeps, double omega, const int ss[3], const int nn[3], double *rel,
                    const double *bb);
 
+/* ---------------- multigrid V cycle (preconditioner 1) ---------------- */
+/* Hierarchy, coefficients and work arrays of every level; NULL for BCs other
+ * than walls or periodic pairs.  n_down, n_up, n_bottom: relaxations per level
+ * (prog/config.txt:108-110), n_grids_limit (:116); omega is the reference's 0.98. */
+typedef struct orc_mg orc_mg;
+orc_mg *orc_mg_create(const orc_case *c, int n_grids_limit, int n_down, int n_up, int n_bottom);
+void orc_mg_destroy(orc_mg *m);
+int orc_mg_grids(const orc_mg *m);
+/* NN[3*(g-1)+d] for g = 1..G; returns G */
+int orc_mg_levels(const orc_mg *m, int *NN);
+/* "cdg1".."cdg3" (3*N), "cR1".. (3*N, levels >= 2), "cI1".. (2*N, levels <= G-1),
+ * "xp1".., "xu1".. (N+7): level g's tables; returns the number written or -1 */
+int orc_mg_coeff(const orc_mg *m, int g, const char *name, double *out);
+/* elements of a level-g array, (N1+7)(N2+7)(N3+7) */
+int64_t orc_mg_field_size(const orc_mg *m, int g);
+/* multigridV(.TRUE., 1, bb, phi): phi = M bb; bb's edges of wall pairs are averaged in place */
+void orc_mg_vcycle(orc_mg *m, double *bb, double *phi);
+/* single operations (test hooks): coarse = R (bb - A x) from level g to g+1;
+ * fine += P coarse from level g+1 to g; relaxation on level g */
+void orc_mg_restrict_residual(orc_mg *m, int g, double *bb, double *x, double *coarse);
+void orc_mg_interpolate_add(orc_mg *m, int g, double *coarse, double *fine);
+void orc_mg_relax(orc_mg *m, int g, int init, int n_relax, const double *bb, double *rel);
+/* BiCGstab_orig with preconditioner 1 */
+void orc_bicgstab_mg(const orc_case *c, orc_mg *mg, double eps, int n_it_max, const int ss[3], const int nn[3],
+                     const double *bb, double *phi, orc_bicgstab_stats *st);
+
 /* ---------------- projection step ---------------- */
 typedef struct {
     double Re, dtime;
@@ -118,6 +146,7 @@ typedef struct {
     const uint8_t *mask3;  /* optional: 3 masks (u,v,w grids), 1 = fluid; NULL = none */
     int jor_fp32;          /* 1: the JOR sweeps as FP32 defect correction */
     int precond_sweeps;    /* > 0: BiCGstab preconditioned by this many FP32 JOR sweeps */
+    int precond_poisson;   /* 1: BiCGstab preconditioned by the multigrid V cycle (precond_sweeps 0) */
 } orc_step_params;
 
 typedef struct {

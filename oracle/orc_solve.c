@@ -178,7 +178,7 @@ static double absmax(const orc_case *c, const int ss[3], const int nn[3], const 
 /* ------------------------------------------------------------------------ */
 static void bicgstab_run(const orc_case *c, double eps, int n_it_max, const int ss[3], const int nn[3],
                          const double *bb, double *phi, double *work, int sweeps, double pre_omega,
-                         orc_bicgstab_stats *st)
+                         orc_mg *mg, orc_bicgstab_stats *st)
 {
     const int64_t n = c->ntot;
     double *own = NULL;
@@ -192,9 +192,12 @@ static void bicgstab_run(const orc_case *c, double eps, int n_it_max, const int 
     double *z1 = NULL, *z2 = NULL;
     float *fb = NULL, *fe = NULL, *ft = NULL;
     const float omf = (float)pre_omega;
-    if (sweeps > 0) {
+    const int pre = sweeps > 0 || mg;
+    if (pre) {
         z1 = calloc((size_t)n, sizeof(double));
         z2 = calloc((size_t)n, sizeof(double));
+    }
+    if (sweeps > 0) {
         fb = calloc((size_t)n, sizeof(float));
         fe = calloc((size_t)n, sizeof(float));
         ft = calloc((size_t)n, sizeof(float));
@@ -259,7 +262,11 @@ static void bicgstab_run(const orc_case *c, double eps, int n_it_max, const int 
                 for (int j = ss[1]; j <= nn[1]; ++j)
                     for (int i = ss[0]; i <= nn[0]; ++i) A(pk, i, j, k) = A(rr, i, j, k);
         }
-        if (sweeps > 0) {
+        if (mg) {
+            /* multigridV(.TRUE.,1,pp,z1) (src/mod_solvers.f90:1688) */
+            orc_mg_vcycle(mg, pk, z1);
+            orc_product_div_grad(c, z1, Ap, dig);
+        } else if (sweeps > 0) {
             precond(c, sweeps, omf, ss, nn, pk, z1, fb, fe, ft);
             orc_product_div_grad(c, z1, Ap, dig);
         } else {
@@ -275,7 +282,10 @@ static void bicgstab_run(const orc_case *c, double eps, int n_it_max, const int 
         for (int k = ss[2]; k <= nn[2]; ++k)
             for (int j = ss[1]; j <= nn[1]; ++j)
                 for (int i = ss[0]; i <= nn[0]; ++i) A(rr, i, j, k) = A(rr, i, j, k) - alpha * A(Ap, i, j, k);
-        if (sweeps > 0) {
+        if (mg) {
+            orc_mg_vcycle(mg, rr, z2); /* src/mod_solvers.f90:1751 */
+            orc_product_div_grad(c, z2, Ar, dig);
+        } else if (sweeps > 0) {
             precond(c, sweeps, omf, ss, nn, rr, z2, fb, fe, ft);
             orc_product_div_grad(c, z2, Ar, dig);
         } else {
@@ -293,7 +303,7 @@ static void bicgstab_run(const orc_case *c, double eps, int n_it_max, const int 
         for (int k = ss[2]; k <= nn[2]; ++k)
             for (int j = ss[1]; j <= nn[1]; ++j)
                 for (int i = ss[0]; i <= nn[0]; ++i) {
-                    if (sweeps > 0)
+                    if (pre)
                         A(phi, i, j, k) = A(phi, i, j, k) + omega * A(z2, i, j, k) + alpha * A(z1, i, j, k);
                     else
                         A(phi, i, j, k) = A(phi, i, j, k) + omega * A(rr, i, j, k) + alpha * A(pk, i, j, k);
@@ -317,13 +327,22 @@ static void bicgstab_run(const orc_case *c, double eps, int n_it_max, const int 
 void orc_bicgstab(const orc_case *c, double eps, int n_it_max, const int ss[3], const int nn[3], const double *bb,
                   double *phi, double *work, orc_bicgstab_stats *st)
 {
-    bicgstab_run(c, eps, n_it_max, ss, nn, bb, phi, work, 0, 0.0, st);
+    bicgstab_run(c, eps, n_it_max, ss, nn, bb, phi, work, 0, 0.0, NULL, st);
 }
 
 void orc_bicgstab_mixed(const orc_case *c, double eps, int n_it_max, const int ss[3], const int nn[3],
                         const double *bb, double *phi, int sweeps, double omega, orc_bicgstab_stats *st)
 {
-    bicgstab_run(c, eps, n_it_max, ss, nn, bb, phi, NULL, sweeps, omega, st);
+    bicgstab_run(c, eps, n_it_max, ss, nn, bb, phi, NULL, sweeps, omega, NULL, st);
+}
+
+/* BiCGstab_orig with preconditioner 1 (multigridV, src/mod_solvers.f90:1684-1693,
+ * 1750-1752); bb is not changed (the cycle averages edges of its own copies of
+ * pp and rr only) */
+void orc_bicgstab_mg(const orc_case *c, orc_mg *mg, double eps, int n_it_max, const int ss[3], const int nn[3],
+                     const double *bb, double *phi, orc_bicgstab_stats *st)
+{
+    bicgstab_run(c, eps, n_it_max, ss, nn, bb, phi, NULL, 0, 0.0, mg, st);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -499,6 +518,8 @@ void orc_timestep(orc_case *c, const orc_step_params *p, int step_index, double 
     double *gpre = calloc((size_t)n, sizeof(double));
     double *pp = calloc((size_t)n, sizeof(double));
     double *comp = calloc((size_t)n, sizeof(double));
+    /* preconditioner 1 with the reference configuration's cycle (prog/config.txt:108-116) */
+    orc_mg *mg = p->precond_poisson == 1 ? orc_mg_create(c, 15, 4, 4, 4) : NULL;
     double *bw = calloc((size_t)7 * n, sizeof(double));
     const int ss[3] = {c->Sp[0], c->Sp[1], c->Sp[2]};
     const int nn[3] = {c->Np[0], c->Np[1], c->Np[2]};
@@ -550,15 +571,15 @@ void orc_timestep(orc_case *c, const orc_step_params *p, int step_index, double 
             /* the device keeps the initial guess and restarts plain FP64 BiCGstab
              * from it when the preconditioned iteration does not converge */
             memcpy(comp, pre, sizeof(double) * (size_t)n);
-            bicgstab_run(c, eps, p->bicg_max_it, ss, nn, res, pre, bw, p->precond_sweeps, p->jor_omega, &bst);
+            bicgstab_run(c, eps, p->bicg_max_it, ss, nn, res, pre, bw, p->precond_sweeps, p->jor_omega, NULL, &bst);
             if (bst.exit_reason != 0) {
                 const int mixed_its = bst.iterations;
                 memcpy(pre, comp, sizeof(double) * (size_t)n);
-                bicgstab_run(c, eps, p->bicg_max_it, ss, nn, res, pre, bw, 0, 0.0, &bst);
+                bicgstab_run(c, eps, p->bicg_max_it, ss, nn, res, pre, bw, 0, 0.0, NULL, &bst);
                 bst.iterations += mixed_its;
             }
         } else {
-            bicgstab_run(c, eps, p->bicg_max_it, ss, nn, res, pre, bw, 0, 0.0, &bst);
+            bicgstab_run(c, eps, p->bicg_max_it, ss, nn, res, pre, bw, 0, 0.0, mg, &bst);
         }
         if (st) {
             st->bicg_iterations[s] = bst.iterations;
@@ -589,5 +610,6 @@ void orc_timestep(orc_case *c, const orc_step_params *p, int step_index, double 
     free(gpre);
     free(pp);
     free(comp);
+    orc_mg_destroy(mg);
     free(bw);
 }

@@ -84,6 +84,22 @@ def lib():
         L.orc_fill_corners.argtypes = [vp, dp]
         L.orc_inflow_amplitude.restype = C.c_double
         L.orc_inflow_amplitude.argtypes = [C.c_int, C.c_double, C.c_double]
+        L.orc_mg_create.restype = vp
+        L.orc_mg_create.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.orc_mg_destroy.argtypes = [vp]
+        L.orc_mg_grids.restype = C.c_int
+        L.orc_mg_grids.argtypes = [vp]
+        L.orc_mg_levels.restype = C.c_int
+        L.orc_mg_levels.argtypes = [vp, ip]
+        L.orc_mg_coeff.restype = C.c_int
+        L.orc_mg_coeff.argtypes = [vp, C.c_int, C.c_char_p, dp]
+        L.orc_mg_field_size.restype = C.c_int64
+        L.orc_mg_field_size.argtypes = [vp, C.c_int]
+        L.orc_mg_vcycle.argtypes = [vp, dp, dp]
+        L.orc_mg_restrict_residual.argtypes = [vp, C.c_int, dp, dp, dp]
+        L.orc_mg_interpolate_add.argtypes = [vp, C.c_int, dp, dp]
+        L.orc_mg_relax.argtypes = [vp, C.c_int, C.c_int, C.c_int, dp, dp]
+        L.orc_bicgstab_mg.argtypes = [vp, vp, C.c_double, C.c_int, ip, ip, dp, dp, C.POINTER(BicgStats)]
         _lib = L
     return _lib
 
@@ -97,7 +113,8 @@ class StepParams(C.Structure):
     _fields_ = [("Re", C.c_double), ("dtime", C.c_double), ("jor_sweeps", C.c_int),
                 ("jor_omega", C.c_double), ("bicg_rel_tol", C.c_double), ("bicg_max_it", C.c_int),
                 ("level_pressure", C.c_int), ("inflow_kind", C.c_int), ("inflow_amp", C.c_double),
-                ("mask3", C.POINTER(C.c_uint8)), ("jor_fp32", C.c_int), ("precond_sweeps", C.c_int)]
+                ("mask3", C.POINTER(C.c_uint8)), ("jor_fp32", C.c_int), ("precond_sweeps", C.c_int),
+                ("precond_poisson", C.c_int)]
 
 
 class StepStats(C.Structure):
@@ -227,6 +244,11 @@ class OracleCase:
                                   float(omega), C.byref(st))
         return st
 
+    def bicgstab_mg(self, mg, eps, nmax, ss, nn, bb, phi):
+        st = BicgStats()
+        self.L.orc_bicgstab_mg(self.h, mg.h, eps, nmax, _ip(ss), _ip(nn), _dp(bb), _dp(phi), C.byref(st))
+        return st
+
     def jor_mixed(self, n, omega, ss, nn, rel, bb):
         self.L.orc_jor_mixed(self.h, n, omega, _ip(ss), _ip(nn), _dp(rel), _dp(bb))
 
@@ -249,6 +271,54 @@ class OracleCase:
 
     def fill_corners(self, pre):
         self.L.orc_fill_corners(self.h, _dp(pre))
+
+
+class OracleMG:
+    """The multigrid V cycle of a case (preconditioner 1): levels 1..G."""
+
+    def __init__(self, case: OracleCase, n_grids_limit=15, n_down=4, n_up=4, n_bottom=4):
+        self.L = lib()
+        self.case = case
+        self.h = self.L.orc_mg_create(case.h, n_grids_limit, n_down, n_up, n_bottom)
+        if not self.h:
+            raise ValueError("oracle rejected the multigrid configuration")
+        self.G = int(self.L.orc_mg_grids(self.h))
+        nn = (C.c_int * (3 * self.G))()
+        self.L.orc_mg_levels(self.h, nn)
+        self.NN = [tuple(nn[3 * g:3 * g + 3]) for g in range(self.G)]  # level g+1
+
+    def __del__(self):
+        try:
+            if self.h:
+                self.L.orc_mg_destroy(self.h)
+        except Exception:
+            pass
+
+    def shape(self, g):
+        N = self.NN[g - 1]
+        return (N[2] + 7, N[1] + 7, N[0] + 7)
+
+    def zeros(self, g):
+        return np.zeros(self.shape(g), dtype=np.float64)
+
+    def coeff(self, g, name):
+        buf = np.zeros(3 * max(self.NN[g - 1]) + 16)
+        cnt = self.L.orc_mg_coeff(self.h, g, name.encode(), _dp(buf))
+        if cnt < 0:
+            raise KeyError(name)
+        return buf[:cnt].copy()
+
+    def vcycle(self, bb, phi):
+        self.L.orc_mg_vcycle(self.h, _dp(bb), _dp(phi))
+
+    def restrict_residual(self, g, bb, x, coarse):
+        self.L.orc_mg_restrict_residual(self.h, g, _dp(bb), _dp(x), _dp(coarse))
+
+    def interpolate_add(self, g, coarse, fine):
+        self.L.orc_mg_interpolate_add(self.h, g, _dp(coarse), _dp(fine))
+
+    def relax(self, g, init, n, bb, rel):
+        self.L.orc_mg_relax(self.h, g, int(init), int(n), _dp(bb), _dp(rel))
 
 
 def exact_table(abl, n, dx):

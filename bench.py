@@ -74,10 +74,11 @@ def read_traffic():
         return None
 
 
-def setup_case(ih, mixed=False, precond_sweeps=8):
+def setup_case(ih, mixed=False, precond_sweeps=8, multigrid=False):
     """The bench case; mixed=True runs the Poisson solve in the mixed-precision
     form (FP32 JOR defect correction and an FP32-JOR-preconditioned BiCGstab,
-    residuals, inner products and updates in FP64)."""
+    residuals, inner products and updates in FP64); multigrid=True runs the FP64
+    BiCGstab with the reference's multigrid V cycle as preconditioner 1."""
     c = ih.Case(N, BC, LEN, STRETCH, device=0)
     r_u = tube_radius(c.coords(0, 1))
     r_p = tube_radius(c.coords(0, 0))
@@ -91,7 +92,8 @@ def setup_case(ih, mixed=False, precond_sweeps=8):
     c.fill(ih.PRE, 0.0)
     p = ih.StepParams(Re=RE, dtime=DT, jor_sweeps=JOR_SWEEPS, jor_omega=JOR_OMEGA, bicg_rel_tol=TOL,
                       bicg_max_it=MAXIT, level_pressure=1, inflow_kind=2, inflow_amp=1.0, use_mask=1,
-                      jor_fp32=1 if mixed else 0, precond_sweeps=precond_sweeps if mixed else 0)
+                      jor_fp32=1 if mixed else 0, precond_sweeps=precond_sweeps if mixed else 0,
+                      precond_poisson=1 if multigrid else 0)
     return c, p
 
 
@@ -332,6 +334,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision run")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-relaxation run")
+    ap.add_argument("--no-multigrid", action="store_true", help="skip the multigrid-preconditioned FP64 run")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving trace")
     ap.add_argument("--precond-sweeps", type=int, default=8,
                     help="FP32 JOR sweeps of the mixed-precision BiCGstab preconditioner")
@@ -420,6 +423,23 @@ def main():
             **{f"{k}_vs_fp64": v for k, v in err.items()},
         }
         cm.close()
+    if not args.no_multigrid:
+        cg, pg = setup_case(ih, multigrid=True)
+        ms_g, fams_g, iters_g, _ = run_steps(ih, cg, pg, args.steps, args.warmup, "multigrid")
+        err = rel_errors(ih, cg, c)
+        bg_ms, bg_n = fams_g["bicgstab"]
+        result["multigrid_preconditioner"] = {
+            "form": ("FP64 BiCGstab with preconditioner 1, the multigrid V cycle (levels to 3 points, 4/4/4 "
+                     "Jacobi relaxations, omega 0.98), after the same 200 JOR sweeps"),
+            "steps_per_s": 1000.0 / ms_g,
+            "ms_per_step": ms_g,
+            "speedup_vs_plain": ms_per_step / ms_g,
+            "bicgstab_iterations_per_step": iters_g,
+            "bicgstab_ms_per_iteration": bg_ms / max(bg_n, 1),
+            "steps_compared": args.warmup + args.steps + 1,
+            **{f"{k}_vs_plain": v for k, v in err.items()},
+        }
+        cg.close()
     if not args.no_batch:
         log("batched relaxation ...")
         result["batched_relaxation"] = batched_relaxation(ih)

@@ -317,12 +317,14 @@ double absmax_host(ih_case *c, const Box &b, const double *a, int *rc) {
 }
 
 void drop_mg(ih_case *c) {
-    if (!c->mg && !c->mgmem) return;
+    if (!c->mg && !c->mgmem && !c->mgldev) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     delete c->mg;
     c->mg = nullptr;
     if (c->mgmem) (void)hipFree(c->mgmem);
     c->mgmem = nullptr;
+    if (c->mgldev) (void)hipFree(c->mgldev);
+    c->mgldev = nullptr;
     c->mgh.clear();
 }
 
@@ -446,6 +448,7 @@ void ih_case_destroy(ih_case *c) {
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     delete c->mg;
     if (c->mgmem) (void)hipFree(c->mgmem);
+    if (c->mgldev) (void)hipFree(c->mgldev);
     resolve_timings(c);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -1091,8 +1094,38 @@ static int mg_setup(ih_case *c) {
             L.cI[d] = tb + offI[g][d];
         }
     }
-    // launch shape of each level's sweep (a sweep is a pure function of its inputs)
+    // levels from g0 on run inside one workgroup; their descriptors in device memory
+    for (int g = 2; g <= G; ++g) {
+        const Geo &q = m->L[g].g;
+        if ((int64_t)q.N[0] * q.N[1] * q.N[2] <= MG_FUSED_POINTS) {
+            m->g0 = g;
+            break;
+        }
+    }
+    std::vector<MgLevelDev> ld((size_t)G + 1);
     for (int g = 1; g <= G; ++g) {
+        const MgLevel &L = m->L[g];
+        MgLevelDev &d = ld[g];
+        d.g = L.g;
+        for (int e = 0; e < 3; ++e) {
+            d.dg[e] = L.c.dg[e];
+            d.cR[e] = L.cR[e];
+            d.cI[e] = L.cI[e];
+            d.dn[e] = L.dn[e];
+        }
+        d.A = L.A;
+        d.B = L.B;
+        d.C = L.C;
+        d.D = L.D;
+    }
+    if (hipMalloc(&c->mgldev, sizeof(MgLevelDev) * ld.size()) != hipSuccess) {
+        delete m;
+        return fail(IH_ENOMEM, "out of device memory for the multigrid level descriptors");
+    }
+    HIPCHK(hipMemcpyAsync(c->mgldev, ld.data(), sizeof(MgLevelDev) * ld.size(), hipMemcpyHostToDevice, c->stream));
+    m->Ldev = (const MgLevelDev *)c->mgldev;
+    // launch shape of each level's sweep (a sweep is a pure function of its inputs)
+    for (int g = 1; g < std::min(m->g0, G + 1); ++g) {
         MgLevel &L = m->L[g];
         const double *bb = g == 1 ? L.C : L.A;
         const double *x0 = g == 1 ? L.C : L.B;

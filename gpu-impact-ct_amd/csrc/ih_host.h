@@ -85,6 +85,7 @@ struct ih_case {
     // multigrid V cycle (preconditioner 1), built on first use
     ih::MgDev *mg = nullptr;
     double *mgmem = nullptr;          // level arrays and coefficient tables
+    void *mgldev = nullptr;           // level descriptors of the fused coarse-level kernel
     int mg_cfg[4] = {15, 4, 4, 4};    // n_grids_limit, n_relax_down, n_relax_up, n_relax_bottom
     std::vector<ih::HostMgLevel> mgh;
 };

@@ -24,12 +24,30 @@ struct MgLevel {
     JorLaunch shape;
 };
 
+// A level as the fused coarse-level kernel reads it from device memory.
+struct MgLevelDev {
+    Geo g;
+    const double *dg[3];
+    const double *cR[3];
+    const double *cI[3];
+    int dn[3];
+    double *A, *B, *C, *D;
+};
+
 struct MgDev {
     int G = 0;
     int n_down = 4, n_up = 4, n_bottom = 4;
     double omega = 0.98;  // src/mod_laplace.f90:470
     MgLevel L[MG_LEVELS_MAX + 1];  // 1-based; level 1 uses the case's geometry and cdg
+    // levels g0..G run inside one single-workgroup launch (g0 > G: none);
+    // Ldev: device copy of levels 1..G (index 0 unused)
+    int g0 = MG_LEVELS_MAX + 1;
+    const MgLevelDev *Ldev = nullptr;
 };
+// points up to which a level runs inside the fused coarse-level launch
+// (one workgroup is latency-bound: a level of 2048 points takes two trips of
+// every step per thread, 9.5K points measured slower than its own launches)
+constexpr int64_t MG_FUSED_POINTS = 2048;
 
 // z = M v.  v's edges of wall pairs are averaged in place (restrict,
 // src/mod_solvers.f90:632-661), as the reference does to its pp and rr.

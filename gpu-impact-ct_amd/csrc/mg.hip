@@ -4,16 +4,17 @@
 // relax_restrict (relaxation, product_div_grad_relax, restrict), relax_bottom
 // and interpolate_relax (interpolate, relaxation).  The relaxation is the
 // Jacobi branch of relaxation_div_grad / relaxation_div_grad_inv with omega
-// 0.98 (src/mod_laplace.f90:466-597, 712-790) and runs as the out-of-place JOR
-// sweep of kernels.hip on each level's pressure range; the reference's
-// exchange_relax (src/mod_exchange.f90:1273-1480) becomes wrapped reads across
-// periodic directions.  Every value is formed with the reference's operation
-// order, so the cycle is bit-identical to the loop restatement.
+// 0.98 (src/mod_laplace.f90:466-597, 712-790) on each level's pressure range;
+// the reference's exchange_relax (src/mod_exchange.f90:1273-1480) becomes
+// wrapped reads across periodic directions.  Every value is formed with the
+// reference's operation order, so the cycle is bit-identical to the loop
+// restatement.
 //
-// Level arrays follow the case layout; restriction and interpolation are one
-// thread per output point (coarse point, fine point) reading through the
-// caches: on the finest level they touch each fine value about once, and the
-// coarse levels are 8x, 64x, ... smaller.
+// Large levels run one launch per step (the JOR sweep of kernels.hip, and one
+// thread per output point for the rest).  A launch of a few-microsecond
+// kernel costs about as much as the work of a small level, so every level of
+// at most MG_FUSED_POINTS points runs inside one single-workgroup kernel that
+// walks the rest of the cycle with workgroup barriers between its steps.
 #include "ih_device.h"
 #include "ih_mg.h"
 
@@ -38,139 +39,294 @@ __device__ inline int64_t wix(const Geo &g, int i, int j, int k) {
     return gidx(g, wrapd(g, 0, i), wrapd(g, 1, j), wrapd(g, 2, k));
 }
 
+// ---- per-point steps -------------------------------------------------------
+
 // first relaxation of a level (r == 1 with init_yes): rel = omega bb / diag
-__global__ void k_mg_init(Geo g, Coef c, Range3 r, double omega, const double *__restrict__ bb,
-                          double *__restrict__ x, const int *active) {
-    if (active && !*active) return;
-    IH_POINT(r);
+__device__ inline void pt_init(const Geo &g, const double *const *dg, int i, int j, int k, double omega,
+                               const double *bb, double *x) {
     const int64_t p = gidx(g, i, j, k);
-    const double d = (c.dg[0][g.cst[0] + i] + c.dg[1][g.cst[1] + j]) + c.dg[2][g.cst[2] + k];
+    const double d = (dg[0][g.cst[0] + i] + dg[1][g.cst[1] + j]) + dg[2][g.cst[2] + k];
     x[p] = omega * bb[p] / d;
 }
 
-// product_div_grad_relax (src/mod_laplace.f90:270-280) on the range
-__global__ void k_mg_lap(Geo g, Coef c, Range3 r, const double *__restrict__ x, double *__restrict__ lap,
-                         const int *active) {
-    if (active && !*active) return;
-    IH_POINT(r);
+// one Jacobi sweep point (src/mod_laplace.f90:514-518), out of place
+__device__ inline void pt_jor(const Geo &g, const double *const *dg, int i, int j, int k, double omega,
+                              const double *x, const double *bb, double *y) {
     const int c0 = g.cst[0], c1 = g.cst[1], c2 = g.cst[2];
-    const double *g1 = c.dg[0], *g2 = c.dg[1], *g3 = c.dg[2];
-    double t = g1[i] * x[wix(g, i - 1, j, k)];
-    t = t + g1[2 * c0 + i] * x[wix(g, i + 1, j, k)];
-    t = t + g2[j] * x[wix(g, i, j - 1, k)];
-    t = t + g2[2 * c1 + j] * x[wix(g, i, j + 1, k)];
-    t = t + g3[k] * x[wix(g, i, j, k - 1)];
-    t = t + g3[2 * c2 + k] * x[wix(g, i, j, k + 1)];
     const int64_t p = gidx(g, i, j, k);
-    t = t + ((g1[c0 + i] + g2[c1 + j]) + g3[c2 + k]) * x[p];
+    double t = bb[p];
+    t = t - dg[0][i] * x[wix(g, i - 1, j, k)];
+    t = t - dg[0][2 * c0 + i] * x[wix(g, i + 1, j, k)];
+    t = t - dg[1][j] * x[wix(g, i, j - 1, k)];
+    t = t - dg[1][2 * c1 + j] * x[wix(g, i, j + 1, k)];
+    t = t - dg[2][k] * x[wix(g, i, j, k - 1)];
+    t = t - dg[2][2 * c2 + k] * x[wix(g, i, j, k + 1)];
+    const double diag = (dg[0][c0 + i] + dg[1][c1 + j]) + dg[2][c2 + k];
+    y[p] = omega * t / diag + (1.0 - omega) * x[p];
+}
+
+// product_div_grad_relax (src/mod_laplace.f90:270-280)
+__device__ inline void pt_lap(const Geo &g, const double *const *dg, int i, int j, int k, const double *x,
+                              double *lap) {
+    const int c0 = g.cst[0], c1 = g.cst[1], c2 = g.cst[2];
+    double t = dg[0][i] * x[wix(g, i - 1, j, k)];
+    t = t + dg[0][2 * c0 + i] * x[wix(g, i + 1, j, k)];
+    t = t + dg[1][j] * x[wix(g, i, j - 1, k)];
+    t = t + dg[1][2 * c1 + j] * x[wix(g, i, j + 1, k)];
+    t = t + dg[2][k] * x[wix(g, i, j, k - 1)];
+    t = t + dg[2][2 * c2 + k] * x[wix(g, i, j, k + 1)];
+    const int64_t p = gidx(g, i, j, k);
+    t = t + ((dg[0][c0 + i] + dg[1][c1 + j]) + dg[2][c2 + k]) * x[p];
     lap[p] = t;
+}
+
+// restrict(.TRUE., g, coarse, bb, lap) at coarse point (ii, jj, kk)
+// (src/mod_solvers.f90:699-714); w: the coarse level's weights
+__device__ inline void pt_restrict(const Geo &gf, const Geo &gc, const double *const *w, const int *dd, int ii,
+                                   int jj, int kk, const double *bb, const double *lap, double *out) {
+    const int i = dd[0] * (ii - 1) + 1, j = dd[1] * (jj - 1) + 1, k = dd[2] * (kk - 1) + 1;
+    auto df = [&](int a, int b, int c) {
+        const int64_t p = wix(gf, a, b, c);
+        return bb[p] - lap[p];
+    };
+    const int s1 = gc.cst[0], s2 = gc.cst[1], s3 = gc.cst[2];
+    double t = ((w[0][s1 + ii] + w[1][s2 + jj]) + w[2][s3 + kk]) * df(i, j, k);
+    t = t + w[0][ii] * df(i - 1, j, k);
+    t = t + w[0][2 * s1 + ii] * df(i + 1, j, k);
+    t = t + w[1][jj] * df(i, j - 1, k);
+    t = t + w[1][2 * s2 + jj] * df(i, j + 1, k);
+    t = t + w[2][kk] * df(i, j, k - 1);
+    t = t + w[2][2 * s3 + kk] * df(i, j, k + 1);
+    out[gidx(gc, ii, jj, kk)] = t / 3.0;
+}
+
+// interpolate(.TRUE., g, coarse, fine, work) at fine point (i, j, k)
+// (src/mod_solvers.f90:1046-1141): the injected coarse values, then the z, y
+// and x passes, evaluated with the operands each pass reads, added to fine;
+// w: the fine level's weights, dn: coarsening ratios
+__device__ inline void pt_interp(const Geo &gf, const Geo &gc, const double *const *w, const int *dn, int i, int j,
+                                 int k, const double *coarse, double *x) {
+    auto lat = [&](int d, int t) { return dn[d] == 1 || ((t - 1) & 1) == 0; };
+    auto cix = [&](int d, int t) { return (t - 1) / dn[d] + 1; };
+    auto C = [&](int I, int J, int K) { return coarse[wix(gc, I, J, K)]; };
+    auto wz = [&](int a, int b, int c) {  // z pass (i, j on the coarse lattice)
+        const int I = cix(0, a), J = cix(1, b);
+        if (lat(2, c)) return C(I, J, cix(2, c));
+        const int K = c / 2;
+        return w[2][c] * C(I, J, K) + w[2][gf.cst[2] + c] * C(I, J, K + 1);
+    };
+    auto wy = [&](int a, int b, int c) {  // y pass (i on the lattice)
+        if (lat(1, b)) return wz(a, b, c);
+        return w[1][b] * wz(a, b - 1, c) + w[1][gf.cst[1] + b] * wz(a, b + 1, c);
+    };
+    double v;
+    if (lat(0, i))
+        v = wy(i, j, k);
+    else
+        v = w[0][i] * wy(i - 1, j, k) + w[0][gf.cst[0] + i] * wy(i + 1, j, k);
+    const int64_t p = gidx(gf, i, j, k);
+    x[p] = x[p] + v;
 }
 
 // Averaging on the edges of wall pairs, the reference's statements in order:
 // direction pairs (1,2), (1,3), (2,3), per pair the sides (L,L), (L,U), (U,L),
 // (U,U), each a line over 1..N of the third direction.  three = 0: two
 // neighbours / 2 (restrict, src/mod_solvers.f90:632-661); three = 1: three
-// neighbours / 3 (interpolate, :1028-1043).  One block; a barrier between
-// statements, whose lines share their corner points.
-__global__ __launch_bounds__(256) void k_mg_edges(Geo g, double *f1, double *f2, int three, const int *active) {
-    if (active && !*active) return;
+// neighbours / 3 (interpolate, :1028-1043).  Lines of one pair share no point
+// when every N >= 3, so a pair is one parallel step; pairs are ordered by
+// barriers (a later pair reads corners an earlier one wrote).  Runs on one
+// workgroup of nthr threads (tid); f2 may be null.
+__device__ inline void wg_edges(const Geo &g, double *f1, double *f2, int three, int tid, int nthr) {
     const int pairs[3][3] = {{0, 1, 2}, {0, 2, 1}, {1, 2, 0}};
     for (int pr = 0; pr < 3; ++pr) {
         const int p = pairs[pr][0], q = pairs[pr][1], t = pairs[pr][2];
         if (g.per[p] || g.per[q]) continue;
-        for (int sp = 0; sp < 2; ++sp)
-            for (int sq = 0; sq < 2; ++sq) {
-                const int ep = sp ? g.N[p] : 1, ip = sp ? g.N[p] - 1 : 2;
-                const int eq = sq ? g.N[q] : 1, iq = sq ? g.N[q] - 1 : 2;
-                for (int s = 1 + (int)threadIdx.x; s <= g.N[t]; s += (int)blockDim.x) {
-                    int e[3], a[3], b[3], cc[3];
-                    e[p] = ep, e[q] = eq, e[t] = s;
-                    a[p] = ip, a[q] = eq, a[t] = s;
-                    b[p] = ep, b[q] = iq, b[t] = s;
-                    cc[p] = ip, cc[q] = iq, cc[t] = s;
-                    const int64_t pe = gidx(g, e[0], e[1], e[2]), pa = gidx(g, a[0], a[1], a[2]),
-                                  pb = gidx(g, b[0], b[1], b[2]), pc = gidx(g, cc[0], cc[1], cc[2]);
-                    for (int u = 0; u < 2; ++u) {
-                        double *f = u ? f2 : f1;
-                        if (!f) continue;
-                        if (three)
-                            f[pe] = (f[pa] + f[pb] + f[pc]) / 3.0;
-                        else
-                            f[pe] = (f[pa] + f[pb]) / 2.0;
-                    }
-                }
-                __syncthreads();
+        const int nt = g.N[t];
+        for (int u = tid; u < 4 * nt; u += nthr) {
+            const int side = u / nt, s = 1 + u - side * nt;
+            const int sp = side >> 1, sq = side & 1;
+            const int ep = sp ? g.N[p] : 1, ip = sp ? g.N[p] - 1 : 2;
+            const int eq = sq ? g.N[q] : 1, iq = sq ? g.N[q] - 1 : 2;
+            int e[3], a[3], b[3], cc[3];
+            e[p] = ep, e[q] = eq, e[t] = s;
+            a[p] = ip, a[q] = eq, a[t] = s;
+            b[p] = ep, b[q] = iq, b[t] = s;
+            cc[p] = ip, cc[q] = iq, cc[t] = s;
+            const int64_t pe = gidx(g, e[0], e[1], e[2]), pa = gidx(g, a[0], a[1], a[2]),
+                          pb = gidx(g, b[0], b[1], b[2]), pc = gidx(g, cc[0], cc[1], cc[2]);
+            for (int w = 0; w < 2; ++w) {
+                double *f = w ? f2 : f1;
+                if (!f) continue;
+                if (three)
+                    f[pe] = (f[pa] + f[pb] + f[pc]) / 3.0;
+                else
+                    f[pe] = (f[pa] + f[pb]) / 2.0;
             }
+        }
+        __syncthreads();
     }
 }
 
-struct RestrictW {
-    const double *w[3];  // weights of the coarse level, (ii+1)*cst + ii
-    int cst[3];
+// ---- one launch per step (large levels) ------------------------------------
+
+__global__ void k_mg_init(Geo g, Coef c, Range3 r, double omega, const double *__restrict__ bb,
+                          double *__restrict__ x, const int *active) {
+    if (active && !*active) return;
+    IH_POINT(r);
+    pt_init(g, c.dg, i, j, k, omega, bb, x);
+}
+
+__global__ void k_mg_lap(Geo g, Coef c, Range3 r, const double *__restrict__ x, double *__restrict__ lap,
+                         const int *active) {
+    if (active && !*active) return;
+    IH_POINT(r);
+    pt_lap(g, c.dg, i, j, k, x, lap);
+}
+
+__global__ __launch_bounds__(256) void k_mg_edges(Geo g, double *f1, double *f2, int three, const int *active) {
+    if (active && !*active) return;
+    wg_edges(g, f1, f2, three, (int)threadIdx.x, (int)blockDim.x);
+}
+
+struct LevelW {
+    const double *w[3];
+    int d[3];
 };
 
-// restrict(.TRUE., g, coarse, bb, lap) (src/mod_solvers.f90:699-714): one
-// thread per coarse point of [1, nmax]
-__global__ void k_mg_restrict(Geo gf, Geo gc, RestrictW R, Range3 r, int dd0, int dd1, int dd2,
-                              const double *__restrict__ bb, const double *__restrict__ lap,
-                              double *__restrict__ out, const int *active) {
+__global__ void k_mg_restrict(Geo gf, Geo gc, LevelW R, Range3 r, const double *__restrict__ bb,
+                              const double *__restrict__ lap, double *__restrict__ out, const int *active) {
     if (active && !*active) return;
     const int ii = r.lo0 + (int)(blockIdx.x * BX + threadIdx.x);
     const int jj = r.lo1 + (int)(blockIdx.y * BY + threadIdx.y);
     const int kk = r.lo2 + (int)blockIdx.z;
     if (ii >= r.lo0 + r.n0 || jj >= r.lo1 + r.n1) return;
-    const int i = dd0 * (ii - 1) + 1, j = dd1 * (jj - 1) + 1, k = dd2 * (kk - 1) + 1;
-    auto df = [&](int a, int b, int c) {
-        const int64_t p = wix(gf, a, b, c);
-        return bb[p] - lap[p];
-    };
-    const double *w1 = R.w[0], *w2 = R.w[1], *w3 = R.w[2];
-    const int s1 = R.cst[0], s2 = R.cst[1], s3 = R.cst[2];
-    double t = ((w1[s1 + ii] + w2[s2 + jj]) + w3[s3 + kk]) * df(i, j, k);
-    t = t + w1[ii] * df(i - 1, j, k);
-    t = t + w1[2 * s1 + ii] * df(i + 1, j, k);
-    t = t + w2[jj] * df(i, j - 1, k);
-    t = t + w2[2 * s2 + jj] * df(i, j + 1, k);
-    t = t + w3[kk] * df(i, j, k - 1);
-    t = t + w3[2 * s3 + kk] * df(i, j, k + 1);
-    out[gidx(gc, ii, jj, kk)] = t / 3.0;
+    pt_restrict(gf, gc, R.w, R.d, ii, jj, kk, bb, lap, out);
 }
 
-struct InterpW {
-    const double *w[3];  // weights of the fine level, (t-1)*cst + i
-    int cst[3];
-    int dn[3];
-};
-
-// interpolate(.TRUE., g, coarse, fine, work) (src/mod_solvers.f90:1046-1141):
-// the injected coarse values, then the z, y and x passes, evaluated per fine
-// point with the operands each pass reads, and added to fine over 1..N
-__global__ void k_mg_interp(Geo gf, Geo gc, InterpW W, Range3 r, const double *__restrict__ coarse,
-                            double *__restrict__ x, const int *active) {
+// interpolate-and-add by coarse cells: the thread of cell (I, J, K) writes the
+// fine points dn*(T-1)+1+s, s < dn (s = 0 only at T = Nc), from the eight
+// coarse values around it; every fine value is formed with pt_interp's
+// operands and order
+__global__ void k_mg_interp_cells(Geo gf, Geo gc, LevelW W, const double *__restrict__ coarse,
+                                  double *__restrict__ x, const int *active) {
     if (active && !*active) return;
-    IH_POINT(r);
-    auto lat = [&](int d, int t) { return W.dn[d] == 1 || ((t - 1) & 1) == 0; };
-    auto cix = [&](int d, int t) { return (t - 1) / W.dn[d] + 1; };
-    auto C = [&](int I, int J, int K) { return coarse[wix(gc, I, J, K)]; };
-    // z pass (i, j on the coarse lattice)
-    auto wz = [&](int a, int b, int c) {
-        const int I = cix(0, a), J = cix(1, b);
-        if (lat(2, c)) return C(I, J, cix(2, c));
-        const int K = c / 2;
-        return W.w[2][c] * C(I, J, K) + W.w[2][W.cst[2] + c] * C(I, J, K + 1);
+    const int I = 1 + (int)(blockIdx.x * BX + threadIdx.x);
+    const int J = 1 + (int)(blockIdx.y * BY + threadIdx.y);
+    const int K = 1 + (int)blockIdx.z;
+    if (I > gc.N[0] || J > gc.N[1]) return;
+    const int T[3] = {I, J, K};
+    int ns[3];
+    for (int d = 0; d < 3; ++d) ns[d] = (W.d[d] == 2 && T[d] < gc.N[d]) ? 2 : 1;
+    // coarse values at (I + a, J + b, K + e); a direction with one fine point
+    // per cell reads index 0 only
+    double cv[2][2][2];
+    for (int e = 0; e < 2; ++e)
+        for (int b = 0; b < 2; ++b)
+            for (int a = 0; a < 2; ++a)
+                cv[a][b][e] = coarse[wix(gc, I + (a < ns[0] ? a : 0), J + (b < ns[1] ? b : 0), K + (e < ns[2] ? e : 0))];
+    const int i0 = W.d[0] * (I - 1) + 1, j0 = W.d[1] * (J - 1) + 1, k0 = W.d[2] * (K - 1) + 1;
+    const double *w1 = W.w[0], *w2 = W.w[1], *w3 = W.w[2];
+    const int s1 = gf.cst[0], s2 = gf.cst[1], s3 = gf.cst[2];
+    for (int se = 0; se < ns[2]; ++se) {
+        const int k = k0 + se;
+        // z pass at the four lattice columns (a, b)
+        double wz[2][2];
+        for (int b = 0; b < 2; ++b)
+            for (int a = 0; a < 2; ++a)
+                wz[a][b] = se == 0 ? cv[a][b][0] : w3[k] * cv[a][b][0] + w3[s3 + k] * cv[a][b][1];
+        for (int sb = 0; sb < ns[1]; ++sb) {
+            const int j = j0 + sb;
+            // y pass at the two lattice x positions
+            double wy[2];
+            for (int a = 0; a < 2; ++a) wy[a] = sb == 0 ? wz[a][0] : w2[j] * wz[a][0] + w2[s2 + j] * wz[a][1];
+            for (int sa = 0; sa < ns[0]; ++sa) {
+                const int i = i0 + sa;
+                const double v = sa == 0 ? wy[0] : w1[i] * wy[0] + w1[s1 + i] * wy[1];
+                const int64_t p = gidx(gf, i, j, k);
+                x[p] = x[p] + v;
+            }
+        }
+    }
+}
+
+// ---- levels g0..G in one workgroup ------------------------------------------
+
+constexpr int MG_FUSED_THREADS = 1024;
+
+// every point of [lo, lo + n) (three directions) once, over the workgroup
+template <typename F>
+__device__ inline void wg_points(const int *lo, const int *n, F f) {
+    const int tot = n[0] * n[1] * n[2];
+    for (int q = (int)threadIdx.x; q < tot; q += MG_FUSED_THREADS) {
+        const int a = q % n[0], r = q / n[0];
+        const int b = r % n[1], c = r / n[1];
+        f(lo[0] + a, lo[1] + b, lo[2] + c);
+    }
+    __syncthreads();
+}
+
+// the rest of the cycle from level g0: its right-hand side A is in place; the
+// iterate of g0 ends in B or D as the host's account of the same swaps says
+__global__ __launch_bounds__(MG_FUSED_THREADS) void k_mg_fused(const MgLevelDev *__restrict__ L, int g0, int G,
+                                                               int n_down, int n_up, int n_bottom, double omega,
+                                                               const int *active) {
+    if (active && !*active) return;
+    const int tid = (int)threadIdx.x;
+    double *x[MG_LEVELS_MAX + 1], *y[MG_LEVELS_MAX + 1];
+    for (int g = g0; g <= G; ++g) {
+        x[g] = L[g].B;
+        y[g] = L[g].D;
+    }
+    auto range = [&](int g, int *lo, int *n) {
+        for (int d = 0; d < 3; ++d) {
+            lo[d] = L[g].g.Sp[d];
+            n[d] = L[g].g.Np[d] - L[g].g.Sp[d] + 1;
+        }
     };
-    // y pass (i on the lattice)
-    auto wy = [&](int a, int b, int c) {
-        if (lat(1, b)) return wz(a, b, c);
-        return W.w[1][b] * wz(a, b - 1, c) + W.w[1][W.cst[1] + b] * wz(a, b + 1, c);
+    auto relax = [&](int g, bool init, int nr) {
+        const MgLevelDev &l = L[g];
+        int lo[3], n[3];
+        range(g, lo, n);
+        int q = 1;
+        if (init && nr >= 1) {
+            double *xx = x[g];
+            wg_points(lo, n, [&](int i, int j, int k) { pt_init(l.g, l.dg, i, j, k, omega, l.A, xx); });
+            q = 2;
+        }
+        for (; q <= nr; ++q) {
+            const double *xx = x[g];
+            double *yy = y[g];
+            wg_points(lo, n, [&](int i, int j, int k) { pt_jor(l.g, l.dg, i, j, k, omega, xx, l.A, yy); });
+            double *t = x[g];
+            x[g] = y[g];
+            y[g] = t;
+        }
     };
-    double v;
-    if (lat(0, i))
-        v = wy(i, j, k);
-    else
-        v = W.w[0][i] * wy(i - 1, j, k) + W.w[0][W.cst[0] + i] * wy(i + 1, j, k);
-    const int64_t p = gidx(gf, i, j, k);
-    x[p] = x[p] + v;
+    for (int g = g0; g <= G - 1; ++g) {
+        const MgLevelDev &l = L[g], &lc = L[g + 1];
+        relax(g, true, n_down);
+        int lo[3], n[3];
+        range(g, lo, n);
+        const double *xx = x[g];
+        wg_points(lo, n, [&](int i, int j, int k) { pt_lap(l.g, l.dg, i, j, k, xx, l.C); });
+        wg_edges(l.g, l.A, l.C, 0, tid, MG_FUSED_THREADS);
+        int clo[3] = {1, 1, 1}, cn[3];
+        for (int d = 0; d < 3; ++d) cn[d] = lc.g.N[d] - (l.g.per[d] ? 1 : 0);
+        wg_points(clo, cn, [&](int ii, int jj, int kk) {
+            pt_restrict(l.g, lc.g, lc.cR, l.dn, ii, jj, kk, l.A, l.C, lc.A);
+        });
+    }
+    relax(G, true, n_bottom);
+    relax(G, false, n_bottom);
+    for (int g = G - 1; g >= g0; --g) {
+        const MgLevelDev &l = L[g], &lc = L[g + 1];
+        wg_edges(lc.g, x[g + 1], nullptr, 1, tid, MG_FUSED_THREADS);
+        int flo[3] = {1, 1, 1}, fn[3] = {l.g.N[0], l.g.N[1], l.g.N[2]};
+        const double *co = x[g + 1];
+        double *xx = x[g];
+        wg_points(flo, fn, [&](int i, int j, int k) { pt_interp(l.g, lc.g, l.cI, l.dn, i, j, k, co, xx); });
+        relax(g, false, n_up);
+    }
 }
 
 inline Range3 full_range(const Geo &g) { return Range3{1, 1, 1, g.N[0], g.N[1], g.N[2]}; }
@@ -179,6 +335,7 @@ inline Range3 full_range(const Geo &g) { return Range3{1, 1, 1, g.N[0], g.N[1], 
 
 void launch_mg_vcycle(hipStream_t s, const MgDev &m, double *v, double *z, const int *act) {
     const int G = m.G;
+    const int g0 = m.g0;  // levels g0..G run fused (g0 >= 2)
     double *b[MG_LEVELS_MAX + 1], *x[MG_LEVELS_MAX + 1], *y[MG_LEVELS_MAX + 1];
     // level 1 alternates between z and D over the cycle's sweeps; starting on the
     // buffer that makes the last sweep write z leaves the result in place
@@ -191,57 +348,71 @@ void launch_mg_vcycle(hipStream_t s, const MgDev &m, double *v, double *z, const
         x[g] = m.L[g].B;
         y[g] = m.L[g].D;
     }
-    auto relax = [&](int g, bool init, int n) {
+    // relaxation of level g; launch = false only follows the swaps (the fused kernel runs them)
+    auto relax = [&](int g, bool init, int n, bool launch) {
         const MgLevel &L = m.L[g];
         const Range3 r = make_range(L.pr);
         int q = 1;
         if (init && n >= 1) {
-            hipLaunchKernelGGL(k_mg_init, grid_for(r), dim3(BX, BY, 1), 0, s, L.g, L.c, r, m.omega, b[g], x[g], act);
+            if (launch)
+                hipLaunchKernelGGL(k_mg_init, grid_for(r), dim3(BX, BY, 1), 0, s, L.g, L.c, r, m.omega, b[g], x[g],
+                                   act);
             q = 2;
         }
         for (; q <= n; ++q) {
-            launch_jor(s, L.g, L.c, L.pr, m.omega, x[g], b[g], y[g], act, L.shape);
+            if (launch) launch_jor(s, L.g, L.c, L.pr, m.omega, x[g], b[g], y[g], act, L.shape);
             std::swap(x[g], y[g]);
         }
     };
     if (G == 1) {
-        relax(1, true, m.n_bottom);
-        relax(1, false, m.n_bottom);
+        relax(1, true, m.n_bottom, true);
+        relax(1, false, m.n_bottom, true);
         return;
     }
-    for (int g = 1; g <= G - 1; ++g) {
+    const int gl = std::min(g0, G) - 1;  // last level with launches of its own on the way down
+    for (int g = 1; g <= gl; ++g) {
         const MgLevel &L = m.L[g], &Lc = m.L[g + 1];
-        relax(g, true, m.n_down);
+        relax(g, true, m.n_down, true);
         const Range3 pr = make_range(L.pr);
         hipLaunchKernelGGL(k_mg_lap, grid_for(pr), dim3(BX, BY, 1), 0, s, L.g, L.c, pr, x[g], L.C, act);
         if (!(L.g.per[0] && L.g.per[1] && L.g.per[2]))
             hipLaunchKernelGGL(k_mg_edges, dim3(1), dim3(256), 0, s, L.g, b[g], L.C, 0, act);
-        RestrictW R;
+        LevelW R;
         int nmax[3];
         for (int d = 0; d < 3; ++d) {
             R.w[d] = Lc.cR[d];
-            R.cst[d] = Lc.g.cst[d];
+            R.d[d] = L.dn[d];
             nmax[d] = Lc.g.N[d] - (L.g.per[d] ? 1 : 0);
         }
         const Range3 cr{1, 1, 1, nmax[0], nmax[1], nmax[2]};
-        hipLaunchKernelGGL(k_mg_restrict, grid_for(cr), dim3(BX, BY, 1), 0, s, L.g, Lc.g, R, cr, L.dn[0], L.dn[1],
-                           L.dn[2], b[g], L.C, Lc.A, act);
+        hipLaunchKernelGGL(k_mg_restrict, grid_for(cr), dim3(BX, BY, 1), 0, s, L.g, Lc.g, R, cr, b[g], L.C, Lc.A,
+                           act);
     }
-    relax(G, true, m.n_bottom);
-    relax(G, false, m.n_bottom);
-    for (int g = G - 1; g >= 1; --g) {
+    if (g0 <= G) {
+        hipLaunchKernelGGL(k_mg_fused, dim3(1), dim3(MG_FUSED_THREADS), 0, s, m.Ldev, g0, G, m.n_down, m.n_up,
+                           m.n_bottom, m.omega, act);
+        // the fused levels' swaps, for the buffer that holds level g0's iterate
+        for (int g = g0; g <= G - 1; ++g) relax(g, true, m.n_down, false);
+        relax(G, true, m.n_bottom, false);
+        relax(G, false, m.n_bottom, false);
+        for (int g = G - 1; g >= g0; --g) relax(g, false, m.n_up, false);
+    } else {
+        relax(G, true, m.n_bottom, true);
+        relax(G, false, m.n_bottom, true);
+    }
+    for (int g = gl; g >= 1; --g) {
         const MgLevel &L = m.L[g], &Lc = m.L[g + 1];
         if (!(Lc.g.per[0] && Lc.g.per[1] && Lc.g.per[2]))
             hipLaunchKernelGGL(k_mg_edges, dim3(1), dim3(256), 0, s, Lc.g, x[g + 1], (double *)nullptr, 1, act);
-        InterpW W;
+        LevelW W;
         for (int d = 0; d < 3; ++d) {
             W.w[d] = L.cI[d];
-            W.cst[d] = L.g.cst[d];
-            W.dn[d] = L.dn[d];
+            W.d[d] = L.dn[d];
         }
-        const Range3 fr = full_range(L.g);
-        hipLaunchKernelGGL(k_mg_interp, grid_for(fr), dim3(BX, BY, 1), 0, s, L.g, Lc.g, W, fr, x[g + 1], x[g], act);
-        relax(g, false, m.n_up);
+        const Range3 cr = full_range(Lc.g);
+        hipLaunchKernelGGL(k_mg_interp_cells, grid_for(cr), dim3(BX, BY, 1), 0, s, L.g, Lc.g, W, x[g + 1], x[g],
+                           act);
+        relax(g, false, m.n_up, true);
     }
 }
 

@@ -25,6 +25,8 @@ SPECS = [
     pytest.param(((70, 9, 40), (1, 1, -1, -1, 1, 1), (2.0, 1.0, 1.0), (0.0, 0.0, 1.5)), id="ragged-walls-x"),
     pytest.param(((67, 10, 37), (-1, -1, 1, 1, -1, -1), (1.0, 1.0, 1.0), (0.0, 1.3, 0.0)), id="ragged-periodic-xz"),
     pytest.param(((20, 20, 20), (1, 1, 1, 1, 1, 1), (1.0, 1.0, 1.0), (0.0, 0.0, 0.0)), id="one-level"),
+    # level 2 (33x33x17) above the fused-kernel size: launched level by level, levels 3.. fused
+    pytest.param(((65, 65, 33), (1, 1, -1, -1, 1, 1), (1.0, 1.0, 0.5), (0.0, 0.0, 1.5)), id="launched-levels"),
 ]
 
 
@@ -83,7 +85,7 @@ def test_vcycle_configurations(cfg):
     same(gc.download(ih.PRE)[s], z[s], f"V cycle {cfg}")
 
 
-@pytest.mark.parametrize("spec", SPECS[:5])
+@pytest.mark.parametrize("spec", SPECS[:5] + SPECS[6:])
 def test_bicgstab_mg(spec):
     oc, gc = pair(spec)
     om = O.OracleMG(oc)

@@ -807,7 +807,7 @@ static JorLaunch jor_shape(ih_case *c, const Box &b, bool fp32, F sweep, int lev
     bool have = false, failed = false;
     for (int tpb : {128, 256, 512}) {
         int last_nbz = -1;
-        for (int kc : {6, 8, 10, 12, 14, 16, 20, 24, 32}) {
+        for (int kc : {2, 4, 6, 8, 10, 12, 14, 16, 20, 24, 32}) {  // short chunks suit small boxes
             const int nbz = (n2 + kc - 1) / kc;
             if (nbz == last_nbz) continue;  // same launch as the previous candidate
             last_nbz = nbz;

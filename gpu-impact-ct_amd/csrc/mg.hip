@@ -82,8 +82,8 @@ __device__ inline void pt_lap(const Geo &g, const double *const *dg, int i, int 
 
 // restrict(.TRUE., g, coarse, bb, lap) at coarse point (ii, jj, kk)
 // (src/mod_solvers.f90:699-714); w: the coarse level's weights
-__device__ inline void pt_restrict(const Geo &gf, const Geo &gc, const double *const *w, const int *dd, int ii,
-                                   int jj, int kk, const double *bb, const double *lap, double *out) {
+__device__ inline double pt_restrict(const Geo &gf, const Geo &gc, const double *const *w, const int *dd, int ii,
+                                     int jj, int kk, const double *bb, const double *lap, double *out) {
     const int i = dd[0] * (ii - 1) + 1, j = dd[1] * (jj - 1) + 1, k = dd[2] * (kk - 1) + 1;
     auto df = [&](int a, int b, int c) {
         const int64_t p = wix(gf, a, b, c);
@@ -97,7 +97,9 @@ __device__ inline void pt_restrict(const Geo &gf, const Geo &gc, const double *c
     t = t + w[1][2 * s2 + jj] * df(i, j + 1, k);
     t = t + w[2][kk] * df(i, j, k - 1);
     t = t + w[2][2 * s3 + kk] * df(i, j, k + 1);
-    out[gidx(gc, ii, jj, kk)] = t / 3.0;
+    const double a = t / 3.0;
+    out[gidx(gc, ii, jj, kk)] = a;
+    return a;
 }
 
 // interpolate(.TRUE., g, coarse, fine, work) at fine point (i, j, k)
@@ -193,14 +195,21 @@ struct LevelW {
     int d[3];
 };
 
-__global__ void k_mg_restrict(Geo gf, Geo gc, LevelW R, Range3 r, const double *__restrict__ bb,
-                              const double *__restrict__ lap, double *__restrict__ out, const int *active) {
+// restriction; with xinit the coarse level's first relaxation as well
+// (omega A / diag at the same point, the value pt_init forms)
+__global__ void k_mg_restrict(Geo gf, Geo gc, LevelW R, Coef cc, Range3 r, const double *__restrict__ bb,
+                              const double *__restrict__ lap, double *__restrict__ out, double *__restrict__ xinit,
+                              double omega, const int *active) {
     if (active && !*active) return;
     const int ii = r.lo0 + (int)(blockIdx.x * BX + threadIdx.x);
     const int jj = r.lo1 + (int)(blockIdx.y * BY + threadIdx.y);
     const int kk = r.lo2 + (int)blockIdx.z;
     if (ii >= r.lo0 + r.n0 || jj >= r.lo1 + r.n1) return;
-    pt_restrict(gf, gc, R.w, R.d, ii, jj, kk, bb, lap, out);
+    const double a = pt_restrict(gf, gc, R.w, R.d, ii, jj, kk, bb, lap, out);
+    if (xinit) {
+        const double d = (cc.dg[0][gc.cst[0] + ii] + cc.dg[1][gc.cst[1] + jj]) + cc.dg[2][gc.cst[2] + kk];
+        xinit[gidx(gc, ii, jj, kk)] = omega * a / d;
+    }
 }
 
 // interpolate-and-add by coarse cells: the thread of cell (I, J, K) writes the
@@ -348,13 +357,14 @@ void launch_mg_vcycle(hipStream_t s, const MgDev &m, double *v, double *z, const
         x[g] = m.L[g].B;
         y[g] = m.L[g].D;
     }
-    // relaxation of level g; launch = false only follows the swaps (the fused kernel runs them)
-    auto relax = [&](int g, bool init, int n, bool launch) {
+    // relaxation of level g; launch = false only follows the swaps (the fused kernel
+    // runs them); init_done: the restriction into g already wrote the first relaxation
+    auto relax = [&](int g, bool init, int n, bool launch, bool init_done = false) {
         const MgLevel &L = m.L[g];
         const Range3 r = make_range(L.pr);
         int q = 1;
         if (init && n >= 1) {
-            if (launch)
+            if (launch && !init_done)
                 hipLaunchKernelGGL(k_mg_init, grid_for(r), dim3(BX, BY, 1), 0, s, L.g, L.c, r, m.omega, b[g], x[g],
                                    act);
             q = 2;
@@ -372,7 +382,7 @@ void launch_mg_vcycle(hipStream_t s, const MgDev &m, double *v, double *z, const
     const int gl = std::min(g0, G) - 1;  // last level with launches of its own on the way down
     for (int g = 1; g <= gl; ++g) {
         const MgLevel &L = m.L[g], &Lc = m.L[g + 1];
-        relax(g, true, m.n_down, true);
+        relax(g, true, m.n_down, true, g >= 2);
         const Range3 pr = make_range(L.pr);
         hipLaunchKernelGGL(k_mg_lap, grid_for(pr), dim3(BX, BY, 1), 0, s, L.g, L.c, pr, x[g], L.C, act);
         if (!(L.g.per[0] && L.g.per[1] && L.g.per[2]))
@@ -385,8 +395,11 @@ void launch_mg_vcycle(hipStream_t s, const MgDev &m, double *v, double *z, const
             nmax[d] = Lc.g.N[d] - (L.g.per[d] ? 1 : 0);
         }
         const Range3 cr{1, 1, 1, nmax[0], nmax[1], nmax[2]};
-        hipLaunchKernelGGL(k_mg_restrict, grid_for(cr), dim3(BX, BY, 1), 0, s, L.g, Lc.g, R, cr, b[g], L.C, Lc.A,
-                           act);
+        // a launched coarse level takes its first relaxation from here (the fused
+        // levels run their own); nmax spans that level's pressure range
+        const bool coarse_launched = g + 1 < g0;
+        hipLaunchKernelGGL(k_mg_restrict, grid_for(cr), dim3(BX, BY, 1), 0, s, L.g, Lc.g, R, Lc.c, cr, b[g], L.C,
+                           Lc.A, coarse_launched ? x[g + 1] : (double *)nullptr, m.omega, act);
     }
     if (g0 <= G) {
         hipLaunchKernelGGL(k_mg_fused, dim3(1), dim3(MG_FUSED_THREADS), 0, s, m.Ldev, g0, G, m.n_down, m.n_up,
@@ -397,7 +410,7 @@ void launch_mg_vcycle(hipStream_t s, const MgDev &m, double *v, double *z, const
         relax(G, false, m.n_bottom, false);
         for (int g = G - 1; g >= g0; --g) relax(g, false, m.n_up, false);
     } else {
-        relax(G, true, m.n_bottom, true);
+        relax(G, true, m.n_bottom, true, true);
         relax(G, false, m.n_bottom, true);
     }
     for (int g = gl; g >= 1; --g) {

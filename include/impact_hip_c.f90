@@ -82,6 +82,32 @@ module impact_hip_c
        integer(c_int), value :: rel, bb
      end function ih_jor
 
+     ! BiCGstab_orig with the reference's preconditioner argument (0 none, 1 multigrid V cycle)
+     integer(c_int) function ih_bicgstab_precond(ctx, eps, n_it_max, ss, nn, bb, phi, preconditioner, st) &
+          bind(C, name="ih_bicgstab_precond")
+       import :: c_int, c_double, c_ptr
+       type(c_ptr), value :: ctx
+       real(c_double), value :: eps
+       integer(c_int), value :: n_it_max
+       integer(c_int), intent(in) :: ss(3), nn(3)
+       integer(c_int), value :: bb, phi, preconditioner
+       type(c_ptr), value :: st
+     end function ih_bicgstab_precond
+
+     integer(c_int) function ih_mg_configure(ctx, n_grids_limit, n_relax_down, n_relax_up, n_relax_bottom) &
+          bind(C, name="ih_mg_configure")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: n_grids_limit, n_relax_down, n_relax_up, n_relax_bottom
+     end function ih_mg_configure
+
+     ! multigridV(.TRUE., 1, bb, phi) on resident fields
+     integer(c_int) function ih_multigrid_v(ctx, bb, phi) bind(C, name="ih_multigrid_v")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: bb, phi
+     end function ih_multigrid_v
+
      integer(c_int) function ih_device_jacobi_solver(ctx, grid_id, omega, SS1, SS2, SS3, NN1, NN2, NN3, rel, bb) &
           bind(C, name="ih_device_jacobi_solver")
        import :: c_int, c_double, c_ptr

@@ -92,8 +92,8 @@ def setup_case(ih, mixed=False, precond_sweeps=8, multigrid=False):
     c.fill(ih.PRE, 0.0)
     p = ih.StepParams(Re=RE, dtime=DT, jor_sweeps=JOR_SWEEPS, jor_omega=JOR_OMEGA, bicg_rel_tol=TOL,
                       bicg_max_it=MAXIT, level_pressure=1, inflow_kind=2, inflow_amp=1.0, use_mask=1,
-                      jor_fp32=1 if mixed else 0, precond_sweeps=precond_sweeps if mixed else 0,
-                      precond_poisson=1 if multigrid else 0)
+                      jor_fp32=1 if mixed else 0, precond_sweeps=precond_sweeps if (mixed and not multigrid) else 0,
+                      precond_poisson=1 if multigrid else 0, mg_fp32=1 if (mixed and multigrid) else 0)
     return c, p
 
 
@@ -428,7 +428,7 @@ def main():
         ms_g, fams_g, iters_g, _ = run_steps(ih, cg, pg, args.steps, args.warmup, "multigrid")
         err = rel_errors(ih, cg, c)
         bg_ms, bg_n = fams_g["bicgstab"]
-        result["multigrid_preconditioner"] = {
+        result["multigrid_preconditioner"] = mg_res = {
             "form": ("FP64 BiCGstab with preconditioner 1, the multigrid V cycle (levels to 3 points, 4/4/4 "
                      "Jacobi relaxations, omega 0.98), after the same 200 JOR sweeps"),
             "steps_per_s": 1000.0 / ms_g,
@@ -440,6 +440,24 @@ def main():
             **{f"{k}_vs_plain": v for k, v in err.items()},
         }
         cg.close()
+        # mixed precision with the cycle in FP32
+        cx, px = setup_case(ih, mixed=True, multigrid=True)
+        ms_x, fams_x, iters_x, _ = run_steps(ih, cx, px, args.steps, args.warmup, "mixed multigrid")
+        err = rel_errors(ih, cx, c)
+        bx_ms, bx_n = fams_x["bicgstab"]
+        result["mixed_precision_multigrid"] = {
+            "form": (f"{JOR_SWEEPS} FP32 JOR sweeps as defect correction, then BiCGstab preconditioned by the "
+                     "multigrid V cycle in FP32; residuals, inner products and updates FP64"),
+            "steps_per_s": 1000.0 / ms_x,
+            "ms_per_step": ms_x,
+            "speedup_vs_fp64": ms_per_step / ms_x,
+            "speedup_vs_fp64_multigrid": mg_res["ms_per_step"] / ms_x,
+            "bicgstab_iterations_per_step": iters_x,
+            "bicgstab_ms_per_iteration": bx_ms / max(bx_n, 1),
+            "steps_compared": args.warmup + args.steps + 1,
+            **{f"{k}_vs_fp64": v for k, v in err.items()},
+        }
+        cx.close()
     if not args.no_batch:
         log("batched relaxation ...")
         result["batched_relaxation"] = batched_relaxation(ih)

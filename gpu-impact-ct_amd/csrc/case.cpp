@@ -317,7 +317,7 @@ double absmax_host(ih_case *c, const Box &b, const double *a, int *rc) {
 }
 
 void drop_mg(ih_case *c) {
-    if (!c->mg && !c->mgmem && !c->mgldev) return;
+    if (!c->mg && !c->mgmem && !c->mgldev && !c->mgmem32 && !c->mgldev32) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     delete c->mg;
     c->mg = nullptr;
@@ -325,6 +325,10 @@ void drop_mg(ih_case *c) {
     c->mgmem = nullptr;
     if (c->mgldev) (void)hipFree(c->mgldev);
     c->mgldev = nullptr;
+    if (c->mgmem32) (void)hipFree(c->mgmem32);
+    c->mgmem32 = nullptr;
+    if (c->mgldev32) (void)hipFree(c->mgldev32);
+    c->mgldev32 = nullptr;
     c->mgh.clear();
 }
 
@@ -449,6 +453,8 @@ void ih_case_destroy(ih_case *c) {
     delete c->mg;
     if (c->mgmem) (void)hipFree(c->mgmem);
     if (c->mgldev) (void)hipFree(c->mgldev);
+    if (c->mgmem32) (void)hipFree(c->mgmem32);
+    if (c->mgldev32) (void)hipFree(c->mgldev32);
     resolve_timings(c);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -901,8 +907,9 @@ int ih_residual_cdg(ih_case *c, const int ss[3], const int nn[3], ih_field rel, 
 // iterations and reads the flags once per batch.
 static int mg_setup(ih_case *c);
 
+// mg: 0 no multigrid, 1 the FP64 cycle, 2 the FP32 cycle as M
 static int bicgstab_run(ih_case *c, double eps, int n_it_max, const int ss[3], const int nn[3], ih_field bbf,
-                        ih_field phif, int sweeps, double omega, bool mg, ih_bicgstab_stats *st) {
+                        ih_field phif, int sweeps, double omega, int mg, ih_bicgstab_stats *st) {
     int rc = need_coeffs(c);
     if (rc) return rc;
     if (!ss || !nn || !valid_field(bbf) || !valid_field(phif) || n_it_max < 0 || sweeps < 0)
@@ -941,6 +948,7 @@ static int bicgstab_run(ih_case *c, double eps, int n_it_max, const int ss[3], c
         k.z1 = F(c, IH_Z1);
         k.z2 = F(c, IH_Z2);
         k.mg = c->mg;
+        k.mg_fp32 = mg == 2;
     }
     // rr = bb - A phi, rh = rr, norm and <rr, rh>
     if ((rc = apply_div_grad(c, F(c, phif), k.rr))) return rc;
@@ -975,20 +983,25 @@ static int bicgstab_run(ih_case *c, double eps, int n_it_max, const int ss[3], c
 
 int ih_bicgstab(ih_case *c, double eps, int n_it_max, const int ss[3], const int nn[3], ih_field bbf, ih_field phif,
                 ih_bicgstab_stats *st) {
-    return bicgstab_run(c, eps, n_it_max, ss, nn, bbf, phif, 0, 0.0, false, st);
+    return bicgstab_run(c, eps, n_it_max, ss, nn, bbf, phif, 0, 0.0, 0, st);
 }
 
 int ih_bicgstab_precond(ih_case *c, double eps, int n_it_max, const int ss[3], const int nn[3], ih_field bbf,
                         ih_field phif, int preconditioner, ih_bicgstab_stats *st) {
-    if (preconditioner == 0) return bicgstab_run(c, eps, n_it_max, ss, nn, bbf, phif, 0, 0.0, false, st);
-    if (preconditioner == 1) return bicgstab_run(c, eps, n_it_max, ss, nn, bbf, phif, 0, 0.0, true, st);
+    if (preconditioner == 0) return bicgstab_run(c, eps, n_it_max, ss, nn, bbf, phif, 0, 0.0, 0, st);
+    if (preconditioner == 1) return bicgstab_run(c, eps, n_it_max, ss, nn, bbf, phif, 0, 0.0, 1, st);
     return fail(IH_EINVAL, "preconditioner %d (0: none, 1: multigrid V cycle)", preconditioner);
+}
+
+int ih_bicgstab_mg_f32(ih_case *c, double eps, int n_it_max, const int ss[3], const int nn[3], ih_field bbf,
+                       ih_field phif, ih_bicgstab_stats *st) {
+    return bicgstab_run(c, eps, n_it_max, ss, nn, bbf, phif, 0, 0.0, 2, st);
 }
 
 int ih_bicgstab_mixed(ih_case *c, double eps, int n_it_max, const int ss[3], const int nn[3], ih_field bbf,
                       ih_field phif, int sweeps, double omega, ih_bicgstab_stats *st) {
     if (sweeps < 1) return fail(IH_EINVAL, "the mixed-precision preconditioner needs at least one sweep");
-    return bicgstab_run(c, eps, n_it_max, ss, nn, bbf, phif, sweeps, omega, false, st);
+    return bicgstab_run(c, eps, n_it_max, ss, nn, bbf, phif, sweeps, omega, 0, st);
 }
 
 // ---------------------------------------------------------------------------
@@ -1102,10 +1115,82 @@ static int mg_setup(ih_case *c) {
             break;
         }
     }
-    std::vector<MgLevelDev> ld((size_t)G + 1);
+    // the FP32 cycle: float arrays (level 1 included: its Af is (float) of the
+    // right-hand side) and the tables rounded once
+    int64_t arr32 = 0;
+    int64_t offA32[MG_LEVELS_MAX + 1], offB32[MG_LEVELS_MAX + 1], offC32[MG_LEVELS_MAX + 1],
+        offD32[MG_LEVELS_MAX + 1];
+    for (int g = 1; g <= G; ++g) {
+        const int64_t fe = m->L[g].g.fe;
+        offA32[g] = arr32;
+        offB32[g] = arr32 + fe;
+        offC32[g] = arr32 + 2 * fe;
+        offD32[g] = arr32 + 3 * fe;
+        arr32 += 4 * fe;
+    }
+    std::vector<float> tab32;
+    int64_t offdg32[MG_LEVELS_MAX + 1][3], offR32[MG_LEVELS_MAX + 1][3], offI32[MG_LEVELS_MAX + 1][3];
+    for (int g = 1; g <= G; ++g)
+        for (int d = 0; d < 3; ++d) {
+            const int cst = m->L[g].g.cst[d];
+            auto put32 = [&](int64_t off, int taps) -> int64_t {
+                const int64_t o = (int64_t)tab32.size();
+                for (int64_t q = 0; q < (int64_t)taps * cst; ++q) tab32.push_back((float)tab[(size_t)(off + q)]);
+                return o;
+            };
+            offdg32[g][d] = put32(offdg[g][d], 3);
+            offR32[g][d] = put32(offR[g][d], 3);
+            offI32[g][d] = put32(offI[g][d], 2);
+        }
+    if (hipMalloc(&c->mgmem32, sizeof(float) * (size_t)(arr32 + (int64_t)tab32.size())) != hipSuccess) {
+        delete m;
+        return fail(IH_ENOMEM, "out of device memory for the FP32 multigrid levels");
+    }
+    HIPCHK(hipMemsetAsync(c->mgmem32, 0, sizeof(float) * (size_t)arr32, c->stream));
+    HIPCHK(hipMemcpyAsync(c->mgmem32 + arr32, tab32.data(), sizeof(float) * tab32.size(), hipMemcpyHostToDevice,
+                          c->stream));
+    {
+        float *fb = c->mgmem32, *ft = c->mgmem32 + arr32;
+        for (int g = 1; g <= G; ++g) {
+            MgLevel &L = m->L[g];
+            L.Af = fb + offA32[g];
+            L.Bf = fb + offB32[g];
+            L.Cf = fb + offC32[g];
+            L.Df = fb + offD32[g];
+            for (int d = 0; d < 3; ++d) {
+                if (g >= 2) L.c.dgf[d] = ft + offdg32[g][d];
+                L.cRf[d] = ft + offR32[g][d];
+                L.cIf[d] = ft + offI32[g][d];
+            }
+        }
+    }
+    std::vector<MgLevelDev<float>> ld32((size_t)G + 1);
     for (int g = 1; g <= G; ++g) {
         const MgLevel &L = m->L[g];
-        MgLevelDev &d = ld[g];
+        MgLevelDev<float> &d = ld32[g];
+        d.g = L.g;
+        for (int e = 0; e < 3; ++e) {
+            d.dg[e] = L.c.dgf[e];
+            d.cR[e] = L.cRf[e];
+            d.cI[e] = L.cIf[e];
+            d.dn[e] = L.dn[e];
+        }
+        d.A = L.Af;
+        d.B = L.Bf;
+        d.C = L.Cf;
+        d.D = L.Df;
+    }
+    if (hipMalloc(&c->mgldev32, sizeof(MgLevelDev<float>) * ld32.size()) != hipSuccess) {
+        delete m;
+        return fail(IH_ENOMEM, "out of device memory for the multigrid level descriptors");
+    }
+    HIPCHK(hipMemcpyAsync(c->mgldev32, ld32.data(), sizeof(MgLevelDev<float>) * ld32.size(), hipMemcpyHostToDevice,
+                          c->stream));
+    m->Ldev32 = (const MgLevelDev<float> *)c->mgldev32;
+    std::vector<MgLevelDev<double>> ld((size_t)G + 1);
+    for (int g = 1; g <= G; ++g) {
+        const MgLevel &L = m->L[g];
+        MgLevelDev<double> &d = ld[g];
         d.g = L.g;
         for (int e = 0; e < 3; ++e) {
             d.dg[e] = L.c.dg[e];
@@ -1118,12 +1203,13 @@ static int mg_setup(ih_case *c) {
         d.C = L.C;
         d.D = L.D;
     }
-    if (hipMalloc(&c->mgldev, sizeof(MgLevelDev) * ld.size()) != hipSuccess) {
+    if (hipMalloc(&c->mgldev, sizeof(MgLevelDev<double>) * ld.size()) != hipSuccess) {
         delete m;
         return fail(IH_ENOMEM, "out of device memory for the multigrid level descriptors");
     }
-    HIPCHK(hipMemcpyAsync(c->mgldev, ld.data(), sizeof(MgLevelDev) * ld.size(), hipMemcpyHostToDevice, c->stream));
-    m->Ldev = (const MgLevelDev *)c->mgldev;
+    HIPCHK(hipMemcpyAsync(c->mgldev, ld.data(), sizeof(MgLevelDev<double>) * ld.size(), hipMemcpyHostToDevice,
+                          c->stream));
+    m->Ldev = (const MgLevelDev<double> *)c->mgldev;
     // launch shape of each level's sweep (a sweep is a pure function of its inputs)
     for (int g = 1; g < std::min(m->g0, G + 1); ++g) {
         MgLevel &L = m->L[g];
@@ -1131,6 +1217,9 @@ static int mg_setup(ih_case *c) {
         const double *x0 = g == 1 ? L.C : L.B;
         L.shape = jor_shape(c, L.pr, false, [&](JorLaunch l) {
             launch_jor(c->stream, L.g, L.c, L.pr, m->omega, x0, bb, L.D, l);
+        }, g);
+        L.shape32 = jor_shape(c, L.pr, true, [&](JorLaunch l) {
+            launch_jor_f32(c->stream, L.g, L.c, L.pr, (float)m->omega, L.Bf, L.Af, L.Df, nullptr, l);
         }, g);
     }
     LAUNCHCHK();
@@ -1184,6 +1273,17 @@ int ih_mg_get_coeff(ih_case *c, int level, const char *name, double *out, int64_
     if (!v) return fail(IH_EINVAL, "unknown multigrid table '%s'", name);
     if (out) std::copy(v->begin(), v->end(), out);
     *n = (int64_t)v->size();
+    return IH_OK;
+}
+
+int ih_multigrid_v_f32(ih_case *c, ih_field bb, ih_field phi) {
+    int rc = need_coeffs(c);
+    if (rc) return rc;
+    if (!valid_field(bb) || !valid_field(phi) || bb == phi) return fail(IH_EINVAL, "bad argument");
+    HIPCHK(hipSetDevice(c->device));
+    if ((rc = mg_setup(c))) return rc;
+    launch_mg_vcycle_f32(c->stream, *c->mg, F(c, bb), F(c, phi), nullptr);
+    LAUNCHCHK();
     return IH_OK;
 }
 
@@ -1322,6 +1422,20 @@ static int poisson_solve(ih_case *c, const ih_step_params *p, const Box &b, ih_b
             HIPCHK(hipMemcpyAsync(F(c, IH_PRE), F(c, IH_COMP), sizeof(double) * (size_t)c->g.fe,
                                   hipMemcpyDeviceToDevice, c->stream));
             rc = ih_bicgstab(c, eps, p->bicg_max_it, ss, nn, IH_RES, IH_PRE, bst);
+            if (bst) bst->iterations += mst.iterations;
+        } else if (bst) {
+            *bst = mst;
+        }
+    } else if (p->precond_poisson == 1 && p->mg_fp32) {
+        // the FP32 cycle; a solve that does not converge is redone with the FP64 cycle from the saved guess
+        HIPCHK(hipMemcpyAsync(F(c, IH_COMP), F(c, IH_PRE), sizeof(double) * (size_t)c->g.fe, hipMemcpyDeviceToDevice,
+                              c->stream));
+        ih_bicgstab_stats mst{};
+        rc = ih_bicgstab_mg_f32(c, eps, p->bicg_max_it, ss, nn, IH_RES, IH_PRE, &mst);
+        if (!rc && mst.exit_reason != 0) {
+            HIPCHK(hipMemcpyAsync(F(c, IH_PRE), F(c, IH_COMP), sizeof(double) * (size_t)c->g.fe,
+                                  hipMemcpyDeviceToDevice, c->stream));
+            rc = ih_bicgstab_precond(c, eps, p->bicg_max_it, ss, nn, IH_RES, IH_PRE, 1, bst);
             if (bst) bst->iterations += mst.iterations;
         } else if (bst) {
             *bst = mst;

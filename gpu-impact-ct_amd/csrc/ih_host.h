@@ -86,6 +86,8 @@ struct ih_case {
     ih::MgDev *mg = nullptr;
     double *mgmem = nullptr;          // level arrays and coefficient tables
     void *mgldev = nullptr;           // level descriptors of the fused coarse-level kernel
+    float *mgmem32 = nullptr;         // the FP32 cycle's arrays and tables
+    void *mgldev32 = nullptr;
     int mg_cfg[4] = {15, 4, 4, 4};    // n_grids_limit, n_relax_down, n_relax_up, n_relax_bottom
     std::vector<ih::HostMgLevel> mgh;
 };

@@ -39,6 +39,7 @@ struct KrylovPtrs {
     JorLaunch shape;  // launch shape of the FP32 sweeps
     // right preconditioner M = the multigrid V cycle (preconditioner 1) instead
     const MgDev *mg = nullptr;
+    bool mg_fp32 = false;  // the cycle in FP32 (z = (double) of its result)
 };
 
 // dot modes of the operator kernel

@@ -22,16 +22,24 @@ struct MgLevel {
     const double *cI[3] = {nullptr, nullptr, nullptr};  // interpolation weights, (t-1)*cst + i (levels < G)
     int dn[3] = {1, 1, 1};  // coarsening ratio to the next level
     JorLaunch shape;
+    // the FP32 cycle: float arrays (level 1: Af holds (float) of the right-hand
+    // side) and tables rounded once (c.dgf holds cdg)
+    float *Af = nullptr, *Bf = nullptr, *Cf = nullptr, *Df = nullptr;
+    const float *cRf[3] = {nullptr, nullptr, nullptr};
+    const float *cIf[3] = {nullptr, nullptr, nullptr};
+    JorLaunch shape32;
 };
 
-// A level as the fused coarse-level kernel reads it from device memory.
+// A level as the fused coarse-level kernel reads it from device memory, for
+// the arithmetic type T of the cycle.
+template <typename T>
 struct MgLevelDev {
     Geo g;
-    const double *dg[3];
-    const double *cR[3];
-    const double *cI[3];
+    const T *dg[3];
+    const T *cR[3];
+    const T *cI[3];
     int dn[3];
-    double *A, *B, *C, *D;
+    T *A, *B, *C, *D;
 };
 
 struct MgDev {
@@ -42,7 +50,8 @@ struct MgDev {
     // levels g0..G run inside one single-workgroup launch (g0 > G: none);
     // Ldev: device copy of levels 1..G (index 0 unused)
     int g0 = MG_LEVELS_MAX + 1;
-    const MgLevelDev *Ldev = nullptr;
+    const MgLevelDev<double> *Ldev = nullptr;
+    const MgLevelDev<float> *Ldev32 = nullptr;
 };
 // points up to which a level runs inside the fused coarse-level launch
 // (one workgroup is latency-bound: a level of 2048 points takes two trips of
@@ -52,5 +61,8 @@ constexpr int64_t MG_FUSED_POINTS = 2048;
 // z = M v.  v's edges of wall pairs are averaged in place (restrict,
 // src/mod_solvers.f90:632-661), as the reference does to its pp and rr.
 void launch_mg_vcycle(hipStream_t s, const MgDev &m, double *v, double *z, const int *active);
+// the same cycle in FP32 on b32 = (float) v, z = (double) of the result; v's
+// edges are averaged in FP64 as well (no counterpart in the reference)
+void launch_mg_vcycle_f32(hipStream_t s, const MgDev &m, double *v, double *z, const int *active);
 
 }  // namespace ih

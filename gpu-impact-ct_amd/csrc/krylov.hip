@@ -651,7 +651,10 @@ static void launch_precond(hipStream_t s, const Geo &g, const Coef &c, const Box
     const int *act = k.fl + KF_ACTIVE;
     if (k.mg) {
         // multigridV(.TRUE.,1,v,z) (src/mod_solvers.f90:1688, 1751)
-        launch_mg_vcycle(s, *k.mg, v, z, act);
+        if (k.mg_fp32)
+            launch_mg_vcycle_f32(s, *k.mg, v, z, act);
+        else
+            launch_mg_vcycle(s, *k.mg, v, z, act);
         return;
     }
     if (k.sweeps == 1) {

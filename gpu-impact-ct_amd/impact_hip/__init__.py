@@ -49,7 +49,8 @@ class StepParams(C.Structure):
     _fields_ = [("Re", C.c_double), ("dtime", C.c_double), ("jor_sweeps", C.c_int), ("jor_omega", C.c_double),
                 ("bicg_rel_tol", C.c_double), ("bicg_max_it", C.c_int), ("level_pressure", C.c_int),
                 ("inflow_kind", C.c_int), ("inflow_amp", C.c_double), ("use_mask", C.c_int), ("jor_fp32", C.c_int),
-                ("precond_sweeps", C.c_int), ("precond_poisson", C.c_int)]
+                ("precond_sweeps", C.c_int), ("precond_poisson", C.c_int),
+                ("mg_fp32", C.c_int)]
 
 
 class Request(C.Structure):
@@ -132,6 +133,8 @@ def load(path: os.PathLike | None = None):
         "ih_mg_levels": (C.c_int, [vp, ip, ip]),
         "ih_mg_get_coeff": (C.c_int, [vp, C.c_int, C.c_char_p, dp, C.POINTER(C.c_int64)]),
         "ih_multigrid_v": (C.c_int, [vp, C.c_int, C.c_int]),
+        "ih_multigrid_v_f32": (C.c_int, [vp, C.c_int, C.c_int]),
+        "ih_bicgstab_mg_f32": (C.c_int, [vp, C.c_double, C.c_int, ip, ip, C.c_int, C.c_int, C.POINTER(BicgStats)]),
         "ih_fill_corners": (C.c_int, [vp, C.c_int]),
         "ih_level_pressure": (C.c_int, [vp, C.c_int]),
         "ih_set_tube_mask": (C.c_int, [vp, C.c_double, C.c_double, dp, dp]),
@@ -382,6 +385,16 @@ class Case:
     def multigrid_v(self, bb, phi):
         """phi = M bb (one V cycle); bb's edges of wall pairs are averaged in place."""
         check(self.lib.ih_multigrid_v(self.h, bb, phi), "multigrid_v")
+
+    def multigrid_v_f32(self, bb, phi):
+        """The FP32 cycle: phi = (double) M32 (float) bb; bb's edges averaged in FP64."""
+        check(self.lib.ih_multigrid_v_f32(self.h, bb, phi), "multigrid_v_f32")
+
+    def bicgstab_mg_f32(self, eps, nmax, ss, nn, bb=RES, phi=PRE) -> BicgStats:
+        st = BicgStats()
+        check(self.lib.ih_bicgstab_mg_f32(self.h, float(eps), int(nmax), _ints(ss), _ints(nn), bb, phi,
+                                          C.byref(st)), "bicgstab_mg_f32")
+        return st
 
     def fill_corners(self, f=PRE):
         check(self.lib.ih_fill_corners(self.h, f), "fill_corners")

@@ -183,6 +183,13 @@ int ih_mg_levels(ih_case *c, int *n_grids, int *NN);
 int ih_mg_get_coeff(ih_case *c, int level, const char *name, double *out, int64_t *n);
 /* phi = M bb, one V cycle multigridV(.TRUE., 1, bb, phi); bb's edges of wall pairs are averaged */
 int ih_multigrid_v(ih_case *c, ih_field bb, ih_field phi);
+/* Mixed precision (no counterpart in the reference): the same cycle in FP32 on
+ * (float) bb, phi = (double) of its result; bb's edges are averaged in FP64
+ * as the FP64 cycle does.  ih_bicgstab_mg_f32 is BiCGstab_orig with it as M;
+ * residuals, inner products and updates stay FP64. */
+int ih_multigrid_v_f32(ih_case *c, ih_field bb, ih_field phi);
+int ih_bicgstab_mg_f32(ih_case *c, double eps, int n_it_max, const int ss[3], const int nn[3], ih_field bb,
+                       ih_field phi, ih_bicgstab_stats *st);
 
 int ih_fill_corners(ih_case *c, ih_field f);
 int ih_level_pressure(ih_case *c, ih_field pre);
@@ -209,6 +216,7 @@ typedef struct {
     int jor_fp32;   /* 1: JOR sweeps in FP32 (residual and update kept FP64) */
     int precond_sweeps; /* > 0: BiCGstab right-preconditioned by this many FP32 JOR sweeps */
     int precond_poisson; /* with precond_sweeps 0: BiCGstab's preconditioner, 0 none, 1 multigrid V cycle */
+    int mg_fp32;         /* with precond_poisson 1: the cycle in FP32 (an FP64-cycle solve redoes a failure) */
 } ih_step_params;
 
 typedef struct {

@@ -129,6 +129,11 @@ void orc_mg_vcycle(orc_mg *m, double *bb, double *phi);
 void orc_mg_restrict_residual(orc_mg *m, int g, double *bb, double *x, double *coarse);
 void orc_mg_interpolate_add(orc_mg *m, int g, double *coarse, double *fine);
 void orc_mg_relax(orc_mg *m, int g, int init, int n_relax, const double *bb, double *rel);
+/* Mixed precision (no reference counterpart): the same cycle in binary32 on
+ * b32 = (float) bb, phi = (double) result; bb's edges are averaged in FP64 too */
+void orc_mg_vcycle_f32(orc_mg *m, double *bb, double *phi);
+void orc_bicgstab_mg_f32(const orc_case *c, orc_mg *mg, double eps, int n_it_max, const int ss[3], const int nn[3],
+                         const double *bb, double *phi, orc_bicgstab_stats *st);
 /* BiCGstab_orig with preconditioner 1 */
 void orc_bicgstab_mg(const orc_case *c, orc_mg *mg, double eps, int n_it_max, const int ss[3], const int nn[3],
                      const double *bb, double *phi, orc_bicgstab_stats *st);
@@ -147,6 +152,7 @@ typedef struct {
     int jor_fp32;          /* 1: the JOR sweeps as FP32 defect correction */
     int precond_sweeps;    /* > 0: BiCGstab preconditioned by this many FP32 JOR sweeps */
     int precond_poisson;   /* 1: BiCGstab preconditioned by the multigrid V cycle (precond_sweeps 0) */
+    int mg_fp32;           /* with precond_poisson 1: the cycle in FP32 (mixed precision) */
 } orc_step_params;
 
 typedef struct {

@@ -53,6 +53,9 @@ typedef struct {
     double *cI[3];  /* (1:2, 1:N) at (t-1) + 2(i-1); levels <= G-1 */
     double *xp[3], *xu[3]; /* level coordinates (-3:N+3) at [i+3] */
     double *A, *B, *C, *D; /* bb, phi, work (residual product / interpolation), relaxation copy */
+    /* the FP32 cycle: its tables (rounded once) and arrays; level 1's Af holds (float) bb */
+    float *cdgf[3], *cRf[3], *cIf[3];
+    float *Af, *Bf, *Cf, *Df;
 } mg_level;
 
 struct orc_mg {
@@ -272,6 +275,21 @@ orc_mg *orc_mg_create(const orc_case *c, int n_grids_limit, int n_down, int n_up
         }
         L->C = calloc((size_t)L->ntot, sizeof(double));
         L->D = calloc((size_t)L->ntot, sizeof(double));
+        for (int d = 0; d < 3; ++d) {
+            const int N = L->N[d];
+            L->cdgf[d] = calloc(3 * (size_t)N, sizeof(float));
+            L->cRf[d] = calloc(3 * (size_t)N, sizeof(float));
+            L->cIf[d] = calloc(2 * (size_t)N, sizeof(float));
+            for (int t = 0; t < 3 * N; ++t) {
+                L->cdgf[d][t] = (float)L->cdg[d][t];
+                L->cRf[d][t] = (float)L->cR[d][t];
+            }
+            for (int t = 0; t < 2 * N; ++t) L->cIf[d][t] = (float)L->cI[d][t];
+        }
+        L->Af = calloc((size_t)L->ntot, sizeof(float));
+        L->Bf = calloc((size_t)L->ntot, sizeof(float));
+        L->Cf = calloc((size_t)L->ntot, sizeof(float));
+        L->Df = calloc((size_t)L->ntot, sizeof(float));
     }
     return m;
 }
@@ -287,7 +305,14 @@ void orc_mg_destroy(orc_mg *m)
             free(L->cI[d]);
             free(L->xp[d]);
             free(L->xu[d]);
+            free(L->cdgf[d]);
+            free(L->cRf[d]);
+            free(L->cIf[d]);
         }
+        free(L->Af);
+        free(L->Bf);
+        free(L->Cf);
+        free(L->Df);
         free(L->A);
         free(L->B);
         free(L->C);
@@ -338,205 +363,38 @@ int64_t orc_mg_field_size(const orc_mg *m, int g)
     return (g >= 1 && g <= m->G) ? m->L[g].ntot : -1;
 }
 
-/* exchange_relax (src/mod_exchange.f90:1273-1480), single block: periodic
- * ghosts from the opposite side; lower (fb >= 0) and/or upper (fb <= 0) */
-static void lvl_exchange(const mg_level *L, double *phi, int lower, int upper)
-{
-    const int N1 = L->N[0], N2 = L->N[1], N3 = L->N[2];
-    if (L->per[0]) {
-        const int S = L->SB[0][0], Nn = L->SB[0][1];
-        for (int k = 1; k <= N3; ++k)
-            for (int j = 1; j <= N2; ++j) {
-                if (lower) F(L, phi, S - 1, j, k) = F(L, phi, Nn, j, k);
-                if (upper) F(L, phi, Nn + 1, j, k) = F(L, phi, S, j, k);
-            }
-    }
-    if (L->per[1]) {
-        const int S = L->SB[1][0], Nn = L->SB[1][1];
-        for (int k = 1; k <= N3; ++k)
-            for (int i = 1; i <= N1; ++i) {
-                if (lower) F(L, phi, i, S - 1, k) = F(L, phi, i, Nn, k);
-                if (upper) F(L, phi, i, Nn + 1, k) = F(L, phi, i, S, k);
-            }
-    }
-    if (L->per[2]) {
-        const int S = L->SB[2][0], Nn = L->SB[2][1];
-        for (int j = 1; j <= N2; ++j)
-            for (int i = 1; i <= N1; ++i) {
-                if (lower) F(L, phi, i, j, S - 1) = F(L, phi, i, j, Nn);
-                if (upper) F(L, phi, i, j, Nn + 1) = F(L, phi, i, j, S);
-            }
-    }
-}
-
 static int in_snf(const mg_level *L, int i, int j, int k)
 {
     return i >= L->SNF[0][0] && i <= L->SNF[0][1] && j >= L->SNF[1][0] && j <= L->SNF[1][1] &&
            k >= L->SNF[2][0] && k <= L->SNF[2][1];
 }
 
-/* relaxation_div_grad / relaxation_div_grad_inv, Jacobi branch */
-static void mg_relax(const orc_mg *m, const mg_level *L, int init, int n_relax, const double *bb, double *rel)
-{
-    const double *g1 = L->cdg[0], *g2 = L->cdg[1], *g3 = L->cdg[2];
-    const double omega = m->omega;
-    double *comp = L->D;
-    for (int r = 1; r <= n_relax; ++r) {
-        if (r == 1 && init) {
-            for (int k = 1; k <= L->N[2]; ++k)
-                for (int j = 1; j <= L->N[1]; ++j)
-                    for (int i = 1; i <= L->N[0]; ++i) {
-                        if (in_snf(L, i, j, k))
-                            F(L, rel, i, j, k) =
-                                omega * F(L, bb, i, j, k) / ((CDG3(g1, 0, i) + CDG3(g2, 0, j)) + CDG3(g3, 0, k));
-                        else
-                            F(L, rel, i, j, k) = 0.0;
-                    }
-        } else {
-            lvl_exchange(L, rel, 1, 1);
-            for (int k = L->SNF[2][0]; k <= L->SNF[2][1]; ++k)
-                for (int j = L->SNF[1][0]; j <= L->SNF[1][1]; ++j)
-                    for (int i = L->SNF[0][0]; i <= L->SNF[0][1]; ++i) {
-                        double t = F(L, bb, i, j, k);
-                        t = t - CDG3(g1, -1, i) * F(L, rel, i - 1, j, k);
-                        t = t - CDG3(g1, 1, i) * F(L, rel, i + 1, j, k);
-                        t = t - CDG3(g2, -1, j) * F(L, rel, i, j - 1, k);
-                        t = t - CDG3(g2, 1, j) * F(L, rel, i, j + 1, k);
-                        t = t - CDG3(g3, -1, k) * F(L, rel, i, j, k - 1);
-                        t = t - CDG3(g3, 1, k) * F(L, rel, i, j, k + 1);
-                        const double diag = (CDG3(g1, 0, i) + CDG3(g2, 0, j)) + CDG3(g3, 0, k);
-                        F(L, comp, i, j, k) = omega * t / diag + (1.0 - omega) * F(L, rel, i, j, k);
-                    }
-            for (int k = 1; k <= L->N[2]; ++k)
-                for (int j = 1; j <= L->N[1]; ++j)
-                    for (int i = 1; i <= L->N[0]; ++i)
-                        F(L, rel, i, j, k) = in_snf(L, i, j, k) ? F(L, comp, i, j, k) : 0.0;
-        }
-    }
-}
-
-/* product_div_grad_relax (src/mod_laplace.f90:200-300) */
-static void mg_lap(const mg_level *L, double *phi, double *lap)
-{
-    const double *g1 = L->cdg[0], *g2 = L->cdg[1], *g3 = L->cdg[2];
-    lvl_exchange(L, phi, 1, 1);
-    for (int k = 1; k <= L->N[2]; ++k)
-        for (int j = 1; j <= L->N[1]; ++j)
-            for (int i = 1; i <= L->N[0]; ++i) {
-                if (!in_snf(L, i, j, k)) {
-                    F(L, lap, i, j, k) = 0.0;
-                    continue;
-                }
-                double t = CDG3(g1, -1, i) * F(L, phi, i - 1, j, k);
-                t = t + CDG3(g1, 1, i) * F(L, phi, i + 1, j, k);
-                t = t + CDG3(g2, -1, j) * F(L, phi, i, j - 1, k);
-                t = t + CDG3(g2, 1, j) * F(L, phi, i, j + 1, k);
-                t = t + CDG3(g3, -1, k) * F(L, phi, i, j, k - 1);
-                t = t + CDG3(g3, 1, k) * F(L, phi, i, j, k + 1);
-                t = t + ((CDG3(g1, 0, i) + CDG3(g2, 0, j)) + CDG3(g3, 0, k)) * F(L, phi, i, j, k);
-                F(L, lap, i, j, k) = t;
-            }
-}
-
-/* Edge averaging on the edges of wall pairs: restrict (three = 0, two
- * neighbours / 2, src/mod_solvers.f90:632-661) and interpolate (three = 1,
- * three neighbours / 3, src/mod_solvers.f90:1028-1043).  Statements in the
- * reference's order: direction pairs (1,2), (1,3), (2,3); per pair the sides
- * (L,L), (L,U), (U,L), (U,U); each a line over the whole range 1..N of the
- * third direction. */
-static void mg_edges(const mg_level *L, double *f, int three)
-{
-    static const int pairs[3][3] = {{0, 1, 2}, {0, 2, 1}, {1, 2, 0}}; /* p, q, line t */
-    for (int pr = 0; pr < 3; ++pr) {
-        const int p = pairs[pr][0], q = pairs[pr][1], t = pairs[pr][2];
-        for (int sp = 0; sp < 2; ++sp)
-            for (int sq = 0; sq < 2; ++sq) {
-                if (!(L->bc[p][sp] > 0 && L->bc[q][sq] > 0)) continue;
-                const int ep = sp ? L->N[p] : 1, ip = sp ? L->N[p] - 1 : 2;
-                const int eq = sq ? L->N[q] : 1, iq = sq ? L->N[q] - 1 : 2;
-                for (int s = 1; s <= L->N[t]; ++s) {
-                    int e[3], a[3], b[3], cc[3];
-                    e[p] = ep; e[q] = eq; e[t] = s;
-                    a[p] = ip; a[q] = eq; a[t] = s;
-                    b[p] = ep; b[q] = iq; b[t] = s;
-                    cc[p] = ip; cc[q] = iq; cc[t] = s;
-                    if (three)
-                        F(L, f, e[0], e[1], e[2]) =
-                            (F(L, f, a[0], a[1], a[2]) + F(L, f, b[0], b[1], b[2]) + F(L, f, cc[0], cc[1], cc[2])) /
-                            3.0;
-                    else
-                        F(L, f, e[0], e[1], e[2]) = (F(L, f, a[0], a[1], a[2]) + F(L, f, b[0], b[1], b[2])) / 2.0;
-                }
-            }
-    }
-}
-
-/* restrict(.TRUE., g, coarse, fine1, fine2): coarse = R (fine1 - fine2) */
-static void mg_restrict(const mg_level *L, const mg_level *Lc, double *coarse, double *fine1, double *fine2)
-{
-    mg_edges(L, fine1, 0);
-    mg_edges(L, fine2, 0);
-    int nmax[3], dd[3];
-    for (int d = 0; d < 3; ++d) {
-        nmax[d] = Lc->N[d];
-        dd[d] = (L->N[d] - 1) / (nmax[d] - 1);
-        if (L->bc[d][1] == -1 || L->bc[d][1] == 0) nmax[d] = nmax[d] - 1; /* ls = -1 */
-    }
-    lvl_exchange(L, fine1, 1, 1);
-    lvl_exchange(L, fine2, 1, 1);
-    const double *r1 = Lc->cR[0], *r2 = Lc->cR[1], *r3 = Lc->cR[2];
-#define DF(i, j, k) (F(L, fine1, i, j, k) - F(L, fine2, i, j, k))
-    for (int kk = 1; kk <= nmax[2]; ++kk) {
-        const int k = dd[2] * (kk - 1) + 1;
-        for (int jj = 1; jj <= nmax[1]; ++jj) {
-            const int j = dd[1] * (jj - 1) + 1;
-            for (int ii = 1; ii <= nmax[0]; ++ii) {
-                const int i = dd[0] * (ii - 1) + 1;
-                double t = ((CDG3(r1, 0, ii) + CDG3(r2, 0, jj)) + CDG3(r3, 0, kk)) * DF(i, j, k);
-                t = t + CDG3(r1, -1, ii) * DF(i - 1, j, k);
-                t = t + CDG3(r1, 1, ii) * DF(i + 1, j, k);
-                t = t + CDG3(r2, -1, jj) * DF(i, j - 1, k);
-                t = t + CDG3(r2, 1, jj) * DF(i, j + 1, k);
-                t = t + CDG3(r3, -1, kk) * DF(i, j, k - 1);
-                t = t + CDG3(r3, 1, kk) * DF(i, j, k + 1);
-                F(Lc, coarse, ii, jj, kk) = t / 3.0;
-            }
-        }
-    }
-#undef DF
-}
-
-/* interpolate(.TRUE., g, coarse, fine, work): fine += P coarse */
-static void mg_interpolate(const mg_level *L, const mg_level *Lc, double *coarse, double *fine, double *work)
-{
-    mg_edges(Lc, coarse, 1);
-    lvl_exchange(Lc, coarse, 0, 1); /* ex = -1: upper ghosts only */
-    const int imax = L->N[0], jmax = L->N[1], kmax = L->N[2];
-    const int di = (imax - 1) / (Lc->N[0] - 1), dj = (jmax - 1) / (Lc->N[1] - 1), dk = (kmax - 1) / (Lc->N[2] - 1);
-    for (int kk = 1; kk <= Lc->N[2]; ++kk)
-        for (int jj = 1; jj <= Lc->N[1]; ++jj)
-            for (int ii = 1; ii <= Lc->N[0]; ++ii)
-                F(L, work, di * (ii - 1) + 1, dj * (jj - 1) + 1, dk * (kk - 1) + 1) = F(Lc, coarse, ii, jj, kk);
-    const double *c1 = L->cI[0], *c2 = L->cI[1], *c3 = L->cI[2];
-    if (dk != 1)
-        for (int k = 2; k <= kmax - 1; k += dk)
-            for (int j = 1; j <= jmax; j += dj)
-                for (int i = 1; i <= imax; i += di)
-                    F(L, work, i, j, k) = CI2(c3, 1, k) * F(L, work, i, j, k - 1) + CI2(c3, 2, k) * F(L, work, i, j, k + 1);
-    if (dj != 1)
-        for (int k = 1; k <= kmax; ++k)
-            for (int j = 2; j <= jmax - 1; j += dj)
-                for (int i = 1; i <= imax; i += di)
-                    F(L, work, i, j, k) = CI2(c2, 1, j) * F(L, work, i, j - 1, k) + CI2(c2, 2, j) * F(L, work, i, j + 1, k);
-    if (di != 1)
-        for (int k = 1; k <= kmax; ++k)
-            for (int j = 1; j <= jmax; ++j)
-                for (int i = 2; i <= imax - 1; i += di)
-                    F(L, work, i, j, k) = CI2(c1, 1, i) * F(L, work, i - 1, j, k) + CI2(c1, 2, i) * F(L, work, i + 1, j, k);
-    for (int k = 1; k <= kmax; ++k)
-        for (int j = 1; j <= jmax; ++j)
-            for (int i = 1; i <= imax; ++i) F(L, fine, i, j, k) = F(L, fine, i, j, k) + F(L, work, i, j, k);
-}
+#define REAL double
+#define FN(name) name##_d
+#define LCDG(L) ((L)->cdg)
+#define LCR(L) ((L)->cR)
+#define LCI(L) ((L)->cI)
+#define LD(L) ((L)->D)
+#include "orc_mg_cycle.h"
+#undef REAL
+#undef FN
+#undef LCDG
+#undef LCR
+#undef LCI
+#undef LD
+#define REAL float
+#define FN(name) name##_f
+#define LCDG(L) ((L)->cdgf)
+#define LCR(L) ((L)->cRf)
+#define LCI(L) ((L)->cIf)
+#define LD(L) ((L)->Df)
+#include "orc_mg_cycle.h"
+#undef REAL
+#undef FN
+#undef LCDG
+#undef LCR
+#undef LCI
+#undef LD
 
 /* multigridV(.TRUE., 1, bb, phi) (src/mod_solvers.f90:456-585) */
 void orc_mg_vcycle(orc_mg *m, double *bb, double *phi)
@@ -544,40 +402,78 @@ void orc_mg_vcycle(orc_mg *m, double *bb, double *phi)
     const int G = m->G;
     mg_level *L = m->L;
     if (G == 1) {
-        mg_relax(m, &L[1], 1, m->n_bottom, bb, phi);
-        mg_relax(m, &L[1], 0, m->n_bottom, bb, phi);
+        mg_relax_d(m, &L[1], 1, m->n_bottom, bb, phi);
+        mg_relax_d(m, &L[1], 0, m->n_bottom, bb, phi);
         return;
     }
     for (int g = 1; g <= G - 1; ++g) {
         double *b = g == 1 ? bb : L[g].A, *x = g == 1 ? phi : L[g].B;
-        mg_relax(m, &L[g], 1, m->n_down, b, x);
-        mg_lap(&L[g], x, L[g].C);
-        mg_restrict(&L[g], &L[g + 1], L[g + 1].A, b, L[g].C);
+        mg_relax_d(m, &L[g], 1, m->n_down, b, x);
+        mg_lap_d(&L[g], x, L[g].C);
+        mg_restrict_d(&L[g], &L[g + 1], L[g + 1].A, b, L[g].C);
     }
-    mg_relax(m, &L[G], 1, m->n_bottom, L[G].A, L[G].B);
-    mg_relax(m, &L[G], 0, m->n_bottom, L[G].A, L[G].B);
+    mg_relax_d(m, &L[G], 1, m->n_bottom, L[G].A, L[G].B);
+    mg_relax_d(m, &L[G], 0, m->n_bottom, L[G].A, L[G].B);
     for (int g = G - 1; g >= 1; --g) {
         double *b = g == 1 ? bb : L[g].A, *x = g == 1 ? phi : L[g].B;
-        mg_interpolate(&L[g], &L[g + 1], L[g + 1].B, x, L[g].C);
-        mg_relax(m, &L[g], 0, m->n_up, b, x);
+        mg_interpolate_d(&L[g], &L[g + 1], L[g + 1].B, x, L[g].C);
+        mg_relax_d(m, &L[g], 0, m->n_up, b, x);
     }
+}
+
+/* The FP32 cycle of the mixed-precision solve (no counterpart in the
+ * reference): the same steps in binary32 on float copies of the tables, from
+ * b32 = (float) bb over the pressure range; phi = (double) of the level-1
+ * result.  Where the FP64 cycle averages bb's edges (restrict on level 1) the
+ * double bb is averaged as well, so BiCGstab sees the same change to pp and rr. */
+void orc_mg_vcycle_f32(orc_mg *m, double *bb, double *phi)
+{
+    const int G = m->G;
+    mg_level *L = m->L;
+    mg_level *L1 = &L[1];
+    float *b1 = L1->Af, *x1 = L1->Bf;
+    memset(b1, 0, sizeof(float) * (size_t)L1->ntot);
+    for (int k = L1->SNF[2][0]; k <= L1->SNF[2][1]; ++k)
+        for (int j = L1->SNF[1][0]; j <= L1->SNF[1][1]; ++j)
+            for (int i = L1->SNF[0][0]; i <= L1->SNF[0][1]; ++i) F(L1, b1, i, j, k) = (float)F(L1, bb, i, j, k);
+    if (G == 1) {
+        mg_relax_f(m, L1, 1, m->n_bottom, b1, x1);
+        mg_relax_f(m, L1, 0, m->n_bottom, b1, x1);
+    } else {
+        for (int g = 1; g <= G - 1; ++g) {
+            float *b = L[g].Af, *x = L[g].Bf;
+            mg_relax_f(m, &L[g], 1, m->n_down, b, x);
+            mg_lap_f(&L[g], x, L[g].Cf);
+            if (g == 1) mg_edges_d(L1, bb, 0);
+            mg_restrict_f(&L[g], &L[g + 1], L[g + 1].Af, b, L[g].Cf);
+        }
+        mg_relax_f(m, &L[G], 1, m->n_bottom, L[G].Af, L[G].Bf);
+        mg_relax_f(m, &L[G], 0, m->n_bottom, L[G].Af, L[G].Bf);
+        for (int g = G - 1; g >= 1; --g) {
+            mg_interpolate_f(&L[g], &L[g + 1], L[g + 1].Bf, L[g].Bf, L[g].Cf);
+            mg_relax_f(m, &L[g], 0, m->n_up, L[g].Af, L[g].Bf);
+        }
+    }
+    for (int k = L1->SNF[2][0]; k <= L1->SNF[2][1]; ++k)
+        for (int j = L1->SNF[1][0]; j <= L1->SNF[1][1]; ++j)
+            for (int i = L1->SNF[0][0]; i <= L1->SNF[0][1]; ++i) F(L1, phi, i, j, k) = (double)F(L1, x1, i, j, k);
 }
 
 /* test hooks: the single operations of the cycle on level g */
 void orc_mg_restrict_residual(orc_mg *m, int g, double *bb, double *x, double *coarse)
 {
-    mg_lap(&m->L[g], x, m->L[g].C);
-    mg_restrict(&m->L[g], &m->L[g + 1], coarse, bb, m->L[g].C);
+    mg_lap_d(&m->L[g], x, m->L[g].C);
+    mg_restrict_d(&m->L[g], &m->L[g + 1], coarse, bb, m->L[g].C);
 }
 
 void orc_mg_interpolate_add(orc_mg *m, int g, double *coarse, double *fine)
 {
-    mg_interpolate(&m->L[g], &m->L[g + 1], coarse, fine, m->L[g].C);
+    mg_interpolate_d(&m->L[g], &m->L[g + 1], coarse, fine, m->L[g].C);
 }
 
 void orc_mg_relax(orc_mg *m, int g, int init, int n_relax, const double *bb, double *rel)
 {
-    mg_relax(m, &m->L[g], init, n_relax, bb, rel);
+    mg_relax_d(m, &m->L[g], init, n_relax, bb, rel);
 }
 
 int orc_mg_grids(const orc_mg *m) { return m->G; }

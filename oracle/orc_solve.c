@@ -178,7 +178,7 @@ static double absmax(const orc_case *c, const int ss[3], const int nn[3], const 
 /* ------------------------------------------------------------------------ */
 static void bicgstab_run(const orc_case *c, double eps, int n_it_max, const int ss[3], const int nn[3],
                          const double *bb, double *phi, double *work, int sweeps, double pre_omega,
-                         orc_mg *mg, orc_bicgstab_stats *st)
+                         orc_mg *mg, int mg_fp32, orc_bicgstab_stats *st)
 {
     const int64_t n = c->ntot;
     double *own = NULL;
@@ -264,7 +264,10 @@ static void bicgstab_run(const orc_case *c, double eps, int n_it_max, const int 
         }
         if (mg) {
             /* multigridV(.TRUE.,1,pp,z1) (src/mod_solvers.f90:1688) */
-            orc_mg_vcycle(mg, pk, z1);
+            if (mg_fp32)
+                orc_mg_vcycle_f32(mg, pk, z1);
+            else
+                orc_mg_vcycle(mg, pk, z1);
             orc_product_div_grad(c, z1, Ap, dig);
         } else if (sweeps > 0) {
             precond(c, sweeps, omf, ss, nn, pk, z1, fb, fe, ft);
@@ -283,7 +286,10 @@ static void bicgstab_run(const orc_case *c, double eps, int n_it_max, const int 
             for (int j = ss[1]; j <= nn[1]; ++j)
                 for (int i = ss[0]; i <= nn[0]; ++i) A(rr, i, j, k) = A(rr, i, j, k) - alpha * A(Ap, i, j, k);
         if (mg) {
-            orc_mg_vcycle(mg, rr, z2); /* src/mod_solvers.f90:1751 */
+            if (mg_fp32)
+                orc_mg_vcycle_f32(mg, rr, z2);
+            else
+                orc_mg_vcycle(mg, rr, z2); /* src/mod_solvers.f90:1751 */
             orc_product_div_grad(c, z2, Ar, dig);
         } else if (sweeps > 0) {
             precond(c, sweeps, omf, ss, nn, rr, z2, fb, fe, ft);
@@ -327,13 +333,13 @@ static void bicgstab_run(const orc_case *c, double eps, int n_it_max, const int 
 void orc_bicgstab(const orc_case *c, double eps, int n_it_max, const int ss[3], const int nn[3], const double *bb,
                   double *phi, double *work, orc_bicgstab_stats *st)
 {
-    bicgstab_run(c, eps, n_it_max, ss, nn, bb, phi, work, 0, 0.0, NULL, st);
+    bicgstab_run(c, eps, n_it_max, ss, nn, bb, phi, work, 0, 0.0, NULL, 0, st);
 }
 
 void orc_bicgstab_mixed(const orc_case *c, double eps, int n_it_max, const int ss[3], const int nn[3],
                         const double *bb, double *phi, int sweeps, double omega, orc_bicgstab_stats *st)
 {
-    bicgstab_run(c, eps, n_it_max, ss, nn, bb, phi, NULL, sweeps, omega, NULL, st);
+    bicgstab_run(c, eps, n_it_max, ss, nn, bb, phi, NULL, sweeps, omega, NULL, 0, st);
 }
 
 /* BiCGstab_orig with preconditioner 1 (multigridV, src/mod_solvers.f90:1684-1693,
@@ -342,7 +348,14 @@ void orc_bicgstab_mixed(const orc_case *c, double eps, int n_it_max, const int s
 void orc_bicgstab_mg(const orc_case *c, orc_mg *mg, double eps, int n_it_max, const int ss[3], const int nn[3],
                      const double *bb, double *phi, orc_bicgstab_stats *st)
 {
-    bicgstab_run(c, eps, n_it_max, ss, nn, bb, phi, NULL, 0, 0.0, mg, st);
+    bicgstab_run(c, eps, n_it_max, ss, nn, bb, phi, NULL, 0, 0.0, mg, 0, st);
+}
+
+/* the same with the FP32 cycle (orc_mg_vcycle_f32) as M */
+void orc_bicgstab_mg_f32(const orc_case *c, orc_mg *mg, double eps, int n_it_max, const int ss[3], const int nn[3],
+                         const double *bb, double *phi, orc_bicgstab_stats *st)
+{
+    bicgstab_run(c, eps, n_it_max, ss, nn, bb, phi, NULL, 0, 0.0, mg, 1, st);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -571,15 +584,26 @@ void orc_timestep(orc_case *c, const orc_step_params *p, int step_index, double 
             /* the device keeps the initial guess and restarts plain FP64 BiCGstab
              * from it when the preconditioned iteration does not converge */
             memcpy(comp, pre, sizeof(double) * (size_t)n);
-            bicgstab_run(c, eps, p->bicg_max_it, ss, nn, res, pre, bw, p->precond_sweeps, p->jor_omega, NULL, &bst);
+            bicgstab_run(c, eps, p->bicg_max_it, ss, nn, res, pre, bw, p->precond_sweeps, p->jor_omega, NULL, 0, &bst);
             if (bst.exit_reason != 0) {
                 const int mixed_its = bst.iterations;
                 memcpy(pre, comp, sizeof(double) * (size_t)n);
-                bicgstab_run(c, eps, p->bicg_max_it, ss, nn, res, pre, bw, 0, 0.0, NULL, &bst);
+                bicgstab_run(c, eps, p->bicg_max_it, ss, nn, res, pre, bw, 0, 0.0, NULL, 0, &bst);
+                bst.iterations += mixed_its;
+            }
+        } else if (mg && p->mg_fp32) {
+            /* the FP32 cycle: as with the FP32 sweeps, a solve that does not converge
+             * is redone in FP64 (here with the FP64 cycle) from the saved guess */
+            memcpy(comp, pre, sizeof(double) * (size_t)n);
+            bicgstab_run(c, eps, p->bicg_max_it, ss, nn, res, pre, bw, 0, 0.0, mg, 1, &bst);
+            if (bst.exit_reason != 0) {
+                const int mixed_its = bst.iterations;
+                memcpy(pre, comp, sizeof(double) * (size_t)n);
+                bicgstab_run(c, eps, p->bicg_max_it, ss, nn, res, pre, bw, 0, 0.0, mg, 0, &bst);
                 bst.iterations += mixed_its;
             }
         } else {
-            bicgstab_run(c, eps, p->bicg_max_it, ss, nn, res, pre, bw, 0, 0.0, mg, &bst);
+            bicgstab_run(c, eps, p->bicg_max_it, ss, nn, res, pre, bw, 0, 0.0, mg, 0, &bst);
         }
         if (st) {
             st->bicg_iterations[s] = bst.iterations;

@@ -100,6 +100,8 @@ def lib():
         L.orc_mg_interpolate_add.argtypes = [vp, C.c_int, dp, dp]
         L.orc_mg_relax.argtypes = [vp, C.c_int, C.c_int, C.c_int, dp, dp]
         L.orc_bicgstab_mg.argtypes = [vp, vp, C.c_double, C.c_int, ip, ip, dp, dp, C.POINTER(BicgStats)]
+        L.orc_bicgstab_mg_f32.argtypes = [vp, vp, C.c_double, C.c_int, ip, ip, dp, dp, C.POINTER(BicgStats)]
+        L.orc_mg_vcycle_f32.argtypes = [vp, dp, dp]
         _lib = L
     return _lib
 
@@ -114,7 +116,7 @@ class StepParams(C.Structure):
                 ("jor_omega", C.c_double), ("bicg_rel_tol", C.c_double), ("bicg_max_it", C.c_int),
                 ("level_pressure", C.c_int), ("inflow_kind", C.c_int), ("inflow_amp", C.c_double),
                 ("mask3", C.POINTER(C.c_uint8)), ("jor_fp32", C.c_int), ("precond_sweeps", C.c_int),
-                ("precond_poisson", C.c_int)]
+                ("precond_poisson", C.c_int), ("mg_fp32", C.c_int)]
 
 
 class StepStats(C.Structure):
@@ -249,6 +251,11 @@ class OracleCase:
         self.L.orc_bicgstab_mg(self.h, mg.h, eps, nmax, _ip(ss), _ip(nn), _dp(bb), _dp(phi), C.byref(st))
         return st
 
+    def bicgstab_mg_f32(self, mg, eps, nmax, ss, nn, bb, phi):
+        st = BicgStats()
+        self.L.orc_bicgstab_mg_f32(self.h, mg.h, eps, nmax, _ip(ss), _ip(nn), _dp(bb), _dp(phi), C.byref(st))
+        return st
+
     def jor_mixed(self, n, omega, ss, nn, rel, bb):
         self.L.orc_jor_mixed(self.h, n, omega, _ip(ss), _ip(nn), _dp(rel), _dp(bb))
 
@@ -310,6 +317,9 @@ class OracleMG:
 
     def vcycle(self, bb, phi):
         self.L.orc_mg_vcycle(self.h, _dp(bb), _dp(phi))
+
+    def vcycle_f32(self, bb, phi):
+        self.L.orc_mg_vcycle_f32(self.h, _dp(bb), _dp(phi))
 
     def restrict_residual(self, g, bb, x, coarse):
         self.L.orc_mg_restrict_residual(self.h, g, _dp(bb), _dp(x), _dp(coarse))

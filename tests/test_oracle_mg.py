@@ -269,3 +269,37 @@ def test_bicgstab_mg_converges_in_fewer_iterations(N, bc, st):
         assert np.abs((bb - r)[s]).max() < 1.01 * eps
         its[name] = st_.iterations
     assert its["mg"] < its["plain"]
+
+
+@pytest.mark.parametrize("bc,st", [(WALLS, (0.0, 1.5, 1.5)), (PER_XZ, (0.0, 1.5, 0.0))])
+def test_fp32_cycle_tracks_the_fp64_cycle(bc, st):
+    """The mixed-precision cycle is the FP64 cycle in binary32: its result lies
+    within single-precision rounding of the FP64 one (a wrong table or step
+    would not), and it still preconditions BiCGstab: fewer iterations than
+    without a preconditioner."""
+    c = O.OracleCase((33, 17, 17), bc, (2.0, 1.0, 1.0), st)
+    m = O.OracleMG(c)
+    s = prange(c)
+    b = c.zeros()
+    b[s] = rng(7).standard_normal(b[s].shape)
+    b64, b32 = b.copy(), b.copy()
+    z64, z32 = c.zeros(), c.zeros()
+    m.vcycle(b64, z64)
+    m.vcycle_f32(b32, z32)
+    assert np.array_equal(b64[s], b32[s])  # the same edge averaging of bb, in FP64
+    d = np.linalg.norm(z32[s] - z64[s]) / np.linalg.norm(z64[s])
+    assert 0 < d < 1e-5
+    xt = c.zeros()
+    xt[s] = rng(8).standard_normal(xt[s].shape)
+    bb = c.zeros()
+    c.product_div_grad(xt.copy(), bb, c.zeros())
+    b2 = c.zeros()
+    b2[s] = bb[s]
+    eps = 1e-8 * np.abs(b2).max()
+    p64, p32 = c.zeros(), c.zeros()
+    s64 = c.bicgstab_mg(m, eps, 3000, c.Sp, c.Np, b2.copy(), p64)
+    s32 = c.bicgstab_mg_f32(m, eps, 3000, c.Sp, c.Np, b2.copy(), p32)
+    assert s64.exit_reason == 0 and s32.exit_reason == 0
+    pp = c.zeros()
+    s0 = c.bicgstab(eps, 3000, c.Sp, c.Np, b2.copy(), pp)
+    assert s64.iterations < s32.iterations < s0.iterations or s32.iterations <= s64.iterations

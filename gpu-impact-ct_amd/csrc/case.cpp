@@ -438,6 +438,7 @@ void ih_case_destroy(ih_case *c) {
     for (int q = 0; q < 3; ++q)
         if (c->f32[q]) (void)hipFree(c->f32[q]);
     if (c->mask) (void)hipFree(c->mask);
+    if (c->ctlab) (void)hipFree(c->ctlab);
     if (c->dcoef) (void)hipFree(c->dcoef);
     if (c->dcoef32) (void)hipFree(c->dcoef32);
     if (c->part) (void)hipFree(c->part);
@@ -1360,6 +1361,152 @@ int ih_get_mask(ih_case *c, uint8_t *host3) {
     return IH_OK;
 }
 
+// ---------------------------------------------------------------------------
+// CT geometry: read_vtk_3D (src/usr_func.f90:187-555) and apply_CT_forcing's
+// CT-based branch (src/usr_func.f90:909-945)
+// ---------------------------------------------------------------------------
+int ih_ct_load_vtk(ih_case *c, const char *path, int x_offset, int y_offset, int z_offset, const double *wave_t_ms,
+                   const double *wave_q, int npts, ih_ct_info *info) {
+    if (!c || !path || !wave_t_ms || !wave_q) return fail(IH_EINVAL, "null argument");
+    if (npts < 2) return fail(IH_EINVAL, "the waveform needs at least two points");
+    if (c->hg.xp[0].empty()) return fail(IH_ESTATE, "coordinates not generated (ih_case_generate_coeffs)");
+    HIPCHK(hipSetDevice(c->device));
+    const Geo &g = c->g;
+    const int M1 = g.N[0], M2 = g.N[1], M3 = g.N[2];
+    int dims[3];
+    std::vector<double> img;
+    std::string err;
+    if (!vtk_read_ascii(path, dims, img, err)) return fail(IH_EINVAL, "%s", err.c_str());
+    // the image's first axis is the grid's y, its second the grid's x
+    if (x_offset < 0 || y_offset < 0 || dims[1] + y_offset > M1 || dims[0] + x_offset > M2 ||
+        dims[2] + z_offset > M3 || z_offset < 10 || z_offset + dims[2] + 10 > M3)
+        return fail(IH_EINVAL, "the image with its offsets and the 10-plane inlet and outlet buffers must fit the grid");
+    double ref[3];
+    std::vector<double> tconv, qconv;
+    waveform_convert(wave_t_ms, wave_q, npts, M1, tconv, qconv, ref);
+    std::vector<uint8_t> lab((size_t)g.fe, 0);
+    auto L = [&](int i, int j, int k) -> uint8_t & { return lab[(size_t)gidx(g, i, j, k)]; };
+    for (int k = 1; k <= M3; ++k)
+        for (int j = 1; j <= M2; ++j)
+            for (int i = 1; i <= M1; ++i) L(i, j, k) = 2;
+    for (int i = 1; i <= dims[0]; ++i)
+        for (int j = 1; j <= dims[1]; ++j)
+            for (int k = 1; k <= dims[2]; ++k) {
+                const double v = img[(size_t)(k - 1) * dims[1] * dims[0] + (size_t)(j - 1) * dims[0] + (i - 1)];
+                if (!(v >= 0.0 && v <= 255.0 && v == std::floor(v)))
+                    return fail(IH_EINVAL, "image value %g is not an integer label 0..255", v);
+                L(j + y_offset, i + x_offset, k + z_offset) = (uint8_t)v;
+            }
+    // inlet (left half) and outlet (right half, from floor(M1/2)) on the image's
+    // first plane, the second outlet on its last plane: label band [0.5, 2)
+    const std::vector<double> &xp = c->hg.xp[0], &yp = c->hg.xp[1];
+    const int half = M1 / 2, ks = z_offset + 1, ks2 = z_offset + dims[2];
+    auto fluid = [&](int i, int j, int k) { return L(i, j, k) == 1; };
+    std::vector<int> in, out, out2;
+    double xi = 0.0, yi = 0.0, xo = 0.0, yo = 0.0;
+    for (int j = 1; j <= M2; ++j)
+        for (int i = 1; i <= half; ++i)
+            if (fluid(i, j, ks)) {
+                xi = xi + xp[i + 3];
+                yi = yi + yp[j + 3];
+                in.push_back(i);
+                in.push_back(j);
+            }
+    for (int j = 1; j <= M2; ++j)
+        for (int i = half; i <= M1; ++i)
+            if (fluid(i, j, ks)) {
+                xo = xo + xp[i + 3];
+                yo = yo + yp[j + 3];
+                out.push_back(i);
+                out.push_back(j);
+            }
+    for (int j = 1; j <= M2; ++j)
+        for (int i = 1; i <= M1; ++i)
+            if (fluid(i, j, ks2)) {
+                out2.push_back(i);
+                out2.push_back(j);
+            }
+    const int ni = (int)in.size() / 2, no = (int)out.size() / 2, no2 = (int)out2.size() / 2;
+    if (ni == 0 || no == 0) return fail(IH_EINVAL, "no fluid cells on the inlet or outlet side of the first plane");
+    xi = xi / ni;
+    yi = yi / ni;
+    xo = xo / no;
+    yo = yo / no;
+    // the incoming jet: cells within R_inlet of the inlet centre, ten planes below
+    const double R_inlet = std::sqrt(0.0004 / (4.0 * std::atan(1.0) * ref[0])) / ref[1];
+    int jet = 0;
+    for (int q = 0; q < ni; ++q)
+        for (int k = -10; k <= -1; ++k) {
+            const int i = in[2 * q], j = in[2 * q + 1];
+            L(i, j, ks + k) = 7;
+            const double dx = xp[i + 3] - xi, dy = yp[j + 3] - yi;
+            if (std::sqrt(dx * dx + dy * dy) <= R_inlet) {
+                L(i, j, ks + k) = 5;
+                if (k == -1) ++jet;
+            }
+        }
+    for (int q = 0; q < no; ++q)
+        for (int k = -10; k <= -1; ++k) L(out[2 * q], out[2 * q + 1], ks + k) = 9;
+    for (int q = 0; q < no2; ++q)
+        for (int k = 1; k <= 10; ++k) L(out2[2 * q], out2[2 * q + 1], ks2 + k) = 11;
+    if (!c->ctlab && hipMalloc(&c->ctlab, (size_t)g.fe) != hipSuccess) {
+        c->ctlab = nullptr;
+        return fail(IH_ENOMEM, "out of device memory for the CT labels");
+    }
+    HIPCHK(hipMemcpy(c->ctlab, lab.data(), lab.size(), hipMemcpyHostToDevice));
+    c->ct_t = tconv;
+    c->ct_q = qconv;
+    if (info) {
+        info->inlet_cells = ni;
+        info->outlet_cells = no;
+        info->outlet2_cells = no2;
+        info->inlet_jet_voxels = jet;
+        info->k_start = ks;
+        info->k_start2 = ks2;
+        info->inlet_centre[0] = xi;
+        info->inlet_centre[1] = yi;
+        info->outlet_centre[0] = xo;
+        info->outlet_centre[1] = yo;
+        info->reference_velocity = ref[0];
+        info->reference_length = ref[1];
+        info->reference_time = ref[2];
+    }
+    return IH_OK;
+}
+
+int ih_ct_get_labels(ih_case *c, uint8_t *host) {
+    if (!c || !host) return fail(IH_EINVAL, "null argument");
+    if (!c->ctlab) return fail(IH_ESTATE, "no CT labels loaded");
+    const Geo &g = c->g;
+    std::vector<uint8_t> m((size_t)g.fe);
+    HIPCHK(hipMemcpy(m.data(), c->ctlab, m.size(), hipMemcpyDeviceToHost));
+    const int64_t nx = g.N[0] + 7, ny = g.N[1] + 7, nz = g.N[2] + 7;
+    for (int64_t k = 0; k < nz; ++k)
+        for (int64_t j = 0; j < ny; ++j)
+            for (int64_t i = 0; i < nx; ++i)
+                host[i + nx * (j + ny * k)] = m[(size_t)gidx(g, (int)i - 3, (int)j - 3, (int)k - 3)];
+    return IH_OK;
+}
+
+int ih_ct_inflow(ih_case *c, double time, double *q, int *found) {
+    if (!c || !q || !found) return fail(IH_EINVAL, "null argument");
+    if (c->ct_t.size() < 2) return fail(IH_ESTATE, "no waveform loaded");
+    bool f = false;
+    *q = waveform_inflow(c->ct_t, c->ct_q, time, f);
+    *found = f ? 1 : 0;
+    return IH_OK;
+}
+
+int ih_ct_forcing(ih_case *c, double time) {
+    if (!c) return fail(IH_EINVAL, "null case");
+    if (!c->ctlab || c->ct_t.size() < 2) return fail(IH_ESTATE, "no CT labels loaded");
+    bool found = false;
+    const double q = waveform_inflow(c->ct_t, c->ct_q, time, found);
+    launch_ct_forcing(c->stream, c->g, c->ctlab, q, found ? 1 : 0, F(c, IH_VEL1), F(c, IH_VEL2), F(c, IH_VEL3));
+    LAUNCHCHK();
+    return IH_OK;
+}
+
 int ih_apply_mask(ih_case *c) {
     if (!c) return fail(IH_EINVAL, "null case");
     launch_apply_mask(c->stream, c->g, c->mask, F(c, IH_VEL1), F(c, IH_VEL2), F(c, IH_VEL3));
@@ -1518,6 +1665,9 @@ int ih_timestep(ih_case *c, const ih_step_params *p, double time, ih_step_stats 
         if (!c->has_mask) return fail(IH_ESTATE, "no mask set");
         if ((rc = ih_apply_mask(c))) return rc;
     }
+    // apply_CT_forcing follows the time update (src/mod_timeint.f90:405-416)
+    if (p->use_ct)
+        if ((rc = ih_ct_forcing(c, time + p->dtime))) return rc;
     if (p->level_pressure)
         if ((rc = ih_level_pressure(c, IH_PRE))) return rc;
     if (st) {

@@ -41,6 +41,12 @@ constexpr int MG_MAX_LEVELS = 15;
 // levels 1..G of the grid (index 0 holds level 1), at most n_grids_limit
 void host_mg_levels(const HostGrid &hg, int n_grids_limit, std::vector<HostMgLevel> &out);
 
+// CT geometry (csrc/ct.cpp)
+bool vtk_read_ascii(const char *path, int dims[3], std::vector<double> &values, std::string &err);
+void waveform_convert(const double *t_ms, const double *q, int npts, int M1, std::vector<double> &t_out,
+                      std::vector<double> &q_out, double ref[3]);
+double waveform_inflow(const std::vector<double> &t, const std::vector<double> &q, double time, bool &found);
+
 struct KernelTiming {
     double total_ms = 0.0;
     int64_t launches = 0;
@@ -82,6 +88,9 @@ struct ih_case {
     std::map<std::string, ih::KernelTiming> ktimes;
     std::map<std::string, std::vector<ih::PendingTiming>> kpending;
     std::map<std::vector<int>, ih::JorLaunch> jor_shape;  // (fp32, box, level) -> launch shape of the JOR sweep
+    // CT label volume (uint8 labels on the field layout) and the converted waveform
+    uint8_t *ctlab = nullptr;
+    std::vector<double> ct_t, ct_q;
     // multigrid V cycle (preconditioner 1), built on first use
     ih::MgDev *mg = nullptr;
     double *mgmem = nullptr;          // level arrays and coefficient tables

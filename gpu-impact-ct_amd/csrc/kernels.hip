@@ -1306,6 +1306,39 @@ void launch_sub_box(hipStream_t s, const Geo &g, const Box &b, double *f, const 
     hipLaunchKernelGGL(k_sub_box, grid_for(r), dim3(BX, BY, 1), 0, s, g, r, f, val);
 }
 
+// apply_CT_forcing, CT-based branch (src/usr_func.f90:918-945): over the
+// component-1 velocity range, all three components; labels 0 (solid, < 0.5)
+// and 7 set the velocity to 0, label 5 (inlet jet) to (0, 0, q) when the
+// waveform holds the time (found).  Positions outside 1..N read label 2.
+__global__ void k_ct_forcing(Geo g, Range3 r, const uint8_t *__restrict__ lab, double q, int found,
+                             double *__restrict__ u, double *__restrict__ v, double *__restrict__ w) {
+    IH_POINT(r);
+    const bool inside = i >= 1 && i <= g.N[0] && j >= 1 && j <= g.N[1] && k >= 1 && k <= g.N[2];
+    const int64_t p = gidx(g, i, j, k);
+    const int L = inside ? lab[p] : 2;
+    if (L < 1 || L == 7) {
+        u[p] = 0.0;
+        v[p] = 0.0;
+        w[p] = 0.0;
+    } else if (L == 5 && found) {
+        w[p] = q;
+        v[p] = 0.0;
+        u[p] = 0.0;
+    }
+}
+
+void launch_ct_forcing(hipStream_t s, const Geo &g, const uint8_t *lab, double q, int found, double *u, double *v,
+                       double *w) {
+    Box b;
+    for (int d = 0; d < 3; ++d) {
+        b.lo[d] = g.SB[0][d];
+        b.hi[d] = g.NB[0][d];
+    }
+    const Range3 r = R(b);
+    if (empty(r)) return;
+    hipLaunchKernelGGL(k_ct_forcing, grid_for(r), dim3(BX, BY, 1), 0, s, g, r, lab, q, found, u, v, w);
+}
+
 void launch_apply_mask(hipStream_t s, const Geo &g, const uint8_t *mask, double *u, double *v, double *w) {
     double *f[3] = {u, v, w};
     const int64_t ntot = g.fe;

@@ -45,12 +45,20 @@ class BicgStats(C.Structure):
                 ("exit_reason", C.c_int)]
 
 
+class CtInfo(C.Structure):
+    _fields_ = [("inlet_cells", C.c_int), ("outlet_cells", C.c_int), ("outlet2_cells", C.c_int),
+                ("inlet_jet_voxels", C.c_int), ("k_start", C.c_int), ("k_start2", C.c_int),
+                ("inlet_centre", C.c_double * 2), ("outlet_centre", C.c_double * 2),
+                ("reference_velocity", C.c_double), ("reference_length", C.c_double),
+                ("reference_time", C.c_double)]
+
+
 class StepParams(C.Structure):
     _fields_ = [("Re", C.c_double), ("dtime", C.c_double), ("jor_sweeps", C.c_int), ("jor_omega", C.c_double),
                 ("bicg_rel_tol", C.c_double), ("bicg_max_it", C.c_int), ("level_pressure", C.c_int),
                 ("inflow_kind", C.c_int), ("inflow_amp", C.c_double), ("use_mask", C.c_int), ("jor_fp32", C.c_int),
                 ("precond_sweeps", C.c_int), ("precond_poisson", C.c_int),
-                ("mg_fp32", C.c_int)]
+                ("mg_fp32", C.c_int), ("use_ct", C.c_int)]
 
 
 class Request(C.Structure):
@@ -135,6 +143,11 @@ def load(path: os.PathLike | None = None):
         "ih_multigrid_v": (C.c_int, [vp, C.c_int, C.c_int]),
         "ih_multigrid_v_f32": (C.c_int, [vp, C.c_int, C.c_int]),
         "ih_bicgstab_mg_f32": (C.c_int, [vp, C.c_double, C.c_int, ip, ip, C.c_int, C.c_int, C.POINTER(BicgStats)]),
+        "ih_ct_load_vtk": (C.c_int, [vp, C.c_char_p, C.c_int, C.c_int, C.c_int, dp, dp, C.c_int,
+                                     C.POINTER(CtInfo)]),
+        "ih_ct_get_labels": (C.c_int, [vp, C.POINTER(C.c_uint8)]),
+        "ih_ct_inflow": (C.c_int, [vp, C.c_double, dp, ip]),
+        "ih_ct_forcing": (C.c_int, [vp, C.c_double]),
         "ih_fill_corners": (C.c_int, [vp, C.c_int]),
         "ih_level_pressure": (C.c_int, [vp, C.c_int]),
         "ih_set_tube_mask": (C.c_int, [vp, C.c_double, C.c_double, dp, dp]),
@@ -395,6 +408,28 @@ class Case:
         check(self.lib.ih_bicgstab_mg_f32(self.h, float(eps), int(nmax), _ints(ss), _ints(nn), bb, phi,
                                           C.byref(st)), "bicgstab_mg_f32")
         return st
+
+    def ct_load_vtk(self, path, offsets, wave_t_ms, wave_q) -> CtInfo:
+        """read_vtk_3D: the CT label image at offsets (x, y, z) and the echo waveform."""
+        t = np.ascontiguousarray(wave_t_ms, dtype=np.float64)
+        q = np.ascontiguousarray(wave_q, dtype=np.float64)
+        info = CtInfo()
+        check(self.lib.ih_ct_load_vtk(self.h, str(path).encode(), int(offsets[0]), int(offsets[1]), int(offsets[2]),
+                                      _dp(t), _dp(q), len(t), C.byref(info)), "ct_load_vtk")
+        return info
+
+    def ct_labels(self) -> np.ndarray:
+        out = np.zeros(self.shape, dtype=np.uint8)
+        check(self.lib.ih_ct_get_labels(self.h, out.ctypes.data_as(C.POINTER(C.c_uint8))), "ct_get_labels")
+        return out
+
+    def ct_inflow(self, time):
+        q, found = C.c_double(0.0), C.c_int(0)
+        check(self.lib.ih_ct_inflow(self.h, float(time), C.byref(q), C.byref(found)), "ct_inflow")
+        return q.value, bool(found.value)
+
+    def ct_forcing(self, time):
+        check(self.lib.ih_ct_forcing(self.h, float(time)), "ct_forcing")
 
     def fill_corners(self, f=PRE):
         check(self.lib.ih_fill_corners(self.h, f), "fill_corners")

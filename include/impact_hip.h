@@ -202,6 +202,30 @@ int ih_set_tube_mask(ih_case *c, double yc, double zc, const double *r_u, const 
 int ih_get_mask(ih_case *c, uint8_t *host3);
 int ih_apply_mask(ih_case *c);
 
+/* ---- CT geometry (src/usr_func.f90:187-651, 909-945) --------------------- */
+typedef struct {
+    int inlet_cells, outlet_cells, outlet2_cells; /* fluid cells found on the image's first / last plane */
+    int inlet_jet_voxels;                         /* inlet cells within the jet radius (label 5 at k_start-1) */
+    int k_start, k_start2;                        /* z_offset + 1, z_offset + image depth */
+    double inlet_centre[2], outlet_centre[2];     /* mean x, y of the inlet / outlet cells */
+    double reference_velocity, reference_length, reference_time;
+} ih_ct_info;
+/* read_vtk_3D: a legacy ASCII VTK label image placed at the offsets with the
+ * reference's x/y transpose (image axis 1 -> grid y, axis 2 -> grid x), the
+ * rest of the grid label 2; inlet cells (label 1, left half of the first
+ * plane) get ten planes of 7 below them, 5 within the jet radius; outlet cells
+ * (right half) ten planes of 9, the last plane's cells ten planes of 11 above.
+ * The echo waveform (t in ms, velocity in m/s) is converted to reference units
+ * as convert_waveform does for a CT image.  Labels live on the device. */
+int ih_ct_load_vtk(ih_case *c, const char *path, int x_offset, int y_offset, int z_offset, const double *wave_t_ms,
+                   const double *wave_q, int npts, ih_ct_info *info);
+/* labels as uint8 (N1+7)(N2+7)(N3+7), Fortran element order */
+int ih_ct_get_labels(ih_case *c, uint8_t *host);
+/* the inflow velocity the forcing uses at `time` (reference units); found = 0 outside the waveform */
+int ih_ct_inflow(ih_case *c, double time, double *q, int *found);
+/* apply_CT_forcing (CT-based branch) on VEL1..3 at `time` */
+int ih_ct_forcing(ih_case *c, double time);
+
 /* ---- projection time step ------------------------------------------------ */
 typedef struct {
     double Re, dtime;
@@ -217,6 +241,7 @@ typedef struct {
     int precond_sweeps; /* > 0: BiCGstab right-preconditioned by this many FP32 JOR sweeps */
     int precond_poisson; /* with precond_sweeps 0: BiCGstab's preconditioner, 0 none, 1 multigrid V cycle */
     int mg_fp32;         /* with precond_poisson 1: the cycle in FP32 (an FP64-cycle solve redoes a failure) */
+    int use_ct;          /* 1: apply_CT_forcing with the loaded CT labels at the step's end time */
 } ih_step_params;
 
 typedef struct {

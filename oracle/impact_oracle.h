@@ -23,6 +23,7 @@
  *   - projection / RK3    src/mod_solvers.f90:64-346, src/mod_rhs.f90:35-420,
  *                         src/mod_timeint.f90:275-377, src/mod_lib.f90:507-575
  *   - IBM volume mask     src/mod_sharp_interface_ibm.f90:585-640
+ *   - CT labels, forcing  src/usr_func.f90:187-555, 574-651, 753-760, 893-945 (orc_ct.c)
  *
  * Arrays are Fortran-ordered with halo 3: phi(-3:N1+3,-3:N2+3,-3:N3+3),
  * element (i,j,k) at (i+3) + (j+3)*nx + (k+3)*nx*ny, nx = N1+7, ny = N2+7.
@@ -138,6 +139,23 @@ void orc_bicgstab_mg_f32(const orc_case *c, orc_mg *mg, double eps, int n_it_max
 void orc_bicgstab_mg(const orc_case *c, orc_mg *mg, double eps, int n_it_max, const int ss[3], const int nn[3],
                      const double *bb, double *phi, orc_bicgstab_stats *st);
 
+/* ---------------- CT label volume and forcing ---------------- */
+/* read_vtk_3D's file reading: returns np and the values (malloc'd), or -1 */
+int orc_vtk_read(const char *path, int dims[3], double **values);
+/* convert_waveform (CT-based): t_out, q_out in reference units; ref = velocity, length, time */
+void orc_waveform_convert(int npts, const double *t_ms, const double *q, int M1, double *t_out, double *q_out,
+                          double ref[3]);
+/* read_vtk_3D's placement (x/y transpose, offsets), inlet / outlet detection and labels
+ * 7, 5, 9, 11 into lab (uint8, field layout); info: inlet, outlet, outlet-2 cells, jet
+ * voxels, k_start, k_start_2; centre: inlet x, y, outlet x, y.  0 or -1 */
+int orc_ct_build(const orc_case *c, const int dims[3], const double *x, int x_offset, int y_offset, int z_offset,
+                 double reference_velocity, double reference_length, uint8_t *lab, int info[6], double centre[4]);
+/* Q(t) of the converted waveform after the pulse shift; *found = 0 if no interval holds it */
+double orc_ct_inflow(int npts, const double *t_conv, const double *q_conv, double time, int *found);
+/* apply_CT_forcing (CT branch) on vel (3 fields, contiguous) */
+void orc_ct_forcing(const orc_case *c, const uint8_t *lab, int npts, const double *t_conv, const double *q_conv,
+                    double time, double *vel);
+
 /* ---------------- projection step ---------------- */
 typedef struct {
     double Re, dtime;
@@ -153,6 +171,9 @@ typedef struct {
     int precond_sweeps;    /* > 0: BiCGstab preconditioned by this many FP32 JOR sweeps */
     int precond_poisson;   /* 1: BiCGstab preconditioned by the multigrid V cycle (precond_sweeps 0) */
     int mg_fp32;           /* with precond_poisson 1: the cycle in FP32 (mixed precision) */
+    const uint8_t *ct_lab; /* optional: CT labels (orc_ct_build); apply_CT_forcing at the step's end time */
+    int ct_npts;
+    const double *ct_t, *ct_q; /* converted waveform */
 } orc_step_params;
 
 typedef struct {

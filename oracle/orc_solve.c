@@ -622,6 +622,9 @@ void orc_timestep(orc_case *c, const orc_step_params *p, int step_index, double 
     }
     if (p->mask3)
         orc_apply_mask(c, p->mask3, vel);
+    /* apply_CT_forcing follows the time update (src/mod_timeint.f90:405-416) */
+    if (p->ct_lab)
+        orc_ct_forcing(c, p->ct_lab, p->ct_npts, p->ct_t, p->ct_q, time + p->dtime, vel);
     if (p->level_pressure)
         orc_level_pressure(c, pre);
     if (st) {

@@ -102,6 +102,15 @@ def lib():
         L.orc_bicgstab_mg.argtypes = [vp, vp, C.c_double, C.c_int, ip, ip, dp, dp, C.POINTER(BicgStats)]
         L.orc_bicgstab_mg_f32.argtypes = [vp, vp, C.c_double, C.c_int, ip, ip, dp, dp, C.POINTER(BicgStats)]
         L.orc_mg_vcycle_f32.argtypes = [vp, dp, dp]
+        L.orc_vtk_read.restype = C.c_int
+        L.orc_vtk_read.argtypes = [C.c_char_p, ip, C.POINTER(dp)]
+        L.orc_waveform_convert.argtypes = [C.c_int, dp, dp, C.c_int, dp, dp, dp]
+        L.orc_ct_build.restype = C.c_int
+        L.orc_ct_build.argtypes = [vp, ip, dp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
+                                   C.POINTER(C.c_uint8), ip, dp]
+        L.orc_ct_inflow.restype = C.c_double
+        L.orc_ct_inflow.argtypes = [C.c_int, dp, dp, C.c_double, ip]
+        L.orc_ct_forcing.argtypes = [vp, C.POINTER(C.c_uint8), C.c_int, dp, dp, C.c_double, dp]
         _lib = L
     return _lib
 
@@ -116,7 +125,8 @@ class StepParams(C.Structure):
                 ("jor_omega", C.c_double), ("bicg_rel_tol", C.c_double), ("bicg_max_it", C.c_int),
                 ("level_pressure", C.c_int), ("inflow_kind", C.c_int), ("inflow_amp", C.c_double),
                 ("mask3", C.POINTER(C.c_uint8)), ("jor_fp32", C.c_int), ("precond_sweeps", C.c_int),
-                ("precond_poisson", C.c_int), ("mg_fp32", C.c_int)]
+                ("precond_poisson", C.c_int), ("mg_fp32", C.c_int), ("ct_lab", C.POINTER(C.c_uint8)),
+                ("ct_npts", C.c_int), ("ct_t", C.POINTER(C.c_double)), ("ct_q", C.POINTER(C.c_double))]
 
 
 class StepStats(C.Structure):
@@ -329,6 +339,49 @@ class OracleMG:
 
     def relax(self, g, init, n, bb, rel):
         self.L.orc_mg_relax(self.h, g, int(init), int(n), _dp(bb), _dp(rel))
+
+
+def vtk_read(path):
+    """read_vtk_3D's reader: (dims, values) or None."""
+    dims = (C.c_int * 3)()
+    ptr = C.POINTER(C.c_double)()
+    n = lib().orc_vtk_read(str(path).encode(), dims, C.byref(ptr))
+    if n < 0:
+        return None
+    vals = np.ctypeslib.as_array(ptr, shape=(n,)).copy()
+    C.CDLL(None).free(ptr)
+    return tuple(dims), vals
+
+
+def waveform_convert(t_ms, q, M1):
+    t_ms = np.ascontiguousarray(t_ms, dtype=np.float64)
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    to, qo, ref = np.zeros_like(t_ms), np.zeros_like(q), np.zeros(3)
+    lib().orc_waveform_convert(len(t_ms), _dp(t_ms), _dp(q), int(M1), _dp(to), _dp(qo), _dp(ref))
+    return to, qo, ref
+
+
+def ct_build(case, dims, values, offsets, ref_velocity, ref_length):
+    lab = np.zeros(case.shape, dtype=np.uint8)
+    info = (C.c_int * 6)()
+    centre = np.zeros(4)
+    rc = lib().orc_ct_build(case.h, _ip(dims), _dp(np.ascontiguousarray(values, dtype=np.float64)), *offsets,
+                            float(ref_velocity), float(ref_length), lab.ctypes.data_as(C.POINTER(C.c_uint8)), info,
+                            _dp(centre))
+    if rc != 0:
+        raise ValueError("oracle rejected the CT volume")
+    return lab, list(info), centre
+
+
+def ct_inflow(t_conv, q_conv, time):
+    found = (C.c_int * 1)()
+    v = lib().orc_ct_inflow(len(t_conv), _dp(t_conv), _dp(q_conv), float(time), found)
+    return v, bool(found[0])
+
+
+def ct_forcing(case, lab, t_conv, q_conv, time, vel3):
+    lib().orc_ct_forcing(case.h, lab.ctypes.data_as(C.POINTER(C.c_uint8)), len(t_conv), _dp(t_conv), _dp(q_conv),
+                         float(time), _dp(vel3))
 
 
 def exact_table(abl, n, dx):

@@ -237,6 +237,47 @@ def serving(ih, n_requests=16, rate=4.0, seed=5, budget_gb=64, max_active=4):
     }
 
 
+def ct_geometry(ih, launches=50):
+    """The CT path at the bench grid: a synthetic legacy-VTK label image (a
+    96 x 200 x 96 box with two lumen disks on its first plane and one on its
+    last) and the reference's idealized echo waveform; times the load (parse,
+    placement, detection, upload) and the forcing kernel."""
+    import tempfile
+    d1, d2, d3 = 96, 200, 96
+    offs = (16, 28, 11)
+    i = np.arange(1, d1 + 1)[:, None] + offs[0]   # grid y
+    j = np.arange(1, d2 + 1)[None, :] + offs[1]   # grid x
+    disk = lambda cx, cy, r: (j - cx) ** 2 + (i - cy) ** 2 <= r * r
+    lab = np.zeros((d1, d2, d3), dtype=np.int64)
+    lab[:, :, :-1] = (disk(70, 64, 24) | disk(190, 64, 20))[:, :, None]
+    lab[:, :, -1] = disk(128, 64, 14)
+    wave = ROOT / "tests" / "golden" / "waveform_idealized.csv"
+    w = np.loadtxt(wave, delimiter=",", usecols=(0, 1))
+    c = ih.Case(N, BC, LEN, STRETCH, device=0)
+    with tempfile.TemporaryDirectory() as td:
+        path = Path(td) / "labels.vtk"
+        vals = lab.transpose(2, 1, 0).reshape(-1)
+        with open(path, "w") as f:
+            f.write("# vtk DataFile Version 3.0\nsynthetic labels\nASCII\nDATASET STRUCTURED_POINTS\n")
+            f.write(f"DIMENSIONS {d1} {d2} {d3}\nSPACING 1 1 1\nORIGIN 0 0 0\nPOINT_DATA {vals.size}\n")
+            f.write("SCALARS labels int 1\nLOOKUP_TABLE default\n")
+            f.write("\n".join(" ".join(map(str, row)) for row in vals.reshape(-1, d1)) + "\n")
+        t0 = time.perf_counter()
+        info = c.ct_load_vtk(path, offs, w[:, 0], w[:, 1])
+        load_s = time.perf_counter() - t0
+    c.ct_forcing(0.1)
+    c.sync()
+    c.timer_start()
+    for q in range(launches):
+        c.ct_forcing(0.01 * q)
+    ms = c.timer_stop() / launches
+    c.close()
+    return {"image": [d1, d2, d3], "offsets": list(offs), "inlet_cells": info.inlet_cells,
+            "jet_voxels": info.inlet_jet_voxels, "outlet_cells": info.outlet_cells,
+            "branch_cells": info.outlet2_cells, "load_s": load_s, "forcing_ms": ms,
+            "forcing_points_per_s": (N[0] + 1) * N[1] * N[2] / (ms * 1e-3)}
+
+
 def cpu_baseline(iters_per_step):
     """The oracle (the C restatement of the reference loops, point loops on
     OpenMP threads, dot products serial) on a bounded sample at the full size:
@@ -335,6 +376,7 @@ def main():
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision run")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-relaxation run")
     ap.add_argument("--no-multigrid", action="store_true", help="skip the multigrid-preconditioned FP64 run")
+    ap.add_argument("--no-ct", action="store_true", help="skip the CT-geometry load and forcing timing")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving trace")
     ap.add_argument("--precond-sweeps", type=int, default=8,
                     help="FP32 JOR sweeps of the mixed-precision BiCGstab preconditioner")
@@ -458,6 +500,9 @@ def main():
             **{f"{k}_vs_fp64": v for k, v in err.items()},
         }
         cx.close()
+    if not args.no_ct:
+        log("CT geometry ...")
+        result["ct_geometry"] = ct_geometry(ih)
     if not args.no_batch:
         log("batched relaxation ...")
         result["batched_relaxation"] = batched_relaxation(ih)

@@ -95,16 +95,17 @@ void init_limits(Geo &g, const int bc[6]) {
     }
 }
 
-// both partial buffers hold at least nb block partials
+// the four partial buffers hold at least nb block partials
 int ensure_part(ih_case *c, int64_t nb) {
-    if (nb <= c->part_cap && c->part2) return IH_OK;
+    if (nb <= c->part_cap && c->part4) return IH_OK;
     nb = std::max(nb, c->part_cap);
     HIPCHK(hipStreamSynchronize(c->stream));
-    if (c->part) (void)hipFree(c->part);
-    if (c->part2) (void)hipFree(c->part2);
-    c->part = c->part2 = nullptr;
-    HIPCHK(hipMalloc(&c->part, sizeof(double) * (size_t)nb));
-    HIPCHK(hipMalloc(&c->part2, sizeof(double) * (size_t)nb));
+    for (double **q : {&c->part, &c->part2, &c->part3, &c->part4}) {
+        if (*q) (void)hipFree(*q);
+        *q = nullptr;
+    }
+    for (double **q : {&c->part, &c->part2, &c->part3, &c->part4})
+        HIPCHK(hipMalloc(q, sizeof(double) * (size_t)nb));
     c->part_cap = nb;
     return IH_OK;
 }
@@ -443,6 +444,8 @@ void ih_case_destroy(ih_case *c) {
     if (c->dcoef32) (void)hipFree(c->dcoef32);
     if (c->part) (void)hipFree(c->part);
     if (c->part2) (void)hipFree(c->part2);
+    if (c->part3) (void)hipFree(c->part3);
+    if (c->part4) (void)hipFree(c->part4);
     if (c->ksc) (void)hipFree(c->ksc);
     if (c->kfl) (void)hipFree(c->kfl);
     if (c->hksc) (void)hipHostFree(c->hksc);
@@ -928,6 +931,8 @@ static int bicgstab_run(ih_case *c, double eps, int n_it_max, const int ss[3], c
     hipStream_t s = c->stream;
     KrylovPtrs k{F(c, IH_RR), F(c, IH_RH), F(c, IH_PK), F(c, IH_AP), F(c, IH_AR), F(c, phif), c->part, c->part2,
                  c->ksc, c->kfl};
+    k.part3 = c->part3;
+    k.part4 = c->part4;
     if (sweeps > 0) {
         k.z1 = F(c, IH_Z1);
         k.z2 = F(c, IH_Z2);

@@ -75,6 +75,8 @@ struct ih_case {
     double *dscal = nullptr;   // device scalars
     double *hscal = nullptr;   // pinned mirror
     double *part2 = nullptr;   // second partial buffer (fused dots)
+    double *part3 = nullptr;   // partials of the BiCGstab solution update (read while part/part2 are reduced)
+    double *part4 = nullptr;
     double *ksc = nullptr;     // BiCGstab scalars (device)
     int *kfl = nullptr;        // BiCGstab flags (device)
     double *hksc = nullptr;    // pinned mirrors

@@ -40,6 +40,9 @@ struct KrylovPtrs {
     // right preconditioner M = the multigrid V cycle (preconditioner 1) instead
     const MgDev *mg = nullptr;
     bool mg_fp32 = false;  // the cycle in FP32 (z = (double) of its result)
+    // max-norm and <r, rh> partials of the solution update: their own buffers,
+    // because that kernel reduces the operator's partials in part1/part2 itself
+    double *part3 = nullptr, *part4 = nullptr;
 };
 
 // dot modes of the operator kernel

@@ -408,46 +408,6 @@ __global__ void k_p_update(Geo g, Range3 r, const double *rr, double *pk, const 
         pk[x] = rr[x];
 }
 
-__global__ void k_s_update(Geo g, Range3 r, double *rr, const double *Ap, const double *sc, const int *fl) {
-    if (!fl[KF_ACTIVE]) return;
-    IH_POINT(r);
-    const int64_t x = gidx(g, i, j, k);
-    rr[x] = rr[x] - sc[KS_ALPHA] * Ap[x];
-}
-
-// phi = phi + omega*ra + alpha*pa; rr = rr - omega*Ar (ra = rr, pa = pk without a
-// preconditioner; ra = z2, pa = z1 with one, src/mod_solvers.f90:1873-1899)
-__global__ __launch_bounds__(256) void k_x_update(Geo g, Range3 r, double *phi, double *rr, const double *ra,
-                                                  const double *pa, const double *Ar, const double *rh,
-                                                  const double *sc, const int *fl, double *pmax, double *pdot) {
-    if (!fl[KF_ACTIVE]) return;
-    __shared__ double sm[8];
-    const double om = sc[KS_OMEGA], alpha = sc[KS_ALPHA];
-    const int i = r.lo0 + (int)(blockIdx.x * RED_TX + threadIdx.x);
-    const int j = r.lo1 + (int)(blockIdx.y * RED_TY + threadIdx.y);
-    double m = 0.0, d = 0.0;
-    if (i < r.lo0 + r.n0 && j < r.lo1 + r.n1) {
-        int kc0, kc1;
-        red_chunk(r, (int)blockIdx.z, (int)gridDim.z, kc0, kc1);
-        for (int k = kc0; k <= kc1; ++k) {
-            const int64_t x = gidx(g, i, j, k);
-            phi[x] = phi[x] + om * ra[x] + alpha * pa[x];
-            const double v = rr[x] - om * Ar[x];
-            rr[x] = v;
-            const double a = fabs(v);
-            m = (a > m) ? a : m;
-            d = d + v * rh[x];
-        }
-    }
-    const double bm = block_max(m, sm);
-    __syncthreads();
-    const double bs = block_sum(d, sm + 4);
-    if (threadIdx.x == 0 && threadIdx.y == 0) {
-        pmax[part_index()] = bm;
-        pdot[part_index()] = bs;
-    }
-}
-
 __device__ inline double reduce_sum_parts(const double *part, int64_t nb, double *sm) {
     const int tid = threadIdx.x + threadIdx.y * blockDim.x;
     double acc = 0.0;
@@ -550,36 +510,113 @@ __global__ void k_ctl_resume(double *sc, int *fl) {
     ctl_head_tail(sc, fl, fl[KF_COUNTER]);
 }
 
-__global__ __launch_bounds__(256) void k_ctl_alpha(const double *part, int64_t nb, double *sc, int *fl) {
-    if (!fl[KF_ACTIVE]) return;
+// Every workgroup of the two update kernels below first reduces the partials
+// of the operator kernel before it (the same fixed tree as the single-workgroup
+// control kernels), so the scalar step of BiCGstab needs no launch of its own.
+// Workgroup 0 publishes the scalar, or the breakdown flags, in which case no
+// workgroup updates.  The active flag is read once per workgroup: workgroup 0
+// may clear it while later workgroups start, and either value they see leads
+// to the same decision.
+__device__ inline bool wg_active(const int *fl, int *s_go) {
+    if (threadIdx.x == 0 && threadIdx.y == 0) *s_go = fl[KF_ACTIVE];
+    __syncthreads();
+    return *s_go != 0;
+}
+
+__device__ inline void breakdown(int *fl) {
+    fl[KF_EXIT] = 2;
+    fl[KF_ACTIVE] = 0;
+    fl[KF_DONE] = 1;
+}
+
+// alpha = rhr/<rh, Ap> (a vanishing <rh, Ap> is a breakdown), then rr = rr - alpha*Ap
+// (src/mod_solvers.f90:1709-1740)
+__global__ __launch_bounds__(256) void k_s_update_alpha(Geo g, Range3 r, double *rr, const double *Ap,
+                                                        const double *part, int64_t nb, double *sc, int *fl) {
     __shared__ double sm[4];
+    __shared__ double s_alpha;
+    __shared__ int s_go;
+    if (!wg_active(fl, &s_go)) return;
     const double rhAp = reduce_sum_parts(part, nb, sm);
     if (threadIdx.x == 0 && threadIdx.y == 0) {
-        if (fabs(rhAp) == 0.0) {
-            fl[KF_EXIT] = 2;
-            fl[KF_ACTIVE] = 0;
-            fl[KF_DONE] = 1;
-        } else {
-            sc[KS_ALPHA] = sc[KS_RHR] / rhAp;
+        const bool ok = !(fabs(rhAp) == 0.0);
+        const double alpha = ok ? sc[KS_RHR] / rhAp : 0.0;
+        s_alpha = alpha;
+        s_go = ok;
+        if (part_index() == 0) {
+            if (ok)
+                sc[KS_ALPHA] = alpha;
+            else
+                breakdown(fl);
         }
+    }
+    __syncthreads();
+    if (!s_go) return;
+    const double alpha = s_alpha;
+    const int i = r.lo0 + (int)(blockIdx.x * RED_TX + threadIdx.x);
+    const int j = r.lo1 + (int)(blockIdx.y * RED_TY + threadIdx.y);
+    if (i >= r.lo0 + r.n0 || j >= r.lo1 + r.n1) return;
+    int kc0, kc1;
+    red_chunk(r, (int)blockIdx.z, (int)gridDim.z, kc0, kc1);
+    for (int k = kc0; k <= kc1; ++k) {
+        const int64_t x = gidx(g, i, j, k);
+        rr[x] = rr[x] - alpha * Ap[x];
     }
 }
 
-__global__ __launch_bounds__(256) void k_ctl_omega(const double *p1, const double *p2, int64_t nb, double *sc,
-                                                   int *fl) {
-    if (!fl[KF_ACTIVE]) return;
+// omega = <r, Ar>/<Ar, Ar> (a vanishing <Ar, Ar> is a breakdown), then
+// phi = phi + omega*ra + alpha*pa; rr = rr - omega*Ar, with the max-norm and
+// <rr, rh> partials of the new residual in pmax/pdot (ra = rr, pa = pk without
+// a preconditioner; ra = z2, pa = z1 with one, src/mod_solvers.f90:1782-1797, 1855-1899).
+// pmax/pdot must not alias p1/p2, which other workgroups may still be reducing.
+__global__ __launch_bounds__(256) void k_x_update_omega(Geo g, Range3 r, double *phi, double *rr, const double *ra,
+                                                        const double *pa, const double *Ar, const double *rh,
+                                                        const double *p1, const double *p2, int64_t nb, double *sc,
+                                                        int *fl, double *pmax, double *pdot) {
     __shared__ double sm[8];
+    __shared__ double s_om;
+    __shared__ int s_go;
+    if (!wg_active(fl, &s_go)) return;
     const double rAr = reduce_sum_parts(p1, nb, sm);
     __syncthreads();
     const double ArAr = reduce_sum_parts(p2, nb, sm + 4);
     if (threadIdx.x == 0 && threadIdx.y == 0) {
-        if (fabs(ArAr) == 0.0) {
-            fl[KF_EXIT] = 2;
-            fl[KF_ACTIVE] = 0;
-            fl[KF_DONE] = 1;
-        } else {
-            sc[KS_OMEGA] = rAr / ArAr;
+        const bool ok = !(fabs(ArAr) == 0.0);
+        const double om = ok ? rAr / ArAr : 0.0;
+        s_om = om;
+        s_go = ok;
+        if (part_index() == 0) {
+            if (ok)
+                sc[KS_OMEGA] = om;
+            else
+                breakdown(fl);
         }
+    }
+    __syncthreads();
+    if (!s_go) return;
+    const double om = s_om, alpha = sc[KS_ALPHA];
+    const int i = r.lo0 + (int)(blockIdx.x * RED_TX + threadIdx.x);
+    const int j = r.lo1 + (int)(blockIdx.y * RED_TY + threadIdx.y);
+    double m = 0.0, d = 0.0;
+    if (i < r.lo0 + r.n0 && j < r.lo1 + r.n1) {
+        int kc0, kc1;
+        red_chunk(r, (int)blockIdx.z, (int)gridDim.z, kc0, kc1);
+        for (int k = kc0; k <= kc1; ++k) {
+            const int64_t x = gidx(g, i, j, k);
+            phi[x] = phi[x] + om * ra[x] + alpha * pa[x];
+            const double v = rr[x] - om * Ar[x];
+            rr[x] = v;
+            const double a = fabs(v);
+            m = (a > m) ? a : m;
+            d = d + v * rh[x];
+        }
+    }
+    const double bm = block_max(m, sm);
+    __syncthreads();
+    const double bs = block_sum(d, sm + 4);
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+        pmax[part_index()] = bm;
+        pdot[part_index()] = bs;
     }
 }
 
@@ -677,18 +714,18 @@ void launch_krylov_iteration(hipStream_t s, const Geo &g, const Coef &c, const B
     hipLaunchKernelGGL(k_p_update, grid_for(r), dim3(BX, BY, 1), 0, s, g, r, k.rr, k.pk, k.Ap, k.sc, k.fl);
     if (pre) launch_precond(s, g, c, b, k.pk, k.z1, k);
     launch_apply_A(s, g, c, b, pre ? k.z1 : k.pk, k.Ap, k.rh, DOT_W, k.part1, nullptr, k.fl + KF_ACTIVE);
-    hipLaunchKernelGGL(k_ctl_alpha, dim3(1), dim3(RED_TX, RED_TY, 1), 0, s, k.part1, nb, k.sc, k.fl);
-    hipLaunchKernelGGL(k_s_update, grid_for(r), dim3(BX, BY, 1), 0, s, g, r, k.rr, k.Ap, k.sc, k.fl);
+    hipLaunchKernelGGL(k_s_update_alpha, red_grid(r), dim3(RED_TX, RED_TY, 1), 0, s, g, r, k.rr, k.Ap, k.part1, nb,
+                       k.sc, k.fl);
     if (pre) {
         launch_precond(s, g, c, b, k.rr, k.z2, k);
         launch_apply_A(s, g, c, b, k.z2, k.Ar, k.rr, DOT_PAIR, k.part1, k.part2, k.fl + KF_ACTIVE);
     } else {
         launch_apply_A(s, g, c, b, k.rr, k.Ar, nullptr, DOT_SELF, k.part1, k.part2, k.fl + KF_ACTIVE);
     }
-    hipLaunchKernelGGL(k_ctl_omega, dim3(1), dim3(RED_TX, RED_TY, 1), 0, s, k.part1, k.part2, nb, k.sc, k.fl);
-    hipLaunchKernelGGL(k_x_update, red_grid(r), dim3(RED_TX, RED_TY, 1), 0, s, g, r, k.phi, k.rr, pre ? k.z2 : k.rr,
-                       pre ? k.z1 : k.pk, k.Ar, k.rh, k.sc, k.fl, k.part1, k.part2);
-    hipLaunchKernelGGL(k_ctl_end, dim3(1), dim3(RED_TX, RED_TY, 1), 0, s, k.part1, k.part2, nb, k.sc, k.fl);
+    hipLaunchKernelGGL(k_x_update_omega, red_grid(r), dim3(RED_TX, RED_TY, 1), 0, s, g, r, k.phi, k.rr,
+                       pre ? k.z2 : k.rr, pre ? k.z1 : k.pk, k.Ar, k.rh, k.part1, k.part2, nb, k.sc, k.fl, k.part3,
+                       k.part4);
+    hipLaunchKernelGGL(k_ctl_end, dim3(1), dim3(RED_TX, RED_TY, 1), 0, s, k.part3, k.part4, nb, k.sc, k.fl);
 }
 
 void launch_krylov_resume(hipStream_t s, const Geo &g, const Box &b, const KrylovPtrs &k) {

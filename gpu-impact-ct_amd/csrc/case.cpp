@@ -592,6 +592,19 @@ int ih_grid_coords(const ih_grid_desc *gd, int d, int which, double *out) {
     return IH_OK;
 }
 
+int ih_case_get_grid(const ih_case *c, ih_grid_desc *out) {
+    if (!c || !out) return fail(IH_EINVAL, "null argument");
+    out->version = IH_ABI_VERSION;
+    for (int d = 0; d < 3; ++d) {
+        out->N[d] = c->hg.N[d];
+        out->bc[2 * d] = c->hg.bc[d][0];
+        out->bc[2 * d + 1] = c->hg.bc[d][1];
+        out->L[d] = c->hg.L[d];
+        out->stretch[d] = c->hg.stretch[d];
+    }
+    return IH_OK;
+}
+
 int ih_case_get_coords(ih_case *c, int d, int which, double *out) {
     if (!c || !out || d < 0 || d > 2) return fail(IH_EINVAL, "bad argument");
     const std::vector<double> &v = which ? c->hg.xu[d] : c->hg.xp[d];
@@ -656,6 +669,50 @@ int ih_field_download(ih_case *c, ih_field f, double *host) {
                             (size_t)(g.N[1] + 7) * (g.N[2] + 7), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     return IH_OK;
+}
+
+// The strided box lo, lo+s, .. <= hi of a field as a dense host array, x fastest
+// (the hyperslab write_hdf writes, src/mod_inout.f90:570-575, 657-667).  The box
+// is gathered on the device, so only its bytes cross the host link.
+static int box_transfer(ih_case *c, ih_field f, const int lo[3], const int hi[3], const int stride[3], double *host,
+                        bool to_device) {
+    if (!c || !host || !lo || !hi || !stride || !valid_field(f)) return fail(IH_EINVAL, "bad argument");
+    int n[3];
+    int64_t cnt = 1;
+    for (int d = 0; d < 3; ++d) {
+        if (stride[d] < 1 || lo[d] > hi[d] || lo[d] < -3 || hi[d] > c->g.N[d] + 3)
+            return fail(IH_EINVAL, "box %d..%d stride %d outside the field's index range -3..%d along %d", lo[d], hi[d],
+                        stride[d], c->g.N[d] + 3, d + 1);
+        n[d] = (hi[d] - lo[d]) / stride[d] + 1;
+        cnt *= n[d];
+    }
+    HIPCHK(hipSetDevice(c->device));
+    double *buf = nullptr;
+    HIPCHK(hipMalloc(&buf, sizeof(double) * (size_t)cnt));
+    hipError_t e = hipSuccess;
+    if (to_device) {
+        e = hipMemcpyAsync(buf, host, sizeof(double) * (size_t)cnt, hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess) launch_unpack_box(c->stream, c->g, lo, n, stride, buf, c->fields[f]);
+    } else {
+        launch_pack_box(c->stream, c->g, lo, n, stride, c->fields[f], buf);
+        e = hipGetLastError();
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(host, buf, sizeof(double) * (size_t)cnt, hipMemcpyDeviceToHost, c->stream);
+    }
+    if (e == hipSuccess) e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(buf);
+    if (e != hipSuccess) return fail(IH_EHIP, "box transfer: %s", hipGetErrorString(e));
+    return IH_OK;
+}
+
+int ih_field_get_box(ih_case *c, ih_field f, const int lo[3], const int hi[3], const int stride[3], double *host) {
+    return box_transfer(c, f, lo, hi, stride, host, false);
+}
+
+int ih_field_set_box(ih_case *c, ih_field f, const int lo[3], const int hi[3], const int stride[3],
+                     const double *host) {
+    return box_transfer(c, f, lo, hi, stride, const_cast<double *>(host), true);
 }
 
 int ih_field_fill(ih_case *c, ih_field f, double value) {

@@ -106,6 +106,11 @@ void launch_jor_f32(hipStream_t s, const Geo &g, const Coef &c, const Box &b, fl
 void launch_residual_cdg(hipStream_t s, const Geo &g, const Coef &c, const Box &b, const double *rel,
                          const double *bb, double *out_res /* may be null */, double *partial_max);
 void launch_copy_box(hipStream_t s, const Geo &g, const Box &b, const double *src, double *dst);
+// strided box lo + st*(0..n-1) of a field <-> a dense array of n[0]*n[1]*n[2] (x fastest)
+void launch_pack_box(hipStream_t s, const Geo &g, const int lo[3], const int n[3], const int st[3],
+                     const double *field, double *dense);
+void launch_unpack_box(hipStream_t s, const Geo &g, const int lo[3], const int n[3], const int st[3],
+                       const double *dense, double *field);
 void launch_axpby(hipStream_t s, const Geo &g, double a, double b, const double *x, const double *y, double *z,
                   int64_t n);
 void launch_fill(hipStream_t s, double *f, double v, int64_t n);

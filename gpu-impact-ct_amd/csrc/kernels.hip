@@ -855,6 +855,27 @@ __global__ void k_copy_box(Geo g, Range3 r, const double *src, double *dst) {
     dst[x] = src[x];
 }
 
+// A strided box of a field <-> a dense array, x fastest: the hyperslab of
+// write_hdf / read2_hdf (src/mod_inout.f90:657-667, 1452-1720).  One thread per
+// packed element; the host checks the box against the field's index range.
+struct PackBox {
+    int lo[3], n[3], s[3];
+};
+
+__global__ void k_pack_box(Geo g, PackBox b, const double *field, double *dense) {
+    const int i = (int)(blockIdx.x * BX + threadIdx.x), j = (int)(blockIdx.y * BY + threadIdx.y), k = (int)blockIdx.z;
+    if (i >= b.n[0] || j >= b.n[1]) return;
+    const int64_t q = i + (int64_t)b.n[0] * (j + (int64_t)b.n[1] * k);
+    dense[q] = field[gidx(g, b.lo[0] + b.s[0] * i, b.lo[1] + b.s[1] * j, b.lo[2] + b.s[2] * k)];
+}
+
+__global__ void k_unpack_box(Geo g, PackBox b, const double *dense, double *field) {
+    const int i = (int)(blockIdx.x * BX + threadIdx.x), j = (int)(blockIdx.y * BY + threadIdx.y), k = (int)blockIdx.z;
+    if (i >= b.n[0] || j >= b.n[1]) return;
+    const int64_t q = i + (int64_t)b.n[0] * (j + (int64_t)b.n[1] * k);
+    field[gidx(g, b.lo[0] + b.s[0] * i, b.lo[1] + b.s[1] * j, b.lo[2] + b.s[2] * k)] = dense[q];
+}
+
 __global__ void k_axpby(double a, double b, const double *x, const double *y, double *z, int64_t n) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t < n) z[t] = a * x[t] + b * y[t];
@@ -1264,6 +1285,30 @@ void launch_axpby(hipStream_t s, const Geo &g, double a, double b, const double 
 
 void launch_fill(hipStream_t s, double *f, double v, int64_t n) {
     hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, f, v, n);
+}
+
+static PackBox pack_box(const int lo[3], const int n[3], const int st[3]) {
+    PackBox b;
+    for (int d = 0; d < 3; ++d) {
+        b.lo[d] = lo[d];
+        b.n[d] = n[d];
+        b.s[d] = st[d];
+    }
+    return b;
+}
+
+void launch_pack_box(hipStream_t s, const Geo &g, const int lo[3], const int n[3], const int st[3],
+                     const double *field, double *dense) {
+    if (n[0] <= 0 || n[1] <= 0 || n[2] <= 0) return;
+    const dim3 grid((unsigned)((n[0] + BX - 1) / BX), (unsigned)((n[1] + BY - 1) / BY), (unsigned)n[2]);
+    hipLaunchKernelGGL(k_pack_box, grid, dim3(BX, BY, 1), 0, s, g, pack_box(lo, n, st), field, dense);
+}
+
+void launch_unpack_box(hipStream_t s, const Geo &g, const int lo[3], const int n[3], const int st[3],
+                       const double *dense, double *field) {
+    if (n[0] <= 0 || n[1] <= 0 || n[2] <= 0) return;
+    const dim3 grid((unsigned)((n[0] + BX - 1) / BX), (unsigned)((n[1] + BY - 1) / BY), (unsigned)n[2]);
+    hipLaunchKernelGGL(k_unpack_box, grid, dim3(BX, BY, 1), 0, s, g, pack_box(lo, n, st), dense, field);
 }
 
 void launch_fill_f32(hipStream_t s, float *f, float v, int64_t n, const int *active) {

@@ -18,6 +18,8 @@ import numpy as np
 
 PKG_ROOT = Path(__file__).resolve().parent.parent
 LIB_PATH = PKG_ROOT / "lib" / "libimpact_hip.so"
+IO_LIB_PATH = PKG_ROOT / "lib" / "libimpact_io.so"
+IH_IO_ABI_VERSION = 1
 
 IH_ABI_VERSION = 3
 
@@ -59,6 +61,22 @@ class StepParams(C.Structure):
                 ("inflow_kind", C.c_int), ("inflow_amp", C.c_double), ("use_mask", C.c_int), ("jor_fp32", C.c_int),
                 ("precond_sweeps", C.c_int), ("precond_poisson", C.c_int),
                 ("mg_fp32", C.c_int), ("use_ct", C.c_int)]
+
+
+class IoState(C.Structure):
+    """ih_io_state (include/impact_io.h): the run state of the HDF5 files."""
+    _fields_ = ([("version", C.c_int)] +
+                [(n, C.c_int) for n in ("restart", "write_count", "timestep", "write_out_vect", "write_out_scal",
+                                         "new_dtime", "Euler_yes", "twostep_yes", "mapping_yes", "upwind_yes")] +
+                [(n, C.c_double) for n in ("time", "dtime", "time_out_vect", "time_out_scal", "dtime_out_vect",
+                                            "dtime_out_scal", "Re", "CFL", "thetaL", "epsU", "p_end_systole_in",
+                                            "p_end_systole_out", "p_end_systole_out_2", "U_ref", "L_ref")])
+
+    def __init__(self, **kw):
+        super().__init__()
+        self.version = IH_IO_ABI_VERSION
+        for k, v in kw.items():
+            setattr(self, k, v)
 
 
 class Request(C.Structure):
@@ -117,6 +135,9 @@ def load(path: os.PathLike | None = None):
         "ih_field_download": (C.c_int, [vp, C.c_int, dp]),
         "ih_field_fill": (C.c_int, [vp, C.c_int, C.c_double]),
         "ih_field_copy": (C.c_int, [vp, C.c_int, C.c_int]),
+        "ih_field_get_box": (C.c_int, [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), dp]),
+        "ih_field_set_box": (C.c_int, [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), dp]),
+        "ih_case_get_grid": (C.c_int, [vp, C.POINTER(GridDesc)]),
         "ih_synchronize": (C.c_int, [vp]),
         "ih_exchange": (C.c_int, [vp, C.c_int, C.c_int, C.c_int]),
         "ih_exchange_relax": (C.c_int, [vp, C.c_int]),
@@ -185,6 +206,48 @@ def load(path: os.PathLike | None = None):
         raise IHError("ABI version mismatch")
     _lib = L
     return L
+
+
+_io = None
+
+
+def load_io(path: os.PathLike | None = None):
+    """libimpact_io.so (HDF5 output and restart); raises if it is not built."""
+    global _io
+    if _io is not None:
+        return _io
+    load()
+    p = Path(path) if path else IO_LIB_PATH
+    if not p.exists():
+        raise IHError(f"{p} is not built (make -C gpu-impact-ct_amd)")
+    L = C.CDLL(str(p), mode=C.RTLD_GLOBAL)
+    vp, ip, cs = C.c_void_p, C.POINTER(C.c_int), C.c_char_p
+    sp = C.POINTER(IoState)
+    sig = {
+        "ih_io_abi_version": (C.c_int, []),
+        "ih_io_last_error": (C.c_int, [C.c_char_p, C.c_size_t]),
+        "ih_io_write_hdf": (C.c_int, [vp, cs, cs, C.c_int, C.c_int, ip, sp]),
+        "ih_io_read_hdf": (C.c_int, [vp, cs, cs, C.c_int, C.c_int, sp]),
+        "ih_io_write_restart": (C.c_int, [vp, cs, sp]),
+        "ih_io_read_restart": (C.c_int, [vp, cs, C.c_int, sp]),
+        "ih_io_write_fields": (C.c_int, [vp, cs, ip, ip, sp]),
+        "ih_io_write_xdmf": (C.c_int, [vp, cs, ip, ip, sp]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    if L.ih_io_abi_version() != IH_IO_ABI_VERSION:
+        raise IHError("I/O ABI version mismatch")
+    _io = L
+    return L
+
+
+def io_check(rc: int, what: str = ""):
+    if rc != 0:
+        buf = C.create_string_buffer(1024)
+        load_io().ih_io_last_error(buf, 1024)
+        raise IHError(f"{what} failed ({rc}): {buf.value.decode(errors='replace')}")
 
 
 def exported_symbols():
@@ -312,6 +375,45 @@ class Case:
         out = np.zeros(n.value)
         check(self.lib.ih_case_get_coeff(self.h, name.encode(), _dp(out), C.byref(n)), "coeff")
         return out
+
+    def get_box(self, f: int, lo, hi, stride=(1, 1, 1)) -> np.ndarray:
+        """The strided box of a field, shape (n3, n2, n1) (x fastest)."""
+        n = [(int(hi[d]) - int(lo[d])) // int(stride[d]) + 1 for d in range(3)]
+        out = np.zeros((n[2], n[1], n[0]))
+        check(self.lib.ih_field_get_box(self.h, f, _ints(lo), _ints(hi), _ints(stride), _dp(out)), "get_box")
+        return out
+
+    def set_box(self, f: int, lo, hi, stride, data: np.ndarray):
+        data = np.ascontiguousarray(data, dtype=np.float64)
+        check(self.lib.ih_field_set_box(self.h, f, _ints(lo), _ints(hi), _ints(stride), _dp(data)), "set_box")
+
+    def grid(self) -> GridDesc:
+        g = GridDesc()
+        check(self.lib.ih_case_get_grid(self.h, C.byref(g)), "grid")
+        return g
+
+    # --- HDF5 output and restart (libimpact_io) ---------------------------
+    def write_hdf(self, filename: str, dsetname: str, f: int, vel_dir: int, stride, st: IoState):
+        io_check(load_io().ih_io_write_hdf(self.h, str(filename).encode(), dsetname.encode(), f, vel_dir,
+                                           _ints(stride), C.byref(st)), "write_hdf")
+
+    def read_hdf(self, filename: str, dsetname: str, f: int, vel_dir: int, st: IoState | None = None):
+        io_check(load_io().ih_io_read_hdf(self.h, str(filename).encode(), dsetname.encode(), f, vel_dir,
+                                          C.byref(st) if st is not None else None), "read_hdf")
+
+    def write_restart(self, prefix: str, st: IoState):
+        io_check(load_io().ih_io_write_restart(self.h, str(prefix).encode(), C.byref(st)), "write_restart")
+
+    def read_restart(self, prefix: str, restart: int, st: IoState):
+        io_check(load_io().ih_io_read_restart(self.h, str(prefix).encode(), restart, C.byref(st)), "read_restart")
+
+    def write_fields(self, prefix: str, sizes, strides, st: IoState):
+        io_check(load_io().ih_io_write_fields(self.h, str(prefix).encode(), _ints(sizes),
+                                              _ints([s for t in strides for s in t]), C.byref(st)), "write_fields")
+
+    def write_xdmf(self, prefix: str, sizes, strides, st: IoState):
+        io_check(load_io().ih_io_write_xdmf(self.h, str(prefix).encode(), _ints(sizes),
+                                            _ints([s for t in strides for s in t]), C.byref(st)), "write_xdmf")
 
     def coords(self, d: int, which: int) -> np.ndarray:
         out = np.zeros(self.N[d] + 7)

@@ -102,6 +102,8 @@ int  ih_case_generate_coeffs(ih_case *c);
 /* Export one generated coefficient array by reference name ("cGp1", "cdg2", ...);
  * *n receives the element count.  out may be NULL to query the size. */
 int  ih_case_get_coeff(ih_case *c, const char *name, double *out, int64_t *n);
+/* The grid the case was created with. */
+int  ih_case_get_grid(const ih_case *c, ih_grid_desc *out);
 /* Coordinates x?p (which = 0) or x?u (which = 1) of direction d, (-3:N+3). */
 int  ih_case_get_coords(ih_case *c, int d, int which, double *out);
 /* Host-only forms of the two above (no device needed): coefficients and
@@ -120,6 +122,13 @@ int ih_field_upload(ih_case *c, ih_field f, const double *host);
 int ih_field_download(ih_case *c, ih_field f, double *host);
 int ih_field_fill(ih_case *c, ih_field f, double value);
 int ih_field_copy(ih_case *c, ih_field dst, ih_field src);
+/* The strided box lo, lo+stride, .. <= hi (reference indices, within -3..N+3)
+ * of field f as a dense host array, x fastest: the hyperslab that write_hdf
+ * writes and read2_hdf reads (src/mod_inout.f90:570-575, 657-667, 1452-1720).
+ * Gathered and scattered on the device: only the box crosses the host link. */
+int ih_field_get_box(ih_case *c, ih_field f, const int lo[3], const int hi[3], const int stride[3], double *host);
+int ih_field_set_box(ih_case *c, ih_field f, const int lo[3], const int hi[3], const int stride[3],
+                     const double *host);
 int ih_synchronize(ih_case *c);
 
 /* ---- device-resident operators ------------------------------------------ */

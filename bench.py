@@ -36,6 +36,7 @@ sys.path.insert(0, str(ROOT / "gpu-impact-ct_amd"))
 METRIC = "time steps/s + HBM TB/s at 256×128² FP64; mixed-prec speedup; p50/p99 latency"
 HBM_PEAK_GBS = 8000.0
 JOR_BYTES_PER_POINT = 24  # read x, read b, write x' (FP64)
+OPERATOR_BYTES_PER_POINT = 16  # read p, write Ap (FP64); the dot forms add one read of w
 JOR32_BYTES_PER_POINT = 12  # the same in FP32
 
 N = (257, 129, 129)
@@ -58,9 +59,9 @@ def tube_radius(x, R=0.3, x0=1.0, ell=0.4, delta=1.0 - 1.0 / math.sqrt(2.0)):
     return r
 
 
-def read_traffic():
-    """HBM bytes per JOR launch from the committed rocprofv3 counter summary."""
-    p = ROOT / "profiles" / "jor_sweep_256x128x128_counters.txt"
+def read_traffic(name="jor_sweep_256x128x128_counters.txt"):
+    """HBM bytes per launch from a committed rocprofv3 counter summary."""
+    p = ROOT / "profiles" / name
     if not p.exists():
         return None
     vals = {}
@@ -72,6 +73,41 @@ def read_traffic():
         return float(vals["hbm_bytes_per_launch"])
     except (KeyError, ValueError):
         return None
+
+
+def operator_roofline(ih, c, launches=40):
+    """Average duration of the fused Poisson operator kernel (k_apply without a
+    dot product) on the bench grid, from HIP events on the case stream:
+    `launches` operator calls (three ghost fills and the kernel each) minus
+    `launches` rounds of the three ghost fills alone.  Two of these kernels,
+    with a fused dot product, are half of every BiCGstab iteration."""
+    for _ in range(3):
+        c.product_div_grad(ih.TMP1, ih.TMP2)
+    c.sync()
+    c.timer_start()
+    for _ in range(launches):
+        c.product_div_grad(ih.TMP1, ih.TMP2)
+    t_all = c.timer_stop()
+    c.timer_start()
+    for _ in range(launches):
+        for d in (1, 2, 3):
+            c.exchange(d, 0, ih.TMP1)
+    t_fill = c.timer_stop()
+    ms = (t_all - t_fill) / launches
+    nbytes = OPERATOR_BYTES_PER_POINT * N[0] * N[1] * N[2]
+    achieved = nbytes / (ms * 1e-3) / 1e9
+    return {
+        "kernel": "k_apply<16, 16, 32, 0, false>",
+        "bound": "hbm",
+        "achieved": achieved,
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": achieved / HBM_PEAK_GBS,
+        "traffic": read_traffic("poisson_operator_256x128x128_counters.txt"),
+        "bytes_per_launch": nbytes,
+        "avg_launch_ms": ms,
+        "method": f"{launches} operator calls minus {launches} x 3 ghost fills, HIP events on the case stream",
+    }
 
 
 def setup_case(ih, mixed=False, precond_sweeps=8, multigrid=False):
@@ -509,6 +545,8 @@ def main():
     if not args.no_serve:
         log("serving trace ...")
         result["serving"] = serving(ih)
+    log("operator roofline ...")
+    result["roofline_bicgstab_operator"] = operator_roofline(ih, c)
     if not args.no_cpu_baseline and rank == 0:
         log("cpu baseline: timing the oracle on a bounded sample ...")
         result["cpu_baseline"] = cpu_baseline(iters_per_step)

@@ -376,15 +376,23 @@ class Case:
         check(self.lib.ih_case_get_coeff(self.h, name.encode(), _dp(out), C.byref(n)), "coeff")
         return out
 
+    @staticmethod
+    def _box_shape(lo, hi, stride):
+        """(n3, n2, n1) of a valid box; the library rejects an invalid one
+        before it touches the buffer, which then only needs to exist."""
+        n = [max((int(hi[d]) - int(lo[d])) // max(int(stride[d]), 1) + 1, 1) for d in range(3)]
+        return (n[2], n[1], n[0])
+
     def get_box(self, f: int, lo, hi, stride=(1, 1, 1)) -> np.ndarray:
         """The strided box of a field, shape (n3, n2, n1) (x fastest)."""
-        n = [(int(hi[d]) - int(lo[d])) // int(stride[d]) + 1 for d in range(3)]
-        out = np.zeros((n[2], n[1], n[0]))
+        out = np.zeros(self._box_shape(lo, hi, stride))
         check(self.lib.ih_field_get_box(self.h, f, _ints(lo), _ints(hi), _ints(stride), _dp(out)), "get_box")
         return out
 
     def set_box(self, f: int, lo, hi, stride, data: np.ndarray):
         data = np.ascontiguousarray(data, dtype=np.float64)
+        if data.size != int(np.prod(self._box_shape(lo, hi, stride))):
+            raise ValueError("data does not have the box's size")
         check(self.lib.ih_field_set_box(self.h, f, _ints(lo), _ints(hi), _ints(stride), _dp(data)), "set_box")
 
     def grid(self) -> GridDesc:

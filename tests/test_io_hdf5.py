@@ -290,3 +290,31 @@ def test_write_hdf_rejects_what_the_reference_rejects(tmp_path):
     with pytest.raises(ih.IHError, match="Cannot open file"):
         gc.read_hdf(str(tmp_path / "missing"), "pre", ih.PRE, 0)
     assert not list(tmp_path.iterdir())
+
+
+@pytest.mark.gpu
+def test_field_box_transfer():
+    """ih_field_get_box / ih_field_set_box: the strided box equals the numpy
+    slice of the field; boxes outside -3..N+3 and strides below 1 are rejected
+    before anything runs on the device."""
+    from test_gpu_parity import rnd
+    oc, gc = _pair(PERIODIC_XZ)
+    a = rnd(oc.shape, 430)
+    gc.upload(ih.TMP2, a)
+    for lo, hi, st in (((-3, -3, -3), (oc.N[0] + 3, oc.N[1] + 3, oc.N[2] + 3), (1, 1, 1)),
+                       ((1, 0, 2), (31, 17, 30), (3, 2, 4)), ((5, 5, 5), (5, 5, 5), (1, 1, 1))):
+        got = gc.get_box(ih.TMP2, lo, hi, st)
+        want = a[tuple(slice(lo[d] + 3, hi[d] + 4, st[d]) for d in (2, 1, 0))]
+        assert np.array_equal(got, want), (lo, hi, st)
+    # scatter a box back: only its points change
+    lo, hi, st = (2, 1, 3), (30, 16, 27), (2, 3, 4)
+    box_vals = np.full(gc.get_box(ih.TMP2, lo, hi, st).shape, 7.5)
+    gc.set_box(ih.TMP2, lo, hi, st, box_vals)
+    b = a.copy()
+    b[tuple(slice(lo[d] + 3, hi[d] + 4, st[d]) for d in (2, 1, 0))] = 7.5
+    assert np.array_equal(gc.download(ih.TMP2), b)
+    for lo, hi, st in (((-4, 0, 0), (1, 1, 1), (1, 1, 1)), ((0, 0, 0), (1, 1, oc.N[2] + 4), (1, 1, 1)),
+                       ((0, 0, 0), (1, 1, 1), (1, 0, 1)), ((2, 0, 0), (1, 1, 1), (1, 1, 1))):
+        with pytest.raises(ih.IHError, match="outside the field's index range"):
+            gc.get_box(ih.TMP2, lo, hi, st)
+    assert np.array_equal(gc.download(ih.TMP2), b)

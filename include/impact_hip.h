@@ -260,7 +260,11 @@ typedef struct {
 } ih_step_stats;
 
 /* One time step (3 RK substeps) of the fields VEL, PRE, NL in place.
- * stats may be NULL; when given, div_max_after costs one extra divergence. */
+ * stats may be NULL; when given, div_max_after costs one extra divergence.
+ * div_max_after is max |div u| over the pressure range at the end of the step:
+ * after the mask and the CT forcing, which set velocities in the wall and the
+ * jet and so leave divergence at those interfaces; without them it is what
+ * the projection leaves. */
 int ih_timestep(ih_case *c, const ih_step_params *p, double time, ih_step_stats *st);
 
 /* ---- reference-signature entry points (host arrays in, host arrays out) --- */

@@ -422,8 +422,8 @@ def main():
     ap.add_argument("--launches", type=int, default=20)
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs one GPU; --gpus {args.gpus} is not supported "
-              "(the block decomposition across GPUs is described in DESIGN.md, not built).", file=sys.stderr)
+        print(f"bench.py: the projection step runs on one GPU as a single block; --gpus {args.gpus} is not "
+              "supported (a block decomposition across GPUs is not built).", file=sys.stderr)
         sys.exit(2)
     rank = int(os.environ.get("RANK", "0"))
 

@@ -38,6 +38,14 @@ HBM_PEAK_GBS = 8000.0
 JOR_BYTES_PER_POINT = 24  # read x, read b, write x' (FP64)
 OPERATOR_BYTES_PER_POINT = 16  # read p, write Ap (FP64); the dot forms add one read of w
 JOR32_BYTES_PER_POINT = 12  # the same in FP32
+# algorithmic bytes per pressure point of each kernel of a BiCGstab iteration (FP64):
+# p = r + beta p - beta omega Ap (read r, p, Ap; write p); Ap and <rh, Ap> (read p, rh;
+# write Ap); s = r - alpha Ap (read r, Ap; write r); As, <s, As>, <As, As> (read s;
+# write As); phi += alpha p + omega s, r = s - omega As, |r|, <r, rh> (read phi, p, s,
+# As, rh; write phi, r) -- 152 bytes an iteration
+KERNEL_BYTES = {"jor": 24, "p_update": 32, "apply_w": 24, "s_update": 24, "apply_self": 16, "x_update": 56}
+ITERATION_BYTES = 152
+HBM_CASE = (385, 193, 193)  # 384x192x192 cells: every kernel's working set exceeds the 256 MB last-level cache
 
 N = (257, 129, 129)
 BC = (1, 1, 1, 1, 1, 1)
@@ -75,38 +83,43 @@ def read_traffic(name="jor_sweep_256x128x128_counters.txt"):
         return None
 
 
-def operator_roofline(ih, c, launches=40):
-    """Average duration of the fused Poisson operator kernel (k_apply without a
-    dot product) on the bench grid, from HIP events on the case stream:
-    `launches` operator calls (three ghost fills and the kernel each) minus
-    `launches` rounds of the three ghost fills alone.  Two of these kernels,
-    with a fused dot product, are half of every BiCGstab iteration."""
-    for _ in range(3):
-        c.product_div_grad(ih.TMP1, ih.TMP2)
-    c.sync()
-    c.timer_start()
-    for _ in range(launches):
-        c.product_div_grad(ih.TMP1, ih.TMP2)
-    t_all = c.timer_stop()
-    c.timer_start()
-    for _ in range(launches):
-        for d in (1, 2, 3):
-            c.exchange(d, 0, ih.TMP1)
-    t_fill = c.timer_stop()
-    ms = (t_all - t_fill) / launches
-    nbytes = OPERATOR_BYTES_PER_POINT * N[0] * N[1] * N[2]
-    achieved = nbytes / (ms * 1e-3) / 1e9
+def kernel_table(ih, grid, launches=40):
+    """ms per launch and algorithmic GB/s of the JOR sweep and of each kernel of
+    a BiCGstab iteration on a case of the given grid (random fields), from HIP
+    events around back-to-back launches on the case's stream."""
+    c = ih.Case(grid, BC, LEN, STRETCH, device=0)
+    rng = np.random.default_rng(1)
+    for f in (ih.PRE, ih.RES, ih.RR, ih.RH, ih.PK, ih.AP, ih.AR, ih.TMP4):
+        c.upload(f, np.ascontiguousarray(rng.standard_normal(c.shape)))
+    P = grid[0] * grid[1] * grid[2]
+    out = {}
+    for name, bpp in KERNEL_BYTES.items():
+        ms = c.bench_kernel(name, launches)
+        out[name] = {"ms": ms, "bytes": bpp * P, "GBs": bpp * P / (ms * 1e-3) / 1e9}
+    c.close()
+    return out
+
+
+def operator_roofline(table, grid, traffic=None):
+    """roofline of the fused Poisson operator k_apply in the two forms a BiCGstab
+    iteration launches (Ap with <rh, Ap>: 24 B/point; As with <s, As>, <As, As>:
+    16 B/point): their bytes over their summed durations."""
+    P = grid[0] * grid[1] * grid[2]
+    t = table["apply_w"]["ms"] + table["apply_self"]["ms"]
+    nbytes = (KERNEL_BYTES["apply_w"] + KERNEL_BYTES["apply_self"]) * P
+    achieved = nbytes / (t * 1e-3) / 1e9
     return {
-        "kernel": "k_apply<16, 16, 32, 0, false>",
+        "kernel": "k_apply<16, 16, 32, 1, false> + k_apply<16, 16, 32, 2, false> (the two operator launches "
+                  "of a BiCGstab iteration)",
         "bound": "hbm",
         "achieved": achieved,
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": achieved / HBM_PEAK_GBS,
-        "traffic": read_traffic("poisson_operator_256x128x128_counters.txt"),
-        "bytes_per_launch": nbytes,
-        "avg_launch_ms": ms,
-        "method": f"{launches} operator calls minus {launches} x 3 ghost fills, HIP events on the case stream",
+        "traffic": traffic,
+        "bytes_per_launch": nbytes // 2,
+        "avg_launch_ms": t / 2,
+        "method": "HIP events around back-to-back launches of each form on the case stream (ih_bench_kernel)",
     }
 
 
@@ -314,12 +327,36 @@ def ct_geometry(ih, launches=50):
             "forcing_points_per_s": (N[0] + 1) * N[1] * N[2] / (ms * 1e-3)}
 
 
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _oracle_tube_state(O, oc):
+    import ctypes
+    r_u = tube_radius(oc.coords(0, 1))
+    r_p = tube_radius(oc.coords(0, 0))
+    mask = oc.tube_mask(0.5, 0.5, r_u, r_p)
+    vel = oc.zeros(3)
+    vel[0] = mask[0].astype(np.float64)
+    return mask, np.ascontiguousarray(vel), oc.zeros(), oc.zeros(3)
+
+
 def cpu_baseline(iters_per_step):
-    """The oracle (the C restatement of the reference loops, point loops on
-    OpenMP threads, dot products serial) on a bounded sample at the full size:
-    200 JOR sweeps, 180 BiCGstab iterations and one step with the Poisson
-    solve cut to one iteration (about 10 s of work on 16 cores); the per-step
-    time is composed with the GPU run's iteration counts."""
+    """The oracle (the C restatement of the reference loops: point loops on
+    OpenMP threads, sums serial in the reference's order) timed as whole
+    projection steps on the host, on all cores and on one core:
+    - the bench case (257x129x129 stenosed tube): two whole steps, one with
+      BiCGstab cut at 2 and one at 12 iterations per substep (200 JOR sweeps
+      each); their difference gives the time of an iteration, and a step is
+      the 2-iteration step plus the GPU run's remaining iterations;
+    - the 65^3 channel (walls, parabolic inflow, N(0, 0.01) noise, Re 100,
+      dt 1e-3, 200 sweeps + BiCGstab to 1e-8): one whole step."""
     sys.path.insert(0, str(ROOT / "tests"))
     import ctypes
 
@@ -332,74 +369,84 @@ def cpu_baseline(iters_per_step):
         cores = os.cpu_count() or 1
     if os.environ.get("OMP_NUM_THREADS", "").isdigit():
         cores = min(cores, int(os.environ["OMP_NUM_THREADS"]))
+    out = {"cpu_model": _cpu_model()}
 
-    oc = O.OracleCase(N, BC, LEN, STRETCH)
-    oc.set_reduction_mode(0)  # the reference's sequential loop order
-    r_u = tube_radius(oc.coords(0, 1))
-    r_p = tube_radius(oc.coords(0, 0))
-    mask = oc.tube_mask(0.5, 0.5, r_u, r_p)
-    vel = oc.zeros(3)
-    vel[0] = mask[0].astype(np.float64)
-    vel = np.ascontiguousarray(vel)
-    pre, nl = oc.zeros(), oc.zeros(3)
-    rng = np.random.default_rng(0)
-    bb = np.ascontiguousarray(rng.standard_normal(oc.shape))
-    rel = oc.zeros()
-    t0 = time.perf_counter()
-    n_jor, n_bi = 200, 180
-    oc.jor(n_jor, JOR_OMEGA, oc.Sp, oc.Np, rel, bb, oc.zeros())
-    t_jor = (time.perf_counter() - t0) / n_jor
-    log(f"cpu: {t_jor * 1e3:.1f} ms per JOR sweep")
-    t0 = time.perf_counter()
-    st = oc.bicgstab(0.0, n_bi, oc.Sp, oc.Np, bb, rel)
-    t_bi = (time.perf_counter() - t0) / max(st.iterations, 1)
-    log(f"cpu: {t_bi * 1e3:.1f} ms per BiCGstab iteration (incl. start-up)")
-    po = O.StepParams(Re=RE, dtime=DT, jor_sweeps=0, jor_omega=JOR_OMEGA, bicg_rel_tol=TOL, bicg_max_it=1,
-                      level_pressure=1, inflow_kind=2, inflow_amp=1.0,
-                      mask3=mask.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)))
-    t0 = time.perf_counter()
-    oc.timestep(po, 0, 0.0, vel, pre, nl)
-    t_rest = time.perf_counter() - t0
-    log(f"cpu: {t_rest:.2f} s for a step with one BiCGstab iteration per substep")
-    step = t_rest + 3 * JOR_SWEEPS * t_jor + max(iters_per_step - 3, 0) * t_bi
-    sample = t_rest + n_jor * t_jor + st.iterations * t_bi
+    def tube_step(max_it):
+        oc = O.OracleCase(N, BC, LEN, STRETCH)
+        oc.set_reduction_mode(0)  # the reference's sequential loop order
+        mask, vel, pre, nl = _oracle_tube_state(O, oc)
+        po = O.StepParams(Re=RE, dtime=DT, jor_sweeps=JOR_SWEEPS, jor_omega=JOR_OMEGA, bicg_rel_tol=TOL,
+                          bicg_max_it=max_it, level_pressure=1, inflow_kind=2, inflow_amp=1.0,
+                          mask3=mask.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)))
+        t0 = time.perf_counter()
+        st = oc.timestep(po, 0, 0.0, vel, pre, nl)
+        return time.perf_counter() - t0, sum(st.bicg_iterations)
+
+    def channel_step():
+        oc = O.OracleCase((65, 65, 65), BC, (1.0, 1.0, 1.0), (0.0, 0.0, 0.0))
+        oc.set_reduction_mode(0)
+        rng = np.random.default_rng(2026)
+        yp, zp = oc.coords(1, 0), oc.coords(2, 0)
+        vel = oc.zeros(3)
+        vel[0] = (16 * yp[None, :, None] * (1 - yp[None, :, None]) * zp[:, None, None] * (1 - zp[:, None, None])
+                  * np.ones(oc.shape))
+        for q in range(3):
+            vel[q] += rng.normal(0.0, 0.01, size=oc.shape)
+        vel = np.ascontiguousarray(vel)
+        po = O.StepParams(Re=100.0, dtime=1e-3, jor_sweeps=200, jor_omega=0.8, bicg_rel_tol=1e-8,
+                          bicg_max_it=20000, level_pressure=1, inflow_kind=1, inflow_amp=1.0)
+        t0 = time.perf_counter()
+        st = oc.timestep(po, 0, 0.0, vel, oc.zeros(), oc.zeros(3))
+        return time.perf_counter() - t0, sum(st.bicg_iterations)
+
+    work = 0.0
+    for label, threads in (("all_cores", cores), ("one_core", 1)):
+        O.set_threads(threads)
+        t2, n2 = tube_step(2)
+        t12, n12 = tube_step(12)
+        t_it = (t12 - t2) / max(n12 - n2, 1)
+        step = t2 + max(iters_per_step - n2, 0) * t_it
+        tc, nc = channel_step()
+        work += t2 + t12 + tc
+        log(f"cpu ({threads} threads): tube steps {t2:.2f} s ({n2} its), {t12:.2f} s ({n12} its) -> "
+            f"{t_it * 1e3:.1f} ms an iteration; channel 65^3 step {tc:.2f} s ({nc} its)")
+        out[label] = {"threads": threads, "tube_steps_per_s": 1.0 / step, "tube_step_s": step,
+                      "tube_step_2_its_s": t2, "tube_step_12_its_s": t12, "tube_iteration_s": t_it,
+                      "channel65_steps_per_s": 1.0 / tc, "channel65_step_s": tc,
+                      "channel65_bicgstab_iterations": nc}
+    O.set_threads(cores)
+    a = out["all_cores"]
     return {
-        "value": 1.0 / step,
+        "value": a["tube_steps_per_s"],
         "unit": "time steps/s",
         "cores": cores,
         "kind": "port",
-        "sample": (f"oracle at 257x129x129 on {cores} threads: {n_jor} JOR sweeps ({t_jor * 1e3:.1f} ms each), "
-                   f"{st.iterations} BiCGstab iterations ({t_bi:.3f} s each), one step with 1 BiCGstab iteration per substep "
-                   f"({t_rest:.1f} s); step = that + 600 sweeps + {iters_per_step - 3} iterations "
-                   f"(the GPU run's count); {sample:.0f} s of CPU work"),
+        "sample": (f"oracle at 257x129x129 on {cores} threads: whole steps with 200 JOR sweeps and BiCGstab cut "
+                   f"at 2 and 12 iterations per substep ({a['tube_step_2_its_s']:.1f} s, "
+                   f"{a['tube_step_12_its_s']:.1f} s); a step = the first plus the GPU run's other "
+                   f"{iters_per_step - 6} iterations at {a['tube_iteration_s'] * 1e3:.0f} ms; one core and the 65^3 "
+                   f"channel under one_core / channel65_*; {work:.0f} s of CPU work"),
+        **out,
     }
 
 
-def kernel_only(ih, which, launches):
-    """Run `launches` launches of one kernel family on the bench grid and one
-    device-to-device copy of a whole field (fe doubles read and written), so
-    that per-dispatch hardware counters of the kernel can be scaled by those of
-    a copy of known size."""
-    c = ih.Case(N, BC, LEN, STRETCH, device=0)
+def kernel_only(ih, which, launches, grid=N):
+    """`launches` launches of one kernel (ih_bench_kernel) on a case of `grid`,
+    then one device-to-device copy of a whole field (fe doubles read and
+    written), so that per-dispatch hardware counters of the kernel can be
+    scaled by those of a copy of known size."""
+    c = ih.Case(grid, BC, LEN, STRETCH, device=0)
     rng = np.random.default_rng(1)
-    for f in (ih.PRE, ih.RES, ih.TMP1):
+    for f in (ih.PRE, ih.RES, ih.RR, ih.RH, ih.PK, ih.AP, ih.AR, ih.TMP1, ih.TMP4):
         c.upload(f, np.ascontiguousarray(rng.standard_normal(c.shape)))
-    if which == "jor":
-        c.jor(1, JOR_OMEGA, c.Sp, c.Np)  # the first sweep of a case also times its launch shapes
-    c.sync()
-    c.timer_start()
-    if which == "jor":
-        c.jor(launches, JOR_OMEGA, c.Sp, c.Np)
-    else:
-        for _ in range(launches):
-            c.product_div_grad(ih.TMP1, ih.TMP2)
-    ms = c.timer_stop()
-    check = c.lib.ih_field_copy(c.h, ih.TMP3, ih.TMP1)
-    ih.check(check, "field copy")
+    ms = c.bench_kernel(which, launches)
+    ih.check(c.lib.ih_field_copy(c.h, ih.TMP3, ih.TMP1), "field copy")
     c.sync()
     fe = int(np.prod(c.shape))
-    print(json.dumps({"kernel_only": which, "launches": launches, "ms_per_launch": ms / launches,
-                      "points": N[0] * N[1] * N[2], "copy_bytes_each_way": 8 * fe}), flush=True)
+    P = grid[0] * grid[1] * grid[2]
+    print(json.dumps({"kernel_only": which, "grid": list(grid), "launches": launches, "ms_per_launch": ms,
+                      "points": P, "algorithmic_bytes": KERNEL_BYTES.get(which, 16) * P,
+                      "copy_bytes_each_way": 8 * fe}), flush=True)
     c.close()
 
 
@@ -414,11 +461,14 @@ def main():
     ap.add_argument("--no-multigrid", action="store_true", help="skip the multigrid-preconditioned FP64 run")
     ap.add_argument("--no-ct", action="store_true", help="skip the CT-geometry load and forcing timing")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving trace")
+    ap.add_argument("--no-hbm-case", action="store_true", help="skip the kernel table at 385x193x193")
     ap.add_argument("--precond-sweeps", type=int, default=8,
                     help="FP32 JOR sweeps of the mixed-precision BiCGstab preconditioner")
-    ap.add_argument("--only", choices=("jor", "operator"), default=None,
-                    help="launch one kernel family on the bench grid (for counter profiles): JOR sweeps or the "
-                         "fused Poisson operator, followed by one plain device copy of a field for calibration")
+    ap.add_argument("--only", choices=("jor", "apply", "apply_w", "apply_self", "p_update", "s_update", "x_update"),
+                    default=None,
+                    help="launch one kernel on the bench grid (for counter profiles), followed by one plain device "
+                         "copy of a field for calibration")
+    ap.add_argument("--grid", type=int, nargs=3, default=list(N), help="grid of --only (default the bench grid)")
     ap.add_argument("--launches", type=int, default=20)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -430,7 +480,7 @@ def main():
     import impact_hip as ih
 
     if args.only:
-        return kernel_only(ih, args.only, args.launches)
+        return kernel_only(ih, args.only, args.launches, tuple(args.grid))
     log(f"bench: case {N}, walls, tanh {STRETCH[1]}, stenosed tube, Re {RE}, dt {DT}")
     c, p = setup_case(ih)
     log(f"bench: {c.bytes() / 2**30:.2f} GiB resident")
@@ -443,6 +493,9 @@ def main():
     jor_avg_ms = jor_ms / max(jor_n, 1)
     achieved = jor_bytes / (jor_avg_ms * 1e-3) / 1e9
     traffic = read_traffic()
+    log("kernel table ...")
+    table = kernel_table(ih, N)
+    it_ms = bi_ms / max(bi_n, 1)
     result = {
         "metric": METRIC,
         "value": value,
@@ -464,7 +517,8 @@ def main():
             "bicgstab_iterations_per_step": iters_per_step,
             "parallelism": "single GPU",
         },
-        "roofline": {
+        "roofline": operator_roofline(table, N, read_traffic("poisson_operator_256x128x128_counters.txt")),
+        "roofline_jor": {
             "kernel": "k_jor_vec<double, double>",
             "bound": "hbm",
             "achieved": achieved,
@@ -474,13 +528,26 @@ def main():
             "traffic": traffic,
             "bytes_per_launch": jor_bytes,
             "avg_launch_ms": jor_avg_ms,
+            "method": "HIP events on the case stream around the sweeps of the timed steps",
         },
+        "roofline_bicgstab_iteration": {
+            "kernels": "k_p_update, k_apply (Ap, <rh, Ap>), k_s_update_alpha, k_apply (As, dots), "
+                       "k_x_update_omega, k_ctl_end",
+            "bytes_per_iteration": ITERATION_BYTES * P,
+            "ms_per_iteration": it_ms,
+            "achieved": ITERATION_BYTES * P / (it_ms * 1e-3) / 1e9,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": ITERATION_BYTES * P / (it_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "method": "BiCGstab phase time of the timed steps (HIP events) over its iterations",
+        },
+        "kernels_257x129x129": table,
         "phases_ms_per_step": {
             "jor": jor_ms / args.steps,
             "bicgstab": bi_ms / args.steps,
             "other": ms_per_step - (jor_ms + bi_ms) / args.steps,
         },
-        "bicgstab_ms_per_iteration": bi_ms / max(bi_n, 1),
+        "bicgstab_ms_per_iteration": it_ms,
         "wall_s_timed": wall,
     }
     if not args.no_mixed:
@@ -545,8 +612,11 @@ def main():
     if not args.no_serve:
         log("serving trace ...")
         result["serving"] = serving(ih)
-    log("operator roofline ...")
-    result["roofline_bicgstab_operator"] = operator_roofline(ih, c)
+    if not args.no_hbm_case:
+        log(f"kernels at {HBM_CASE} ...")
+        t_big = kernel_table(ih, HBM_CASE)
+        result["kernels_385x193x193"] = t_big
+        result["roofline_385x193x193"] = operator_roofline(t_big, HBM_CASE)
     if not args.no_cpu_baseline and rank == 0:
         log("cpu baseline: timing the oracle on a bounded sample ...")
         result["cpu_baseline"] = cpu_baseline(iters_per_step)

@@ -7,6 +7,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <string>
 
 #include "ih_host.h"
@@ -317,6 +318,38 @@ double absmax_host(ih_case *c, const Box &b, const double *a, int *rc) {
     return c->hscal[S_MAX];
 }
 
+// set_pressure_BC_kernel over the points pressure_bc_launcher's grid reaches:
+// b = (N-1)/8 blocks of t = (N-1)/b threads per direction cover 1..b*t; a
+// direction without a block, or a block of more than 1024 threads, is a launch
+// that fails in the reference and writes nothing.  The plane is Fortran
+// k_start_index - 4 (storage offset k+4 compared with k_start_index - 1).
+int pressure_bc(ih_case *c, double *rel, int type1, int type2, int type3, double p_in, double p_out, double p_out2) {
+    int cover[3];
+    int64_t threads = 1;
+    for (int d = 0; d < 3; ++d) {
+        const int m = c->g.N[d] - 1;
+        const int blocks = m / 8;
+        if (blocks < 1) return IH_OK;
+        const int t = m / blocks;
+        threads *= t;
+        cover[d] = blocks * t;
+    }
+    if (threads > 1024) return IH_OK;
+    launch_set_pressure_bc(c->stream, c->g, cover, c->ct.k_start - 4, c->ctlab, type1, type2, type3, p_in, p_out,
+                           p_out2, rel);
+    LAUNCHCHK();
+    return IH_OK;
+}
+
+// device_jacobi_solver's substep-3 pressure (src/jacobi_solver_host_v4.f90:
+// 302-305): types 1, 1, 0, the pressures times (aRK + bRK) dtime
+int jacobi_pressure_bc(ih_case *c, double *rel, int substep, double dtime) {
+    if (!c->ctlab || substep != 3) return IH_OK;
+    const double f = aRK[substep - 1] + bRK[substep - 1];
+    const double *pr = c->ct.pres;
+    return pressure_bc(c, rel, 1, 1, 0, pr[0] * f * dtime, pr[1] * f * dtime, pr[2] * f * dtime);
+}
+
 void drop_mg(ih_case *c) {
     if (!c->mg && !c->mgmem && !c->mgldev && !c->mgmem32 && !c->mgldev32) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
@@ -331,11 +364,14 @@ void drop_mg(ih_case *c) {
     if (c->mgldev32) (void)hipFree(c->mgldev32);
     c->mgldev32 = nullptr;
     c->mgh.clear();
+    c->bytes -= c->mg_bytes;
+    c->mg_bytes = 0;
 }
 
 }  // namespace
 
 static int grid_from_desc(const ih_grid_desc *gd, HostGrid &hg);
+static int64_t mg_alloc_bytes(const std::vector<HostMgLevel> &mgh);
 
 // ===========================================================================
 extern "C" {
@@ -649,6 +685,17 @@ int64_t ih_grid_bytes(const ih_grid_desc *gd) {
     return IH_NFIELDS * 8 * fe + 3 * 4 * fe + 3 * fe + tables + parts + (1 << 20);
 }
 
+int64_t ih_grid_bytes_step(const ih_grid_desc *gd, const ih_step_params *p) {
+    const int64_t base = ih_grid_bytes(gd);
+    if (!base || !p || p->precond_poisson != 1 || p->precond_sweeps > 0) return base;
+    HostGrid hg;
+    if (grid_from_desc(gd, hg) != IH_OK) return 0;
+    host_coords(hg);
+    std::vector<HostMgLevel> mgh;
+    host_mg_levels(hg, 15, mgh);
+    return base + mg_alloc_bytes(mgh);
+}
+
 int ih_field_upload(ih_case *c, ih_field f, const double *host) {
     if (!c || !host || !valid_field(f)) return fail(IH_EINVAL, "bad argument");
     HIPCHK(hipSetDevice(c->device));
@@ -906,13 +953,11 @@ static JorLaunch jor_shape(ih_case *c, const Box &b, bool fp32, F sweep, int lev
 }
 }
 
-int ih_jor(ih_case *c, int n_sweeps, double omega, const int ss[3], const int nn[3], ih_field rel, ih_field bb) {
-    int rc = need_coeffs(c);
-    if (rc) return rc;
-    if (!ss || !nn || n_sweeps < 0 || !valid_field(rel) || !valid_field(bb) || rel == IH_COMP || bb == IH_COMP)
-        return fail(IH_EINVAL, "bad argument");
-    Box b;
-    if ((rc = to_box(c, ss, nn, b))) return rc;
+// n_sweeps JOR sweeps of rel on bb over box b, out of place through COMP;
+// pbc_substep 3 (with CT labels) applies device_jacobi_solver's Dirichlet
+// pressure before every sweep and once after the last.
+static int jor_run(ih_case *c, int n_sweeps, double omega, const Box &b, ih_field rel, ih_field bb, int pbc_substep,
+                   double pbc_dtime) {
     double *x = F(c, rel), *y = F(c, IH_COMP);
     const double *B = F(c, bb);
     const JorLaunch sh = n_sweeps > 0 ? jor_shape(c, b, false, [&](JorLaunch l) {
@@ -929,13 +974,17 @@ int ih_jor(ih_case *c, int n_sweeps, double omega, const int ss[3], const int nn
     for (int d = 0; d < 3; ++d) whole = whole && b.lo[d] == pb.lo[d] && b.hi[d] == pb.hi[d];
     if (n_sweeps > 1 && !whole)
         HIPCHK(hipMemcpyAsync(y, x, sizeof(double) * (size_t)c->g.fe, hipMemcpyDeviceToDevice, c->stream));
+    const bool with_bc = pbc_substep == 3 && c->ctlab;
+    int rc;
     PhaseTimer pt(c, "jor");
     for (int s = 0; s < n_sweeps; ++s) {
+        if (with_bc && (rc = jacobi_pressure_bc(c, x, pbc_substep, pbc_dtime))) return rc;
         launch_jor(c->stream, c->g, c->c, b, omega, x, B, y, sh);
         std::swap(x, y);
     }
     pt.done(n_sweeps);
     LAUNCHCHK();
+    if (with_bc && (rc = jacobi_pressure_bc(c, x, pbc_substep, pbc_dtime))) return rc;
     if (x != F(c, rel)) {
         launch_copy_box(c->stream, c->g, b, x, F(c, rel));
         LAUNCHCHK();
@@ -943,6 +992,16 @@ int ih_jor(ih_case *c, int n_sweeps, double omega, const int ss[3], const int nn
     launch_exchange_relax(c->stream, c->g, F(c, rel));
     LAUNCHCHK();
     return IH_OK;
+}
+
+int ih_jor(ih_case *c, int n_sweeps, double omega, const int ss[3], const int nn[3], ih_field rel, ih_field bb) {
+    int rc = need_coeffs(c);
+    if (rc) return rc;
+    if (!ss || !nn || n_sweeps < 0 || !valid_field(rel) || !valid_field(bb) || rel == IH_COMP || bb == IH_COMP)
+        return fail(IH_EINVAL, "bad argument");
+    Box b;
+    if ((rc = to_box(c, ss, nn, b))) return rc;
+    return jor_run(c, n_sweeps, omega, b, rel, bb, 0, 0.0);
 }
 
 int ih_residual_cdg(ih_case *c, const int ss[3], const int nn[3], ih_field rel, ih_field bb, double *out) {
@@ -968,15 +1027,20 @@ int ih_residual_cdg(ih_case *c, const int ss[3], const int nn[3], ih_field rel, 
 // iterations and reads the flags once per batch.
 static int mg_setup(ih_case *c);
 
-// mg: 0 no multigrid, 1 the FP64 cycle, 2 the FP32 cycle as M
+// mg: 0 no multigrid, 1 the FP64 cycle, 2 the FP32 cycle as M; sweeps > 0: M =
+// that many JOR sweeps from zero, in FP32, or in FP64 when fp64_sweeps
 static int bicgstab_run(ih_case *c, double eps, int n_it_max, const int ss[3], const int nn[3], ih_field bbf,
-                        ih_field phif, int sweeps, double omega, int mg, ih_bicgstab_stats *st) {
+                        ih_field phif, int sweeps, double omega, int mg, ih_bicgstab_stats *st,
+                        bool fp64_sweeps = false) {
     int rc = need_coeffs(c);
     if (rc) return rc;
     if (!ss || !nn || !valid_field(bbf) || !valid_field(phif) || n_it_max < 0 || sweeps < 0)
         return fail(IH_EINVAL, "bad argument");
     for (int f : {IH_RR, IH_RH, IH_PK, IH_AP, IH_AR, IH_DIG, IH_Z1, IH_Z2})
         if (bbf == f || phif == f) return fail(IH_EINVAL, "bb/phi may not be a BiCGstab work field");
+    if (fp64_sweeps)
+        for (int f : {IH_TMP3, IH_TMP4})
+            if (bbf == f || phif == f) return fail(IH_EINVAL, "bb/phi may not be the FP64 preconditioner's work fields");
     Box b;
     if ((rc = to_box(c, ss, nn, b))) return rc;
     const Box pb = pbox(c->g);
@@ -990,7 +1054,22 @@ static int bicgstab_run(ih_case *c, double eps, int n_it_max, const int ss[3], c
                  c->ksc, c->kfl};
     k.part3 = c->part3;
     k.part4 = c->part4;
-    if (sweeps > 0) {
+    if (sweeps > 0 && fp64_sweeps) {
+        k.z1 = F(c, IH_Z1);
+        k.z2 = F(c, IH_Z2);
+        k.sweeps64 = sweeps;
+        k.omega64 = omega;
+        k.e64a = F(c, IH_TMP3);
+        k.e64b = F(c, IH_TMP4);
+        k.zero64 = F(c, IH_DIG);
+        k.shape64 = jor_shape(c, b, false, [&](JorLaunch l) {
+            launch_jor(s, c->g, c->c, b, omega, F(c, IH_DIG), F(c, IH_TMP3), F(c, IH_TMP4), l);
+        });
+        // the zero start and finite ghost layers of the ping-pong fields
+        launch_fill(s, F(c, IH_DIG), 0.0, c->g.fe);
+        launch_fill(s, k.e64a, 0.0, c->g.fe);
+        launch_fill(s, k.e64b, 0.0, c->g.fe);
+    } else if (sweeps > 0) {
         k.z1 = F(c, IH_Z1);
         k.z2 = F(c, IH_Z2);
         k.e0 = c->f32[0];
@@ -1067,10 +1146,35 @@ int ih_bicgstab_mixed(ih_case *c, double eps, int n_it_max, const int ss[3], con
     return bicgstab_run(c, eps, n_it_max, ss, nn, bbf, phif, sweeps, omega, 0, st);
 }
 
+int ih_bicgstab_jor64(ih_case *c, double eps, int n_it_max, const int ss[3], const int nn[3], ih_field bbf,
+                      ih_field phif, int sweeps, double omega, ih_bicgstab_stats *st) {
+    if (sweeps < 1) return fail(IH_EINVAL, "the JOR preconditioner needs at least one sweep");
+    return bicgstab_run(c, eps, n_it_max, ss, nn, bbf, phif, sweeps, omega, 0, st, true);
+}
+
 // ---------------------------------------------------------------------------
 // Multigrid V cycle (preconditioner 1): levels, arrays and tables on the device
 // ---------------------------------------------------------------------------
 static void mg_free(ih_case *c) { drop_mg(c); }
+
+// Device bytes mg_setup allocates for the levels of mgh: the FP64 level arrays
+// (2 fields on level 1, 4 below), the FP64 tables (cdg, cR: 3 taps, cI: 2 taps
+// per direction) and 64 zeros, the FP32 arrays (4 per level) and tables, and
+// the two descriptor arrays.
+static int64_t mg_alloc_bytes(const std::vector<HostMgLevel> &mgh) {
+    const int G = (int)mgh.size();
+    int64_t arr = 0, tab = 0, arr32 = 0;
+    for (int g = 1; g <= G; ++g) {
+        const int *N = mgh[g - 1].N;
+        const int64_t px = ((N[0] + 7) + 15) / 16 * 16;
+        const int64_t fe = ((12 + px * (N[1] + 7) * (N[2] + 7)) + 63) / 64 * 64;
+        arr += (g >= 2 ? 4 : 2) * fe;
+        arr32 += 4 * fe;
+        for (int d = 0; d < 3; ++d) tab += 8 * (int64_t)(((N[d] + 1) + 7) / 8 * 8);
+    }
+    return (int64_t)sizeof(double) * (arr + tab + 64) + (int64_t)sizeof(float) * (arr32 + tab) +
+           (int64_t)(G + 1) * (int64_t)(sizeof(MgLevelDev<float>) + sizeof(MgLevelDev<double>));
+}
 
 static int mg_setup(ih_case *c) {
     if (c->mg) return IH_OK;
@@ -1273,6 +1377,8 @@ static int mg_setup(ih_case *c) {
     HIPCHK(hipMemcpyAsync(c->mgldev, ld.data(), sizeof(MgLevelDev<double>) * ld.size(), hipMemcpyHostToDevice,
                           c->stream));
     m->Ldev = (const MgLevelDev<double> *)c->mgldev;
+    c->mg_bytes = mg_alloc_bytes(c->mgh);
+    c->bytes += c->mg_bytes;
     // launch shape of each level's sweep (a sweep is a pure function of its inputs)
     for (int g = 1; g < std::min(m->g0, G + 1); ++g) {
         MgLevel &L = m->L[g];
@@ -1445,7 +1551,7 @@ int ih_ct_load_vtk(ih_case *c, const char *path, int x_offset, int y_offset, int
         return fail(IH_EINVAL, "the image with its offsets and the 10-plane inlet and outlet buffers must fit the grid");
     double ref[3];
     std::vector<double> tconv, qconv;
-    waveform_convert(wave_t_ms, wave_q, npts, M1, tconv, qconv, ref);
+    waveform_convert(wave_t_ms, wave_q, npts, M1, c->ct.idealized != 0, tconv, qconv, ref);
     std::vector<uint8_t> lab((size_t)g.fe, 0);
     auto L = [&](int i, int j, int k) -> uint8_t & { return lab[(size_t)gidx(g, i, j, k)]; };
     for (int k = 1; k <= M3; ++k)
@@ -1511,13 +1617,36 @@ int ih_ct_load_vtk(ih_case *c, const char *path, int x_offset, int y_offset, int
         for (int k = -10; k <= -1; ++k) L(out[2 * q], out[2 * q + 1], ks + k) = 9;
     for (int q = 0; q < no2; ++q)
         for (int k = 1; k <= 10; ++k) L(out2[2 * q], out2[2 * q + 1], ks2 + k) = 11;
-    if (!c->ctlab && hipMalloc(&c->ctlab, (size_t)g.fe) != hipSuccess) {
-        c->ctlab = nullptr;
-        return fail(IH_ENOMEM, "out of device memory for the CT labels");
+    if (!c->ctlab) {
+        if (hipMalloc(&c->ctlab, (size_t)g.fe) != hipSuccess) {
+            c->ctlab = nullptr;
+            return fail(IH_ENOMEM, "out of device memory for the CT labels");
+        }
+        c->bytes += g.fe;
     }
     HIPCHK(hipMemcpy(c->ctlab, lab.data(), lab.size(), hipMemcpyHostToDevice));
-    c->ct_t = tconv;
-    c->ct_q = qconv;
+    CtState &ct = c->ct;
+    ct.t_ms.assign(wave_t_ms, wave_t_ms + npts);
+    ct.q_raw.assign(wave_q, wave_q + npts);
+    ct.t_conv = tconv;
+    ct.q_conv = qconv;
+    for (int d = 0; d < 3; ++d) ct.ref[d] = ref[d];
+    ct.inlet_voxels = jet;
+    ct.k_start = ks;
+    // labels present in the forcing's range (the component-1 velocity range;
+    // points outside 1..N read label 2): the idealized branch evaluates a
+    // Windkessel model only where a point of its label lies in that range
+    ct.has5 = ct.has9 = ct.has11 = false;
+    for (int k = g.SB[0][2]; k <= g.NB[0][2]; ++k)
+        for (int j = g.SB[0][1]; j <= g.NB[0][1]; ++j)
+            for (int i = g.SB[0][0]; i <= g.NB[0][0]; ++i) {
+                const bool inside = i >= 1 && i <= M1 && j >= 1 && j <= M2 && k >= 1 && k <= M3;
+                const int Lb = inside ? L(i, j, k) : 2;
+                ct.has5 = ct.has5 || Lb == 5;
+                ct.has9 = ct.has9 || Lb == 9;
+                ct.has11 = ct.has11 || Lb == 11;
+            }
+    for (int q = 0; q < 3; ++q) ct.wk_state[q] = ct.pres[q] = 0.0;
     if (info) {
         info->inlet_cells = ni;
         info->outlet_cells = no;
@@ -1552,20 +1681,75 @@ int ih_ct_get_labels(ih_case *c, uint8_t *host) {
 
 int ih_ct_inflow(ih_case *c, double time, double *q, int *found) {
     if (!c || !q || !found) return fail(IH_EINVAL, "null argument");
-    if (c->ct_t.size() < 2) return fail(IH_ESTATE, "no waveform loaded");
+    if (c->ct.t_conv.size() < 2) return fail(IH_ESTATE, "no waveform loaded");
     bool f = false;
-    *q = waveform_inflow(c->ct_t, c->ct_q, time, f);
+    *q = waveform_inflow(c->ct.t_conv, c->ct.q_conv, time, f);
     *found = f ? 1 : 0;
+    return IH_OK;
+}
+
+int ih_ct_configure(ih_case *c, int ct_based, int idealized) {
+    if (!c || (ct_based != 0 && ct_based != 1) || (idealized != 0 && idealized != 1))
+        return fail(IH_EINVAL, "ct_based and idealized are 0 or 1");
+    c->ct.ct_based = ct_based;
+    c->ct.idealized = idealized;
     return IH_OK;
 }
 
 int ih_ct_forcing(ih_case *c, double time) {
     if (!c) return fail(IH_EINVAL, "null case");
-    if (!c->ctlab || c->ct_t.size() < 2) return fail(IH_ESTATE, "no CT labels loaded");
+    CtState &ct = c->ct;
+    if (!c->ctlab || ct.t_conv.size() < 2) return fail(IH_ESTATE, "no CT labels loaded");
     bool found = false;
-    const double q = waveform_inflow(c->ct_t, c->ct_q, time, found);
-    launch_ct_forcing(c->stream, c->g, c->ctlab, q, found ? 1 : 0, F(c, IH_VEL1), F(c, IH_VEL2), F(c, IH_VEL3));
+    double q = 0.0, w_ideal = 0.0;
+    if (ct.ct_based) {
+        wk_echo(ct.t_ms, ct.q_raw, ct.t_conv, time, ct.ref[0], ct.pres[0], ct.pres[1]);
+        q = waveform_inflow(ct.t_conv, ct.q_conv, time, found);
+    }
+    if (ct.idealized) {
+        double q_inlet = 0.0, q_other = 0.0;
+        if (ct.has5) wk_pressure_flow(1, time, ct.ref, ct.wk_state, ct.pres[0], q_inlet);
+        if (ct.has9) wk_pressure_flow(2, time, ct.ref, ct.wk_state, ct.pres[1], q_other);
+        if (ct.has11) wk_pressure_flow(3, time, ct.ref, ct.wk_state, ct.pres[2], q_other);
+        const double h = c->hg.L[0] / (double)(c->g.N[0] - 1);
+        w_ideal = q_inlet / ((double)ct.inlet_voxels * (h * h));
+    }
+    if (!ct.ct_based && !ct.idealized) return IH_OK;
+    launch_ct_forcing(c->stream, c->g, c->ctlab, q, found ? 1 : 0, ct.idealized, w_ideal, F(c, IH_VEL1),
+                      F(c, IH_VEL2), F(c, IH_VEL3));
     LAUNCHCHK();
+    return IH_OK;
+}
+
+int ih_ct_get_state(ih_case *c, double pres[3], double wk[3]) {
+    if (!c || !pres || !wk) return fail(IH_EINVAL, "null argument");
+    for (int q = 0; q < 3; ++q) {
+        pres[q] = c->ct.pres[q];
+        wk[q] = c->ct.wk_state[q];
+    }
+    return IH_OK;
+}
+
+int ih_ct_set_state(ih_case *c, const double pres[3], const double wk[3]) {
+    if (!c || !pres || !wk) return fail(IH_EINVAL, "null argument");
+    for (int q = 0; q < 3; ++q) {
+        c->ct.pres[q] = pres[q];
+        c->ct.wk_state[q] = wk[q];
+    }
+    return IH_OK;
+}
+
+int ih_set_pressure_bc(ih_case *c, ih_field rel, int type1, int type2, int type3, double p_in, double p_out,
+                       double p_out2) {
+    if (!c || !valid_field(rel)) return fail(IH_EINVAL, "bad argument");
+    if (!c->ctlab) return fail(IH_ESTATE, "no CT labels loaded");
+    return pressure_bc(c, F(c, rel), type1, type2, type3, p_in, p_out, p_out2);
+}
+
+int ih_case_set_substep(ih_case *c, int substep, double dtime) {
+    if (!c || substep < 0 || substep > 3) return fail(IH_EINVAL, "substep is 0..3");
+    c->ct.substep = substep;
+    c->ct.dtime = dtime;
     return IH_OK;
 }
 
@@ -1581,7 +1765,7 @@ int ih_apply_mask(ih_case *c) {
 // (src/mod_rhs.f90:35-241) and explicit (src/mod_solvers.f90:64-346); after
 // the step the mask (apply_CT_forcing pattern) and level_pressure.
 // ---------------------------------------------------------------------------
-static int poisson_solve(ih_case *c, const ih_step_params *p, const Box &b, ih_bicgstab_stats *bst) {
+static int poisson_solve(ih_case *c, const ih_step_params *p, const Box &b, int substep, ih_bicgstab_stats *bst) {
     int rc;
     const int ss[3] = {b.lo[0], b.lo[1], b.lo[2]};
     const int nn[3] = {b.hi[0], b.hi[1], b.hi[2]};
@@ -1611,7 +1795,7 @@ static int poisson_solve(ih_case *c, const ih_step_params *p, const Box &b, ih_b
             launch_exchange_relax(c->stream, g, F(c, IH_PRE));
             LAUNCHCHK();
         } else {
-            rc = ih_jor(c, p->jor_sweeps, p->jor_omega, ss, nn, IH_PRE, IH_RES);
+            rc = jor_run(c, p->jor_sweeps, p->jor_omega, b, IH_PRE, IH_RES, p->pressure_bc ? substep : 0, p->dtime);
             if (rc) return rc;
         }
     }
@@ -1626,7 +1810,11 @@ static int poisson_solve(ih_case *c, const ih_step_params *p, const Box &b, ih_b
         HIPCHK(hipMemcpyAsync(F(c, IH_COMP), F(c, IH_PRE), sizeof(double) * (size_t)c->g.fe, hipMemcpyDeviceToDevice,
                               c->stream));
         ih_bicgstab_stats mst{};
-        rc = ih_bicgstab_mixed(c, eps, p->bicg_max_it, ss, nn, IH_RES, IH_PRE, p->precond_sweeps, p->jor_omega, &mst);
+        rc = p->precond_fp64
+                 ? ih_bicgstab_jor64(c, eps, p->bicg_max_it, ss, nn, IH_RES, IH_PRE, p->precond_sweeps, p->jor_omega,
+                                     &mst)
+                 : ih_bicgstab_mixed(c, eps, p->bicg_max_it, ss, nn, IH_RES, IH_PRE, p->precond_sweeps, p->jor_omega,
+                                     &mst);
         if (!rc && mst.exit_reason != 0) {
             HIPCHK(hipMemcpyAsync(F(c, IH_PRE), F(c, IH_COMP), sizeof(double) * (size_t)c->g.fe,
                                   hipMemcpyDeviceToDevice, c->stream));
@@ -1661,6 +1849,9 @@ int ih_timestep(ih_case *c, const ih_step_params *p, double time, ih_step_stats 
     if (rc) return rc;
     if (!p || !(p->dtime > 0.0) || !(p->Re > 0.0)) return fail(IH_EINVAL, "bad step parameters");
     if (p->inflow_kind == 2 && !c->has_mask) return fail(IH_ESTATE, "pulsatile inflow needs a mask");
+    if (p->pressure_bc && (!c->ctlab || p->jor_fp32))
+        return fail(p->jor_fp32 ? IH_EINVAL : IH_ESTATE,
+                    "the Dirichlet pressure of the Jacobi sweeps needs CT labels and FP64 sweeps");
     HIPCHK(hipSetDevice(c->device));
     const Geo &g = c->g;
     hipStream_t s = c->stream;
@@ -1709,7 +1900,7 @@ int ih_timestep(ih_case *c, const ih_step_params *p, double time, ih_step_stats 
         for (int q = 0; q < 3; ++q) std::swap(c->fields[IH_VEL1 + q], c->fields[IH_RHS1 + q]);
         if ((rc = ih_divergence2(c, IH_VEL1, IH_VEL2, IH_VEL3, IH_RES))) return rc;
         ih_bicgstab_stats bst{};
-        if ((rc = poisson_solve(c, p, pb, &bst))) return rc;
+        if ((rc = poisson_solve(c, p, pb, sub + 1, &bst))) return rc;
         if (st) {
             st->bicg_iterations[sub] = bst.iterations;
             st->bicg_final[sub] = bst.norm_inf_final;
@@ -1808,44 +1999,124 @@ int ih_device_interpolate2(ih_case *c, int dimension_id, double *phi, double *in
     return ih_field_download(c, IH_TMP2, inter);
 }
 
-// device_jacobi_solver (src/jacobi_solver_host_v4.f90:5-717): JOR sweeps with
-// the reference's stopping test every 20 sweeps (:308-414); grid_id must be 1.
+// device_jacobi_solver (src/jacobi_solver_host_v4.f90:5-717) in the CPU
+// relaxation's out-of-place form.  Each trip applies the substep-3 Dirichlet
+// pressure (:302-305); every 20th trip (iter mod 20 == 1) compares err_tot =
+// sum |rel - rel_old| over the box (fixed reduction tree) with the previous
+// value and stops when |log(err_tot_old / err_tot)| is within the threshold
+// (:357-414); otherwise rel_old = rel and one sweep follows.  rel_old and
+// err_tot_old are undefined at the reference's entry, so its first test
+// compares with undefined values; here the first test only records them.  A
+// later err_tot of 0 (a fixed point, where the reference's log is undefined and
+// it never stops) stops; so does the sweep limit, with IH_ENOTCONV.  grid_id
+// must be 1.
 int ih_device_jacobi_solver(ih_case *c, int grid_id, double omega, int SS1, int SS2, int SS3, int NN1, int NN2,
                             int NN3, double *rel, double *bb) {
     int rc = need_coeffs(c);
     if (rc) return rc;
     if (grid_id != 1) return fail(IH_EINVAL, "only the finest grid (grid_id 1) is supported");
-    if ((rc = ih_field_upload(c, IH_TMP1, rel))) return rc;
-    if ((rc = ih_field_upload(c, IH_TMP2, bb))) return rc;
+    if (!rel || !bb) return fail(IH_EINVAL, "null argument");
     const int ss[3] = {SS1, SS2, SS3}, nn[3] = {NN1, NN2, NN3};
     Box b;
     if ((rc = to_box(c, ss, nn, b))) return rc;
+    if ((rc = ih_field_upload(c, IH_TMP1, rel))) return rc;
+    if ((rc = ih_field_upload(c, IH_TMP2, bb))) return rc;
     const int mode_j = 20;
     const double thresh = 20. * mode_j * 8. * 1. / 3.0 * 0.8 * 3.1415 * 3.1415 *
                           (1. / (double)(c->g.N[0] * c->g.N[0]) + 1. / (double)(c->g.N[1] * c->g.N[1]) +
                            1. / (double)(c->g.N[2] * c->g.N[2]));
+    const int substep = c->ct.substep;
+    const double dtime = c->ct.dtime;
     double err_tot_old = 0.0;
     launch_fill(c->stream, F(c, IH_TMP3), 0.0, c->g.fe);
+    const int64_t nb = red_blocks(b);
+    if ((rc = ensure_part(c, nb))) return rc;
     const int max_iter = 200000;
+    bool stopped = false;
     for (int iter = 1; iter <= max_iter; ++iter) {
+        if ((rc = jacobi_pressure_bc(c, F(c, IH_TMP1), substep, dtime))) return rc;
         if (iter % mode_j == 1) {
-            // err_tot = sum |rel - rel_old| (canonical tree), stop on a stalled log-ratio
-            const int64_t nb = red_blocks(b);
-            if ((rc = ensure_part(c, nb))) return rc;
             launch_absdiff_partial(c->stream, c->g, b, F(c, IH_TMP1), F(c, IH_TMP3), c->part);
             launch_final_sum(c->stream, c->part, nb, c->dscal + S_SUM);
             LAUNCHCHK();
             if ((rc = read_scalars(c, S_SUM, 1))) return rc;
             const double err_tot = c->hscal[S_SUM];
-            const double rel_err_tot = std::fabs(std::log(err_tot_old / err_tot));
-            if (rel_err_tot <= thresh) break;
+            if (iter > 1 && (err_tot == 0.0 || std::fabs(std::log(err_tot_old / err_tot)) <= thresh)) {
+                stopped = true;
+                break;
+            }
             HIPCHK(hipMemcpyAsync(F(c, IH_TMP3), F(c, IH_TMP1), sizeof(double) * (size_t)c->g.fe,
                                   hipMemcpyDeviceToDevice, c->stream));
             err_tot_old = err_tot;
         }
-        if ((rc = ih_jor(c, 1, omega, ss, nn, IH_TMP1, IH_TMP2))) return rc;
+        if ((rc = jor_run(c, 1, omega, b, IH_TMP1, IH_TMP2, 0, 0.0))) return rc;
     }
-    return ih_field_download(c, IH_TMP1, rel);
+    if (!stopped && (rc = jacobi_pressure_bc(c, F(c, IH_TMP1), substep, dtime))) return rc;
+    launch_exchange_relax(c->stream, c->g, F(c, IH_TMP1));
+    LAUNCHCHK();
+    if ((rc = ih_field_download(c, IH_TMP1, rel))) return rc;
+    if (!stopped) return fail(IH_ENOTCONV, "device_jacobi_solver: no stop after %d sweeps", max_iter);
+    return IH_OK;
+}
+
+// Launches of one kernel of the step, back to back between two events on the
+// case's stream, for benchmarks and counter profiles.  The BiCGstab kernels run
+// on the case's Krylov fields with the iteration flagged active (and scalars
+// of 0.5); their results are meaningless.
+int ih_bench_kernel(ih_case *c, const char *name, int launches, double *ms_per_launch) {
+    int rc = need_coeffs(c);
+    if (rc) return rc;
+    if (!name || launches < 1 || !ms_per_launch) return fail(IH_EINVAL, "bad argument");
+    HIPCHK(hipSetDevice(c->device));
+    const Box b = pbox(c->g);
+    hipStream_t s = c->stream;
+    const std::string n(name);
+    static const std::pair<const char *, int> parts[] = {{"p_update", KPART_P_UPDATE}, {"apply_w", KPART_APPLY_W},
+                                                         {"s_update", KPART_S_UPDATE}, {"apply_self", KPART_APPLY_SELF},
+                                                         {"x_update", KPART_X_UPDATE}};
+    int part = -1;
+    for (const auto &pp : parts)
+        if (n == pp.first) part = pp.second;
+    std::function<void()> one;
+    KrylovPtrs k{F(c, IH_RR), F(c, IH_RH), F(c, IH_PK), F(c, IH_AP), F(c, IH_AR), F(c, IH_TMP4), c->part, c->part2,
+                 c->ksc, c->kfl};
+    k.part3 = c->part3;
+    k.part4 = c->part4;
+    if (n == "jor") {
+        const JorLaunch sh = jor_shape(c, b, false, [&](JorLaunch l) {
+            launch_jor(s, c->g, c->c, b, 0.8, F(c, IH_PRE), F(c, IH_RES), F(c, IH_COMP), l);
+        });
+        one = [&]() { launch_jor(s, c->g, c->c, b, 0.8, F(c, IH_PRE), F(c, IH_RES), F(c, IH_COMP), sh); };
+    } else if (n == "apply") {
+        one = [&]() {
+            launch_apply_A(s, c->g, c->c, b, F(c, IH_PK), F(c, IH_AP), nullptr, KDOT_NONE, nullptr, nullptr, nullptr);
+        };
+    } else if (part >= 0) {
+        if ((rc = ensure_part(c, red_blocks(b)))) return rc;
+        for (int q = 0; q < KS_NSLOTS; ++q) c->hksc[q] = 0.5;
+        for (int q = 0; q < KF_NSLOTS; ++q) c->hkfl[q] = 0;
+        c->hkfl[KF_ACTIVE] = 1;
+        c->hkfl[KF_MODE] = 1;
+        HIPCHK(hipMemcpyAsync(c->ksc, c->hksc, sizeof(double) * KS_NSLOTS, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(c->kfl, c->hkfl, sizeof(int) * KF_NSLOTS, hipMemcpyHostToDevice, s));
+        // the update kernels reduce the partials of the operator form before them
+        if (part == KPART_S_UPDATE) launch_krylov_part(s, c->g, c->c, b, k, KPART_APPLY_W);
+        if (part == KPART_X_UPDATE) launch_krylov_part(s, c->g, c->c, b, k, KPART_APPLY_SELF);
+        one = [&]() { launch_krylov_part(s, c->g, c->c, b, k, part); };
+    } else {
+        return fail(IH_EINVAL, "unknown kernel '%s' (jor, apply, apply_w, apply_self, p_update, s_update, x_update)",
+                    name);
+    }
+    one();  // warm
+    HIPCHK(hipEventRecord(c->ev0, s));
+    for (int q = 0; q < launches; ++q) one();
+    HIPCHK(hipEventRecord(c->ev1, s));
+    LAUNCHCHK();
+    HIPCHK(hipEventSynchronize(c->ev1));
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    *ms_per_launch = (double)ms / launches;
+    return IH_OK;
 }
 
 // ---------------------------------------------------------------------------

@@ -2,7 +2,9 @@
 // outlet detection, and the echo waveform in reference units.
 //
 //   read_vtk_3D       src/usr_func.f90:187-555
-//   convert_waveform  src/usr_func.f90:599-651 (CT-based branch)
+//   convert_waveform  src/usr_func.f90:599-651 (both branches' scales)
+//   get_pressure_waveform_echo_WKM  src/usr_func.f90:654-750
+//   get_pressure_and_flow_rate_WKM  src/usr_func.f90:762-855
 //   get_pulse_number_and_length, get_shifted_time_in_pulse, the waveform
 //   lookup of apply_CT_forcing  src/usr_func.f90:753-760, 893-903, 930-937
 //
@@ -81,14 +83,19 @@ bool vtk_read_ascii(const char *path, int dims[3], std::vector<double> &values, 
     return true;
 }
 
-void waveform_convert(const double *t_ms, const double *q, int npts, int M1, std::vector<double> &t_out,
-                      std::vector<double> &q_out, double ref[3]) {
-    // CT pixel width 0.0004 m, upsampling 1, peak inflow 1 in reference units
+void waveform_convert(const double *t_ms, const double *q, int npts, int M1, bool idealized,
+                      std::vector<double> &t_out, std::vector<double> &q_out, double ref[3]) {
+    // CT pixel width 0.0004 m, upsampling 1, peak inflow 1 in reference units;
+    // the idealized aorta overrides the scales with 0.2 m and 2.7 m/s (:637-640)
     const double pixel = 0.0004, pixel_star = (1.0 / M1) * 1.0;
     double vmax = q[0];
     for (int i = 1; i < npts; ++i) vmax = q[i] > vmax ? q[i] : vmax;
-    const double velocity = vmax / 1.0;
-    const double length = pixel / pixel_star;
+    double velocity = vmax / 1.0;
+    double length = pixel / pixel_star;
+    if (idealized) {
+        length = 0.2;
+        velocity = 2.7;
+    }
     const double time = length / velocity;
     t_out.resize((size_t)npts);
     q_out.resize((size_t)npts);
@@ -116,6 +123,85 @@ double waveform_inflow(const std::vector<double> &t, const std::vector<double> &
             found = true;
         }
     return v;
+}
+
+namespace {
+constexpr double kBloodDensity = 1060.;  // src/mod_vars.f90:236
+}
+
+// get_pressure_and_flow_rate_WKM: the reference's closed form of a two-element
+// Windkessel under a half-sine systolic inflow (T_beat 1 s, systole 0.4 s,
+// 80 mmHg at the start), with its constants per branch (1 inflow, 2 outflow,
+// 3 the supra-aortic outflow).  The physiological time is time x reference
+// time, not wrapped into a beat.  state[branch-1] (p_end_systole_*) is written
+// in systole and read in diastole.  p and q come back nondimensional.  The
+// expressions keep the reference's operation order; the sine term of the
+// systolic pressure is the reference's, which lacks the Q R T_systole factor
+// of the ODE's particular solution.
+void wk_pressure_flow(int branch, double time, const double ref[3], double state[3], double &p_out, double &q_out) {
+    const double pi = 4. * std::atan(1.);
+    const double T_systole = 0.4 * 1.;
+    const double p_diastole = 80.;
+    double C, R, Qp;
+    if (branch == 1) {
+        C = 1.5496 / 1000000.;
+        R = 0.9758 * 1000000.;
+        Qp = 0.0004;
+    } else if (branch == 2) {
+        C = 1.5 / 1000000.;
+        R = 1.12 * 1000000.;
+        Qp = 0.0004 * 0.85;
+    } else {
+        C = 0.059 / 1000000.;
+        R = 19.68 * 1000000.;
+        Qp = 0.0004 * 0.05;
+    }
+    const double tp = time * ref[2];
+    double p, q;
+    if (tp <= T_systole) {
+        const double den = std::pow(T_systole, 2.) + std::pow(C, 2.) * std::pow(pi, 2.) * std::pow(R, 2.);
+        const double homog = p_diastole + Qp * T_systole * R * C * pi * R / den;
+        const double forced = T_systole * Qp * R * (C * pi * R * std::cos(pi * tp / T_systole)) -
+                              T_systole * std::sin(pi * tp / T_systole);
+        p = homog * std::exp(-(1. * tp) / (R * C)) - forced / den;
+        q = Qp * std::sin(pi * (tp) / T_systole);
+        state[branch - 1] = p;
+    } else {
+        p = state[branch - 1] * std::exp(-(1. * (tp - T_systole)) / (R * C));
+        q = 0.;
+    }
+    p_out = p / (kBloodDensity * std::pow(ref[0], 2.)) * 133.322;
+    q_out = q / (std::pow(ref[1], 3.) / ref[2]);
+}
+
+// get_pressure_waveform_echo_WKM: from 80 mmHg, the step p (1 - dt/(C R)) -
+// dt/C q_in for each waveform interval whose converted times hold the
+// pulse-shifted time (dt the interval in seconds, q_in the interval's start
+// velocity times q_max / max velocity), for the inlet and the outlet models;
+// nondimensional.
+void wk_echo(const std::vector<double> &t_ms, const std::vector<double> &q_raw, const std::vector<double> &t_conv,
+             double time, double reference_velocity, double &pre_inlet, double &pre_outlet) {
+    const size_t n = t_conv.size();
+    const double pulse = t_conv[n - 1];
+    const double number = std::floor(time / pulse) + 1.0;
+    const double shifted = time - (number - 1.0) * pulse;
+    double vel_max = q_raw[0];
+    for (size_t i = 1; i < n; ++i) vel_max = q_raw[i] > vel_max ? q_raw[i] : vel_max;
+    for (int side = 0; side < 2; ++side) {
+        const double R = side == 0 ? 0.78 * 1000000. : 0.898 * 1000000.;
+        const double C = side == 0 ? 2.166 / 1000000. : 2.066 / 1000000.;
+        const double q_max = side == 0 ? 0.0004 * 1.0 : 0.0004 * 0.85 * 1.0;
+        const double area = q_max / vel_max;
+        double pre = 80.;
+        for (size_t i = 0; i + 1 < n; ++i)
+            if (shifted >= t_conv[i] && shifted <= t_conv[i + 1]) {
+                const double dt = t_ms[i + 1] / 1000. - t_ms[i] / 1000.;
+                const double q_in = q_raw[i] * area;
+                pre = pre * (1. - dt / (C * R)) - dt / C * q_in;
+            }
+        pre = pre / (kBloodDensity * std::pow(reference_velocity, 2.)) * 133.322;
+        (side == 0 ? pre_inlet : pre_outlet) = pre;
+    }
 }
 
 }  // namespace ih

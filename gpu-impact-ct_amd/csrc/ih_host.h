@@ -43,9 +43,29 @@ void host_mg_levels(const HostGrid &hg, int n_grids_limit, std::vector<HostMgLev
 
 // CT geometry (csrc/ct.cpp)
 bool vtk_read_ascii(const char *path, int dims[3], std::vector<double> &values, std::string &err);
-void waveform_convert(const double *t_ms, const double *q, int npts, int M1, std::vector<double> &t_out,
-                      std::vector<double> &q_out, double ref[3]);
+void waveform_convert(const double *t_ms, const double *q, int npts, int M1, bool idealized,
+                      std::vector<double> &t_out, std::vector<double> &q_out, double ref[3]);
 double waveform_inflow(const std::vector<double> &t, const std::vector<double> &q, double time, bool &found);
+void wk_pressure_flow(int branch, double time, const double ref[3], double state[3], double &p, double &q);
+void wk_echo(const std::vector<double> &t_ms, const std::vector<double> &q_raw, const std::vector<double> &t_conv,
+             double time, double reference_velocity, double &pre_inlet, double &pre_outlet);
+
+// The module state of the reference's CT setup that the forcing and the GPU
+// Jacobi solver read: branch switches, the waveform as read and converted, the
+// reference scales, the inlet jet and labels present in the forcing range, the
+// Windkessel state, the boundary pressures, and the RK substep and time step.
+struct CtState {
+    int ct_based = 1, idealized = 0;  // CT_based_aorta, idealized_aorta
+    std::vector<double> t_ms, q_raw, t_conv, q_conv;
+    double ref[3] = {0.0, 0.0, 0.0};  // reference velocity, length, time
+    int inlet_voxels = 0;             // inlet_voxels_count
+    int k_start = 0;                  // k_start_index
+    bool has5 = false, has9 = false, has11 = false;
+    double wk_state[3] = {0.0, 0.0, 0.0};  // p_end_systole_in, _out, _out_2
+    double pres[3] = {0.0, 0.0, 0.0};      // pre_inlet, pre_outlet, pre_outlet_2
+    int substep = 0;
+    double dtime = 0.0;
+};
 
 struct KernelTiming {
     double total_ms = 0.0;
@@ -92,7 +112,7 @@ struct ih_case {
     std::map<std::vector<int>, ih::JorLaunch> jor_shape;  // (fp32, box, level) -> launch shape of the JOR sweep
     // CT label volume (uint8 labels on the field layout) and the converted waveform
     uint8_t *ctlab = nullptr;
-    std::vector<double> ct_t, ct_q;
+    ih::CtState ct;
     // multigrid V cycle (preconditioner 1), built on first use
     ih::MgDev *mg = nullptr;
     double *mgmem = nullptr;          // level arrays and coefficient tables
@@ -100,5 +120,6 @@ struct ih_case {
     float *mgmem32 = nullptr;         // the FP32 cycle's arrays and tables
     void *mgldev32 = nullptr;
     int mg_cfg[4] = {15, 4, 4, 4};    // n_grids_limit, n_relax_down, n_relax_up, n_relax_bottom
+    int64_t mg_bytes = 0;             // device bytes of the multigrid levels (in bytes while built)
     std::vector<ih::HostMgLevel> mgh;
 };

@@ -175,7 +175,9 @@ void launch_copy_field_box(hipStream_t s, const Geo &g, const Box &b, const doub
 void launch_fill_corners(hipStream_t s, const Geo &g, double *f);
 void launch_sub_box(hipStream_t s, const Geo &g, const Box &b, double *f, const double *val);
 void launch_apply_mask(hipStream_t s, const Geo &g, const uint8_t *mask, double *u, double *v, double *w);
-void launch_ct_forcing(hipStream_t s, const Geo &g, const uint8_t *lab, double q, int found, double *u, double *v,
-                       double *w);
+void launch_ct_forcing(hipStream_t s, const Geo &g, const uint8_t *lab, double q, int found, int ideal,
+                       double w_ideal, double *u, double *v, double *w);
+void launch_set_pressure_bc(hipStream_t s, const Geo &g, const int cover[3], int kplane, const uint8_t *lab,
+                            int type1, int type2, int type3, double p_in, double p_out, double p_out2, double *rel);
 
 }  // namespace ih

@@ -37,6 +37,13 @@ struct KrylovPtrs {
     int sweeps = 0;
     float omega = 0.8f;
     JorLaunch shape;  // launch shape of the FP32 sweeps
+    // the same M with the sweeps in FP64 (sweeps64 > 0 instead of sweeps): e64a/e64b
+    // ping-pong, zero64 a field of zeros (the first sweep's start), omega64 the factor
+    int sweeps64 = 0;
+    double omega64 = 0.8;
+    double *e64a = nullptr, *e64b = nullptr;
+    const double *zero64 = nullptr;
+    JorLaunch shape64;
     // right preconditioner M = the multigrid V cycle (preconditioner 1) instead
     const MgDev *mg = nullptr;
     bool mg_fp32 = false;  // the cycle in FP32 (z = (double) of its result)
@@ -56,5 +63,8 @@ void launch_krylov_init(hipStream_t s, const Geo &g, const Box &b, const double 
 // of the first one runs in launch_krylov_init's last kernel
 void launch_krylov_iteration(hipStream_t s, const Geo &g, const Coef &c, const Box &b, const KrylovPtrs &k);
 void launch_krylov_resume(hipStream_t s, const Geo &g, const Box &b, const KrylovPtrs &k);
+// one kernel of the unpreconditioned iteration, for timing it alone
+enum { KPART_P_UPDATE = 0, KPART_APPLY_W, KPART_S_UPDATE, KPART_APPLY_SELF, KPART_X_UPDATE };
+void launch_krylov_part(hipStream_t s, const Geo &g, const Coef &c, const Box &b, const KrylovPtrs &k, int which);
 
 }  // namespace ih

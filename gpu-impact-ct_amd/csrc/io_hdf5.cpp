@@ -295,6 +295,9 @@ int ih_io_read_hdf(ih_case *c, const char *filename, const char *dsetname, ih_fi
         }
         if (any && (hsize_t)dd[d] > fdims[2 - d]) return io_fail(IH_EINVAL, "%s: extent below its attributes", dsetname);
     }
+    // phi = 0. over the whole array before the hyperslab read (src/mod_inout.f90:1675):
+    // ghosts and points outside the file's box do not keep earlier values
+    if ((rc = ih_field_fill(c, f, 0.0))) return lib_fail(rc, "field fill");
     if (any) {
         const hsize_t cnt[3] = {(hsize_t)dd[2], (hsize_t)dd[1], (hsize_t)dd[0]};
         const hsize_t off[3] = {0, 0, 0};

@@ -1355,8 +1355,13 @@ void launch_sub_box(hipStream_t s, const Geo &g, const Box &b, double *f, const 
 // component-1 velocity range, all three components; labels 0 (solid, < 0.5)
 // and 7 set the velocity to 0, label 5 (inlet jet) to (0, 0, q) when the
 // waveform holds the time (found).  Positions outside 1..N read label 2.
-__global__ void k_ct_forcing(Geo g, Range3 r, const uint8_t *__restrict__ lab, double q, int found,
-                             double *__restrict__ u, double *__restrict__ v, double *__restrict__ w) {
+// apply_CT_forcing (src/usr_func.f90:909-1048) over the component-1 velocity
+// range: both branches set solid (label < 1) and label-7 points to rest; label
+// 5 takes (0, 0, q) from the CT branch's waveform when `found`, then, when the
+// idealized branch runs, (0, 0, w_ideal) over it -- the reference runs the two
+// branches one after the other, so the idealized value wins where both run.
+__global__ void k_ct_forcing(Geo g, Range3 r, const uint8_t *__restrict__ lab, double q, int found, int ideal,
+                             double w_ideal, double *__restrict__ u, double *__restrict__ v, double *__restrict__ w) {
     IH_POINT(r);
     const bool inside = i >= 1 && i <= g.N[0] && j >= 1 && j <= g.N[1] && k >= 1 && k <= g.N[2];
     const int64_t p = gidx(g, i, j, k);
@@ -1365,15 +1370,15 @@ __global__ void k_ct_forcing(Geo g, Range3 r, const uint8_t *__restrict__ lab, d
         u[p] = 0.0;
         v[p] = 0.0;
         w[p] = 0.0;
-    } else if (L == 5 && found) {
-        w[p] = q;
+    } else if (L == 5 && (found || ideal)) {
+        w[p] = ideal ? w_ideal : q;
         v[p] = 0.0;
         u[p] = 0.0;
     }
 }
 
-void launch_ct_forcing(hipStream_t s, const Geo &g, const uint8_t *lab, double q, int found, double *u, double *v,
-                       double *w) {
+void launch_ct_forcing(hipStream_t s, const Geo &g, const uint8_t *lab, double q, int found, int ideal,
+                       double w_ideal, double *u, double *v, double *w) {
     Box b;
     for (int d = 0; d < 3; ++d) {
         b.lo[d] = g.SB[0][d];
@@ -1381,7 +1386,41 @@ void launch_ct_forcing(hipStream_t s, const Geo &g, const uint8_t *lab, double q
     }
     const Range3 r = R(b);
     if (empty(r)) return;
-    hipLaunchKernelGGL(k_ct_forcing, grid_for(r), dim3(BX, BY, 1), 0, s, g, r, lab, q, found, u, v, w);
+    hipLaunchKernelGGL(k_ct_forcing, grid_for(r), dim3(BX, BY, 1), 0, s, g, r, lab, q, found, ideal, w_ideal, u, v,
+                       w);
+}
+
+// set_pressure_BC_kernel (src/set_pressure_BC_kernel.cu:1-42): over the box
+// 1..cover[d] that pressure_bc_launcher's grid reaches, label 5 (type1) and
+// label 9 (type2) points of plane kplane take p_in / p_out and label 11 points
+// (type3) take p_out2.  The box and plane are worked out on the host.
+__global__ void k_set_pressure_bc(Geo g, Range3 r, int kplane, const uint8_t *__restrict__ lab, int type1, int type2,
+                                  int type3, double p_in, double p_out, double p_out2, double *__restrict__ rel) {
+    IH_POINT(r);
+    const int64_t p = gidx(g, i, j, k);
+    const int L = lab[p];
+    const bool plane = k == kplane;
+    if (type1 == 1 && L == 5 && plane) rel[p] = p_in;
+    if (type2 == 1 && L == 9 && plane) rel[p] = p_out;
+    if (type3 == 1 && L == 11) rel[p] = p_out2;
+}
+
+void launch_set_pressure_bc(hipStream_t s, const Geo &g, const int cover[3], int kplane, const uint8_t *lab,
+                            int type1, int type2, int type3, double p_in, double p_out, double p_out2, double *rel) {
+    Box b;
+    for (int d = 0; d < 3; ++d) {
+        b.lo[d] = 1;
+        b.hi[d] = cover[d];
+    }
+    if (type3 != 1) {
+        // only the plane can change
+        if (kplane < 1 || kplane > cover[2]) return;
+        b.lo[2] = b.hi[2] = kplane;
+    }
+    const Range3 r = R(b);
+    if (empty(r)) return;
+    hipLaunchKernelGGL(k_set_pressure_bc, grid_for(r), dim3(BX, BY, 1), 0, s, g, r, kplane, lab, type1, type2, type3,
+                       p_in, p_out, p_out2, rel);
 }
 
 void launch_apply_mask(hipStream_t s, const Geo &g, const uint8_t *mask, double *u, double *v, double *w) {

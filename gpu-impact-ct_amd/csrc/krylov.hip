@@ -694,6 +694,17 @@ static void launch_precond(hipStream_t s, const Geo &g, const Coef &c, const Box
             launch_mg_vcycle(s, *k.mg, v, z, act);
         return;
     }
+    if (k.sweeps64 > 0) {
+        // FP64 sweeps on A e = v from e = 0 (the like-for-like FP64 form of the FP32 M):
+        // the first reads the zero field, the last writes z
+        const double *x = k.zero64;
+        for (int q = 0; q < k.sweeps64; ++q) {
+            double *y = q == k.sweeps64 - 1 ? z : (q % 2 == 0 ? k.e64a : k.e64b);
+            launch_jor(s, g, c, b, k.omega64, x, v, y, act, k.shape64);
+            x = y;
+        }
+        return;
+    }
     if (k.sweeps == 1) {
         launch_jor_f32_first(s, g, c, b, k.omega, v, k.b32, nullptr, z, act);
         return;
@@ -710,7 +721,7 @@ static void launch_precond(hipStream_t s, const Geo &g, const Coef &c, const Box
 void launch_krylov_iteration(hipStream_t s, const Geo &g, const Coef &c, const Box &b, const KrylovPtrs &k) {
     Range3 r = Rk(b);
     const int64_t nb = red_blocks(b);
-    const bool pre = k.sweeps > 0 || k.mg;
+    const bool pre = k.sweeps > 0 || k.sweeps64 > 0 || k.mg;
     hipLaunchKernelGGL(k_p_update, grid_for(r), dim3(BX, BY, 1), 0, s, g, r, k.rr, k.pk, k.Ap, k.sc, k.fl);
     if (pre) launch_precond(s, g, c, b, k.pk, k.z1, k);
     launch_apply_A(s, g, c, b, pre ? k.z1 : k.pk, k.Ap, k.rh, DOT_W, k.part1, nullptr, k.fl + KF_ACTIVE);
@@ -726,6 +737,32 @@ void launch_krylov_iteration(hipStream_t s, const Geo &g, const Coef &c, const B
                        pre ? k.z2 : k.rr, pre ? k.z1 : k.pk, k.Ar, k.rh, k.part1, k.part2, nb, k.sc, k.fl, k.part3,
                        k.part4);
     hipLaunchKernelGGL(k_ctl_end, dim3(1), dim3(RED_TX, RED_TY, 1), 0, s, k.part3, k.part4, nb, k.sc, k.fl);
+}
+
+void launch_krylov_part(hipStream_t s, const Geo &g, const Coef &c, const Box &b, const KrylovPtrs &k, int which) {
+    Range3 r = Rk(b);
+    const int64_t nb = red_blocks(b);
+    switch (which) {
+    case KPART_P_UPDATE:
+        hipLaunchKernelGGL(k_p_update, grid_for(r), dim3(BX, BY, 1), 0, s, g, r, k.rr, k.pk, k.Ap, k.sc, k.fl);
+        break;
+    case KPART_APPLY_W:
+        launch_apply_A(s, g, c, b, k.pk, k.Ap, k.rh, DOT_W, k.part1, nullptr, k.fl + KF_ACTIVE);
+        break;
+    case KPART_S_UPDATE:
+        hipLaunchKernelGGL(k_s_update_alpha, red_grid(r), dim3(RED_TX, RED_TY, 1), 0, s, g, r, k.rr, k.Ap, k.part1,
+                           nb, k.sc, k.fl);
+        break;
+    case KPART_APPLY_SELF:
+        launch_apply_A(s, g, c, b, k.rr, k.Ar, nullptr, DOT_SELF, k.part1, k.part2, k.fl + KF_ACTIVE);
+        break;
+    case KPART_X_UPDATE:
+        hipLaunchKernelGGL(k_x_update_omega, red_grid(r), dim3(RED_TX, RED_TY, 1), 0, s, g, r, k.phi, k.rr, k.rr,
+                           k.pk, k.Ar, k.rh, k.part1, k.part2, nb, k.sc, k.fl, k.part3, k.part4);
+        break;
+    default:
+        break;
+    }
 }
 
 void launch_krylov_resume(hipStream_t s, const Geo &g, const Box &b, const KrylovPtrs &k) {

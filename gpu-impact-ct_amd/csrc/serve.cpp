@@ -1,12 +1,22 @@
 // Serving loop: a trace of simulation requests (stenosed tubes of different
-// sizes and lengths) admitted FIFO by an HBM budget and a cap on concurrent
-// simulations, each driven by its own host thread on its own case (and so its
-// own stream), so that the work of admitted requests overlaps on the device.
-// Built on the public C ABI only.
+// sizes and lengths) run on a pool of resident case slots.
+//
+// Set-up (before the first request is timed): the trace's grids are grouped
+// into size classes, and each class gets slots -- cases created, coefficients
+// generated, multigrid levels built and launch shapes tuned -- one class at a
+// time in turn while the pool fits the HBM budget, up to max_active slots or
+// the class's request count.  Serving then allocates and frees nothing on the
+// device: a request is admitted as soon as a slot of its class is free and
+// fewer than max_active run (classes do not share slots, so a request never
+// waits behind one of another class); its thread resets the slot's fields,
+// builds its mask and advances it on the slot's stream, so the kernels of the
+// admitted requests overlap on the device.  Built on the public C ABI only.
 #include <algorithm>
 #include <atomic>
 #include <chrono>
 #include <cmath>
+#include <deque>
+#include <map>
 #include <thread>
 #include <vector>
 
@@ -28,42 +38,69 @@ double tube_radius(double x, double R, double stenosis, double x0, double ell) {
     return R * (1.0 - dr / 2.0 * (1.0 + std::cos(M_PI * (x - x0) / ell)));
 }
 
-struct Slot {
-    std::thread th;
-    std::atomic<int> done{0};
-    int index = -1;
-    int64_t bytes = 0;
-};
-
-void run_request(const ih_serve_config &cfg, const ih_request &rq, ih_request_result &res, Clock::time_point t0) {
-    res.start_s = seconds_since(t0);
-    res.status = IH_OK;
-    res.bicg_iterations = 0;
+ih_grid_desc tube_grid(const int N[3]) {
     ih_grid_desc d{};
     d.version = IH_ABI_VERSION;
-    for (int q = 0; q < 3; ++q) d.N[q] = rq.N[q];
+    for (int q = 0; q < 3; ++q) d.N[q] = N[q];
     for (int q = 0; q < 6; ++q) d.bc[q] = 1;
     d.L[0] = 2.0;
     d.L[1] = d.L[2] = 1.0;
     d.stretch[0] = 0.0;
     d.stretch[1] = d.stretch[2] = 1.5;
+    return d;
+}
+
+struct Slot {
     ih_case *c = nullptr;
-    int rc = ih_case_create(&d, cfg.device, &c);
-    if (!rc) rc = ih_case_generate_coeffs(c);
-    if (!rc) {
-        std::vector<double> xu(rq.N[0] + 7), xp(rq.N[0] + 7), ru(rq.N[0] + 7), rp(rq.N[0] + 7);
-        rc = ih_case_get_coords(c, 0, 1, xu.data());
-        if (!rc) rc = ih_case_get_coords(c, 0, 0, xp.data());
-        for (int i = 0; i < rq.N[0] + 7; ++i) {
-            ru[i] = tube_radius(xu[i], rq.radius, rq.stenosis, 0.5 * d.L[0], 0.4);
-            rp[i] = tube_radius(xp[i], rq.radius, rq.stenosis, 0.5 * d.L[0], 0.4);
-        }
-        if (!rc) rc = ih_set_tube_mask(c, 0.5, 0.5, ru.data(), rp.data());
+    int cls = -1;
+    int64_t bytes = 0;
+    std::vector<double> xu, xp;  // x coordinates of the u and pressure grids
+    std::thread th;
+    std::atomic<int> done{0};
+    int index = -1;  // request running on the slot, -1 when free
+};
+
+// a slot's case, ready to serve: coefficients, the step's multigrid levels and
+// launch shapes (a JOR sweep on the zero fields tunes them without changing them)
+int prepare_slot(const ih_serve_config &cfg, const int N[3], Slot &s) {
+    ih_grid_desc d = tube_grid(N);
+    int rc = ih_case_create(&d, cfg.device, &s.c);
+    if (!rc) rc = ih_case_generate_coeffs(s.c);
+    s.xu.assign(N[0] + 7, 0.0);
+    s.xp.assign(N[0] + 7, 0.0);
+    if (!rc) rc = ih_case_get_coords(s.c, 0, 1, s.xu.data());
+    if (!rc) rc = ih_case_get_coords(s.c, 0, 0, s.xp.data());
+    int r48[48] = {};
+    if (!rc) rc = ih_case_get_ranges(s.c, r48);
+    if (!rc && cfg.step.jor_sweeps > 0) rc = ih_jor(s.c, 1, cfg.step.jor_omega, r48, r48 + 3, IH_PRE, IH_RES);
+    if (!rc && cfg.step.precond_poisson == 1 && cfg.step.precond_sweeps == 0) {
+        int G = 0, NN[45];
+        rc = ih_mg_levels(s.c, &G, NN);
+        // the levels are built on first use: one cycle of zeros builds them
+        if (!rc) rc = ih_multigrid_v(s.c, IH_TMP1, IH_TMP2);
     }
-    // plug flow inside the tube, at rest outside; no history, zero pressure
+    if (!rc) rc = ih_synchronize(s.c);
+    if (!rc) s.bytes = ih_case_bytes(s.c);
+    return rc;
+}
+
+void run_request(const ih_serve_config &cfg, const ih_request &rq, Slot &slot, ih_request_result &res,
+                 Clock::time_point t0) {
+    res.start_s = seconds_since(t0);
+    res.status = IH_OK;
+    res.bicg_iterations = 0;
+    ih_case *c = slot.c;
+    const int n = (int)slot.xu.size();
+    std::vector<double> ru(n), rp(n);
+    for (int i = 0; i < n; ++i) {
+        ru[i] = tube_radius(slot.xu[i], rq.radius, rq.stenosis, 0.5 * 2.0, 0.4);
+        rp[i] = tube_radius(slot.xp[i], rq.radius, rq.stenosis, 0.5 * 2.0, 0.4);
+    }
+    int rc = ih_set_tube_mask(c, 0.5, 0.5, ru.data(), rp.data());
+    // plug flow inside the tube, at rest outside; no history, zero pressure:
+    // every field a step reads before writing starts as it would in a new case
     if (!rc) rc = ih_field_fill(c, IH_VEL1, 1.0);
-    for (int f : {IH_VEL2, IH_VEL3, IH_NL1, IH_NL2, IH_NL3, IH_PRE})
-        if (!rc) rc = ih_field_fill(c, (ih_field)f, 0.0);
+    for (int f = IH_VEL2; !rc && f < IH_NFIELDS; ++f) rc = ih_field_fill(c, (ih_field)f, 0.0);
     if (!rc) rc = ih_apply_mask(c);
     ih_step_stats st{};
     for (int s = 0; !rc && s < rq.steps; ++s) {
@@ -71,19 +108,14 @@ void run_request(const ih_serve_config &cfg, const ih_request &rq, ih_request_re
         if (!rc) res.bicg_iterations += st.bicg_iterations[0] + st.bicg_iterations[1] + st.bicg_iterations[2];
     }
     if (!rc) {
-        int ss[3], nn[3], r48[48];
+        int r48[48];
         rc = ih_case_get_ranges(c, r48);
-        for (int q = 0; q < 3; ++q) {
-            ss[q] = r48[q];
-            nn[q] = r48[3 + q];
-        }
         double a = 0.0, b = 0.0;
-        if (!rc) rc = ih_dot(c, ss, nn, IH_VEL1, IH_VEL1, &a);
-        if (!rc) rc = ih_dot(c, ss, nn, IH_PRE, IH_PRE, &b);
+        if (!rc) rc = ih_dot(c, r48, r48 + 3, IH_VEL1, IH_VEL1, &a);
+        if (!rc) rc = ih_dot(c, r48, r48 + 3, IH_PRE, IH_PRE, &b);
         res.checksum_vel = a;
         res.checksum_pre = b;
     }
-    if (c) ih_case_destroy(c);
     res.status = rc;
     res.finish_s = seconds_since(t0);
 }
@@ -94,26 +126,62 @@ extern "C" int ih_serve(const ih_serve_config *cfg, const ih_request *req, int n
                         ih_serve_summary *sum) {
     if (!cfg || (!req && n > 0) || (!out && n > 0) || n < 0 || cfg->max_active < 1 || cfg->hbm_budget <= 0)
         return IH_EINVAL;
+    if (sum) *sum = ih_serve_summary{};
     std::vector<int> order(n);
     for (int q = 0; q < n; ++q) order[q] = q;
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return req[a].arrival_s < req[b].arrival_s; });
-    std::vector<int64_t> need(n);
-    for (int q = 0; q < n; ++q) {
-        ih_grid_desc d{};
-        d.version = IH_ABI_VERSION;
-        for (int k = 0; k < 3; ++k) d.N[k] = req[q].N[k];
-        for (int k = 0; k < 6; ++k) d.bc[k] = 1;
-        need[q] = ih_grid_bytes(&d);
+    // size classes in order of first arrival
+    std::map<std::vector<int>, int> cls_of;
+    std::vector<std::vector<int>> cls_N;
+    std::vector<int> cls(n), demand;
+    for (int q : order) {
+        const std::vector<int> key{req[q].N[0], req[q].N[1], req[q].N[2]};
+        auto it = cls_of.find(key);
+        if (it == cls_of.end()) {
+            it = cls_of.emplace(key, (int)cls_N.size()).first;
+            cls_N.push_back(key);
+            demand.push_back(0);
+        }
+        cls[q] = it->second;
+        ++demand[it->second];
         out[q] = ih_request_result{};
         out[q].status = IH_OK;
     }
-    const double scale = cfg->time_scale > 0.0 ? cfg->time_scale : 1.0;
-    std::vector<Slot> slots(cfg->max_active);
-    int next = 0, running = 0, finished = 0;
+    const int C = (int)cls_N.size();
+    std::vector<int64_t> est(C);
+    for (int k = 0; k < C; ++k) {
+        const ih_grid_desc d = tube_grid(cls_N[k].data());
+        est[k] = ih_grid_bytes_step(&d, &cfg->step);
+    }
+    // the pool: one more slot per class in turn while it fits the budget
+    const Clock::time_point ts = Clock::now();
+    std::deque<Slot> slots;  // stable addresses: each running request's thread holds its slot
+    std::vector<int> have(C, 0);
+    int64_t pool = 0;
+    int status = IH_OK;
+    for (bool grew = true; grew && status == IH_OK;) {
+        grew = false;
+        for (int k = 0; k < C && status == IH_OK; ++k) {
+            if (have[k] >= std::min(demand[k], cfg->max_active) || est[k] <= 0 || pool + est[k] > cfg->hbm_budget)
+                continue;
+            slots.emplace_back();
+            Slot &s = slots.back();
+            s.cls = k;
+            status = prepare_slot(*cfg, cls_N[k].data(), s);
+            if (status) break;
+            pool += std::max(s.bytes, est[k]);
+            ++have[k];
+            grew = true;
+        }
+    }
+    const double setup_s = seconds_since(ts);
+    int next = 0, running = 0, finished = 0, peak_active = 0;
     int64_t used = 0, peak = 0;
-    int peak_active = 0;
+    double admit_wait_max = 0.0, admit_wait_sum = 0.0;
+    const double scale = cfg->time_scale > 0.0 ? cfg->time_scale : 1.0;
+    std::vector<int> queue;  // arrived, not yet admitted, in arrival order
     const Clock::time_point t0 = Clock::now();
-    while (finished < n) {
+    while (status == IH_OK && finished < n) {
         // retire
         for (Slot &s : slots)
             if (s.index >= 0 && s.done.load(std::memory_order_acquire)) {
@@ -123,55 +191,72 @@ extern "C" int ih_serve(const ih_serve_config *cfg, const ih_request *req, int n
                 --running;
                 ++finished;
             }
-        // admit, first come first served
         const double now = seconds_since(t0);
-        while (next < n) {
-            const int q = order[next];
-            if (req[q].arrival_s * scale > now) break;
-            if (need[q] <= 0 || need[q] > cfg->hbm_budget) {  // can never fit
+        while (next < n && req[order[next]].arrival_s * scale <= now) {
+            const int q = order[next++];
+            if (have[cls[q]] == 0) {  // no slot of its size fits the budget
                 out[q].status = IH_ENOMEM;
                 out[q].admit_s = out[q].start_s = out[q].finish_s = now;
-                out[q].latency_s = now - req[q].arrival_s * scale;
-                ++next;
                 ++finished;
                 continue;
             }
-            if (running >= cfg->max_active || used + need[q] > cfg->hbm_budget) break;
+            queue.push_back(q);
+        }
+        // admit in arrival order; a request whose class has no free slot does not hold back other classes
+        for (size_t w = 0; w < queue.size() && running < cfg->max_active;) {
+            const int q = queue[w];
             Slot *s = nullptr;
             for (Slot &x : slots)
-                if (x.index < 0) {
+                if (x.cls == cls[q] && x.index < 0) {
                     s = &x;
                     break;
                 }
+            if (!s) {
+                ++w;
+                continue;
+            }
+            queue.erase(queue.begin() + (long)w);
             s->index = q;
-            s->bytes = need[q];
             s->done.store(0);
-            used += need[q];
+            used += s->bytes;
             peak = std::max(peak, used);
             ++running;
             peak_active = std::max(peak_active, running);
             out[q].admit_s = now;
+            const double wait = now - req[q].arrival_s * scale;
+            admit_wait_max = std::max(admit_wait_max, wait);
+            admit_wait_sum += wait;
             ih_request_result *r = &out[q];
             const ih_request *rq = &req[q];
             std::atomic<int> *flag = &s->done;
-            s->th = std::thread([cfg, rq, r, t0, flag]() {
-                run_request(*cfg, *rq, *r, t0);
+            s->th = std::thread([cfg, rq, s, r, t0, flag]() {
+                run_request(*cfg, *rq, *s, *r, t0);
                 flag->store(1, std::memory_order_release);
             });
-            ++next;
         }
         std::this_thread::sleep_for(std::chrono::microseconds(200));
     }
     const double makespan = seconds_since(t0);
-    int rc = IH_OK;
+    for (Slot &s : slots)
+        if (s.th.joinable()) s.th.join();
+    for (Slot &s : slots)
+        if (s.c) ih_case_destroy(s.c);
+    int rc = status;
+    int admitted = 0;
     for (int q = 0; q < n; ++q) {
         out[q].latency_s = out[q].finish_s - req[q].arrival_s * scale;
-        if (out[q].status != IH_OK) rc = out[q].status;
+        if (rc == IH_OK && out[q].status != IH_OK) rc = out[q].status;
+        if (out[q].status == IH_OK) ++admitted;
     }
     if (sum) {
         sum->makespan_s = makespan;
         sum->peak_bytes = peak;
         sum->peak_active = peak_active;
+        sum->setup_s = setup_s;
+        sum->pool_bytes = pool;
+        sum->pool_slots = (int)slots.size();
+        sum->admit_wait_max_s = admit_wait_max;
+        sum->admit_wait_mean_s = admitted ? admit_wait_sum / admitted : 0.0;
     }
     return rc;
 }

@@ -21,7 +21,7 @@ LIB_PATH = PKG_ROOT / "lib" / "libimpact_hip.so"
 IO_LIB_PATH = PKG_ROOT / "lib" / "libimpact_io.so"
 IH_IO_ABI_VERSION = 1
 
-IH_ABI_VERSION = 3
+IH_ABI_VERSION = 4
 
 # field ids (include/impact_hip.h)
 (VEL1, VEL2, VEL3, NL1, NL2, NL3, RHS1, RHS2, RHS3, WORK1, WORK2, WORK3, PRE, RES, GPRE, PP, COMP, DIG,
@@ -29,6 +29,9 @@ IH_ABI_VERSION = 3
 
 
 class IHError(RuntimeError):
+IH_ENOTCONV = -1002
+
+
     pass
 
 
@@ -60,7 +63,7 @@ class StepParams(C.Structure):
                 ("bicg_rel_tol", C.c_double), ("bicg_max_it", C.c_int), ("level_pressure", C.c_int),
                 ("inflow_kind", C.c_int), ("inflow_amp", C.c_double), ("use_mask", C.c_int), ("jor_fp32", C.c_int),
                 ("precond_sweeps", C.c_int), ("precond_poisson", C.c_int),
-                ("mg_fp32", C.c_int), ("use_ct", C.c_int)]
+                ("mg_fp32", C.c_int), ("use_ct", C.c_int), ("pressure_bc", C.c_int), ("precond_fp64", C.c_int)]
 
 
 class IoState(C.Structure):
@@ -100,7 +103,9 @@ class ServeConfig(C.Structure):
 
 
 class ServeSummary(C.Structure):
-    _fields_ = [("makespan_s", C.c_double), ("peak_bytes", C.c_int64), ("peak_active", C.c_int)]
+    _fields_ = [("makespan_s", C.c_double), ("peak_bytes", C.c_int64), ("peak_active", C.c_int),
+                ("setup_s", C.c_double), ("pool_bytes", C.c_int64), ("pool_slots", C.c_int),
+                ("admit_wait_max_s", C.c_double), ("admit_wait_mean_s", C.c_double)]
 
 
 _lib = None
@@ -160,6 +165,8 @@ def load(path: os.PathLike | None = None):
                                           C.POINTER(BicgStats)]),
         "ih_mg_configure": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int]),
         "ih_mg_levels": (C.c_int, [vp, ip, ip]),
+        "ih_bicgstab_jor64": (C.c_int, [vp, C.c_double, C.c_int, ip, ip, C.c_int, C.c_int, C.c_int, C.c_double,
+                                        C.POINTER(BicgStats)]),
         "ih_mg_get_coeff": (C.c_int, [vp, C.c_int, C.c_char_p, dp, C.POINTER(C.c_int64)]),
         "ih_multigrid_v": (C.c_int, [vp, C.c_int, C.c_int]),
         "ih_multigrid_v_f32": (C.c_int, [vp, C.c_int, C.c_int]),
@@ -173,6 +180,12 @@ def load(path: os.PathLike | None = None):
         "ih_level_pressure": (C.c_int, [vp, C.c_int]),
         "ih_set_tube_mask": (C.c_int, [vp, C.c_double, C.c_double, dp, dp]),
         "ih_get_mask": (C.c_int, [vp, C.POINTER(C.c_uint8)]),
+        "ih_ct_configure": (C.c_int, [vp, C.c_int, C.c_int]),
+        "ih_ct_get_state": (C.c_int, [vp, dp, dp]),
+        "ih_ct_set_state": (C.c_int, [vp, dp, dp]),
+        "ih_set_pressure_bc": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
+                                         C.c_double]),
+        "ih_case_set_substep": (C.c_int, [vp, C.c_int, C.c_double]),
         "ih_apply_mask": (C.c_int, [vp]),
         "ih_timestep": (C.c_int, [vp, C.POINTER(StepParams), C.c_double, C.POINTER(StepStats)]),
         "ih_device_jacobi_solver": (C.c_int, [vp, C.c_int, C.c_double] + [C.c_int] * 6 + [dp, dp]),
@@ -195,12 +208,14 @@ def load(path: os.PathLike | None = None):
                                C.POINTER(ServeSummary)]),
         "ih_timer_start": (C.c_int, [vp]),
         "ih_timer_stop": (C.c_int, [vp, dp]),
+        "ih_grid_bytes_step": (C.c_int64, [C.POINTER(GridDesc), C.POINTER(StepParams)]),
         "ih_kernel_timing_enable": (C.c_int, [vp, C.c_int]),
         "ih_kernel_timing_get": (C.c_int, [vp, C.c_char_p, dp, i64p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
         fn.restype = res
+        "ih_bench_kernel": (C.c_int, [vp, C.c_char_p, C.c_int, dp]),
         fn.argtypes = args
     if L.ih_abi_version() != IH_ABI_VERSION:
         raise IHError("ABI version mismatch")
@@ -499,6 +514,13 @@ class Case:
         return [tuple(nn[3 * g:3 * g + 3]) for g in range(G.value)]
 
     def mg_coeff(self, level, name) -> np.ndarray:
+    def bicgstab_jor64(self, eps, nmax, ss, nn, sweeps, omega=0.8, bb=RES, phi=PRE) -> BicgStats:
+        """the FP64-swept form of bicgstab_mixed (the like-for-like FP64 baseline)"""
+        st = BicgStats()
+        check(self.lib.ih_bicgstab_jor64(self.h, float(eps), int(nmax), _ints(ss), _ints(nn), bb, phi, int(sweeps),
+                                         float(omega), C.byref(st)), "bicgstab_jor64")
+        return st
+
         n = C.c_int64(0)
         check(self.lib.ih_mg_get_coeff(self.h, int(level), name.encode(), None, C.byref(n)), "mg_get_coeff")
         out = np.zeros(n.value)
@@ -559,6 +581,27 @@ class Case:
     def apply_mask(self):
         check(self.lib.ih_apply_mask(self.h), "apply_mask")
 
+    def ct_configure(self, ct_based: int, idealized: int):
+        """apply_CT_forcing's branches (CT_based_aorta, idealized_aorta); before ct_load_vtk"""
+        check(self.lib.ih_ct_configure(self.h, int(ct_based), int(idealized)), "ct_configure")
+
+    def ct_state(self):
+        """(pre_inlet, pre_outlet, pre_outlet_2), (p_end_systole_in, _out, _out_2)"""
+        pres, wk = np.zeros(3), np.zeros(3)
+        check(self.lib.ih_ct_get_state(self.h, _dp(pres), _dp(wk)), "ct_get_state")
+        return pres, wk
+
+    def ct_set_state(self, pres, wk):
+        check(self.lib.ih_ct_set_state(self.h, _dp(np.ascontiguousarray(pres, dtype=np.float64)),
+                                       _dp(np.ascontiguousarray(wk, dtype=np.float64))), "ct_set_state")
+
+    def set_pressure_bc(self, rel, types, p_in, p_out, p_out2):
+        check(self.lib.ih_set_pressure_bc(self.h, rel, int(types[0]), int(types[1]), int(types[2]), float(p_in),
+                                          float(p_out), float(p_out2)), "set_pressure_bc")
+
+    def set_substep(self, substep: int, dtime: float):
+        check(self.lib.ih_case_set_substep(self.h, int(substep), float(dtime)), "set_substep")
+
     def timestep(self, params: StepParams, time: float, stats: bool = True) -> StepStats | None:
         st = StepStats()
         check(self.lib.ih_timestep(self.h, C.byref(params), float(time), C.byref(st) if stats else None), "timestep")
@@ -586,9 +629,13 @@ class Case:
     def device_interpolate2(self, m, phi, inter):
         check(self.lib.ih_device_interpolate2(self.h, m, _dp(phi), _dp(inter)), "device_interpolate2")
 
-    def device_jacobi_solver(self, omega, ss, nn, rel, bb):
-        check(self.lib.ih_device_jacobi_solver(self.h, 1, float(omega), *[int(x) for x in ss],
-                                               *[int(x) for x in nn], _dp(rel), _dp(bb)), "device_jacobi_solver")
+    def device_jacobi_solver(self, omega, ss, nn, rel, bb) -> int:
+        """returns the status: 0, or IH_ENOTCONV when the sweep limit stopped it (rel is still written)"""
+        rc = self.lib.ih_device_jacobi_solver(self.h, 1, float(omega), *[int(x) for x in ss], *[int(x) for x in nn],
+                                              _dp(rel), _dp(bb))
+        if rc not in (0, IH_ENOTCONV):
+            check(rc, "device_jacobi_solver")
+        return rc
 
     # --- timing ------------------------------------------------------------
     def timer_start(self):
@@ -609,6 +656,12 @@ class Batch:
         arr = (C.c_void_p * len(self.cases))(*[c.h for c in self.cases])
         h = C.c_void_p()
         check(self.lib.ih_batch_create(arr, len(self.cases), C.byref(h)), "batch_create")
+    def bench_kernel(self, name: str, launches: int) -> float:
+        """ms per launch of one kernel, back to back on the case's stream"""
+        ms = C.c_double(0.0)
+        check(self.lib.ih_bench_kernel(self.h, name.encode(), int(launches), C.byref(ms)), "bench_kernel")
+        return ms.value
+
         self.h = h
 
     def close(self):

@@ -34,14 +34,15 @@
 extern "C" {
 #endif
 
-#define IH_ABI_VERSION 3
+#define IH_ABI_VERSION 4
 
 enum {
     IH_OK = 0,
     IH_EINVAL = -22,
     IH_ENOMEM = -12,
     IH_EHIP = -1000,
-    IH_ESTATE = -1001
+    IH_ESTATE = -1001,
+    IH_ENOTCONV = -1002 /* an iteration stopped at its limit without meeting its test (results are kept) */
 };
 
 /* Field ids.  Vectors occupy three consecutive ids. */
@@ -182,6 +183,12 @@ int ih_bicgstab_precond(ih_case *c, double eps, int n_it_max, const int ss[3], c
 /* Levels and relaxations per level (prog/config.txt:108-116: 15, 4, 4, 4, the
  * defaults).  The cycle follows multigridV (src/mod_solvers.f90:456-585) with
  * Jacobi smoothing (omega 0.98, src/mod_laplace.f90:470) on each level's
+/* No counterpart in the reference: the same right-preconditioned BiCGstab with
+ * M = `sweeps` JOR sweeps from zero in FP64 -- the like-for-like FP64 baseline
+ * of ih_bicgstab_mixed, which isolates what the FP32 sweeps gain.  Uses TMP3,
+ * TMP4 and DIG as work fields. */
+int ih_bicgstab_jor64(ih_case *c, double eps, int n_it_max, const int ss[3], const int nn[3], ih_field bb,
+                      ih_field phi, int sweeps, double omega, ih_bicgstab_stats *st);
  * pressure range; the hierarchy halves a direction while N-1 is even and N >= 5
  * (src/mod_setup.f90:96-143).  Walls or periodic pairs in every direction. */
 int ih_mg_configure(ih_case *c, int n_grids_limit, int n_relax_down, int n_relax_up, int n_relax_bottom);
@@ -232,7 +239,26 @@ int ih_ct_load_vtk(ih_case *c, const char *path, int x_offset, int y_offset, int
 int ih_ct_get_labels(ih_case *c, uint8_t *host);
 /* the inflow velocity the forcing uses at `time` (reference units); found = 0 outside the waveform */
 int ih_ct_inflow(ih_case *c, double time, double *q, int *found);
-/* apply_CT_forcing (CT-based branch) on VEL1..3 at `time` */
+/* The two branches of apply_CT_forcing (src/usr_func.f90:909-1048): the
+ * reference's switches CT_based_aorta and idealized_aorta (src/mod_vars.f90:260;
+ * its compile-time default is the idealized aorta alone).  Set before
+ * ih_ct_load_vtk: the idealized aorta also overrides the reference scales of
+ * convert_waveform (0.2 m, 2.7 m/s, src/usr_func.f90:637-640), which the jet
+ * radius of the labels depends on.  The library's default is the CT branch alone. */
+int ih_ct_configure(ih_case *c, int ct_based, int idealized);
+/* apply_CT_forcing on VEL1..3 at `time` (after the time update, as
+ * src/mod_timeint.f90:405-416 calls it), the configured branches in the
+ * reference's order:
+ *   CT-based:  get_pressure_waveform_echo_WKM (:654-750) sets pre_inlet and
+ *              pre_outlet; solid and label-7 points to rest; label 5 to
+ *              (0, 0, Q(t)) of the waveform;
+ *   idealized: get_pressure_and_flow_rate_WKM (:762-855) for the inflow (label
+ *              5 present: pre_inlet, q_inlet), the outflow (label 9: pre_outlet)
+ *              and the second outflow (label 11: pre_outlet_2); solid and label 7
+ *              to rest; label 5 to (0, 0, q_inlet / (inlet_voxels_count h^2)),
+ *              h = L1/(M1-1).
+ * The Windkessel keeps p_end_systole_* between calls as the reference's module
+ * variables do (zero after ih_ct_load_vtk). */
 int ih_ct_forcing(ih_case *c, double time);
 
 /* ---- projection time step ------------------------------------------------ */
@@ -246,6 +272,28 @@ typedef struct {
     int inflow_kind; /* 0 none, 1 parabola 16y(1-y)z(1-z), 2 pulsatile plug in the mask */
     double inflow_amp;
     int use_mask;
+/* The module state the forcing leaves: pres = pre_inlet, pre_outlet,
+ * pre_outlet_2 (nondimensional) and wk = p_end_systole_in, _out, _out_2 (mmHg).
+ * set: for a host that computes them itself, or a restart. */
+int ih_ct_get_state(ih_case *c, double pres[3], double wk[3]);
+int ih_ct_set_state(ih_case *c, const double pres[3], const double wk[3]);
+/* set_pressure_BC_kernel (src/set_pressure_BC_kernel.cu:1-42) on field rel, as
+ * pressure_bc_launcher (src/set_pressure_BC_kernel_launcher.cu:12-57) launches
+ * it: over the points its grid reaches (1..b*t per direction, b = (N-1)/8 blocks
+ * of t = (N-1)/b threads; none when a direction has no block or a block would
+ * exceed 1024 threads), label-5 points (type1 = 1) and label-9 points
+ * (type2 = 1) of plane k_start_index - 4 take p_in and p_out, label-11 points
+ * (type3 = 1) take p_out2.  The kernel compares its storage offset k+4 with
+ * k_start_index - 1; the array starts at index -3, so that is Fortran plane
+ * k_start_index - 4.  Needs the CT labels. */
+int ih_set_pressure_bc(ih_case *c, ih_field rel, int type1, int type2, int type3, double p_in, double p_out,
+                       double p_out2);
+/* The module variables device_jacobi_solver reads besides its arguments: the RK
+ * substep (1..3; 0 = outside a time step) and dtime.  With CT labels loaded,
+ * substep 3 applies set_pressure_BC (types 1, 1, 0) with the pressures of the
+ * last forcing times (aRK + bRK) dtime before every sweep
+ * (src/jacobi_solver_host_v4.f90:302-305). */
+int ih_case_set_substep(ih_case *c, int substep, double dtime);
     int jor_fp32;   /* 1: JOR sweeps in FP32 (residual and update kept FP64) */
     int precond_sweeps; /* > 0: BiCGstab right-preconditioned by this many FP32 JOR sweeps */
     int precond_poisson; /* with precond_sweeps 0: BiCGstab's preconditioner, 0 none, 1 multigrid V cycle */
@@ -263,6 +311,9 @@ typedef struct {
  * stats may be NULL; when given, div_max_after costs one extra divergence.
  * div_max_after is max |div u| over the pressure range at the end of the step:
  * after the mask and the CT forcing, which set velocities in the wall and the
+    int pressure_bc;     /* 1 (with CT labels, FP64 sweeps): set_pressure_BC before every JOR sweep of
+                            substep 3 and after the last, as device_jacobi_solver does */
+    int precond_fp64;    /* with precond_sweeps > 0: the preconditioner's sweeps in FP64 (ih_bicgstab_jor64) */
  * jet and so leave divergence at those interfaces; without them it is what
  * the projection leaves. */
 int ih_timestep(ih_case *c, const ih_step_params *p, double time, ih_step_stats *st);
@@ -280,6 +331,12 @@ int ih_device_axpby(ih_case *c, double a, double b, double *x, double *y, double
 int ih_device_interpolate(ih_case *c, int dimension_id, double *phi, double *inter);
 int ih_device_interpolate2(ih_case *c, int dimension_id, double *phi, double *inter);
 
+/* device_jacobi_solver: JOR sweeps with the reference's stopping test every 20
+ * sweeps (|log(err_tot_old / err_tot)| against its threshold, err_tot the sum
+ * of |rel - rel_old| over the box in the fixed reduction order) and, on
+ * substep 3 with CT labels, the Dirichlet pressure (ih_case_set_substep).  An
+ * err_tot of 0 (a fixed point) stops.  After 200000 sweeps it stops with
+ * IH_ENOTCONV, rel holding the last iterate. */
 /* ---- batched relaxation of many cases ------------------------------------ */
 /* Many cases (uneven grids, one device) relaxed together: each sweep is one
  * launch over the tiles of all cases.  No reference counterpart: the reference
@@ -344,21 +401,35 @@ typedef struct {
 } ih_serve_config;
 
 typedef struct {
-    double makespan_s;
-    int64_t peak_bytes;  /* largest sum of admitted cases' HBM */
+    double makespan_s;         /* first arrival to last finish (set-up excluded) */
+    int64_t peak_bytes;        /* largest sum of the running requests' slot bytes */
     int peak_active;
 } ih_serve_summary;
 
 /* HBM a case of grid g holds (fields, FP32 buffers, mask, tables). */
 int64_t ih_grid_bytes(const ih_grid_desc *g);
-/* Runs the trace to completion; out[q] belongs to req[q].  Returns IH_OK or the
- * status of a failed request. */
+/* The same for a case that runs steps with p: the multigrid levels and their
+ * FP32 copies are added when p selects preconditioner 1. */
+int64_t ih_grid_bytes_step(const ih_grid_desc *g, const ih_step_params *p);
+/* Runs the trace to completion; out[q] belongs to req[q].  Before the trace
+ * starts, the trace's grids get a pool of resident slots (cases with their
+ * coefficients, multigrid levels and tuned launch shapes), one more per size
+ * class in turn while the pool fits hbm_budget, at most max_active or the
+ * class's request count per class; serving then allocates nothing.  A request
+ * is admitted when a slot of its size is free and fewer than max_active run;
+ * a request whose size got no slot fails with IH_ENOMEM.  Returns IH_OK or
+ * the status of a failed request. */
 int ih_serve(const ih_serve_config *cfg, const ih_request *req, int n, ih_request_result *out,
              ih_serve_summary *summary);
 
 /* ---- timing of the case's stream (for benchmarks) ------------------------ */
 int ih_timer_start(ih_case *c);
 int ih_timer_stop(ih_case *c, double *ms);
+    double setup_s;            /* building the slot pool, before the trace starts */
+    int64_t pool_bytes;        /* HBM held by the pool (at most hbm_budget) */
+    int pool_slots;
+    double admit_wait_max_s;   /* arrival to admission */
+    double admit_wait_mean_s;
 /* Per-kernel-family launch statistics collected with HIP events when enabled. */
 int ih_kernel_timing_enable(ih_case *c, int on);
 int ih_kernel_timing_get(ih_case *c, const char *family, double *total_ms, int64_t *launches);
@@ -367,3 +438,10 @@ int ih_kernel_timing_get(ih_case *c, const char *family, double *total_ms, int64
 }
 #endif
 #endif
+/* `launches` launches of one kernel back to back between two events on the
+ * case's stream: "jor" (an FP64 sweep over the pressure range), "apply" (the
+ * fused operator alone), and the five kernels of a BiCGstab iteration --
+ * "p_update", "apply_w" (operator with <rh, Ap>), "s_update", "apply_self"
+ * (operator with <r, Ar>, <Ar, Ar>), "x_update" -- on the case's Krylov
+ * fields (their results are meaningless). */
+int ih_bench_kernel(ih_case *c, const char *name, int launches, double *ms_per_launch);

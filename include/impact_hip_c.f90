@@ -9,7 +9,7 @@ module impact_hip_c
   use iso_c_binding
   implicit none
 
-  integer(c_int), parameter :: IH_ABI_VERSION = 3
+  integer(c_int), parameter :: IH_ABI_VERSION = 4
   integer(c_int), parameter :: IH_OK = 0
 
   ! fields owned by a case (enum ih_field)

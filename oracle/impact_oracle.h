@@ -86,6 +86,8 @@ void orc_product_div_grad(const orc_case *c, double *phi, double *lap, double *d
 /* mode 0: sequential k,j,i loop order (reference order);
  * mode 1: the fixed two-level tree the device kernels use. */
 void orc_set_reduction_mode(orc_case *c, int mode);
+/* threads of the OpenMP build's point loops (n > 0 sets them); returns the count in use (1 when serial) */
+int orc_set_threads(int n);
 double orc_dot(const orc_case *c, const int ss[3], const int nn[3], const double *a, const double *b);
 
 /* ---------------- Poisson ---------------- */
@@ -107,6 +109,10 @@ void orc_bicgstab_mixed(const orc_case *c, double eps, int n_it_max, const int s
                         const double *bb, double *phi, int sweeps, double omega, orc_bicgstab_stats *st);
 void orc_jor_mixed(const orc_case *c, int n_sweeps, double omega, const int ss[3], const int nn[3], double *rel,
                    const double *bb);
+/* The same right-preconditioned BiCGstab with the sweeps of M in FP64 (the
+ * like-for-like FP64 baseline of orc_bicgstab_mixed; no reference counterpart). */
+void orc_bicgstab_jor64(const orc_case *c, double eps, int n_it_max, const int ss[3], const int nn[3],
+                        const double *bb, double *phi, int sweeps, double omega, orc_bicgstab_stats *st);
 
 /* ---------------- multigrid V cycle (preconditioner 1) ---------------- */
 /* Hierarchy, coefficients and work arrays of every level; NULL for BCs other
@@ -156,6 +162,47 @@ double orc_ct_inflow(int npts, const double *t_conv, const double *q_conv, doubl
 void orc_ct_forcing(const orc_case *c, const uint8_t *lab, int npts, const double *t_conv, const double *q_conv,
                     double time, double *vel);
 
+/* CT state of the two apply_CT_forcing branches (src/usr_func.f90:909-1048) and
+ * of the label-driven pressure of the GPU Jacobi solver: the module variables
+ * the reference keeps (CT_based_aorta, idealized_aorta, the waveform as read and
+ * converted, the reference scales, inlet_voxels_count, k_start_index,
+ * p_end_systole_in/out/out_2, pre_inlet/outlet/outlet_2). */
+typedef struct {
+    int ct_based, idealized;
+    int npts;
+    const double *t_ms, *q_raw;   /* waveform as read (ms, m/s) */
+    const double *t_conv, *q_conv; /* converted (reference units) */
+    double ref[3];                 /* reference velocity, length, time */
+    int inlet_voxels;              /* inlet_voxels_count */
+    int k_start;                   /* k_start_index */
+    double wk_state[3];            /* p_end_systole_in, _out, _out_2 */
+    double pres[3];                /* pre_inlet, pre_outlet, pre_outlet_2 (nondimensional) */
+} orc_ct_state;
+
+/* convert_waveform with the idealized-aorta override of the scales when idealized != 0 */
+void orc_waveform_convert_branch(int npts, const double *t_ms, const double *q, int M1, int idealized,
+                                 double *t_out, double *q_out, double ref[3]);
+/* get_pressure_and_flow_rate_WKM (branch 1 inflow, 2 outflow, 3 outflow 2); state = p_end_systole_* */
+void orc_wk_pressure_flow(int branch, double time, const double ref[3], double state[3], double *p, double *q);
+/* get_pressure_waveform_echo_WKM */
+void orc_wk_echo(int npts, const double *t_ms, const double *q_raw, const double *t_conv, double time,
+                 double reference_velocity, double *pre_inlet, double *pre_outlet);
+/* apply_CT_forcing, the branches st->ct_based and st->idealized in the reference's order; updates st->pres */
+void orc_ct_apply_forcing(const orc_case *c, const uint8_t *lab, orc_ct_state *st, double time, double *vel);
+/* set_pressure_BC_kernel as pressure_bc_launcher launches it (plane k_start_index-4, see orc_ct.c) */
+void orc_set_pressure_bc(const orc_case *c, const uint8_t *lab, int type1, int type2, int type3, int k_start_index,
+                         double p_in, double p_out, double p_out2, double *rel);
+
+/* device_jacobi_solver (src/jacobi_solver_host_v4.f90:5-717): JOR sweeps on
+ * [ss, nn] with the stopping test every 20 sweeps; on substep 3 with CT labels
+ * (lab != NULL) the Dirichlet pressure before every sweep and before the exit
+ * (types 1 and 2, values pres * (aRK + bRK) * dtime).  The sum err_tot follows
+ * the case's reduction mode.  Returns 0 when the test stopped it (or err_tot is
+ * 0), 1 at max_iter; *sweeps = sweeps done. */
+int orc_device_jacobi_solver(const orc_case *c, double omega, const int ss[3], const int nn[3], double *rel,
+                             const double *bb, const uint8_t *lab, int k_start, int substep, double dtime,
+                             const double pres[3], int max_iter, int *sweeps);
+
 /* ---------------- projection step ---------------- */
 typedef struct {
     double Re, dtime;
@@ -174,6 +221,9 @@ typedef struct {
     const uint8_t *ct_lab; /* optional: CT labels (orc_ct_build); apply_CT_forcing at the step's end time */
     int ct_npts;
     const double *ct_t, *ct_q; /* converted waveform */
+    orc_ct_state *ct_state; /* optional (with ct_lab): both forcing branches, the Windkessel state and pressures */
+    int pressure_bc;        /* 1 (with ct_state): set_pressure_BC before each JOR sweep of substep 3 and after the last */
+    int precond_fp64;       /* with precond_sweeps > 0: the preconditioner's sweeps in FP64 */
 } orc_step_params;
 
 typedef struct {

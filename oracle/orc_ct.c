@@ -256,3 +256,205 @@ void orc_ct_forcing(const orc_case *c, const uint8_t *lab, int npts, const doubl
                 }
             }
 }
+
+/* ------------------------------------------------------------------------ */
+/* Windkessel models, both branches of apply_CT_forcing and the label-driven */
+/* Dirichlet pressure of the GPU Jacobi solver.                              */
+/* ------------------------------------------------------------------------ */
+
+/* blood properties, src/mod_vars.f90:235-236 */
+#define ORC_BLOOD_NU 0.0000035
+#define ORC_BLOOD_RHO 1060.
+
+/* convert_waveform (src/usr_func.f90:599-651) with the idealized-aorta
+ * override of the reference scales (:637-640: length 0.2, velocity 2.7) when
+ * idealized != 0; the converted waveform uses the final scales. */
+void orc_waveform_convert_branch(int npts, const double *t_ms, const double *q, int M1, int idealized,
+                                 double *t_out, double *q_out, double ref[3])
+{
+    const double CT_pixel_width = 0.0004;
+    const double upsampling_factor = 1.0;
+    const double CT_pixel_width_star = (1.0 / M1) * upsampling_factor;
+    const double maximum_inflow_velocity_star = 1.0;
+    double vmax = q[0];
+    for (int i = 1; i < npts; ++i) vmax = q[i] > vmax ? q[i] : vmax;
+    double reference_velocity = vmax / maximum_inflow_velocity_star;
+    double reference_length = CT_pixel_width / CT_pixel_width_star;
+    if (idealized) {
+        reference_length = 0.2;
+        reference_velocity = 2.7;
+    }
+    const double reference_time = reference_length / reference_velocity;
+    for (int i = 0; i < npts; ++i) {
+        t_out[i] = t_ms[i] / (1000.0 * reference_time);
+        q_out[i] = q[i] / reference_velocity;
+    }
+    ref[0] = reference_velocity;
+    ref[1] = reference_length;
+    ref[2] = reference_time;
+}
+
+/* get_pressure_and_flow_rate_WKM (src/usr_func.f90:762-855): the reference's
+ * closed form of a two-element Windkessel driven by a half-sine systolic flow
+ * (T_beat 1 s, T_systole 0.4 s, p_diastole 80 mmHg), statement by statement.
+ * The physiological time is time * reference_time (not wrapped into a beat).
+ * state[branch-1] is p_end_systole_in / _out / _out_2: written on every call in
+ * systole, read in diastole.  p and q are returned nondimensional. */
+void orc_wk_pressure_flow(int branch, double time, const double ref[3], double state[3], double *p_out, double *q_out)
+{
+    const double pi = 4. * atan(1.);
+    const double T_beat = 1.;
+    const double T_systole = 0.4 * T_beat;
+    const double p_diastole = 80.;
+    double CWK, RWK, Q_peak_systole;
+    if (branch == 1) {
+        CWK = 1.5496 / 1000000.;
+        RWK = 0.9758 * 1000000.;
+        Q_peak_systole = 0.0004;
+    } else if (branch == 2) {
+        CWK = 1.5 / 1000000.;
+        RWK = 1.12 * 1000000.;
+        Q_peak_systole = 0.0004 * 0.85;
+    } else {
+        CWK = 0.059 / 1000000.;
+        RWK = 19.68 * 1000000.;
+        Q_peak_systole = 0.0004 * 0.05;
+    }
+    const double reference_velocity = ref[0], reference_length = ref[1], reference_time = ref[2];
+    const double time_phys = time * reference_time;
+    double p, q;
+    if (time_phys <= T_systole) {
+        const double den = pow(T_systole, 2.) + pow(CWK, 2.) * pow(pi, 2.) * pow(RWK, 2.);
+        p = (p_diastole + Q_peak_systole * T_systole * RWK * CWK * pi * RWK / den) * exp(-(1. * time_phys) / (RWK * CWK)) -
+            (T_systole * Q_peak_systole * RWK * (CWK * pi * RWK * cos(pi * time_phys / T_systole)) -
+             T_systole * sin(pi * time_phys / T_systole)) / den;
+        q = Q_peak_systole * sin(pi * (time_phys) / T_systole);
+        state[branch - 1] = p;
+    } else {
+        p = state[branch - 1] * exp(-(1. * (time_phys - T_systole)) / (RWK * CWK));
+        q = 0.;
+    }
+    p = p / (ORC_BLOOD_RHO * pow(reference_velocity, 2.)) * 133.322;
+    q = q / (pow(reference_length, 3.) / reference_time);
+    *p_out = p;
+    *q_out = q;
+}
+
+/* get_pressure_waveform_echo_WKM (src/usr_func.f90:654-750): from 80 mmHg, one
+ * step p = p (1 - dt/(C R)) - dt/C q_in for every waveform interval i (1..n-1)
+ * whose converted times hold the pulse-shifted time, dt the interval in s and
+ * q_in = Q_i * q_max / max Q; inlet (R 0.78e6, C 2.166e-6, q_max 4e-4) and
+ * outlet (R 0.898e6, C 2.066e-6, q_max 0.85 * 4e-4); nondimensional. */
+void orc_wk_echo(int npts, const double *t_ms, const double *q_raw, const double *t_conv, double time,
+                 double reference_velocity, double *pre_inlet, double *pre_outlet)
+{
+    const double pulse_length = t_conv[npts - 1];
+    const double pulse_number = floor(time / pulse_length) + 1.0;
+    const double shifted = time - (pulse_number - 1.0) * pulse_length;
+    const double cardiac_output_factor = 1.0;
+    double vel_max = q_raw[0];
+    for (int i = 1; i < npts; ++i) vel_max = q_raw[i] > vel_max ? q_raw[i] : vel_max;
+    for (int side = 0; side < 2; ++side) {
+        const double R = side == 0 ? 0.78 * 1000000. : 0.898 * 1000000.;
+        const double Cw = side == 0 ? 2.166 / 1000000. : 2.066 / 1000000.;
+        const double q_max = side == 0 ? 0.0004 * cardiac_output_factor : 0.0004 * 0.85 * cardiac_output_factor;
+        const double inlet_area = q_max / vel_max;
+        double pre = 80.;
+        for (int i = 0; i + 1 < npts; ++i)
+            if (shifted >= t_conv[i] && shifted <= t_conv[i + 1]) {
+                const double dt = t_ms[i + 1] / 1000. - t_ms[i] / 1000.;
+                const double q_in = q_raw[i] * inlet_area;
+                pre = pre * (1. - dt / (Cw * R)) - dt / Cw * q_in;
+            }
+        pre = pre / (ORC_BLOOD_RHO * pow(reference_velocity, 2.)) * 133.322;
+        if (side == 0)
+            *pre_inlet = pre;
+        else
+            *pre_outlet = pre;
+    }
+}
+
+/* apply_CT_forcing (src/usr_func.f90:909-1048), both branches in the
+ * reference's order over the component-1 velocity range.  CT-based branch
+ * (st->ct_based): the echo Windkessel pressures, then solid / label 7 to zero
+ * and label 5 to (0, 0, Q(t)).  Idealized-aorta branch (st->idealized): solid /
+ * label 7 to zero; label 5 to (0, 0, q_inlet / (inlet_voxels_count h^2)),
+ * h = L1/(M1-1), with the inflow Windkessel's pressure and flow; labels 9 and
+ * 11 set the outflow Windkesssel pressures (each model evaluated when a point
+ * of its label lies in the range, as the per-point calls of the reference). */
+void orc_ct_apply_forcing(const orc_case *c, const uint8_t *lab, orc_ct_state *st, double time, double *vel)
+{
+    const int64_t n = c->ntot;
+    if (st->ct_based) {
+        orc_wk_echo(st->npts, st->t_ms, st->q_raw, st->t_conv, time, st->ref[0], &st->pres[0], &st->pres[1]);
+        orc_ct_forcing(c, lab, st->npts, st->t_conv, st->q_conv, time, vel);
+    }
+    if (!st->idealized) return;
+    int has5 = 0, has9 = 0, has11 = 0;
+    for (int k = c->SB[0][2]; k <= c->NB[0][2]; ++k)
+        for (int j = c->SB[0][1]; j <= c->NB[0][1]; ++j)
+            for (int i = c->SB[0][0]; i <= c->NB[0][0]; ++i) {
+                const int inside = i >= 1 && i <= c->N[0] && j >= 1 && j <= c->N[1] && k >= 1 && k <= c->N[2];
+                const int L = inside ? LBL(c, lab, i, j, k) : 2;
+                has5 |= L == 5;
+                has9 |= L == 9;
+                has11 |= L == 11;
+            }
+    double q_inlet = 0.0, q_dummy;
+    if (has5) orc_wk_pressure_flow(1, time, st->ref, st->wk_state, &st->pres[0], &q_inlet);
+    if (has9) orc_wk_pressure_flow(2, time, st->ref, st->wk_state, &st->pres[1], &q_dummy);
+    if (has11) orc_wk_pressure_flow(3, time, st->ref, st->wk_state, &st->pres[2], &q_dummy);
+    const double h = c->L[0] / (double)(c->N[0] - 1);
+    const double w_in = q_inlet / ((double)st->inlet_voxels * (h * h));
+    for (int k = c->SB[0][2]; k <= c->NB[0][2]; ++k)
+        for (int j = c->SB[0][1]; j <= c->NB[0][1]; ++j)
+            for (int i = c->SB[0][0]; i <= c->NB[0][0]; ++i) {
+                const int inside = i >= 1 && i <= c->N[0] && j >= 1 && j <= c->N[1] && k >= 1 && k <= c->N[2];
+                const int L = inside ? LBL(c, lab, i, j, k) : 2;
+                const int64_t x = ORC_IDX(c, i, j, k);
+                if (L < 1 || L == 7) {
+                    vel[x] = 0.0;
+                    vel[n + x] = 0.0;
+                    vel[2 * n + x] = 0.0;
+                } else if (L == 5) {
+                    vel[2 * n + x] = w_in;
+                    vel[n + x] = 0.;
+                    vel[x] = 0.;
+                }
+            }
+}
+
+/* set_pressure_BC_kernel (src/set_pressure_BC_kernel.cu:1-42) as
+ * pressure_bc_launcher launches it (src/set_pressure_BC_kernel_launcher.cu:16-57):
+ * blocks (n-1)/8 and threads (n-1)/blocks per direction (n = N+7 the padded
+ * extent), so thread (i, j, k) reaches Fortran point (i+1, j+1, k+1) for
+ * i < blocks*threads; no point is written when a direction has no block or a
+ * block would exceed 1024 threads (the launch fails).  The kernel compares the
+ * storage offset k+4 with k_start_index-1; the array starts at index -3, so
+ * offset k+4 is Fortran k+1 and the written plane is k_start_index-4.
+ * Label 5 (type1) and label 9 (type2) points of that plane get p_in and p_out;
+ * label 11 points (type3) anywhere in the covered box get p_out2. */
+void orc_set_pressure_bc(const orc_case *c, const uint8_t *lab, int type1, int type2, int type3, int k_start_index,
+                         double p_in, double p_out, double p_out2, double *rel)
+{
+    int cover[3], threads = 1;
+    for (int d = 0; d < 3; ++d) {
+        const int m = c->N[d] - 1; /* (N+7) - 8 */
+        const int blocks = m / 8;
+        if (blocks < 1) return;
+        const int t = m / blocks;
+        threads *= t;
+        cover[d] = blocks * t;
+    }
+    if (threads > 1024) return;
+    for (int k = 1; k <= cover[2]; ++k)
+        for (int j = 1; j <= cover[1]; ++j)
+            for (int i = 1; i <= cover[0]; ++i) {
+                const int L = LBL(c, lab, i, j, k);
+                const int64_t x = ORC_IDX(c, i, j, k);
+                const int plane = (k + 3) == k_start_index - 1;
+                if (type1 == 1 && L == 5 && plane) rel[x] = p_in;
+                if (type2 == 1 && L == 9 && plane) rel[x] = p_out;
+                if (type3 == 1 && L == 11) rel[x] = p_out2;
+            }
+}

@@ -610,3 +610,18 @@ int orc_coeff(const orc_case *c, const char *name, double *out)
 }
 
 void orc_set_reduction_mode(orc_case *c, int mode) { c->red_mode = mode; }
+
+#ifdef ORC_OMP
+#include <omp.h>
+#endif
+/* threads of the OpenMP build's point loops (the serial build ignores it) */
+int orc_set_threads(int n)
+{
+#ifdef ORC_OMP
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
+#else
+    (void)n;
+    return 1;
+#endif
+}

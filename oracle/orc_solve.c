@@ -48,6 +48,75 @@ void orc_jor(const orc_case *c, int n_sweeps, double omega, const int ss[3], con
     orc_exchange_relax(c, rel);
 }
 
+typedef struct {
+    const double *a, *b;
+} absdiff_ctx;
+static double absdiff_pt(const void *ctx, int64_t idx)
+{
+    const absdiff_ctx *d = (const absdiff_ctx *)ctx;
+    return fabs(d->a[idx] - d->b[idx]);
+}
+
+/* the Dirichlet pressure of device_jacobi_solver's substep 3
+ * (src/jacobi_solver_host_v4.f90:302-305): type flags 1, 1, 0 and the
+ * pressures times (aRK + bRK) * dtime */
+static void jacobi_pressure_bc(const orc_case *c, const uint8_t *lab, int k_start, int substep, double dtime,
+                               const double pres[3], double *rel)
+{
+    if (!lab || substep != 3) return;
+    const double f = aRK[substep - 1] + bRK[substep - 1];
+    orc_set_pressure_bc(c, lab, 1, 1, 0, k_start, pres[0] * f * dtime, pres[1] * f * dtime, pres[2] * f * dtime,
+                        rel);
+}
+
+/* device_jacobi_solver (src/jacobi_solver_host_v4.f90:5-717), in the CPU
+ * relaxation's out-of-place form: each trip fills the relaxation ghosts,
+ * applies the substep-3 pressure, and every 20th trip (iter mod 20 == 1)
+ * compares err_tot = sum |rel - rel_old| over [ss, nn] with the previous one:
+ * |log(err_tot_old / err_tot)| <= 20 mode_j 8/3 0.8 3.1415^2 (1/M1^2 + 1/M2^2 +
+ * 1/M3^2) stops (:357-414); otherwise rel_old = rel and one sweep follows.
+ * rel_old and err_tot_old are undefined at the reference's entry, so its first
+ * test compares with undefined values; here the first test only records them.
+ * A later err_tot of 0 (a fixed point, where the reference's log is undefined
+ * and it would never stop) stops. */
+int orc_device_jacobi_solver(const orc_case *c, double omega, const int ss[3], const int nn[3], double *rel,
+                             const double *bb, const uint8_t *lab, int k_start, int substep, double dtime,
+                             const double pres[3], int max_iter, int *sweeps)
+{
+    const int64_t n = c->ntot;
+    double *rel_old = calloc((size_t)n, sizeof(double));
+    double *comp = calloc((size_t)n, sizeof(double));
+    const int mode_j = 20;
+    const int M1 = c->N[0], M2 = c->N[1], M3 = c->N[2];
+    const double thresh = 20. * mode_j * 8. * 1. / 3.0 * 0.8 * 3.1415 * 3.1415 *
+                          (1. / (double)(M1 * M1) + 1. / (double)(M2 * M2) + 1. / (double)(M3 * M3));
+    double err_tot_old = 0.0;
+    int done = 0, status = 1;
+    for (int iter = 1; iter <= max_iter; ++iter) {
+        orc_exchange_relax(c, rel);
+        jacobi_pressure_bc(c, lab, k_start, substep, dtime, pres, rel);
+        if (iter % mode_j == 1) {
+            orc_exchange_relax(c, rel);
+            const absdiff_ctx d = {rel, rel_old};
+            const double err_tot = orc_reduce_sum(c, ss, nn, absdiff_pt, &d);
+            if (iter > 1 && (err_tot == 0.0 || fabs(log(err_tot_old / err_tot)) <= thresh)) {
+                status = 0;
+                break;
+            }
+            memcpy(rel_old, rel, sizeof(double) * (size_t)n);
+            err_tot_old = err_tot;
+        }
+        orc_jor(c, 1, omega, ss, nn, rel, bb, comp);
+        ++done;
+    }
+    if (status == 1) jacobi_pressure_bc(c, lab, k_start, substep, dtime, pres, rel);
+    orc_exchange_relax(c, rel);
+    *sweeps = done;
+    free(rel_old);
+    free(comp);
+    return status;
+}
+
 /* max |bb - A_cdg rel|: src/jacobi_solver_host_v4.f90:357-371 */
 double orc_residual_cdg(const orc_case *c, const int ss[3], const int nn[3], double *rel, const double *bb)
 {
@@ -159,6 +228,18 @@ static void precond(const orc_case *c, int n_sweeps, float omega, const int ss[3
             for (int i = ss[0]; i <= nn[0]; ++i) A(z, i, j, k) = (double)A(res, i, j, k);
 }
 
+/* z = M v with the sweeps in FP64 (the like-for-like FP64 form of precond):
+ * n_sweeps JOR sweeps on A_cdg e = v from e = 0 */
+static void precond64(const orc_case *c, int n_sweeps, double omega, const int ss[3], const int nn[3],
+                      const double *v, double *z, double *e, double *tmp)
+{
+    memset(e, 0, sizeof(double) * (size_t)c->ntot);
+    orc_jor(c, n_sweeps, omega, ss, nn, e, v, tmp);
+    for (int k = ss[2]; k <= nn[2]; ++k)
+        for (int j = ss[1]; j <= nn[1]; ++j)
+            for (int i = ss[0]; i <= nn[0]; ++i) A(z, i, j, k) = A(e, i, j, k);
+}
+
 static double absmax(const orc_case *c, const int ss[3], const int nn[3], const double *a)
 {
     double m = 0.0;
@@ -178,7 +259,7 @@ static double absmax(const orc_case *c, const int ss[3], const int nn[3], const 
 /* ------------------------------------------------------------------------ */
 static void bicgstab_run(const orc_case *c, double eps, int n_it_max, const int ss[3], const int nn[3],
                          const double *bb, double *phi, double *work, int sweeps, double pre_omega,
-                         orc_mg *mg, int mg_fp32, orc_bicgstab_stats *st)
+                         orc_mg *mg, int mg_fp32, int pre64, orc_bicgstab_stats *st)
 {
     const int64_t n = c->ntot;
     double *own = NULL;
@@ -197,7 +278,11 @@ static void bicgstab_run(const orc_case *c, double eps, int n_it_max, const int 
         z1 = calloc((size_t)n, sizeof(double));
         z2 = calloc((size_t)n, sizeof(double));
     }
-    if (sweeps > 0) {
+    double *de = NULL, *dt = NULL;
+    if (sweeps > 0 && pre64) {
+        de = calloc((size_t)n, sizeof(double));
+        dt = calloc((size_t)n, sizeof(double));
+    } else if (sweeps > 0) {
         fb = calloc((size_t)n, sizeof(float));
         fe = calloc((size_t)n, sizeof(float));
         ft = calloc((size_t)n, sizeof(float));
@@ -270,7 +355,10 @@ static void bicgstab_run(const orc_case *c, double eps, int n_it_max, const int 
                 orc_mg_vcycle(mg, pk, z1);
             orc_product_div_grad(c, z1, Ap, dig);
         } else if (sweeps > 0) {
-            precond(c, sweeps, omf, ss, nn, pk, z1, fb, fe, ft);
+            if (pre64)
+                precond64(c, sweeps, pre_omega, ss, nn, pk, z1, de, dt);
+            else
+                precond(c, sweeps, omf, ss, nn, pk, z1, fb, fe, ft);
             orc_product_div_grad(c, z1, Ap, dig);
         } else {
             orc_product_div_grad(c, pk, Ap, dig);
@@ -292,7 +380,10 @@ static void bicgstab_run(const orc_case *c, double eps, int n_it_max, const int 
                 orc_mg_vcycle(mg, rr, z2); /* src/mod_solvers.f90:1751 */
             orc_product_div_grad(c, z2, Ar, dig);
         } else if (sweeps > 0) {
-            precond(c, sweeps, omf, ss, nn, rr, z2, fb, fe, ft);
+            if (pre64)
+                precond64(c, sweeps, pre_omega, ss, nn, rr, z2, de, dt);
+            else
+                precond(c, sweeps, omf, ss, nn, rr, z2, fb, fe, ft);
             orc_product_div_grad(c, z2, Ar, dig);
         } else {
             orc_product_div_grad(c, rr, Ar, dig);
@@ -325,6 +416,8 @@ static void bicgstab_run(const orc_case *c, double eps, int n_it_max, const int 
     free(own);
     free(z1);
     free(z2);
+    free(de);
+    free(dt);
     free(fb);
     free(fe);
     free(ft);
@@ -333,29 +426,35 @@ static void bicgstab_run(const orc_case *c, double eps, int n_it_max, const int 
 void orc_bicgstab(const orc_case *c, double eps, int n_it_max, const int ss[3], const int nn[3], const double *bb,
                   double *phi, double *work, orc_bicgstab_stats *st)
 {
-    bicgstab_run(c, eps, n_it_max, ss, nn, bb, phi, work, 0, 0.0, NULL, 0, st);
+    bicgstab_run(c, eps, n_it_max, ss, nn, bb, phi, work, 0, 0.0, NULL, 0, 0, st);
 }
 
 void orc_bicgstab_mixed(const orc_case *c, double eps, int n_it_max, const int ss[3], const int nn[3],
                         const double *bb, double *phi, int sweeps, double omega, orc_bicgstab_stats *st)
 {
-    bicgstab_run(c, eps, n_it_max, ss, nn, bb, phi, NULL, sweeps, omega, NULL, 0, st);
+    bicgstab_run(c, eps, n_it_max, ss, nn, bb, phi, NULL, sweeps, omega, NULL, 0, 0, st);
 }
 
 /* BiCGstab_orig with preconditioner 1 (multigridV, src/mod_solvers.f90:1684-1693,
  * 1750-1752); bb is not changed (the cycle averages edges of its own copies of
  * pp and rr only) */
+void orc_bicgstab_jor64(const orc_case *c, double eps, int n_it_max, const int ss[3], const int nn[3],
+                        const double *bb, double *phi, int sweeps, double omega, orc_bicgstab_stats *st)
+{
+    bicgstab_run(c, eps, n_it_max, ss, nn, bb, phi, NULL, sweeps, omega, NULL, 0, 1, st);
+}
+
 void orc_bicgstab_mg(const orc_case *c, orc_mg *mg, double eps, int n_it_max, const int ss[3], const int nn[3],
                      const double *bb, double *phi, orc_bicgstab_stats *st)
 {
-    bicgstab_run(c, eps, n_it_max, ss, nn, bb, phi, NULL, 0, 0.0, mg, 0, st);
+    bicgstab_run(c, eps, n_it_max, ss, nn, bb, phi, NULL, 0, 0.0, mg, 0, 0, st);
 }
 
 /* the same with the FP32 cycle (orc_mg_vcycle_f32) as M */
 void orc_bicgstab_mg_f32(const orc_case *c, orc_mg *mg, double eps, int n_it_max, const int ss[3], const int nn[3],
                          const double *bb, double *phi, orc_bicgstab_stats *st)
 {
-    bicgstab_run(c, eps, n_it_max, ss, nn, bb, phi, NULL, 0, 0.0, mg, 1, st);
+    bicgstab_run(c, eps, n_it_max, ss, nn, bb, phi, NULL, 0, 0.0, mg, 1, 0, st);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -575,7 +674,16 @@ void orc_timestep(orc_case *c, const orc_step_params *p, int step_index, double 
         if (p->jor_sweeps > 0) {
             if (p->jor_fp32)
                 orc_jor_mixed(c, p->jor_sweeps, p->jor_omega, ss, nn, pre, res);
-            else
+            else if (p->pressure_bc && p->ct_state && p->ct_lab && s == 2) {
+                /* device_jacobi_solver's substep-3 pressure before every sweep
+                 * and once more after the last (the exit trip applies it too) */
+                for (int r = 0; r < p->jor_sweeps; ++r) {
+                    jacobi_pressure_bc(c, p->ct_lab, p->ct_state->k_start, 3, dtime, p->ct_state->pres, pre);
+                    orc_jor(c, 1, p->jor_omega, ss, nn, pre, res, comp);
+                }
+                jacobi_pressure_bc(c, p->ct_lab, p->ct_state->k_start, 3, dtime, p->ct_state->pres, pre);
+                orc_exchange_relax(c, pre);
+            } else
                 orc_jor(c, p->jor_sweeps, p->jor_omega, ss, nn, pre, res, comp);
         }
         orc_bicgstab_stats bst;
@@ -584,26 +692,27 @@ void orc_timestep(orc_case *c, const orc_step_params *p, int step_index, double 
             /* the device keeps the initial guess and restarts plain FP64 BiCGstab
              * from it when the preconditioned iteration does not converge */
             memcpy(comp, pre, sizeof(double) * (size_t)n);
-            bicgstab_run(c, eps, p->bicg_max_it, ss, nn, res, pre, bw, p->precond_sweeps, p->jor_omega, NULL, 0, &bst);
+            bicgstab_run(c, eps, p->bicg_max_it, ss, nn, res, pre, bw, p->precond_sweeps, p->jor_omega, NULL, 0,
+                         p->precond_fp64, &bst);
             if (bst.exit_reason != 0) {
                 const int mixed_its = bst.iterations;
                 memcpy(pre, comp, sizeof(double) * (size_t)n);
-                bicgstab_run(c, eps, p->bicg_max_it, ss, nn, res, pre, bw, 0, 0.0, NULL, 0, &bst);
+                bicgstab_run(c, eps, p->bicg_max_it, ss, nn, res, pre, bw, 0, 0.0, NULL, 0, 0, &bst);
                 bst.iterations += mixed_its;
             }
         } else if (mg && p->mg_fp32) {
             /* the FP32 cycle: as with the FP32 sweeps, a solve that does not converge
              * is redone in FP64 (here with the FP64 cycle) from the saved guess */
             memcpy(comp, pre, sizeof(double) * (size_t)n);
-            bicgstab_run(c, eps, p->bicg_max_it, ss, nn, res, pre, bw, 0, 0.0, mg, 1, &bst);
+            bicgstab_run(c, eps, p->bicg_max_it, ss, nn, res, pre, bw, 0, 0.0, mg, 1, 0, &bst);
             if (bst.exit_reason != 0) {
                 const int mixed_its = bst.iterations;
                 memcpy(pre, comp, sizeof(double) * (size_t)n);
-                bicgstab_run(c, eps, p->bicg_max_it, ss, nn, res, pre, bw, 0, 0.0, mg, 0, &bst);
+                bicgstab_run(c, eps, p->bicg_max_it, ss, nn, res, pre, bw, 0, 0.0, mg, 0, 0, &bst);
                 bst.iterations += mixed_its;
             }
         } else {
-            bicgstab_run(c, eps, p->bicg_max_it, ss, nn, res, pre, bw, 0, 0.0, mg, 0, &bst);
+            bicgstab_run(c, eps, p->bicg_max_it, ss, nn, res, pre, bw, 0, 0.0, mg, 0, 0, &bst);
         }
         if (st) {
             st->bicg_iterations[s] = bst.iterations;
@@ -623,7 +732,9 @@ void orc_timestep(orc_case *c, const orc_step_params *p, int step_index, double 
     if (p->mask3)
         orc_apply_mask(c, p->mask3, vel);
     /* apply_CT_forcing follows the time update (src/mod_timeint.f90:405-416) */
-    if (p->ct_lab)
+    if (p->ct_lab && p->ct_state)
+        orc_ct_apply_forcing(c, p->ct_lab, p->ct_state, time + p->dtime, vel);
+    else if (p->ct_lab)
         orc_ct_forcing(c, p->ct_lab, p->ct_npts, p->ct_t, p->ct_q, time + p->dtime, vel);
     if (p->level_pressure)
         orc_level_pressure(c, pre);

@@ -18,7 +18,10 @@ LIB_PATH = ORACLE_DIR / "_build" / "liboracle.so"
 OMP_LIB_PATH = ORACLE_DIR / "_build" / "liboracle_omp.so"  # same code, point loops on OpenMP threads
 
 _lib = None
-_path = LIB_PATH
+# The OpenMP build runs the point loops on the host's threads; every point is
+# written by one thread and sums are not split, so its results are the serial
+# build's bit for bit.  It is used when built (the full-size parity tests need it).
+_path = OMP_LIB_PATH if OMP_LIB_PATH.exists() else LIB_PATH
 
 
 def use_parallel_build():
@@ -111,6 +114,19 @@ def lib():
         L.orc_ct_inflow.restype = C.c_double
         L.orc_ct_inflow.argtypes = [C.c_int, dp, dp, C.c_double, ip]
         L.orc_ct_forcing.argtypes = [vp, C.POINTER(C.c_uint8), C.c_int, dp, dp, C.c_double, dp]
+        L.orc_waveform_convert_branch.argtypes = [C.c_int, dp, dp, C.c_int, C.c_int, dp, dp, dp]
+        L.orc_wk_pressure_flow.argtypes = [C.c_int, C.c_double, dp, dp, dp, dp]
+        L.orc_wk_echo.argtypes = [C.c_int, dp, dp, dp, C.c_double, C.c_double, dp, dp]
+        L.orc_ct_apply_forcing.argtypes = [vp, C.POINTER(C.c_uint8), C.POINTER(CtState), C.c_double, dp]
+        L.orc_set_pressure_bc.argtypes = [vp, C.POINTER(C.c_uint8), C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
+                                          C.c_double, C.c_double, dp]
+        L.orc_bicgstab_jor64.argtypes = [vp, C.c_double, C.c_int, ip, ip, dp, dp, C.c_int, C.c_double,
+                                         C.POINTER(BicgStats)]
+        L.orc_set_threads.restype = C.c_int
+        L.orc_set_threads.argtypes = [C.c_int]
+        L.orc_device_jacobi_solver.restype = C.c_int
+        L.orc_device_jacobi_solver.argtypes = [vp, C.c_double, ip, ip, dp, dp, C.POINTER(C.c_uint8), C.c_int,
+                                               C.c_int, C.c_double, dp, C.c_int, ip]
         _lib = L
     return _lib
 
@@ -120,13 +136,35 @@ class BicgStats(C.Structure):
                 ("norm_inf_final", C.c_double), ("exit_reason", C.c_int)]
 
 
+class CtState(C.Structure):
+    """orc_ct_state: the module state of both apply_CT_forcing branches."""
+    _fields_ = [("ct_based", C.c_int), ("idealized", C.c_int), ("npts", C.c_int),
+                ("t_ms", C.POINTER(C.c_double)), ("q_raw", C.POINTER(C.c_double)),
+                ("t_conv", C.POINTER(C.c_double)), ("q_conv", C.POINTER(C.c_double)),
+                ("ref", C.c_double * 3), ("inlet_voxels", C.c_int), ("k_start", C.c_int),
+                ("wk_state", C.c_double * 3), ("pres", C.c_double * 3)]
+
+
+def ct_state(ct_based, idealized, t_ms, q_raw, t_conv, q_conv, ref, inlet_voxels, k_start):
+    """a CtState over the given arrays (kept alive on the object)"""
+    arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (t_ms, q_raw, t_conv, q_conv)]
+    st = CtState(ct_based=int(ct_based), idealized=int(idealized), npts=len(arrs[0]),
+                 t_ms=_dp(arrs[0]), q_raw=_dp(arrs[1]), t_conv=_dp(arrs[2]), q_conv=_dp(arrs[3]),
+                 inlet_voxels=int(inlet_voxels), k_start=int(k_start))
+    for d in range(3):
+        st.ref[d] = float(ref[d])
+    st._keep = arrs
+    return st
+
+
 class StepParams(C.Structure):
     _fields_ = [("Re", C.c_double), ("dtime", C.c_double), ("jor_sweeps", C.c_int),
                 ("jor_omega", C.c_double), ("bicg_rel_tol", C.c_double), ("bicg_max_it", C.c_int),
                 ("level_pressure", C.c_int), ("inflow_kind", C.c_int), ("inflow_amp", C.c_double),
                 ("mask3", C.POINTER(C.c_uint8)), ("jor_fp32", C.c_int), ("precond_sweeps", C.c_int),
                 ("precond_poisson", C.c_int), ("mg_fp32", C.c_int), ("ct_lab", C.POINTER(C.c_uint8)),
-                ("ct_npts", C.c_int), ("ct_t", C.POINTER(C.c_double)), ("ct_q", C.POINTER(C.c_double))]
+                ("ct_npts", C.c_int), ("ct_t", C.POINTER(C.c_double)), ("ct_q", C.POINTER(C.c_double)),
+                ("ct_state", C.POINTER(CtState)), ("pressure_bc", C.c_int), ("precond_fp64", C.c_int)]
 
 
 class StepStats(C.Structure):
@@ -266,6 +304,12 @@ class OracleCase:
         self.L.orc_bicgstab_mg_f32(self.h, mg.h, eps, nmax, _ip(ss), _ip(nn), _dp(bb), _dp(phi), C.byref(st))
         return st
 
+    def bicgstab_jor64(self, eps, nmax, ss, nn, bb, phi, sweeps, omega=0.8):
+        st = BicgStats()
+        self.L.orc_bicgstab_jor64(self.h, eps, nmax, _ip(ss), _ip(nn), _dp(bb), _dp(phi), int(sweeps), float(omega),
+                                  C.byref(st))
+        return st
+
     def jor_mixed(self, n, omega, ss, nn, rel, bb):
         self.L.orc_jor_mixed(self.h, n, omega, _ip(ss), _ip(nn), _dp(rel), _dp(bb))
 
@@ -382,6 +426,57 @@ def ct_inflow(t_conv, q_conv, time):
 def ct_forcing(case, lab, t_conv, q_conv, time, vel3):
     lib().orc_ct_forcing(case.h, lab.ctypes.data_as(C.POINTER(C.c_uint8)), len(t_conv), _dp(t_conv), _dp(q_conv),
                          float(time), _dp(vel3))
+
+
+def waveform_convert_branch(t_ms, q, M1, idealized):
+    t_ms = np.ascontiguousarray(t_ms, dtype=np.float64)
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    to, qo, ref = np.zeros_like(t_ms), np.zeros_like(q), np.zeros(3)
+    lib().orc_waveform_convert_branch(len(t_ms), _dp(t_ms), _dp(q), int(M1), int(idealized), _dp(to), _dp(qo),
+                                      _dp(ref))
+    return to, qo, ref
+
+
+def wk_pressure_flow(branch, time, ref, state):
+    """get_pressure_and_flow_rate_WKM: (p, q) nondimensional; state (3 doubles, p_end_systole_*) is updated"""
+    ref = np.ascontiguousarray(ref, dtype=np.float64)
+    p, q = np.zeros(1), np.zeros(1)
+    lib().orc_wk_pressure_flow(int(branch), float(time), _dp(ref), _dp(state), _dp(p), _dp(q))
+    return p[0], q[0]
+
+
+def wk_echo(t_ms, q_raw, t_conv, time, ref_velocity):
+    pi, po = np.zeros(1), np.zeros(1)
+    lib().orc_wk_echo(len(t_ms), _dp(np.ascontiguousarray(t_ms, dtype=np.float64)),
+                      _dp(np.ascontiguousarray(q_raw, dtype=np.float64)),
+                      _dp(np.ascontiguousarray(t_conv, dtype=np.float64)), float(time), float(ref_velocity),
+                      _dp(pi), _dp(po))
+    return pi[0], po[0]
+
+
+def ct_apply_forcing(case, lab, st: CtState, time, vel3):
+    lib().orc_ct_apply_forcing(case.h, lab.ctypes.data_as(C.POINTER(C.c_uint8)), C.byref(st), float(time), _dp(vel3))
+
+
+def set_pressure_bc(case, lab, types, k_start, p_in, p_out, p_out2, rel):
+    lib().orc_set_pressure_bc(case.h, lab.ctypes.data_as(C.POINTER(C.c_uint8)), int(types[0]), int(types[1]),
+                              int(types[2]), int(k_start), float(p_in), float(p_out), float(p_out2), _dp(rel))
+
+
+def device_jacobi_solver(case, omega, ss, nn, rel, bb, lab=None, k_start=0, substep=0, dtime=0.0, pres=(0, 0, 0),
+                         max_iter=200000):
+    """returns (status, sweeps)"""
+    sweeps = (C.c_int * 1)()
+    pres = np.ascontiguousarray(pres, dtype=np.float64)
+    labp = lab.ctypes.data_as(C.POINTER(C.c_uint8)) if lab is not None else None
+    rc = lib().orc_device_jacobi_solver(case.h, float(omega), _ip(ss), _ip(nn), _dp(rel), _dp(bb), labp,
+                                        int(k_start), int(substep), float(dtime), _dp(pres), int(max_iter), sweeps)
+    return rc, sweeps[0]
+
+
+def set_threads(n: int) -> int:
+    """threads of the OpenMP build's point loops (n <= 0 only queries)"""
+    return int(lib().orc_set_threads(int(n)))
 
 
 def exact_table(abl, n, dx):

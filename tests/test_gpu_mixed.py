@@ -89,3 +89,35 @@ def test_mixed_against_fp64():
     s = box(oc.Sp, oc.Np)
     assert np.linalg.norm(p32[s] - p64[s]) <= 1e-5 * np.linalg.norm(p64[s])
     assert 2 * n32 < n64, (n32, n64)
+
+
+@pytest.mark.parametrize("spec", GRIDS + MULTI)
+def test_bicgstab_jor64(spec):
+    """The like-for-like FP64 baseline: the same right-preconditioned BiCGstab
+    with M = FP64 JOR sweeps, bit for bit against the oracle."""
+    oc, gc = pair(spec)
+    xt = rnd(oc.shape, 192)
+    bb = oc.zeros()
+    oc.product_div_grad(xt.copy(), bb, oc.zeros())
+    phi = np.ascontiguousarray(0.1 * rnd(oc.shape, 193))
+    gc.upload(ih.RES, bb)
+    gc.upload(ih.PRE, phi)
+    eps = 1e-9 * np.abs(bb).max()
+    so = oc.bicgstab_jor64(eps, 500, oc.Sp, oc.Np, bb, phi, 3, 0.8)
+    sg = gc.bicgstab_jor64(eps, 500, oc.Sp, oc.Np, 3, 0.8)
+    assert so.iterations == sg.iterations and so.exit_reason == sg.exit_reason == 0
+    assert so.norm_inf_final == sg.norm_inf_final
+    same(gc.download(ih.PRE), phi, "FP64-preconditioned bicgstab phi")
+
+
+def test_timestep_fp64_preconditioned_bit_exact():
+    oc, gc, vel, pre, nl, po, pg, mask = _tube_case(False)
+    po.precond_sweeps = pg.precond_sweeps = 4
+    po.precond_fp64 = pg.precond_fp64 = 1
+    t = 0.0
+    for step in range(2):
+        so = oc.timestep(po, step, t, vel, pre, nl)
+        sg = gc.timestep(pg, t)
+        t += 5e-4
+        assert list(so.bicg_iterations) == list(sg.bicg_iterations)
+        compare_state(oc, gc, vel, pre, nl, f"FP64-preconditioned step {step}")

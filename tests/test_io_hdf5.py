@@ -323,9 +323,10 @@ def test_field_box_transfer():
 @pytest.mark.gpu
 def test_read_hdf_places_the_file_as_read2_hdf_does(tmp_path):
     """read2_hdf reads the file's S?w..M?w (its attributes) into the overlap with
-    the target range, from file offset 0 (src/mod_inout.f90:1604-1690): a
-    stride-2 output lands in the first points of the range, and a pressure
-    file read as velocity 1 fills the points both ranges share."""
+    the target range, from file offset 0 (src/mod_inout.f90:1604-1690), after
+    setting the whole field to 0 (:1675): a stride-2 output lands in the first
+    points of the range, a pressure file read as velocity 1 fills the points both
+    ranges share, and every other point (ghosts included) reads 0."""
     from test_gpu_parity import rnd
     oc, gc = _pair(WALLS)
     pre = rnd(oc.shape, 440)
@@ -338,7 +339,7 @@ def test_read_hdf_places_the_file_as_read2_hdf_does(tmp_path):
     gc.read_hdf(str(tmp_path / "p2"), "pre", ih.TMP3, 0, st2)
     assert st2.time == st.time and st2.timestep == st.timestep and st2.write_count == st.write_count
     dims = [(oc.N[d] - 1) // 2 + 1 for d in range(3)]
-    want = base.copy()
+    want = np.zeros_like(base)  # read2_hdf sets phi = 0. first (src/mod_inout.f90:1675)
     strided = pre[tuple(slice(4, 4 + 2 * dims[d], 2) for d in (2, 1, 0))]
     want[tuple(slice(4, 4 + dims[d]) for d in (2, 1, 0))] = strided
     assert np.array_equal(gc.download(ih.TMP3), want)
@@ -346,7 +347,7 @@ def test_read_hdf_places_the_file_as_read2_hdf_does(tmp_path):
     gc.write_hdf(str(tmp_path / "p1"), "pre", ih.PRE, 0, (1, 1, 1), st)
     gc.upload(ih.TMP3, base)
     gc.read_hdf(str(tmp_path / "p1"), "pre", ih.TMP3, 1)
-    want = base.copy()
+    want = np.zeros_like(base)
     s = tuple(slice(4, oc.N[d] + 4) for d in (2, 1, 0))
     want[s] = pre[s]
     assert np.array_equal(gc.download(ih.TMP3), want)

@@ -28,10 +28,10 @@ IH_ABI_VERSION = 4
  RR, RH, PK, AP, AR, TMP1, TMP2, TMP3, TMP4, Z1, Z2, NFIELDS) = range(30)
 
 
-class IHError(RuntimeError):
 IH_ENOTCONV = -1002
 
 
+class IHError(RuntimeError):
     pass
 
 
@@ -161,12 +161,12 @@ def load(path: os.PathLike | None = None):
         "ih_bicgstab": (C.c_int, [vp, C.c_double, C.c_int, ip, ip, C.c_int, C.c_int, C.POINTER(BicgStats)]),
         "ih_bicgstab_mixed": (C.c_int, [vp, C.c_double, C.c_int, ip, ip, C.c_int, C.c_int, C.c_int, C.c_double,
                                         C.POINTER(BicgStats)]),
+        "ih_bicgstab_jor64": (C.c_int, [vp, C.c_double, C.c_int, ip, ip, C.c_int, C.c_int, C.c_int, C.c_double,
+                                        C.POINTER(BicgStats)]),
         "ih_bicgstab_precond": (C.c_int, [vp, C.c_double, C.c_int, ip, ip, C.c_int, C.c_int, C.c_int,
                                           C.POINTER(BicgStats)]),
         "ih_mg_configure": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int]),
         "ih_mg_levels": (C.c_int, [vp, ip, ip]),
-        "ih_bicgstab_jor64": (C.c_int, [vp, C.c_double, C.c_int, ip, ip, C.c_int, C.c_int, C.c_int, C.c_double,
-                                        C.POINTER(BicgStats)]),
         "ih_mg_get_coeff": (C.c_int, [vp, C.c_int, C.c_char_p, dp, C.POINTER(C.c_int64)]),
         "ih_multigrid_v": (C.c_int, [vp, C.c_int, C.c_int]),
         "ih_multigrid_v_f32": (C.c_int, [vp, C.c_int, C.c_int]),
@@ -176,16 +176,16 @@ def load(path: os.PathLike | None = None):
         "ih_ct_get_labels": (C.c_int, [vp, C.POINTER(C.c_uint8)]),
         "ih_ct_inflow": (C.c_int, [vp, C.c_double, dp, ip]),
         "ih_ct_forcing": (C.c_int, [vp, C.c_double]),
-        "ih_fill_corners": (C.c_int, [vp, C.c_int]),
-        "ih_level_pressure": (C.c_int, [vp, C.c_int]),
-        "ih_set_tube_mask": (C.c_int, [vp, C.c_double, C.c_double, dp, dp]),
-        "ih_get_mask": (C.c_int, [vp, C.POINTER(C.c_uint8)]),
         "ih_ct_configure": (C.c_int, [vp, C.c_int, C.c_int]),
         "ih_ct_get_state": (C.c_int, [vp, dp, dp]),
         "ih_ct_set_state": (C.c_int, [vp, dp, dp]),
         "ih_set_pressure_bc": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
                                          C.c_double]),
         "ih_case_set_substep": (C.c_int, [vp, C.c_int, C.c_double]),
+        "ih_fill_corners": (C.c_int, [vp, C.c_int]),
+        "ih_level_pressure": (C.c_int, [vp, C.c_int]),
+        "ih_set_tube_mask": (C.c_int, [vp, C.c_double, C.c_double, dp, dp]),
+        "ih_get_mask": (C.c_int, [vp, C.POINTER(C.c_uint8)]),
         "ih_apply_mask": (C.c_int, [vp]),
         "ih_timestep": (C.c_int, [vp, C.POINTER(StepParams), C.c_double, C.POINTER(StepStats)]),
         "ih_device_jacobi_solver": (C.c_int, [vp, C.c_int, C.c_double] + [C.c_int] * 6 + [dp, dp]),
@@ -204,18 +204,18 @@ def load(path: os.PathLike | None = None):
         "ih_batch_get_groups": (C.c_int, [vp, C.POINTER(C.c_int64), ip]),
         "ih_batch_jor": (C.c_int, [vp, C.c_int, C.c_double, C.c_int, C.c_int]),
         "ih_grid_bytes": (C.c_int64, [C.POINTER(GridDesc)]),
+        "ih_grid_bytes_step": (C.c_int64, [C.POINTER(GridDesc), C.POINTER(StepParams)]),
         "ih_serve": (C.c_int, [C.POINTER(ServeConfig), C.POINTER(Request), C.c_int, C.POINTER(RequestResult),
                                C.POINTER(ServeSummary)]),
         "ih_timer_start": (C.c_int, [vp]),
         "ih_timer_stop": (C.c_int, [vp, dp]),
-        "ih_grid_bytes_step": (C.c_int64, [C.POINTER(GridDesc), C.POINTER(StepParams)]),
         "ih_kernel_timing_enable": (C.c_int, [vp, C.c_int]),
         "ih_kernel_timing_get": (C.c_int, [vp, C.c_char_p, dp, i64p]),
+        "ih_bench_kernel": (C.c_int, [vp, C.c_char_p, C.c_int, dp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
         fn.restype = res
-        "ih_bench_kernel": (C.c_int, [vp, C.c_char_p, C.c_int, dp]),
         fn.argtypes = args
     if L.ih_abi_version() != IH_ABI_VERSION:
         raise IHError("ABI version mismatch")
@@ -496,6 +496,13 @@ class Case:
                                          float(omega), C.byref(st)), "bicgstab_mixed")
         return st
 
+    def bicgstab_jor64(self, eps, nmax, ss, nn, sweeps, omega=0.8, bb=RES, phi=PRE) -> BicgStats:
+        """the FP64-swept form of bicgstab_mixed (the like-for-like FP64 baseline)"""
+        st = BicgStats()
+        check(self.lib.ih_bicgstab_jor64(self.h, float(eps), int(nmax), _ints(ss), _ints(nn), bb, phi, int(sweeps),
+                                         float(omega), C.byref(st)), "bicgstab_jor64")
+        return st
+
     def bicgstab_precond(self, eps, nmax, ss, nn, preconditioner, bb=RES, phi=PRE) -> BicgStats:
         """BiCGstab_orig with the reference's preconditioner argument (0 none, 1 multigrid V cycle)."""
         st = BicgStats()
@@ -514,13 +521,6 @@ class Case:
         return [tuple(nn[3 * g:3 * g + 3]) for g in range(G.value)]
 
     def mg_coeff(self, level, name) -> np.ndarray:
-    def bicgstab_jor64(self, eps, nmax, ss, nn, sweeps, omega=0.8, bb=RES, phi=PRE) -> BicgStats:
-        """the FP64-swept form of bicgstab_mixed (the like-for-like FP64 baseline)"""
-        st = BicgStats()
-        check(self.lib.ih_bicgstab_jor64(self.h, float(eps), int(nmax), _ints(ss), _ints(nn), bb, phi, int(sweeps),
-                                         float(omega), C.byref(st)), "bicgstab_jor64")
-        return st
-
         n = C.c_int64(0)
         check(self.lib.ih_mg_get_coeff(self.h, int(level), name.encode(), None, C.byref(n)), "mg_get_coeff")
         out = np.zeros(n.value)
@@ -563,24 +563,6 @@ class Case:
     def ct_forcing(self, time):
         check(self.lib.ih_ct_forcing(self.h, float(time)), "ct_forcing")
 
-    def fill_corners(self, f=PRE):
-        check(self.lib.ih_fill_corners(self.h, f), "fill_corners")
-
-    def level_pressure(self, f=PRE):
-        check(self.lib.ih_level_pressure(self.h, f), "level_pressure")
-
-    def set_tube_mask(self, yc, zc, r_u, r_p):
-        check(self.lib.ih_set_tube_mask(self.h, float(yc), float(zc), _dp(np.ascontiguousarray(r_u, dtype=np.float64)),
-                                        _dp(np.ascontiguousarray(r_p, dtype=np.float64))), "set_tube_mask")
-
-    def get_mask(self) -> np.ndarray:
-        m = np.zeros((3,) + self.shape, dtype=np.uint8)
-        check(self.lib.ih_get_mask(self.h, m.ctypes.data_as(C.POINTER(C.c_uint8))), "get_mask")
-        return m
-
-    def apply_mask(self):
-        check(self.lib.ih_apply_mask(self.h), "apply_mask")
-
     def ct_configure(self, ct_based: int, idealized: int):
         """apply_CT_forcing's branches (CT_based_aorta, idealized_aorta); before ct_load_vtk"""
         check(self.lib.ih_ct_configure(self.h, int(ct_based), int(idealized)), "ct_configure")
@@ -601,6 +583,24 @@ class Case:
 
     def set_substep(self, substep: int, dtime: float):
         check(self.lib.ih_case_set_substep(self.h, int(substep), float(dtime)), "set_substep")
+
+    def fill_corners(self, f=PRE):
+        check(self.lib.ih_fill_corners(self.h, f), "fill_corners")
+
+    def level_pressure(self, f=PRE):
+        check(self.lib.ih_level_pressure(self.h, f), "level_pressure")
+
+    def set_tube_mask(self, yc, zc, r_u, r_p):
+        check(self.lib.ih_set_tube_mask(self.h, float(yc), float(zc), _dp(np.ascontiguousarray(r_u, dtype=np.float64)),
+                                        _dp(np.ascontiguousarray(r_p, dtype=np.float64))), "set_tube_mask")
+
+    def get_mask(self) -> np.ndarray:
+        m = np.zeros((3,) + self.shape, dtype=np.uint8)
+        check(self.lib.ih_get_mask(self.h, m.ctypes.data_as(C.POINTER(C.c_uint8))), "get_mask")
+        return m
+
+    def apply_mask(self):
+        check(self.lib.ih_apply_mask(self.h), "apply_mask")
 
     def timestep(self, params: StepParams, time: float, stats: bool = True) -> StepStats | None:
         st = StepStats()
@@ -638,6 +638,12 @@ class Case:
         return rc
 
     # --- timing ------------------------------------------------------------
+    def bench_kernel(self, name: str, launches: int) -> float:
+        """ms per launch of one kernel, back to back on the case's stream"""
+        ms = C.c_double(0.0)
+        check(self.lib.ih_bench_kernel(self.h, name.encode(), int(launches), C.byref(ms)), "bench_kernel")
+        return ms.value
+
     def timer_start(self):
         check(self.lib.ih_timer_start(self.h), "timer_start")
 
@@ -656,12 +662,6 @@ class Batch:
         arr = (C.c_void_p * len(self.cases))(*[c.h for c in self.cases])
         h = C.c_void_p()
         check(self.lib.ih_batch_create(arr, len(self.cases), C.byref(h)), "batch_create")
-    def bench_kernel(self, name: str, launches: int) -> float:
-        """ms per launch of one kernel, back to back on the case's stream"""
-        ms = C.c_double(0.0)
-        check(self.lib.ih_bench_kernel(self.h, name.encode(), int(launches), C.byref(ms)), "bench_kernel")
-        return ms.value
-
         self.h = h
 
     def close(self):

@@ -171,6 +171,12 @@ int ih_bicgstab(ih_case *c, double eps, int n_it_max, const int ss[3], const int
  * a multigrid V cycle instead. */
 int ih_bicgstab_mixed(ih_case *c, double eps, int n_it_max, const int ss[3], const int nn[3], ih_field bb,
                       ih_field phi, int sweeps, double omega, ih_bicgstab_stats *st);
+/* No counterpart in the reference: the same right-preconditioned BiCGstab with
+ * M = `sweeps` JOR sweeps from zero in FP64 -- the like-for-like FP64 baseline
+ * of ih_bicgstab_mixed, which isolates what the FP32 sweeps gain.  Uses TMP3,
+ * TMP4 and DIG as work fields. */
+int ih_bicgstab_jor64(ih_case *c, double eps, int n_it_max, const int ss[3], const int nn[3], ih_field bb,
+                      ih_field phi, int sweeps, double omega, ih_bicgstab_stats *st);
 
 /* BiCGstab_orig with the reference's preconditioner argument: 0 none (as
  * ih_bicgstab), 1 the multigrid V cycle multigridV (src/mod_solvers.f90:1684-1693,
@@ -183,12 +189,6 @@ int ih_bicgstab_precond(ih_case *c, double eps, int n_it_max, const int ss[3], c
 /* Levels and relaxations per level (prog/config.txt:108-116: 15, 4, 4, 4, the
  * defaults).  The cycle follows multigridV (src/mod_solvers.f90:456-585) with
  * Jacobi smoothing (omega 0.98, src/mod_laplace.f90:470) on each level's
-/* No counterpart in the reference: the same right-preconditioned BiCGstab with
- * M = `sweeps` JOR sweeps from zero in FP64 -- the like-for-like FP64 baseline
- * of ih_bicgstab_mixed, which isolates what the FP32 sweeps gain.  Uses TMP3,
- * TMP4 and DIG as work fields. */
-int ih_bicgstab_jor64(ih_case *c, double eps, int n_it_max, const int ss[3], const int nn[3], ih_field bb,
-                      ih_field phi, int sweeps, double omega, ih_bicgstab_stats *st);
  * pressure range; the hierarchy halves a direction while N-1 is even and N >= 5
  * (src/mod_setup.f90:96-143).  Walls or periodic pairs in every direction. */
 int ih_mg_configure(ih_case *c, int n_grids_limit, int n_relax_down, int n_relax_up, int n_relax_bottom);
@@ -260,18 +260,6 @@ int ih_ct_configure(ih_case *c, int ct_based, int idealized);
  * The Windkessel keeps p_end_systole_* between calls as the reference's module
  * variables do (zero after ih_ct_load_vtk). */
 int ih_ct_forcing(ih_case *c, double time);
-
-/* ---- projection time step ------------------------------------------------ */
-typedef struct {
-    double Re, dtime;
-    int jor_sweeps;
-    double jor_omega;
-    double bicg_rel_tol;
-    int bicg_max_it;
-    int level_pressure;
-    int inflow_kind; /* 0 none, 1 parabola 16y(1-y)z(1-z), 2 pulsatile plug in the mask */
-    double inflow_amp;
-    int use_mask;
 /* The module state the forcing leaves: pres = pre_inlet, pre_outlet,
  * pre_outlet_2 (nondimensional) and wk = p_end_systole_in, _out, _out_2 (mmHg).
  * set: for a host that computes them itself, or a restart. */
@@ -294,11 +282,26 @@ int ih_set_pressure_bc(ih_case *c, ih_field rel, int type1, int type2, int type3
  * last forcing times (aRK + bRK) dtime before every sweep
  * (src/jacobi_solver_host_v4.f90:302-305). */
 int ih_case_set_substep(ih_case *c, int substep, double dtime);
+
+/* ---- projection time step ------------------------------------------------ */
+typedef struct {
+    double Re, dtime;
+    int jor_sweeps;
+    double jor_omega;
+    double bicg_rel_tol;
+    int bicg_max_it;
+    int level_pressure;
+    int inflow_kind; /* 0 none, 1 parabola 16y(1-y)z(1-z), 2 pulsatile plug in the mask */
+    double inflow_amp;
+    int use_mask;
     int jor_fp32;   /* 1: JOR sweeps in FP32 (residual and update kept FP64) */
     int precond_sweeps; /* > 0: BiCGstab right-preconditioned by this many FP32 JOR sweeps */
     int precond_poisson; /* with precond_sweeps 0: BiCGstab's preconditioner, 0 none, 1 multigrid V cycle */
     int mg_fp32;         /* with precond_poisson 1: the cycle in FP32 (an FP64-cycle solve redoes a failure) */
     int use_ct;          /* 1: apply_CT_forcing with the loaded CT labels at the step's end time */
+    int pressure_bc;     /* 1 (with CT labels, FP64 sweeps): set_pressure_BC before every JOR sweep of
+                            substep 3 and after the last, as device_jacobi_solver does */
+    int precond_fp64;    /* with precond_sweeps > 0: the preconditioner's sweeps in FP64 (ih_bicgstab_jor64) */
 } ih_step_params;
 
 typedef struct {
@@ -311,14 +314,17 @@ typedef struct {
  * stats may be NULL; when given, div_max_after costs one extra divergence.
  * div_max_after is max |div u| over the pressure range at the end of the step:
  * after the mask and the CT forcing, which set velocities in the wall and the
-    int pressure_bc;     /* 1 (with CT labels, FP64 sweeps): set_pressure_BC before every JOR sweep of
-                            substep 3 and after the last, as device_jacobi_solver does */
-    int precond_fp64;    /* with precond_sweeps > 0: the preconditioner's sweeps in FP64 (ih_bicgstab_jor64) */
  * jet and so leave divergence at those interfaces; without them it is what
  * the projection leaves. */
 int ih_timestep(ih_case *c, const ih_step_params *p, double time, ih_step_stats *st);
 
 /* ---- reference-signature entry points (host arrays in, host arrays out) --- */
+/* device_jacobi_solver: JOR sweeps with the reference's stopping test every 20
+ * sweeps (|log(err_tot_old / err_tot)| against its threshold, err_tot the sum
+ * of |rel - rel_old| over the box in the fixed reduction order) and, on
+ * substep 3 with CT labels, the Dirichlet pressure (ih_case_set_substep).  An
+ * err_tot of 0 (a fixed point) stops.  After 200000 sweeps it stops with
+ * IH_ENOTCONV, rel holding the last iterate. */
 int ih_device_jacobi_solver(ih_case *c, int grid_id, double omega, int SS1, int SS2, int SS3, int NN1, int NN2,
                             int NN3, double *rel, double *bb);
 int ih_device_gradient(ih_case *c, int dimension_id, double *phi, double *grad);
@@ -331,12 +337,6 @@ int ih_device_axpby(ih_case *c, double a, double b, double *x, double *y, double
 int ih_device_interpolate(ih_case *c, int dimension_id, double *phi, double *inter);
 int ih_device_interpolate2(ih_case *c, int dimension_id, double *phi, double *inter);
 
-/* device_jacobi_solver: JOR sweeps with the reference's stopping test every 20
- * sweeps (|log(err_tot_old / err_tot)| against its threshold, err_tot the sum
- * of |rel - rel_old| over the box in the fixed reduction order) and, on
- * substep 3 with CT labels, the Dirichlet pressure (ih_case_set_substep).  An
- * err_tot of 0 (a fixed point) stops.  After 200000 sweeps it stops with
- * IH_ENOTCONV, rel holding the last iterate. */
 /* ---- batched relaxation of many cases ------------------------------------ */
 /* Many cases (uneven grids, one device) relaxed together: each sweep is one
  * launch over the tiles of all cases.  No reference counterpart: the reference
@@ -404,6 +404,11 @@ typedef struct {
     double makespan_s;         /* first arrival to last finish (set-up excluded) */
     int64_t peak_bytes;        /* largest sum of the running requests' slot bytes */
     int peak_active;
+    double setup_s;            /* building the slot pool, before the trace starts */
+    int64_t pool_bytes;        /* HBM held by the pool (at most hbm_budget) */
+    int pool_slots;
+    double admit_wait_max_s;   /* arrival to admission */
+    double admit_wait_mean_s;
 } ih_serve_summary;
 
 /* HBM a case of grid g holds (fields, FP32 buffers, mask, tables). */
@@ -425,19 +430,9 @@ int ih_serve(const ih_serve_config *cfg, const ih_request *req, int n, ih_reques
 /* ---- timing of the case's stream (for benchmarks) ------------------------ */
 int ih_timer_start(ih_case *c);
 int ih_timer_stop(ih_case *c, double *ms);
-    double setup_s;            /* building the slot pool, before the trace starts */
-    int64_t pool_bytes;        /* HBM held by the pool (at most hbm_budget) */
-    int pool_slots;
-    double admit_wait_max_s;   /* arrival to admission */
-    double admit_wait_mean_s;
 /* Per-kernel-family launch statistics collected with HIP events when enabled. */
 int ih_kernel_timing_enable(ih_case *c, int on);
 int ih_kernel_timing_get(ih_case *c, const char *family, double *total_ms, int64_t *launches);
-
-#ifdef __cplusplus
-}
-#endif
-#endif
 /* `launches` launches of one kernel back to back between two events on the
  * case's stream: "jor" (an FP64 sweep over the pressure range), "apply" (the
  * fused operator alone), and the five kernels of a BiCGstab iteration --
@@ -445,3 +440,8 @@ int ih_kernel_timing_get(ih_case *c, const char *family, double *total_ms, int64
  * (operator with <r, Ar>, <Ar, Ar>), "x_update" -- on the case's Krylov
  * fields (their results are meaningless). */
 int ih_bench_kernel(ih_case *c, const char *name, int launches, double *ms_per_launch);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1844,7 +1844,15 @@ static int poisson_solve(ih_case *c, const ih_step_params *p, const Box &b, int 
     return rc;
 }
 
-int ih_timestep(ih_case *c, const ih_step_params *p, double time, ih_step_stats *st) {
+}  // extern "C"
+
+namespace ih {
+
+// The phases of a time step (ih_timestep and the batched step run the same
+// functions, so a batched case is bit-identical to its own step).
+
+// the checks of ih_timestep
+int step_check(ih_case *c, const ih_step_params *p) {
     int rc = need_coeffs(c);
     if (rc) return rc;
     if (!p || !(p->dtime > 0.0) || !(p->Re > 0.0)) return fail(IH_EINVAL, "bad step parameters");
@@ -1852,72 +1860,85 @@ int ih_timestep(ih_case *c, const ih_step_params *p, double time, ih_step_stats 
     if (p->pressure_bc && (!c->ctlab || p->jor_fp32))
         return fail(p->jor_fp32 ? IH_EINVAL : IH_ESTATE,
                     "the Dirichlet pressure of the Jacobi sweeps needs CT labels and FP64 sweeps");
-    HIPCHK(hipSetDevice(c->device));
+    if (p->use_mask && !c->has_mask) return fail(IH_ESTATE, "no mask set");
+    return IH_OK;
+}
+
+// substep sub (0..2) up to the Poisson right-hand side: ghost fill, velocity
+// interpolation, rhs_vel (nonlinear + Helmholtz fused per component), the
+// Dirichlet planes, the pressure scaled as the increment's guess, vel = rhs and
+// div vel into RES
+int step_pre(ih_case *c, const ih_step_params *p, double time, int sub) {
     const Geo &g = c->g;
     hipStream_t s = c->stream;
     const double dtime = p->dtime;
     const Box pb = pbox(g);
-    for (int sub = 0; sub < 3; ++sub) {
-        double subtime;
-        if (sub == 0)
-            subtime = time + dtime * aRK[0];
-        else if (sub == 1)
-            subtime = time + dtime * (aRK[0] + aRK[1] + bRK[1]);
-        else
-            subtime = time + dtime;
+    int rc;
+    double subtime;
+    if (sub == 0)
+        subtime = time + dtime * aRK[0];
+    else if (sub == 1)
+        subtime = time + dtime * (aRK[0] + aRK[1] + bRK[1]);
+    else
+        subtime = time + dtime;
 
-        // exchange_all_all(.TRUE., vel)
-        double *vel[3] = {F(c, IH_VEL1), F(c, IH_VEL2), F(c, IH_VEL3)};
-        for (int dir = 1; dir <= 3; ++dir) {
-            int vd[3] = {1, 2, 3};
-            launch_exchange(s, g, dir, 3, vel, vd);
-        }
-        LAUNCHCHK();
-        if ((rc = ih_interpolate_vel(c))) return rc;
-
-        // rhs_vel: nonlinear + Helmholtz + rhs assembly fused per component
-        const double mult_b = dtime * bRK[sub];
-        const double mult_a = dtime * aRK[sub];
-        for (int q = 0; q < 3; ++q)
-            launch_nl_helm_rhs(s, g, c->c, q, F(c, IH_WORK1), F(c, IH_WORK2), F(c, IH_WORK3), F(c, IH_VEL1 + q),
-                               F(c, IH_NL1 + q), F(c, IH_RHS1 + q), 1.0 / p->Re, mult_a, mult_b, sub != 0 ? 1 : 0);
-        LAUNCHCHK();
-        double amp = p->inflow_amp;
-        if (p->inflow_kind == 2) {
-            const double pi = 3.14159265358979323846;
-            amp = p->inflow_amp *
-                  (1.0 + 0.5 * std::sin(2.0 * pi * subtime) + 0.2 * std::sin(4.0 * pi * subtime + 0.7) +
-                   0.1 * std::sin(6.0 * pi * subtime + 1.3));
-        }
-        launch_bc_planes(s, g, c->c, F(c, IH_RHS1), F(c, IH_RHS2), F(c, IH_RHS3), p->inflow_kind, amp,
-                         p->inflow_kind == 2 ? c->mask : nullptr);
-        LAUNCHCHK();
-
-        // pressure scaled as the increment's initial guess
-        launch_scale_box(s, g, pb, F(c, IH_PRE), aRK[sub] + bRK[sub], dtime, 0);
-
-        // explicit: vel = rhs (on the B range: swap the buffers), div, Poisson, correction
-        for (int q = 0; q < 3; ++q) std::swap(c->fields[IH_VEL1 + q], c->fields[IH_RHS1 + q]);
-        if ((rc = ih_divergence2(c, IH_VEL1, IH_VEL2, IH_VEL3, IH_RES))) return rc;
-        ih_bicgstab_stats bst{};
-        if ((rc = poisson_solve(c, p, pb, sub + 1, &bst))) return rc;
-        if (st) {
-            st->bicg_iterations[sub] = bst.iterations;
-            st->bicg_final[sub] = bst.norm_inf_final;
-        }
-        for (int m = 1; m <= 3; ++m) {
-            if ((rc = exch(c, m, 0, F(c, IH_PRE)))) return rc;
-            launch_grad_correct(s, g, c->c, m - 1, F(c, IH_PRE), F(c, IH_VEL1 + m - 1));
-        }
-        LAUNCHCHK();
-        launch_scale_box(s, g, pb, F(c, IH_PRE), aRK[sub] + bRK[sub], dtime, 1);
-        launch_fill_corners(s, g, F(c, IH_PRE));
-        LAUNCHCHK();
+    // exchange_all_all(.TRUE., vel)
+    double *vel[3] = {F(c, IH_VEL1), F(c, IH_VEL2), F(c, IH_VEL3)};
+    for (int dir = 1; dir <= 3; ++dir) {
+        int vd[3] = {1, 2, 3};
+        launch_exchange(s, g, dir, 3, vel, vd);
     }
-    if (p->use_mask) {
-        if (!c->has_mask) return fail(IH_ESTATE, "no mask set");
+    LAUNCHCHK();
+    if ((rc = ih_interpolate_vel(c))) return rc;
+
+    // rhs_vel: nonlinear + Helmholtz + rhs assembly fused per component
+    const double mult_b = dtime * bRK[sub];
+    const double mult_a = dtime * aRK[sub];
+    for (int q = 0; q < 3; ++q)
+        launch_nl_helm_rhs(s, g, c->c, q, F(c, IH_WORK1), F(c, IH_WORK2), F(c, IH_WORK3), F(c, IH_VEL1 + q),
+                           F(c, IH_NL1 + q), F(c, IH_RHS1 + q), 1.0 / p->Re, mult_a, mult_b, sub != 0 ? 1 : 0);
+    LAUNCHCHK();
+    double amp = p->inflow_amp;
+    if (p->inflow_kind == 2) {
+        const double pi = 3.14159265358979323846;
+        amp = p->inflow_amp *
+              (1.0 + 0.5 * std::sin(2.0 * pi * subtime) + 0.2 * std::sin(4.0 * pi * subtime + 0.7) +
+               0.1 * std::sin(6.0 * pi * subtime + 1.3));
+    }
+    launch_bc_planes(s, g, c->c, F(c, IH_RHS1), F(c, IH_RHS2), F(c, IH_RHS3), p->inflow_kind, amp,
+                     p->inflow_kind == 2 ? c->mask : nullptr);
+    LAUNCHCHK();
+
+    // pressure scaled as the increment's initial guess
+    launch_scale_box(s, g, pb, F(c, IH_PRE), aRK[sub] + bRK[sub], dtime, 0);
+
+    // explicit: vel = rhs (on the B range: swap the buffers), then div vel
+    for (int q = 0; q < 3; ++q) std::swap(c->fields[IH_VEL1 + q], c->fields[IH_RHS1 + q]);
+    return ih_divergence2(c, IH_VEL1, IH_VEL2, IH_VEL3, IH_RES);
+}
+
+// substep sub after the Poisson solve: u -= G p, the pressure scaled back, corners
+int step_post(ih_case *c, const ih_step_params *p, int sub) {
+    const Geo &g = c->g;
+    hipStream_t s = c->stream;
+    int rc;
+    for (int m = 1; m <= 3; ++m) {
+        if ((rc = exch(c, m, 0, F(c, IH_PRE)))) return rc;
+        launch_grad_correct(s, g, c->c, m - 1, F(c, IH_PRE), F(c, IH_VEL1 + m - 1));
+    }
+    LAUNCHCHK();
+    launch_scale_box(s, g, pbox(g), F(c, IH_PRE), aRK[sub] + bRK[sub], p->dtime, 1);
+    launch_fill_corners(s, g, F(c, IH_PRE));
+    LAUNCHCHK();
+    return IH_OK;
+}
+
+// after the last substep: the IBM mask, the CT forcing, pressure levelling and
+// the step's divergence statistic
+int step_end(ih_case *c, const ih_step_params *p, double time, ih_step_stats *st) {
+    int rc;
+    if (p->use_mask)
         if ((rc = ih_apply_mask(c))) return rc;
-    }
     // apply_CT_forcing follows the time update (src/mod_timeint.f90:405-416)
     if (p->use_ct)
         if ((rc = ih_ct_forcing(c, time + p->dtime))) return rc;
@@ -1925,10 +1946,33 @@ int ih_timestep(ih_case *c, const ih_step_params *p, double time, ih_step_stats 
         if ((rc = ih_level_pressure(c, IH_PRE))) return rc;
     if (st) {
         if ((rc = ih_divergence2(c, IH_VEL1, IH_VEL2, IH_VEL3, IH_TMP2))) return rc;
-        st->div_max_after = absmax_host(c, pb, F(c, IH_TMP2), &rc);
+        st->div_max_after = absmax_host(c, pbox(c->g), F(c, IH_TMP2), &rc);
         if (rc) return rc;
     }
-    HIPCHK(hipStreamSynchronize(s));
+    return IH_OK;
+}
+
+}  // namespace ih
+
+extern "C" {
+
+int ih_timestep(ih_case *c, const ih_step_params *p, double time, ih_step_stats *st) {
+    int rc = step_check(c, p);
+    if (rc) return rc;
+    HIPCHK(hipSetDevice(c->device));
+    const Box pb = pbox(c->g);
+    for (int sub = 0; sub < 3; ++sub) {
+        if ((rc = step_pre(c, p, time, sub))) return rc;
+        ih_bicgstab_stats bst{};
+        if ((rc = poisson_solve(c, p, pb, sub + 1, &bst))) return rc;
+        if (st) {
+            st->bicg_iterations[sub] = bst.iterations;
+            st->bicg_final[sub] = bst.norm_inf_final;
+        }
+        if ((rc = step_post(c, p, sub))) return rc;
+    }
+    if ((rc = step_end(c, p, time, st))) return rc;
+    HIPCHK(hipStreamSynchronize(c->stream));
     return IH_OK;
 }
 
@@ -2186,9 +2230,50 @@ struct ih_batch {
     hipGraphExec_t exec = nullptr;
     int g_sweeps = -1, g_fp32 = -1, g_tpb = -1, g_zchunk = -1, g_layout = -1;
     double g_omega = 0.0;
+    // the batched step's Poisson solve (ih_batch_timestep): the Krylov case
+    // table and tile -> case map, the FP32 preconditioner sweeps' item tables
+    // ([0] M pk -> Z1, [1] M rr -> Z2; the middle sweeps by parity), the flags
+    // of all cases gathered in one place, and the captured iteration graph
+    ih::KBatchCase *kcases = nullptr;
+    int *ktile = nullptr, kntiles = 0;
+    bool k_pz = false, k_nonpz = false, k_px = false;
+    ih::JorFirstItem *pfirst[2] = {};
+    int *pfirst_tile = nullptr, pfirst_ntiles = 0;
+    ih::JorBatchItem *pmid[2] = {}, *plast[2] = {};
+    int *pjor_tile = nullptr, pjor_ntiles = 0;
+    int k_sweeps = -1;
+    std::vector<const void *> k_sig;  // the pointers the tables were built from
+    int *dflags = nullptr, *hflags = nullptr;
+    double *dscal = nullptr, *hscal = nullptr;
+    hipGraph_t kgraph = nullptr;
+    hipGraphExec_t kexec = nullptr;
+    int kg_iters = -1, kg_sweeps = -1;
+    float kg_omega = 0.0f;
 };
 
 namespace {
+
+void batch_drop_step(ih_batch *b) {
+    if (b->kexec) (void)hipGraphExecDestroy(b->kexec);
+    if (b->kgraph) (void)hipGraphDestroy(b->kgraph);
+    b->kexec = nullptr;
+    b->kgraph = nullptr;
+    b->kg_iters = -1;
+    for (void *q : {(void *)b->kcases, (void *)b->ktile, (void *)b->pfirst[0], (void *)b->pfirst[1],
+                    (void *)b->pfirst_tile, (void *)b->pmid[0], (void *)b->pmid[1], (void *)b->plast[0],
+                    (void *)b->plast[1], (void *)b->pjor_tile, (void *)b->dflags, (void *)b->dscal})
+        if (q) (void)hipFree(q);
+    if (b->hflags) (void)hipHostFree(b->hflags);
+    if (b->hscal) (void)hipHostFree(b->hscal);
+    b->kcases = nullptr;
+    b->ktile = b->pfirst_tile = b->pjor_tile = nullptr;
+    b->pfirst[0] = b->pfirst[1] = nullptr;
+    b->pmid[0] = b->pmid[1] = b->plast[0] = b->plast[1] = nullptr;
+    b->dflags = b->hflags = nullptr;
+    b->dscal = b->hscal = nullptr;
+    b->k_sig.clear();
+    b->k_sweeps = -1;
+}
 
 void batch_drop_graph(ih_batch *b) {
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
@@ -2339,6 +2424,7 @@ void ih_batch_destroy(ih_batch *b) {
     (void)hipSetDevice(b->device);
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     batch_drop_graph(b);
+    batch_drop_step(b);
     for (auto &pp : b->items)
         for (auto *q : pp)
             if (q) (void)hipFree(q);
@@ -2421,11 +2507,13 @@ int ih_batch_get_groups(ih_batch *b, int64_t *bytes, int *count) {
     return IH_OK;
 }
 
-int ih_batch_jor(ih_batch *b, int n_sweeps, double omega, int fp32, int use_graph) {
-    if (!b || n_sweeps < 0) return fail(IH_EINVAL, "bad argument");
-    HIPCHK(hipSetDevice(b->device));
+}  // extern "C"
+
+namespace {
+
+// ih_batch_jor's work enqueued on the batch's stream (no synchronisation)
+int batch_jor_enqueue(ih_batch *b, int n_sweeps, double omega, int fp32, int use_graph) {
     int rc;
-    for (ih_case *c : b->cases) HIPCHK(hipStreamSynchronize(c->stream));
     hipStream_t s = b->stream;
     if (fp32) {
         // defect correction per case: r = RES - A_cdg PRE in FP64, A e = r in FP32 from e = 0
@@ -2480,7 +2568,385 @@ int ih_batch_jor(ih_batch *b, int n_sweeps, double omega, int fp32, int use_grap
         launch_exchange_relax(s, c->g, F(c, IH_PRE));
     }
     LAUNCHCHK();
+    return IH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ih_batch_jor(ih_batch *b, int n_sweeps, double omega, int fp32, int use_graph) {
+    if (!b || n_sweeps < 0) return fail(IH_EINVAL, "bad argument");
+    HIPCHK(hipSetDevice(b->device));
+    for (ih_case *c : b->cases) HIPCHK(hipStreamSynchronize(c->stream));
+    int rc = batch_jor_enqueue(b, n_sweeps, omega, fp32, use_graph);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(b->stream));
+    return IH_OK;
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// Batched time step: ih_timestep's sequence for every case of a batch, with
+// the Poisson solves of all cases in shared launches.
+// ===========================================================================
+namespace {
+
+// FP32 preconditioner sweeps in the batch's JOR tiling (16-byte lanes, 4 floats)
+constexpr int PRE_TPB = 256, PRE_ZCHUNK = 10;
+
+// the pointers the device tables hold: a change (a re-allocated partial buffer,
+// a swapped velocity field does not matter) means the tables are rebuilt
+std::vector<const void *> step_signature(const ih_batch *b, int sweeps) {
+    std::vector<const void *> sig;
+    sig.push_back((const void *)(intptr_t)sweeps);
+    for (const ih_case *c : b->cases) {
+        for (int f : {IH_RR, IH_RH, IH_PK, IH_AP, IH_AR, IH_PRE, IH_Z1, IH_Z2, IH_RES}) sig.push_back(c->fields[f]);
+        for (const void *q : {(const void *)c->part, (const void *)c->part2, (const void *)c->part3,
+                              (const void *)c->part4, (const void *)c->ksc, (const void *)c->kfl,
+                              (const void *)c->f32[0], (const void *)c->f32[1], (const void *)c->f32[2],
+                              (const void *)c->dcoef, (const void *)c->dcoef32})
+            sig.push_back(q);
+    }
+    return sig;
+}
+
+template <typename T>
+int upload_table(T **dev, const std::vector<T> &h) {
+    if (*dev) (void)hipFree(*dev);
+    *dev = nullptr;
+    if (h.empty()) return IH_OK;
+    HIPCHK(hipMalloc(dev, sizeof(T) * h.size()));
+    HIPCHK(hipMemcpy(*dev, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice));
+    return IH_OK;
+}
+
+// the Krylov and preconditioner tables of the batch (cases' partial buffers sized first)
+int batch_step_tables(ih_batch *b, int sweeps) {
+    int rc;
+    for (ih_case *c : b->cases)
+        if ((rc = ensure_part(c, red_blocks(pbox(c->g))))) return rc;
+    std::vector<const void *> sig = step_signature(b, sweeps);
+    if (b->kcases && sig == b->k_sig) return IH_OK;
+    HIPCHK(hipStreamSynchronize(b->stream));
+    batch_drop_step(b);
+    const int n = (int)b->cases.size();
+    std::vector<KBatchCase> kc((size_t)n);
+    std::vector<int> ktile;
+    std::vector<JorFirstItem> first[2];
+    std::vector<int> ftile;
+    std::vector<JorBatchItem> mid[2], last[2];
+    std::vector<int> jtile;
+    b->k_pz = b->k_nonpz = b->k_px = false;
+    for (int q = 0; q < n; ++q) {
+        ih_case *c = b->cases[(size_t)q];
+        const Geo &g = c->g;
+        const Box pb = pbox(g);
+        KBatchCase &K = kc[(size_t)q];
+        K.g = g;
+        K.c = c->c;
+        K.r = Range3{pb.lo[0], pb.lo[1], pb.lo[2], pb.hi[0] - pb.lo[0] + 1, pb.hi[1] - pb.lo[1] + 1,
+                     pb.hi[2] - pb.lo[2] + 1};
+        K.nbx = (K.r.n0 + RED_TX - 1) / RED_TX;
+        K.nby = (K.r.n1 + RED_TY - 1) / RED_TY;
+        K.nbz = (K.r.n2 + RED_KC - 1) / RED_KC;
+        K.nb = (int64_t)K.nbx * K.nby * K.nbz;
+        K.tile0 = (int)ktile.size();
+        for (int64_t t = 0; t < K.nb; ++t) ktile.push_back(q);
+        K.rr = F(c, IH_RR);
+        K.rh = F(c, IH_RH);
+        K.pk = F(c, IH_PK);
+        K.Ap = F(c, IH_AP);
+        K.Ar = F(c, IH_AR);
+        K.phi = F(c, IH_PRE);
+        K.z1 = F(c, IH_Z1);
+        K.z2 = F(c, IH_Z2);
+        K.bb = F(c, IH_RES);
+        K.part1 = c->part;
+        K.part2 = c->part2;
+        K.part3 = c->part3;
+        K.part4 = c->part4;
+        K.sc = c->ksc;
+        K.fl = c->kfl;
+        (g.per[2] ? b->k_pz : b->k_nonpz) = true;
+        b->k_px = b->k_px || g.per[0] != 0;
+        if (sweeps > 0) {
+            const int *act = c->kfl + KF_ACTIVE;
+            // both applications of M (to pk and to rr) share the case's tiles
+            JorFirstItem f{};
+            f.g = g;
+            for (int d = 0; d < 3; ++d) f.dgf[d] = c->c.dgf[d];
+            const int ntf = jor_first_tiles(pb, g, f);
+            f.tile0 = (int)ftile.size();
+            f.b32 = c->f32[1];
+            f.active = act;
+            for (int t = 0; t < ntf; ++t) ftile.push_back(q);
+            for (int w = 0; w < 2; ++w) {
+                JorFirstItem fw = f;
+                fw.v = w ? F(c, IH_RR) : F(c, IH_PK);
+                fw.e = sweeps == 1 ? (void *)(w ? F(c, IH_Z2) : F(c, IH_Z1)) : (void *)c->f32[0];
+                first[w].push_back(fw);
+            }
+            JorBatchItem it{};
+            const int nt = jor_batch_tiles(pb, g, 4, PRE_TPB, PRE_ZCHUNK, it.sh, it.ntxy, it.nbz);
+            it.g = g;
+            it.r = K.r;
+            for (int d = 0; d < 3; ++d) {
+                it.dg[d] = c->c.dg[d];
+                it.dgf[d] = c->c.dgf[d];
+            }
+            it.tile0 = (int)jtile.size();
+            it.active = act;
+            for (int t = 0; t < nt; ++t) jtile.push_back(q);
+            float *e0 = c->f32[0], *e1 = c->f32[2];
+            for (int par = 0; par < 2; ++par) {
+                JorBatchItem m = it;
+                m.rel = par ? e1 : e0;
+                m.out = par ? e0 : e1;
+                m.bb = c->f32[1];
+                mid[par].push_back(m);
+            }
+            // the last sweep reads e0 after an even number of middle sweeps
+            float *xl = ((sweeps - 2) % 2 == 0) ? e0 : e1;
+            for (int w = 0; w < 2; ++w) {
+                JorBatchItem m = it;
+                m.rel = xl;
+                m.out = w ? (void *)F(c, IH_Z2) : (void *)F(c, IH_Z1);
+                m.bb = c->f32[1];
+                last[w].push_back(m);
+            }
+        }
+    }
+    // every item's tiles map back to it: a block derives its tile from
+    // t - tile0, which must lie in 0 .. its tile count - 1
+    auto owns = [&](const std::vector<int> &map, int q, int t0, int64_t cnt) {
+        if (t0 < 0 || cnt < 0 || (int64_t)t0 + cnt > (int64_t)map.size()) return false;
+        for (int64_t t = t0; t < t0 + cnt; ++t)
+            if (map[(size_t)t] != q) return false;
+        return true;
+    };
+    for (int q = 0; q < n; ++q) {
+        const KBatchCase &K = kc[(size_t)q];
+        bool ok = owns(ktile, q, K.tile0, K.nb);
+        if (sweeps > 0)
+            for (int w = 0; w < 2; ++w) {
+                const JorFirstItem &f = first[w][(size_t)q];
+                const JorBatchItem &m = mid[w][(size_t)q], &l = last[w][(size_t)q];
+                ok = ok && owns(ftile, q, f.tile0, (int64_t)f.ntile * f.nbz) &&
+                     owns(jtile, q, m.tile0, (int64_t)m.ntxy * m.nbz) && owns(jtile, q, l.tile0, (int64_t)l.ntxy * l.nbz);
+            }
+        if (!ok) return fail(IH_EINVAL, "batched step: tile table of case %d is inconsistent", q);
+    }
+    b->kntiles = (int)ktile.size();
+    b->pfirst_ntiles = (int)ftile.size();
+    b->pjor_ntiles = (int)jtile.size();
+    if ((rc = upload_table(&b->kcases, kc)) || (rc = upload_table(&b->ktile, ktile))) return rc;
+    if (sweeps > 0) {
+        for (int w = 0; w < 2; ++w) {
+            if ((rc = upload_table(&b->pfirst[w], first[w])) || (rc = upload_table(&b->pmid[w], mid[w])) ||
+                (rc = upload_table(&b->plast[w], last[w])))
+                return rc;
+        }
+        if ((rc = upload_table(&b->pfirst_tile, ftile)) || (rc = upload_table(&b->pjor_tile, jtile))) return rc;
+    }
+    HIPCHK(hipMalloc(&b->dflags, sizeof(int) * KF_NSLOTS * (size_t)n));
+    HIPCHK(hipMalloc(&b->dscal, sizeof(double) * KS_NSLOTS * (size_t)n));
+    HIPCHK(hipHostMalloc(&b->hflags, sizeof(int) * KF_NSLOTS * (size_t)n, hipHostMallocDefault));
+    HIPCHK(hipHostMalloc(&b->hscal, sizeof(double) * KS_NSLOTS * (size_t)n, hipHostMallocDefault));
+    b->k_sweeps = sweeps;
+    b->k_sig = sig;
+    return IH_OK;
+}
+
+// z = M v for every case (v = pk -> Z1 with w = 0, v = rr -> Z2 with w = 1): the
+// FP32 sweeps of launch_precond, batched
+void batch_precond(ih_batch *b, int w, int sweeps, float omega) {
+    hipStream_t s = b->stream;
+    launch_jor_f32_first_batch(s, b->pfirst[w], b->pfirst_tile, b->pfirst_ntiles, omega, sweeps == 1);
+    if (sweeps < 2) return;
+    for (int q = 1; q < sweeps - 1; ++q)
+        launch_jor_batch_f32(s, b->pmid[(q - 1) & 1], b->pjor_tile, b->pjor_ntiles, omega, false, b->k_px);
+    launch_jor_batch_f32(s, b->plast[w], b->pjor_tile, b->pjor_ntiles, omega, true, b->k_px);
+}
+
+// one BiCGstab iteration of every case (launch_krylov_iteration's sequence)
+void batch_iteration(ih_batch *b, int sweeps, float omega) {
+    hipStream_t s = b->stream;
+    const bool pre = sweeps > 0;
+    launch_p_update_batch(s, b->kcases, b->ktile, b->kntiles);
+    if (pre) batch_precond(b, 0, sweeps, omega);
+    launch_apply_batch(s, b->kcases, b->ktile, b->kntiles, KB_APPLY_W, pre, b->k_pz, b->k_nonpz);
+    launch_s_update_batch(s, b->kcases, b->ktile, b->kntiles);
+    if (pre) {
+        batch_precond(b, 1, sweeps, omega);
+        launch_apply_batch(s, b->kcases, b->ktile, b->kntiles, KB_APPLY_PAIR, pre, b->k_pz, b->k_nonpz);
+    } else {
+        launch_apply_batch(s, b->kcases, b->ktile, b->kntiles, KB_APPLY_SELF, pre, b->k_pz, b->k_nonpz);
+    }
+    launch_x_update_batch(s, b->kcases, b->ktile, b->kntiles, pre);
+    launch_ctl_end_batch(s, b->kcases, (int)b->cases.size());
+}
+
+// BiCGstab_orig (preconditioner 0, or right-preconditioned by `sweeps` FP32 JOR
+// sweeps) of RES for PRE in every case, from PRE as the guess; the per-case
+// statistics into bst[]
+int batch_bicgstab(ih_batch *b, const ih_step_params *p, int sweeps, std::vector<ih_bicgstab_stats> &bst) {
+    hipStream_t s = b->stream;
+    const int n = (int)b->cases.size();
+    const float omega = (float)p->jor_omega;
+    int rc;
+    if ((rc = batch_step_tables(b, sweeps))) return rc;
+    for (ih_case *c : b->cases) {
+        if (sweeps > 0) {
+            launch_fill_f32(s, c->f32[0], 0.0f, c->g.fe);
+            launch_fill_f32(s, c->f32[2], 0.0f, c->g.fe);
+        }
+        // apply_div_grad's ghost refresh of phi
+        for (int m = 1; m <= 3; ++m)
+            if ((rc = exch(c, m, 0, F(c, IH_PRE)))) return rc;
+    }
+    // rr = A phi, then rr = bb - rr, rh = rr, eps = tol * max|bb| and the first head
+    launch_apply_batch(s, b->kcases, b->ktile, b->kntiles, KB_APPLY_PHI, false, b->k_pz, b->k_nonpz);
+    launch_krylov_init_batch(s, b->kcases, n, b->ktile, b->kntiles, p->bicg_rel_tol, p->bicg_max_it);
+    LAUNCHCHK();
+    // iterations enqueued per convergence check, replayed from a captured graph
+    const int iters = 8;
+    if (!b->kexec || b->kg_iters != iters || b->kg_sweeps != sweeps || b->kg_omega != omega) {
+        if (b->kexec) (void)hipGraphExecDestroy(b->kexec);
+        if (b->kgraph) (void)hipGraphDestroy(b->kgraph);
+        b->kexec = nullptr;
+        b->kgraph = nullptr;
+        HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        for (int q = 0; q < iters; ++q) batch_iteration(b, sweeps, omega);
+        hipGraph_t gr = nullptr;
+        const hipError_t e = hipStreamEndCapture(s, &gr);
+        if (e != hipSuccess) return fail(IH_EHIP, "graph capture: %s", hipGetErrorString(e));
+        b->kgraph = gr;
+        HIPCHK(hipGraphInstantiate(&b->kexec, b->kgraph, nullptr, nullptr, 0));
+        b->kg_iters = iters;
+        b->kg_sweeps = sweeps;
+        b->kg_omega = omega;
+    }
+    for (;;) {
+        HIPCHK(hipGraphLaunch(b->kexec, s));
+        launch_gather_batch(s, b->kcases, n, b->dflags, b->dscal);
+        LAUNCHCHK();
+        HIPCHK(hipMemcpyAsync(b->hflags, b->dflags, sizeof(int) * KF_NSLOTS * (size_t)n, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        bool all = true;
+        for (int q = 0; q < n; ++q) {
+            const int *f = b->hflags + (size_t)q * KF_NSLOTS;
+            if (f[KF_DONE]) continue;
+            all = false;
+            if (f[KF_RESET]) {
+                // rhr == 0: new reference vector rh = rr for that case; its next
+                // iteration is in the next replay
+                ih_case *c = b->cases[(size_t)q];
+                KrylovPtrs k{F(c, IH_RR), F(c, IH_RH), F(c, IH_PK), F(c, IH_AP), F(c, IH_AR), F(c, IH_PRE),
+                             c->part, c->part2, c->ksc, c->kfl};
+                launch_krylov_resume(s, c->g, pbox(c->g), k);
+                LAUNCHCHK();
+            }
+        }
+        if (all) break;
+    }
+    launch_gather_batch(s, b->kcases, n, b->dflags, b->dscal);
+    HIPCHK(hipMemcpyAsync(b->hflags, b->dflags, sizeof(int) * KF_NSLOTS * (size_t)n, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(b->hscal, b->dscal, sizeof(double) * KS_NSLOTS * (size_t)n, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
+    bst.assign((size_t)n, ih_bicgstab_stats{});
+    for (int q = 0; q < n; ++q) {
+        const int *f = b->hflags + (size_t)q * KF_NSLOTS;
+        const double *sc = b->hscal + (size_t)q * KS_NSLOTS;
+        bst[(size_t)q].iterations = f[KF_COUNTER];
+        bst[(size_t)q].norm_inf_initial = sc[KS_NORM0];
+        bst[(size_t)q].norm_inf_final = sc[KS_NORM];
+        bst[(size_t)q].exit_reason = f[KF_EXIT];
+    }
+    return IH_OK;
+}
+
+// poisson_solve for every case of the batch (the FP32 or FP64 JOR sweeps and the
+// plain or FP32-preconditioned BiCGstab, with the plain FP64 redo of a
+// preconditioned solve that stops abnormally)
+int batch_poisson(ih_batch *b, const ih_step_params *p, std::vector<ih_bicgstab_stats> &bst) {
+    hipStream_t s = b->stream;
+    int rc;
+    if (p->jor_sweeps > 0 && (rc = batch_jor_enqueue(b, p->jor_sweeps, p->jor_omega, p->jor_fp32 ? 1 : 0, 1)))
+        return rc;
+    const int sweeps = p->precond_sweeps;
+    if (sweeps > 0)
+        for (ih_case *c : b->cases)
+            HIPCHK(hipMemcpyAsync(F(c, IH_COMP), F(c, IH_PRE), sizeof(double) * (size_t)c->g.fe,
+                                  hipMemcpyDeviceToDevice, s));
+    if ((rc = batch_bicgstab(b, p, sweeps, bst))) return rc;
+    if (sweeps > 0) {
+        for (size_t q = 0; q < b->cases.size(); ++q) {
+            if (bst[q].exit_reason == 0) continue;
+            ih_case *c = b->cases[q];
+            const Box pb = pbox(c->g);
+            const int ss[3] = {pb.lo[0], pb.lo[1], pb.lo[2]}, nn[3] = {pb.hi[0], pb.hi[1], pb.hi[2]};
+            const double eps = b->hscal[q * KS_NSLOTS + KS_EPS];
+            HIPCHK(hipMemcpyAsync(F(c, IH_PRE), F(c, IH_COMP), sizeof(double) * (size_t)c->g.fe,
+                                  hipMemcpyDeviceToDevice, s));
+            const int mixed_its = bst[q].iterations;
+            if ((rc = ih_bicgstab(c, eps, p->bicg_max_it, ss, nn, IH_RES, IH_PRE, &bst[q]))) return rc;
+            bst[q].iterations += mixed_its;
+        }
+    }
+    return IH_OK;
+}
+
+// every case's work on the batch's stream for the duration of a batched step
+struct StreamSwap {
+    ih_batch *b;
+    std::vector<hipStream_t> saved;
+    explicit StreamSwap(ih_batch *bb) : b(bb) {
+        for (ih_case *c : b->cases) {
+            saved.push_back(c->stream);
+            c->stream = b->stream;
+        }
+    }
+    ~StreamSwap() {
+        for (size_t q = 0; q < b->cases.size(); ++q) b->cases[q]->stream = saved[q];
+    }
+};
+
+}  // namespace
+
+extern "C" {
+
+int ih_batch_timestep(ih_batch *b, const ih_step_params *p, double time, ih_step_stats *st) {
+    if (!b || !p) return fail(IH_EINVAL, "bad argument");
+    int rc;
+    for (ih_case *c : b->cases)
+        if ((rc = step_check(c, p))) return rc;
+    if (p->precond_poisson != 0) return fail(IH_EINVAL, "the batched step has no multigrid preconditioner");
+    if (p->pressure_bc) return fail(IH_EINVAL, "the batched step has no Dirichlet pressure in the Jacobi sweeps");
+    if (p->precond_sweeps < 0) return fail(IH_EINVAL, "bad preconditioner sweep count");
+    if (p->precond_sweeps > 0 && p->precond_fp64)
+        return fail(IH_EINVAL, "the batched step's preconditioner sweeps are FP32");
+    HIPCHK(hipSetDevice(b->device));
+    for (ih_case *c : b->cases) HIPCHK(hipStreamSynchronize(c->stream));
+    StreamSwap guard(b);
+    const int n = (int)b->cases.size();
+    std::vector<ih_bicgstab_stats> bst;
+    for (int sub = 0; sub < 3; ++sub) {
+        for (ih_case *c : b->cases)
+            if ((rc = step_pre(c, p, time, sub))) return rc;
+        if ((rc = batch_poisson(b, p, bst))) return rc;
+        if (st)
+            for (int q = 0; q < n; ++q) {
+                st[q].bicg_iterations[sub] = bst[(size_t)q].iterations;
+                st[q].bicg_final[sub] = bst[(size_t)q].norm_inf_final;
+            }
+        for (ih_case *c : b->cases)
+            if ((rc = step_post(c, p, sub))) return rc;
+    }
+    for (int q = 0; q < n; ++q)
+        if ((rc = step_end(b->cases[(size_t)q], p, time, st ? &st[q] : nullptr))) return rc;
+    HIPCHK(hipStreamSynchronize(b->stream));
     return IH_OK;
 }
 

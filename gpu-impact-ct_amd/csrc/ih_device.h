@@ -27,9 +27,12 @@ inline dim3 red_grid(const Range3 &r) {
 }
 
 // Planes kc0..kc1 of z chunk bz out of nbz over the range r (balanced chunks).
+// 32-bit unsigned arithmetic: bz * n2 stays far below 2^32 for any field that
+// fits a device, and a 64-bit division costs over a hundred scalar instructions.
 __device__ inline void red_chunk(const Range3 &r, int bz, int nbz, int &kc0, int &kc1) {
-    kc0 = r.lo2 + (int)(((int64_t)bz * r.n2) / nbz);
-    kc1 = r.lo2 + (int)(((int64_t)(bz + 1) * r.n2) / nbz) - 1;
+    const unsigned n = (unsigned)r.n2, nb = (unsigned)nbz, b = (unsigned)bz;
+    kc0 = r.lo2 + (int)((b * n) / nb);
+    kc1 = r.lo2 + (int)(((b + 1u) * n) / nb) - 1;
 }
 
 #define IH_POINT(r)                                                  \
@@ -96,16 +99,17 @@ constexpr int NXCD = 8;
 
 inline unsigned xcd_grid(int64_t tiles) { return (unsigned)((tiles + NXCD - 1) / NXCD * NXCD); }
 
+// Tile counts stay below 2^31, so the index arithmetic is 32-bit unsigned.
 __device__ inline bool xcd_tile(int nbx, int nby, int nbz, int &bx, int &by, int &bz) {
-    const int64_t total = (int64_t)nbx * nby * nbz;
-    const int64_t padded = (int64_t)gridDim.x;
-    const int64_t per = padded / NXCD;
-    const int64_t id = blockIdx.x;
-    const int64_t logical = (id % NXCD) * per + id / NXCD;
+    const unsigned total = (unsigned)nbx * (unsigned)nby * (unsigned)nbz;
+    const unsigned per = gridDim.x / NXCD;
+    const unsigned id = blockIdx.x;
+    const unsigned logical = (id % NXCD) * per + id / NXCD;
     if (logical >= total) return false;
-    bx = (int)(logical % nbx);
-    by = (int)((logical / nbx) % nby);
-    bz = (int)(logical / ((int64_t)nbx * nby));
+    const unsigned row = logical / (unsigned)nbx;
+    bx = (int)(logical - row * (unsigned)nbx);
+    by = (int)(row % (unsigned)nby);
+    bz = (int)(row / (unsigned)nby);
     return true;
 }
 

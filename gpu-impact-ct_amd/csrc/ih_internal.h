@@ -147,6 +147,19 @@ struct JorBatchItem {
     const void *rel, *bb;
     void *out;
     int ntxy, nbz, tile0, pad;
+    const int *active;  // the case's BiCGstab active flag (nullptr: always sweep)
+};
+// One case of the batched first FP32 preconditioner sweep (k_jor_f32_first):
+// 4-point vectors from i0, nv a row, ntile tiles a chunk, nbz chunks.
+struct JorFirstItem {
+    Geo g;
+    Range3 r;
+    const float *dgf[3];
+    int i0, nv, ntile, nbz, tile0;
+    const double *v;
+    float *b32;
+    void *e;  // float, or double when the first sweep is the last one
+    const int *active;
 };
 // tiles of box b for the batch kernels: V points per lane (2 FP64, 4 FP32),
 // tpb threads per tile, z chunks of at most zchunk planes
@@ -161,6 +174,12 @@ void launch_jor_f32_to64(hipStream_t s, const Geo &g, const Coef &c, const Box &
                          const float *bb, double *out, const int *active, JorLaunch shape = JorLaunch());
 void launch_jor_f32_first(hipStream_t s, const Geo &g, const Coef &c, const Box &b, float omega, const double *v,
                           float *b32, float *e, double *e64, const int *active);
+// batched FP32 preconditioner sweeps (flat tile -> item table over ntiles tiles)
+void launch_jor_batch_f32(hipStream_t s, const JorBatchItem *items, const int *tile_item, int ntiles, float omega,
+                          bool to64, bool any_periodic_x);
+int jor_first_tiles(const Box &b, const Geo &g, JorFirstItem &it);  // sets r, i0, nv, ntile, nbz; returns tiles
+void launch_jor_f32_first_batch(hipStream_t s, const JorFirstItem *items, const int *tile_item, int ntiles,
+                                float omega, bool to64);
 void launch_dot_partial(hipStream_t s, const Geo &g, const Box &b, const double *a, const double *c, double *part);
 void launch_sum_partial(hipStream_t s, const Geo &g, const Box &b, const double *a, double *part);
 void launch_final_sum(hipStream_t s, const double *part, int64_t nb, double *out);

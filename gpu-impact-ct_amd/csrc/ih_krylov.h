@@ -55,6 +55,41 @@ struct KrylovPtrs {
 // dot modes of the operator kernel
 enum { KDOT_NONE = 0, KDOT_W = 1, KDOT_SELF = 2, KDOT_PAIR = 3 };
 
+// One case of a batched BiCGstab: its geometry, coefficients and pressure
+// box, its operator/reduction tiles (16 x 16 columns, z chunks of <= 32
+// planes; nb = nbx*nby*nbz) numbered from tile0 in the launch's global tile
+// range, its work vectors, partial buffers and device scalars/flags.  bb is
+// the right-hand side; z1/z2 the preconditioned directions (or null).
+struct KBatchCase {
+    Geo g;
+    Coef c;
+    Range3 r;
+    int nbx, nby, nbz, tile0;
+    int64_t nb;
+    double *rr, *rh, *pk, *Ap, *Ar, *phi, *z1, *z2, *bb;
+    double *part1, *part2, *part3, *part4;
+    double *sc;
+    int *fl;
+};
+// batched operator forms: A phi -> rr (initial residual, no dot), A (z1 or pk) -> Ap
+// with <Ap, rh>, A rr -> Ar with <rr, Ar>, <Ar, Ar>, A z2 -> Ar with <rr, Ar>, <Ar, Ar>
+enum { KB_APPLY_PHI = 0, KB_APPLY_W = 1, KB_APPLY_SELF = 2, KB_APPLY_PAIR = 3 };
+void launch_apply_batch(hipStream_t s, const KBatchCase *cases, const int *tile_case, int ntiles, int form, bool pre,
+                        bool any_pz, bool any_nonpz);
+// rr = bb - rr (rr holding A phi), rh = rr, the max-norm/<rr, rh> partials, and
+// max|bb| partials in part3; then per case eps = tol * max|bb| and the iteration
+// head (k_ctl_setup's work)
+void launch_krylov_init_batch(hipStream_t s, const KBatchCase *cases, int ncases, const int *tile_case, int ntiles,
+                              double tol, int nmax);
+// one batched iteration: p update, [M pk], A, alpha/s update, [M rr], A, omega/x update,
+// head of the next iteration.  M (pre) is applied by the caller-provided functor
+void launch_p_update_batch(hipStream_t s, const KBatchCase *cases, const int *tile_case, int ntiles);
+void launch_s_update_batch(hipStream_t s, const KBatchCase *cases, const int *tile_case, int ntiles);
+void launch_x_update_batch(hipStream_t s, const KBatchCase *cases, const int *tile_case, int ntiles, bool pre);
+void launch_ctl_end_batch(hipStream_t s, const KBatchCase *cases, int ncases);
+// fl_out[q*KF_NSLOTS + s] = case q's flag s, sc_out[q*KS_NSLOTS + s] its scalar s
+void launch_gather_batch(hipStream_t s, const KBatchCase *cases, int ncases, int *fl_out, double *sc_out);
+
 void launch_apply_A(hipStream_t s, const Geo &g, const Coef &c, const Box &pb, const double *src, double *dst,
                     const double *w, int dot_mode, double *part1, double *part2, const int *active);
 void launch_krylov_init(hipStream_t s, const Geo &g, const Box &b, const double *bb, double *rr, double *rh,

@@ -607,24 +607,21 @@ __global__ __launch_bounds__(TPB) void k_jor_vec(Geo g, Coef c, Range3 r, JorVec
 // short z chunks: the sweep has no z neighbours, and it needs the blocks to fill the device
 constexpr int F32F_TPB = 256, F32F_KC = 4;
 
+// block function of tile tw, z chunk bz (the single-case and the batched kernel)
 template <typename TO>
-__global__ __launch_bounds__(F32F_TPB) void k_jor_f32_first(Geo g, Coef c, Range3 r, int i0, int nv, int ntile,
-                                                            int nbz, float omega, const double *__restrict__ v,
-                                                            float *__restrict__ b32, TO *__restrict__ e,
-                                                            const int *active) {
+__device__ inline void jor_f32_first_block(const Geo &g, const float *g1, const float *g2, const float *g3,
+                                           const Range3 &r, int i0, int nv, int tw, int bz, int nbz, float omega,
+                                           const double *__restrict__ v, float *__restrict__ b32,
+                                           TO *__restrict__ e) {
     typedef double D4 __attribute__((ext_vector_type(2)));
     typedef float F4 __attribute__((ext_vector_type(4)));
     typedef TO O4 __attribute__((ext_vector_type(4)));
-    if (active && !*active) return;
-    int tw, dummy, bz;
-    if (!xcd_tile(ntile, 1, nbz, tw, dummy, bz)) return;
     const int f = tw * F32F_TPB + (int)threadIdx.x;
     if (f >= nv * r.n1) return;
     const int jj = f / nv, q = f - jj * nv;
     const int j = r.lo1 + jj, is = i0 + 4 * q;
     const int hi0 = r.lo0 + r.n0 - 1;
     const bool full = is >= r.lo0 && is + 3 <= hi0;
-    const float *g1 = c.dgf[0], *g2 = c.dgf[1], *g3 = c.dgf[2];
     const int c0 = g.cst[0], c1 = g.cst[1], c2 = g.cst[2];
     float a1m[4], a1p[4], a1d[4];
 #pragma unroll
@@ -695,12 +692,38 @@ __global__ __launch_bounds__(F32F_TPB) void k_jor_f32_first(Geo g, Coef c, Range
     }
 }
 
+template <typename TO>
+__global__ __launch_bounds__(F32F_TPB) void k_jor_f32_first(Geo g, Coef c, Range3 r, int i0, int nv, int ntile,
+                                                            int nbz, float omega, const double *__restrict__ v,
+                                                            float *__restrict__ b32, TO *__restrict__ e,
+                                                            const int *active) {
+    if (active && !*active) return;
+    int tw, dummy, bz;
+    if (!xcd_tile(ntile, 1, nbz, tw, dummy, bz)) return;
+    jor_f32_first_block<TO>(g, c.dgf[0], c.dgf[1], c.dgf[2], r, i0, nv, tw, bz, nbz, omega, v, b32, e);
+}
+
+// the first preconditioner sweep of many cases in one launch (tile -> item table)
+template <typename TO>
+__global__ __launch_bounds__(F32F_TPB) void k_jor_f32_first_batch(const JorFirstItem *__restrict__ items,
+                                                                  const int *__restrict__ tile_item, int ntiles,
+                                                                  float omega) {
+    int t, dummy, dummy2;
+    if (!xcd_tile(ntiles, 1, 1, t, dummy, dummy2)) return;
+    const JorFirstItem &it = items[tile_item[t]];
+    if (it.active && !*it.active) return;
+    const int local = t - it.tile0;
+    const int tw = local % it.ntile, bz = local / it.ntile;
+    jor_f32_first_block<TO>(it.g, it.dgf[0], it.dgf[1], it.dgf[2], it.r, it.i0, it.nv, tw, bz, it.nbz, omega, it.v,
+                            it.b32, (TO *)it.e);
+}
+
 // Batched relaxation: the tiles of many cases (grids of any size) in one
 // launch.  Item q owns the global tiles tile0[q] .. tile0[q+1]-1 and
 // tile_item maps a tile to its item; the block relaxes that tile exactly as
 // the single-case kernel would, so every case's result is bit-identical to its
 // own sweep.
-template <typename T, int TPB, int PXM>
+template <typename T, int TPB, int PXM, typename TO = T>
 __global__ __launch_bounds__(TPB) void k_jor_batch(const JorBatchItem *__restrict__ items,
                                                    const int *__restrict__ tile_item, int tile_base, int ntiles,
                                                    T omega) {
@@ -708,14 +731,15 @@ __global__ __launch_bounds__(TPB) void k_jor_batch(const JorBatchItem *__restric
     if (!xcd_tile(ntiles, 1, 1, t, dummy, dummy2)) return;
     t += tile_base;  // the launch covers the global tiles tile_base .. tile_base + ntiles - 1
     const JorBatchItem &it = items[tile_item[t]];
+    if (it.active && !*it.active) return;
     const int local = t - it.tile0;
     const int tw = local % it.ntxy, bz = local / it.ntxy;
     const void *g1 = sizeof(T) == 8 ? (const void *)it.dg[0] : (const void *)it.dgf[0];
     const void *g2 = sizeof(T) == 8 ? (const void *)it.dg[1] : (const void *)it.dgf[1];
     const void *g3 = sizeof(T) == 8 ? (const void *)it.dg[2] : (const void *)it.dgf[2];
-    jor_vec_block<T, T, TPB, PXM, gconst<T>, gmut<T>>(it.g, it.r, it.sh, tw, bz, it.nbz, omega, (gconst<T>)it.rel,
-                                                 (gconst<T>)it.bb, (gmut<T>)it.out, (gconst<T>)g1, (gconst<T>)g2,
-                                                 (gconst<T>)g3);
+    jor_vec_block<T, TO, TPB, PXM, gconst<T>, gmut<TO>>(it.g, it.r, it.sh, tw, bz, it.nbz, omega,
+                                                         (gconst<T>)it.rel, (gconst<T>)it.bb, (gmut<TO>)it.out,
+                                                         (gconst<T>)g1, (gconst<T>)g2, (gconst<T>)g3);
 }
 
 // Row layout of the same batch: grid (row_tiles, cases); a block's case is its
@@ -1217,6 +1241,57 @@ int launch_jor_batch(hipStream_t s, const JorBatchItem *items, const int *tile_i
     }
 #undef IH_JB
     return 0;
+}
+
+// FP32 preconditioner sweeps of many cases (flat layout, default tile shape):
+// to64 writes the result widened to double
+void launch_jor_batch_f32(hipStream_t s, const JorBatchItem *items, const int *tile_item, int ntiles, float omega,
+                          bool to64, bool any_periodic_x) {
+    if (ntiles <= 0) return;
+    const dim3 flat(xcd_grid(ntiles));
+    constexpr int TPB = JOR_VEC_TPB;
+    if (to64) {
+        if (any_periodic_x)
+            hipLaunchKernelGGL((k_jor_batch<float, TPB, 2, double>), flat, dim3(TPB), 0, s, items, tile_item, 0,
+                               ntiles, omega);
+        else
+            hipLaunchKernelGGL((k_jor_batch<float, TPB, 0, double>), flat, dim3(TPB), 0, s, items, tile_item, 0,
+                               ntiles, omega);
+    } else {
+        if (any_periodic_x)
+            hipLaunchKernelGGL((k_jor_batch<float, TPB, 2>), flat, dim3(TPB), 0, s, items, tile_item, 0, ntiles,
+                               omega);
+        else
+            hipLaunchKernelGGL((k_jor_batch<float, TPB, 0>), flat, dim3(TPB), 0, s, items, tile_item, 0, ntiles,
+                               omega);
+    }
+}
+
+int jor_first_tiles(const Box &b, const Geo &g, JorFirstItem &it) {
+    Range3 r = R(b);
+    it.r = r;
+    if (empty(r)) {
+        it.i0 = it.nv = it.ntile = it.nbz = 0;
+        return 0;
+    }
+    const int a = (int)(((int64_t)g.lead + r.lo0 + 3) % 4);
+    it.i0 = r.lo0 - a;
+    it.nv = (r.lo0 + r.n0 - 1 - it.i0) / 4 + 1;
+    it.ntile = (int)(((int64_t)it.nv * r.n1 + F32F_TPB - 1) / F32F_TPB);
+    it.nbz = (r.n2 + F32F_KC - 1) / F32F_KC;
+    return it.ntile * it.nbz;
+}
+
+void launch_jor_f32_first_batch(hipStream_t s, const JorFirstItem *items, const int *tile_item, int ntiles,
+                                float omega, bool to64) {
+    if (ntiles <= 0) return;
+    const dim3 grid(xcd_grid(ntiles));
+    if (to64)
+        hipLaunchKernelGGL((k_jor_f32_first_batch<double>), grid, dim3(F32F_TPB), 0, s, items, tile_item, ntiles,
+                           omega);
+    else
+        hipLaunchKernelGGL((k_jor_f32_first_batch<float>), grid, dim3(F32F_TPB), 0, s, items, tile_item, ntiles,
+                           omega);
 }
 
 int64_t red_blocks(const Box &b) {

@@ -103,10 +103,12 @@ enum { DOT_NONE = 0, DOT_W = 1, DOT_SELF = 2, DOT_PAIR = 3 };
 // The bound asks for 3 wavefronts per SIMD (at most 168 VGPRs): the pair-dot
 // form otherwise takes 170, which the 8-register allocation rounds to 176, and
 // runs at 2 wavefronts per SIMD, 40% slower.
+// The block function of tile (bx, by, bz); k_apply and the batched operator
+// (k_apply_batch) both run it, so a batched case is bit-identical to its own launch.
 template <int TX, int TY, int AZ, int DM, bool PZ>
-__global__ __launch_bounds__(TX *TY, 3) void k_apply(
-    Geo g, Coef c, Range3 r, int nbx, int nby, int nbz, const double *__restrict__ src, double *__restrict__ dst,
-    const double *__restrict__ w, double *part1, double *part2, const int *active) {
+__device__ inline void apply_block(const Geo &g, const Coef &c, const Range3 &r, int nbx, int nby, int nbz, int bx,
+                                   int by, int bz, const double *__restrict__ src, double *__restrict__ dst,
+                                   const double *__restrict__ w, double *part1, double *part2) {
     constexpr int NT = TX * TY;
     constexpr int PW = TX + 10, PH = TY + 10, PS = PW * PH;
     constexpr int NHX = 10 * TY;
@@ -118,9 +120,6 @@ __global__ __launch_bounds__(TX *TY, 3) void k_apply(
     constexpr int OG2 = TY * G1S;               // offset of the G_2 tile
     constexpr int GS = OG2 + (TY + 5) * TX + 1; // + one dummy slot
     constexpr int ZW = AZ + 7;                  // G_3 positions kc0-4 .. kc0+AZ+2
-    if (active && !*active) return;
-    int bx, by, bz;
-    if (!xcd_tile(nbx, nby, nbz, bx, by, bz)) return;
     __shared__ double sP[2][PS];
     __shared__ double sG[2][GS];
     __shared__ double sCG3[6][ZW];
@@ -134,8 +133,8 @@ __global__ __launch_bounds__(TX *TY, 3) void k_apply(
     const int x = x0 + tx, y = y0 + ty;
     const int xe = r.lo0 + r.n0 - 1, ye = r.lo1 + r.n1 - 1;
     const bool own = (x <= xe) && (y <= ye);
-    const int kc0 = r.lo2 + (int)(((int64_t)bz * r.n2) / nbz);
-    const int kc1 = r.lo2 + (int)(((int64_t)(bz + 1) * r.n2) / nbz) - 1;
+    int kc0, kc1;
+    red_chunk(r, bz, nbz, kc0, kc1);
     const int c0 = g.cst[0], c1 = g.cst[1], c2 = g.cst[2];
     const int64_t pxy = g.pxy;
     // loads outside the field read c.zero, so no value is masked after its load
@@ -365,19 +364,32 @@ __global__ __launch_bounds__(TX *TY, 3) void k_apply(
     }
 }
 
+template <int TX, int TY, int AZ, int DM, bool PZ>
+__global__ __launch_bounds__(TX *TY, 3) void k_apply(
+    Geo g, Coef c, Range3 r, int nbx, int nby, int nbz, const double *__restrict__ src, double *__restrict__ dst,
+    const double *__restrict__ w, double *part1, double *part2, const int *active) {
+    if (active && !*active) return;
+    int bx, by, bz;
+    if (!xcd_tile(nbx, nby, nbz, bx, by, bz)) return;
+    apply_block<TX, TY, AZ, DM, PZ>(g, c, r, nbx, nby, nbz, bx, by, bz, src, dst, w, part1, part2);
+}
+
 // ---------------------------------------------------------------------------
 // BiCGstab vector kernels (reduction tiles, so that fused partials follow
 // the fixed tree) and the one-block control kernels.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_init_rr(Geo g, Range3 r, const double *bb, double *rr, double *rh,
-                                                 double *pmax, double *pdot) {
+// The reduction-tile kernels below are block functions of tile (bx, by, bz)
+// out of nbz z chunks, partial index pi = bx + nbx*(by + nby*bz), run by the
+// single-case kernels (tile = block index) and by the batched ones.
+__device__ inline void init_rr_block(const Geo &g, const Range3 &r, int bx, int by, int bz, int nbz, int64_t pi,
+                                     const double *bb, double *rr, double *rh, double *pmax, double *pdot) {
     __shared__ double sm[8];
-    const int i = r.lo0 + (int)(blockIdx.x * RED_TX + threadIdx.x);
-    const int j = r.lo1 + (int)(blockIdx.y * RED_TY + threadIdx.y);
+    const int i = r.lo0 + bx * RED_TX + (int)threadIdx.x;
+    const int j = r.lo1 + by * RED_TY + (int)threadIdx.y;
     double m = 0.0, d = 0.0;
     if (i < r.lo0 + r.n0 && j < r.lo1 + r.n1) {
         int kc0, kc1;
-        red_chunk(r, (int)blockIdx.z, (int)gridDim.z, kc0, kc1);
+        red_chunk(r, bz, nbz, kc0, kc1);
         for (int k = kc0; k <= kc1; ++k) {
             const int64_t x = gidx(g, i, j, k);
             const double v = bb[x] - rr[x];
@@ -392,9 +404,15 @@ __global__ __launch_bounds__(256) void k_init_rr(Geo g, Range3 r, const double *
     __syncthreads();
     const double bs = block_sum(d, sm + 4);
     if (threadIdx.x == 0 && threadIdx.y == 0) {
-        pmax[part_index()] = bm;
-        pdot[part_index()] = bs;
+        pmax[pi] = bm;
+        pdot[pi] = bs;
     }
+}
+
+__global__ __launch_bounds__(256) void k_init_rr(Geo g, Range3 r, const double *bb, double *rr, double *rh,
+                                                 double *pmax, double *pdot) {
+    init_rr_block(g, r, (int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z, (int)gridDim.z, part_index(), bb, rr, rh,
+                  pmax, pdot);
 }
 
 __global__ void k_p_update(Geo g, Range3 r, const double *rr, double *pk, const double *Ap, const double *sc,
@@ -531,8 +549,9 @@ __device__ inline void breakdown(int *fl) {
 
 // alpha = rhr/<rh, Ap> (a vanishing <rh, Ap> is a breakdown), then rr = rr - alpha*Ap
 // (src/mod_solvers.f90:1709-1740)
-__global__ __launch_bounds__(256) void k_s_update_alpha(Geo g, Range3 r, double *rr, const double *Ap,
-                                                        const double *part, int64_t nb, double *sc, int *fl) {
+__device__ inline void s_update_block(const Geo &g, const Range3 &r, int bx, int by, int bz, int nbz, int64_t pi,
+                                      double *rr, const double *Ap, const double *part, int64_t nb, double *sc,
+                                      int *fl) {
     __shared__ double sm[4];
     __shared__ double s_alpha;
     __shared__ int s_go;
@@ -543,7 +562,7 @@ __global__ __launch_bounds__(256) void k_s_update_alpha(Geo g, Range3 r, double 
         const double alpha = ok ? sc[KS_RHR] / rhAp : 0.0;
         s_alpha = alpha;
         s_go = ok;
-        if (part_index() == 0) {
+        if (pi == 0) {
             if (ok)
                 sc[KS_ALPHA] = alpha;
             else
@@ -553,15 +572,21 @@ __global__ __launch_bounds__(256) void k_s_update_alpha(Geo g, Range3 r, double 
     __syncthreads();
     if (!s_go) return;
     const double alpha = s_alpha;
-    const int i = r.lo0 + (int)(blockIdx.x * RED_TX + threadIdx.x);
-    const int j = r.lo1 + (int)(blockIdx.y * RED_TY + threadIdx.y);
+    const int i = r.lo0 + bx * RED_TX + (int)threadIdx.x;
+    const int j = r.lo1 + by * RED_TY + (int)threadIdx.y;
     if (i >= r.lo0 + r.n0 || j >= r.lo1 + r.n1) return;
     int kc0, kc1;
-    red_chunk(r, (int)blockIdx.z, (int)gridDim.z, kc0, kc1);
+    red_chunk(r, bz, nbz, kc0, kc1);
     for (int k = kc0; k <= kc1; ++k) {
         const int64_t x = gidx(g, i, j, k);
         rr[x] = rr[x] - alpha * Ap[x];
     }
+}
+
+__global__ __launch_bounds__(256) void k_s_update_alpha(Geo g, Range3 r, double *rr, const double *Ap,
+                                                        const double *part, int64_t nb, double *sc, int *fl) {
+    s_update_block(g, r, (int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z, (int)gridDim.z, part_index(), rr, Ap,
+                   part, nb, sc, fl);
 }
 
 // omega = <r, Ar>/<Ar, Ar> (a vanishing <Ar, Ar> is a breakdown), then
@@ -569,10 +594,10 @@ __global__ __launch_bounds__(256) void k_s_update_alpha(Geo g, Range3 r, double 
 // <rr, rh> partials of the new residual in pmax/pdot (ra = rr, pa = pk without
 // a preconditioner; ra = z2, pa = z1 with one, src/mod_solvers.f90:1782-1797, 1855-1899).
 // pmax/pdot must not alias p1/p2, which other workgroups may still be reducing.
-__global__ __launch_bounds__(256) void k_x_update_omega(Geo g, Range3 r, double *phi, double *rr, const double *ra,
-                                                        const double *pa, const double *Ar, const double *rh,
-                                                        const double *p1, const double *p2, int64_t nb, double *sc,
-                                                        int *fl, double *pmax, double *pdot) {
+__device__ inline void x_update_block(const Geo &g, const Range3 &r, int bx, int by, int bz, int nbz, int64_t pi,
+                                      double *phi, double *rr, const double *ra, const double *pa, const double *Ar,
+                                      const double *rh, const double *p1, const double *p2, int64_t nb, double *sc,
+                                      int *fl, double *pmax, double *pdot) {
     __shared__ double sm[8];
     __shared__ double s_om;
     __shared__ int s_go;
@@ -585,7 +610,7 @@ __global__ __launch_bounds__(256) void k_x_update_omega(Geo g, Range3 r, double 
         const double om = ok ? rAr / ArAr : 0.0;
         s_om = om;
         s_go = ok;
-        if (part_index() == 0) {
+        if (pi == 0) {
             if (ok)
                 sc[KS_OMEGA] = om;
             else
@@ -595,12 +620,12 @@ __global__ __launch_bounds__(256) void k_x_update_omega(Geo g, Range3 r, double 
     __syncthreads();
     if (!s_go) return;
     const double om = s_om, alpha = sc[KS_ALPHA];
-    const int i = r.lo0 + (int)(blockIdx.x * RED_TX + threadIdx.x);
-    const int j = r.lo1 + (int)(blockIdx.y * RED_TY + threadIdx.y);
+    const int i = r.lo0 + bx * RED_TX + (int)threadIdx.x;
+    const int j = r.lo1 + by * RED_TY + (int)threadIdx.y;
     double m = 0.0, d = 0.0;
     if (i < r.lo0 + r.n0 && j < r.lo1 + r.n1) {
         int kc0, kc1;
-        red_chunk(r, (int)blockIdx.z, (int)gridDim.z, kc0, kc1);
+        red_chunk(r, bz, nbz, kc0, kc1);
         for (int k = kc0; k <= kc1; ++k) {
             const int64_t x = gidx(g, i, j, k);
             phi[x] = phi[x] + om * ra[x] + alpha * pa[x];
@@ -615,9 +640,17 @@ __global__ __launch_bounds__(256) void k_x_update_omega(Geo g, Range3 r, double 
     __syncthreads();
     const double bs = block_sum(d, sm + 4);
     if (threadIdx.x == 0 && threadIdx.y == 0) {
-        pmax[part_index()] = bm;
-        pdot[part_index()] = bs;
+        pmax[pi] = bm;
+        pdot[pi] = bs;
     }
+}
+
+__global__ __launch_bounds__(256) void k_x_update_omega(Geo g, Range3 r, double *phi, double *rr, const double *ra,
+                                                        const double *pa, const double *Ar, const double *rh,
+                                                        const double *p1, const double *p2, int64_t nb, double *sc,
+                                                        int *fl, double *pmax, double *pdot) {
+    x_update_block(g, r, (int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z, (int)gridDim.z, part_index(), phi, rr,
+                   ra, pa, Ar, rh, p1, p2, nb, sc, fl, pmax, pdot);
 }
 
 __global__ __launch_bounds__(256) void k_ctl_end(const double *pmax, const double *pdot, int64_t nb, double *sc,
@@ -638,6 +671,189 @@ __global__ __launch_bounds__(256) void k_set_rhr(const double *part, int64_t nb,
     __shared__ double sm[4];
     const double v = reduce_sum_parts(part, nb, sm);
     if (threadIdx.x == 0 && threadIdx.y == 0) sc[KS_RHR] = v;
+}
+
+// ---------------------------------------------------------------------------
+// Batched BiCGstab.  One launch covers the tiles of many cases (any grid
+// sizes): tile_case maps a global tile to its case, and a case's tiles are its
+// operator/reduction tiles in the single-case numbering from its tile0.  Every
+// block runs the block function of the single-case kernel for that tile and
+// writes the partial of the same index into the case's own buffers, and every
+// scalar step reads the case's own scalars, so each case's iteration is
+// bit-identical to its own solve.  A case whose solve has stopped (its active
+// flag cleared) makes its blocks return at once.
+// ---------------------------------------------------------------------------
+__device__ inline int batch_tile(const KBatchCase *cases, const int *tile_case, int ntiles, int &bx, int &by,
+                                 int &bz, int64_t &pi) {
+    int t, d1, d2;
+    if (!xcd_tile(ntiles, 1, 1, t, d1, d2)) return -1;
+    const int q = tile_case[t];
+    const int local = t - cases[q].tile0;
+    const int nbx = cases[q].nbx, nby = cases[q].nby;
+    bx = local % nbx;
+    const int rem = local / nbx;
+    by = rem % nby;
+    bz = rem / nby;
+    pi = local;
+    return q;
+}
+
+template <int DM, bool PZ>
+__global__ __launch_bounds__(RED_TX *RED_TY, 3) void k_apply_batch(const KBatchCase *__restrict__ cases,
+                                                                   const int *__restrict__ tile_case, int ntiles,
+                                                                   int form, int pre) {
+    int bx, by, bz;
+    int64_t pi;
+    const int q = batch_tile(cases, tile_case, ntiles, bx, by, bz, pi);
+    if (q < 0) return;
+    const KBatchCase &K = cases[q];
+    if ((K.g.per[2] != 0) != PZ) return;  // the other launch of this form covers it
+    if (form != KB_APPLY_PHI && !K.fl[KF_ACTIVE]) return;
+    const double *src = K.phi, *w = nullptr;
+    double *dst = K.rr;
+    if (form == KB_APPLY_W) {
+        src = pre ? K.z1 : K.pk;
+        dst = K.Ap;
+        w = K.rh;
+    } else if (form == KB_APPLY_SELF) {
+        src = K.rr;
+        dst = K.Ar;
+    } else if (form == KB_APPLY_PAIR) {
+        src = K.z2;
+        dst = K.Ar;
+        w = K.rr;
+    }
+    apply_block<RED_TX, RED_TY, RED_KC, DM, PZ>(K.g, K.c, K.r, K.nbx, K.nby, K.nbz, bx, by, bz, src, dst, w, K.part1,
+                                                K.part2);
+}
+
+// pk = rr + beta*pk + om2*Ap (rr in the first iteration) over a reduction tile
+__global__ __launch_bounds__(256) void k_p_update_batch(const KBatchCase *__restrict__ cases,
+                                                        const int *__restrict__ tile_case, int ntiles) {
+    int bx, by, bz;
+    int64_t pi;
+    const int q = batch_tile(cases, tile_case, ntiles, bx, by, bz, pi);
+    if (q < 0) return;
+    const KBatchCase &K = cases[q];
+    if (!K.fl[KF_ACTIVE]) return;
+    const Range3 &r = K.r;
+    const int i = r.lo0 + bx * RED_TX + (int)threadIdx.x;
+    const int j = r.lo1 + by * RED_TY + (int)threadIdx.y;
+    if (i >= r.lo0 + r.n0 || j >= r.lo1 + r.n1) return;
+    int kc0, kc1;
+    red_chunk(r, bz, K.nbz, kc0, kc1);
+    const bool mode = K.fl[KF_MODE] != 0;
+    const double beta = K.sc[KS_BETA], om2 = K.sc[KS_OM2];
+    for (int k = kc0; k <= kc1; ++k) {
+        const int64_t x = gidx(K.g, i, j, k);
+        if (mode)
+            K.pk[x] = K.rr[x] + beta * K.pk[x] + om2 * K.Ap[x];
+        else
+            K.pk[x] = K.rr[x];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_s_update_batch(const KBatchCase *__restrict__ cases,
+                                                        const int *__restrict__ tile_case, int ntiles) {
+    int bx, by, bz;
+    int64_t pi;
+    const int q = batch_tile(cases, tile_case, ntiles, bx, by, bz, pi);
+    if (q < 0) return;
+    const KBatchCase &K = cases[q];
+    s_update_block(K.g, K.r, bx, by, bz, K.nbz, pi, K.rr, K.Ap, K.part1, K.nb, K.sc, K.fl);
+}
+
+__global__ __launch_bounds__(256) void k_x_update_batch(const KBatchCase *__restrict__ cases,
+                                                        const int *__restrict__ tile_case, int ntiles, int pre) {
+    int bx, by, bz;
+    int64_t pi;
+    const int q = batch_tile(cases, tile_case, ntiles, bx, by, bz, pi);
+    if (q < 0) return;
+    const KBatchCase &K = cases[q];
+    x_update_block(K.g, K.r, bx, by, bz, K.nbz, pi, K.phi, K.rr, pre ? K.z2 : K.rr, pre ? K.z1 : K.pk, K.Ar, K.rh,
+                   K.part1, K.part2, K.nb, K.sc, K.fl, K.part3, K.part4);
+}
+
+// one workgroup per case
+__global__ __launch_bounds__(256) void k_ctl_end_batch(const KBatchCase *__restrict__ cases) {
+    const KBatchCase &K = cases[blockIdx.x];
+    if (!K.fl[KF_ACTIVE]) return;
+    __shared__ double sm[8];
+    const double nm = reduce_max_parts(K.part3, K.nb, sm);
+    __syncthreads();
+    const double dt = reduce_sum_parts(K.part4, K.nb, sm + 4);
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+        K.sc[KS_NORM] = nm;
+        K.sc[KS_RHR_NEXT] = dt;
+        ctl_begin(K.sc, K.fl);
+    }
+}
+
+// rr = bb - rr, rh = rr with their partials, and max|bb| partials (part3)
+__global__ __launch_bounds__(256) void k_init_rr_batch(const KBatchCase *__restrict__ cases,
+                                                       const int *__restrict__ tile_case, int ntiles) {
+    int bx, by, bz;
+    int64_t pi;
+    const int q = batch_tile(cases, tile_case, ntiles, bx, by, bz, pi);
+    if (q < 0) return;
+    const KBatchCase &K = cases[q];
+    init_rr_block(K.g, K.r, bx, by, bz, K.nbz, pi, K.bb, K.rr, K.rh, K.part1, K.part2);
+    __syncthreads();
+    __shared__ double sm[4];
+    const Range3 &r = K.r;
+    const int i = r.lo0 + bx * RED_TX + (int)threadIdx.x;
+    const int j = r.lo1 + by * RED_TY + (int)threadIdx.y;
+    double m = 0.0;
+    if (i < r.lo0 + r.n0 && j < r.lo1 + r.n1) {
+        int kc0, kc1;
+        red_chunk(r, bz, K.nbz, kc0, kc1);
+        for (int k = kc0; k <= kc1; ++k) {
+            const double a = fabs(K.bb[gidx(K.g, i, j, k)]);
+            m = (a > m) ? a : m;
+        }
+    }
+    const double bm = block_max(m, sm);
+    if (threadIdx.x == 0 && threadIdx.y == 0) K.part3[pi] = bm;
+}
+
+// per case: eps = tol * max|bb| (the host's bnorm product, here on the device),
+// then k_ctl_setup's work on the initial residual's partials
+__global__ __launch_bounds__(256) void k_ctl_setup_batch(const KBatchCase *__restrict__ cases, double tol, int nmax) {
+    const KBatchCase &K = cases[blockIdx.x];
+    __shared__ double sm[8];
+    const double bmax = reduce_max_parts(K.part3, K.nb, sm);
+    __syncthreads();
+    const double nm = reduce_max_parts(K.part1, K.nb, sm + 4);
+    __syncthreads();
+    const double dt = reduce_sum_parts(K.part2, K.nb, sm);
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+        double *sc = K.sc;
+        int *fl = K.fl;
+        sc[KS_NORM] = nm;
+        sc[KS_NORM0] = nm;
+        sc[KS_RHR_NEXT] = dt;
+        sc[KS_RHR] = 0.0;
+        sc[KS_ALPHA] = 0.0;
+        sc[KS_OMEGA] = 0.0;
+        sc[KS_EPS] = tol * bmax;
+        fl[KF_ACTIVE] = 1;
+        fl[KF_COUNTER] = 0;
+        fl[KF_NMAX] = nmax;
+        fl[KF_EXIT] = 0;
+        fl[KF_RESET] = 0;
+        fl[KF_MODE] = 0;
+        fl[KF_DONE] = 0;
+        ctl_begin(sc, fl);
+    }
+}
+
+// every case's flags and scalars into one array each, for a single copy to the host
+__global__ void k_gather_batch(const KBatchCase *__restrict__ cases, int ncases, int *fl_out, double *sc_out) {
+    const int q = blockIdx.x;
+    const int t = threadIdx.x;
+    if (q >= ncases) return;
+    if (t < KF_NSLOTS) fl_out[q * KF_NSLOTS + t] = cases[q].fl[t];
+    if (t < KS_NSLOTS) sc_out[q * KS_NSLOTS + t] = cases[q].sc[t];
 }
 
 }  // namespace
@@ -763,6 +979,64 @@ void launch_krylov_part(hipStream_t s, const Geo &g, const Coef &c, const Box &b
     default:
         break;
     }
+}
+
+void launch_apply_batch(hipStream_t s, const KBatchCase *cases, const int *tile_case, int ntiles, int form, bool pre,
+                        bool any_pz, bool any_nonpz) {
+    if (ntiles <= 0) return;
+    const dim3 grid(xcd_grid(ntiles)), block(RED_TX, RED_TY, 1);
+#define IH_APB(DMV)                                                                                               \
+    do {                                                                                                          \
+        if (any_nonpz)                                                                                            \
+            hipLaunchKernelGGL((k_apply_batch<DMV, false>), grid, block, 0, s, cases, tile_case, ntiles, form,      \
+                               (int)pre);                                                                         \
+        if (any_pz)                                                                                               \
+            hipLaunchKernelGGL((k_apply_batch<DMV, true>), grid, block, 0, s, cases, tile_case, ntiles, form,       \
+                               (int)pre);                                                                         \
+    } while (0)
+    if (form == KB_APPLY_W)
+        IH_APB(DOT_W);
+    else if (form == KB_APPLY_SELF)
+        IH_APB(DOT_SELF);
+    else if (form == KB_APPLY_PAIR)
+        IH_APB(DOT_PAIR);
+    else
+        IH_APB(DOT_NONE);
+#undef IH_APB
+}
+
+void launch_krylov_init_batch(hipStream_t s, const KBatchCase *cases, int ncases, const int *tile_case, int ntiles,
+                              double tol, int nmax) {
+    if (ntiles <= 0) return;
+    hipLaunchKernelGGL(k_init_rr_batch, dim3(xcd_grid(ntiles)), dim3(RED_TX, RED_TY, 1), 0, s, cases, tile_case,
+                       ntiles);
+    hipLaunchKernelGGL(k_ctl_setup_batch, dim3(ncases), dim3(RED_TX, RED_TY, 1), 0, s, cases, tol, nmax);
+}
+
+void launch_p_update_batch(hipStream_t s, const KBatchCase *cases, const int *tile_case, int ntiles) {
+    if (ntiles > 0)
+        hipLaunchKernelGGL(k_p_update_batch, dim3(xcd_grid(ntiles)), dim3(RED_TX, RED_TY, 1), 0, s, cases, tile_case,
+                           ntiles);
+}
+
+void launch_s_update_batch(hipStream_t s, const KBatchCase *cases, const int *tile_case, int ntiles) {
+    if (ntiles > 0)
+        hipLaunchKernelGGL(k_s_update_batch, dim3(xcd_grid(ntiles)), dim3(RED_TX, RED_TY, 1), 0, s, cases, tile_case,
+                           ntiles);
+}
+
+void launch_x_update_batch(hipStream_t s, const KBatchCase *cases, const int *tile_case, int ntiles, bool pre) {
+    if (ntiles > 0)
+        hipLaunchKernelGGL(k_x_update_batch, dim3(xcd_grid(ntiles)), dim3(RED_TX, RED_TY, 1), 0, s, cases, tile_case,
+                           ntiles, (int)pre);
+}
+
+void launch_gather_batch(hipStream_t s, const KBatchCase *cases, int ncases, int *fl_out, double *sc_out) {
+    if (ncases > 0) hipLaunchKernelGGL(k_gather_batch, dim3(ncases), dim3(64), 0, s, cases, ncases, fl_out, sc_out);
+}
+
+void launch_ctl_end_batch(hipStream_t s, const KBatchCase *cases, int ncases) {
+    if (ncases > 0) hipLaunchKernelGGL(k_ctl_end_batch, dim3(ncases), dim3(RED_TX, RED_TY, 1), 0, s, cases);
 }
 
 void launch_krylov_resume(hipStream_t s, const Geo &g, const Box &b, const KrylovPtrs &k) {

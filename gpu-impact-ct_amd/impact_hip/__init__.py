@@ -203,6 +203,7 @@ def load(path: os.PathLike | None = None):
         "ih_batch_set_groups": (C.c_int, [vp, C.c_int64]),
         "ih_batch_get_groups": (C.c_int, [vp, C.POINTER(C.c_int64), ip]),
         "ih_batch_jor": (C.c_int, [vp, C.c_int, C.c_double, C.c_int, C.c_int]),
+        "ih_batch_timestep": (C.c_int, [vp, C.POINTER(StepParams), C.c_double, C.POINTER(StepStats)]),
         "ih_grid_bytes": (C.c_int64, [C.POINTER(GridDesc)]),
         "ih_grid_bytes_step": (C.c_int64, [C.POINTER(GridDesc), C.POINTER(StepParams)]),
         "ih_serve": (C.c_int, [C.POINTER(ServeConfig), C.POINTER(Request), C.c_int, C.POINTER(RequestResult),
@@ -697,6 +698,13 @@ class Batch:
 
     def jor(self, n, omega=0.8, fp32=False, graph=False):
         check(self.lib.ih_batch_jor(self.h, int(n), float(omega), 1 if fp32 else 0, 1 if graph else 0), "batch_jor")
+
+    def timestep(self, params: StepParams, time: float, stats: bool = True):
+        """one projection time step of every case (ih_batch_timestep); a list of
+        StepStats, one per case, or None"""
+        st = (StepStats * len(self.cases))() if stats else None
+        check(self.lib.ih_batch_timestep(self.h, C.byref(params), float(time), st), "batch_timestep")
+        return list(st) if stats else None
 
 
 def serve(requests, step: StepParams, hbm_budget: int, max_active: int, time_scale: float = 1.0, device: int = 0):

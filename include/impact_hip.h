@@ -364,6 +364,16 @@ int  ih_batch_get_groups(ih_batch *b, int64_t *bytes, int *count);
  * A_cdg PRE in FP64, n_sweeps FP32 sweeps on A e = r from e = 0, PRE += e).
  * use_graph != 0: the sweeps are captured once into a hipGraph and replayed. */
 int  ih_batch_jor(ih_batch *b, int n_sweeps, double omega, int fp32, int use_graph);
+/* One projection time step of every case (ih_timestep's sequence,
+ * src/mod_timeint.f90:275-377, per case), each case's fields bit-identical to
+ * its own ih_timestep.  The stencil phases run case by case on the batch's
+ * stream; the Poisson solves are shared launches: the JOR sweeps of all cases
+ * (as ih_batch_jor), then BiCGstab -- plain, or right-preconditioned by
+ * p->precond_sweeps FP32 JOR sweeps -- with every kernel of an iteration one
+ * launch over the tiles of all cases, each case stopping on its own test, the
+ * iterations replayed from a captured hipGraph.  st: one entry per case, or
+ * NULL.  Not available here: precond_poisson 1, pressure_bc, precond_fp64. */
+int  ih_batch_timestep(ih_batch *b, const ih_step_params *p, double time, ih_step_stats *st);
 
 /* ---- serving: a trace of simulation requests ------------------------------ */
 /* No reference counterpart (the reference runs one simulation per MPI job).

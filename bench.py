@@ -247,40 +247,147 @@ def batched_relaxation(ih, n_cases=32, sweeps=200, seed=4):
     }
 
 
-SERVE_SIZES = [(97, 97, 97), (129, 129, 129), (257, 129, 129)]  # 96^3, 128^3, 256x128x128 cells
+def _tube(ih, N, R, stenosis):
+    """a stenosed tube of grid N (walls, the bench stretching) with radius R and a
+    cosine constriction removing `stenosis` of the area, plug flow at rest"""
+    delta = 1.0 - math.sqrt(1.0 - stenosis)
+    c = ih.Case(N, BC, LEN, STRETCH, device=0)
+    c.set_tube_mask(0.5, 0.5, tube_radius(c.coords(0, 1), R=R, delta=delta),
+                    tube_radius(c.coords(0, 0), R=R, delta=delta))
+    c.fill(ih.VEL1, 1.0)
+    c.apply_mask()
+    return c
 
 
-def serving(ih, n_requests=16, rate=4.0, seed=5, budget_gb=64, max_active=4):
-    """Config-5 shape, scaled to a bench run: Poisson arrivals at `rate`/s of
-    stenosed tubes (radius U[0.2, 0.35], stenosis U[0.3, 0.7]) of sizes from
-    SERVE_SIZES with 2-5 steps each, admitted by an HBM budget."""
+def batched_steps(ih, n_cases=32, steps=20, compare_steps=3, seed=4):
+    """Config 4: n_cases stenosed tubes seeded per instance (radius U[0.2, 0.35],
+    stenosis U[30 %, 70 %], grid from BATCH_SIZES), mixed-precision Poisson
+    (200 FP32 JOR sweeps, BiCGstab to 1e-8 right-preconditioned by 8 FP32
+    sweeps) and FP64 stencils, all advanced by ih_batch_timestep.  After one
+    untimed step of each set, `compare_steps` steps of the batch are timed
+    against the same steps of an identical set advanced case by case with
+    ih_timestep; the batch then runs to `steps` timed steps."""
+    rng = np.random.default_rng(seed)
+    spec = []
+    for q in range(n_cases):
+        spec.append((BATCH_SIZES[int(rng.integers(len(BATCH_SIZES)))], float(rng.uniform(0.2, 0.35)),
+                     float(rng.uniform(0.3, 0.7))))
+    p = ih.StepParams(Re=RE, dtime=DT, jor_sweeps=JOR_SWEEPS, jor_omega=JOR_OMEGA, bicg_rel_tol=TOL,
+                      bicg_max_it=MAXIT, level_pressure=1, inflow_kind=2, inflow_amp=1.0, use_mask=1, jor_fp32=1,
+                      precond_sweeps=8)
+    pts = sum(N[0] * N[1] * N[2] for N, _, _ in spec)
+    cb = [_tube(ih, *x) for x in spec]
+    cs = [_tube(ih, *x) for x in spec]
+    bt = ih.Batch(cb)
+    t = 0.0
+    st = bt.timestep(p, t)  # builds the tables and captures the graphs
+    for c in cs:
+        c.timestep(p, t, stats=False)
+    t += DT
+    its0 = [sum(x.bicg_iterations) for x in st]
+    cb[0].sync()
+    t0 = time.perf_counter()
+    for s in range(compare_steps):
+        bt.timestep(p, t + s * DT, stats=False)
+    cb[0].sync()
+    t_b = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for s in range(compare_steps):
+        for c in cs:
+            c.timestep(p, t + s * DT, stats=False)
+    t_s = time.perf_counter() - t0
+    # the batch and the case-by-case set must agree bit for bit
+    same = all(np.array_equal(a.download(f), b.download(f)) for a, b in zip(cb, cs) for f in (ih.VEL1, ih.PRE))
+    for c in cs:
+        c.close()
+    t += compare_steps * DT
+    rest = max(steps - compare_steps, 0)
+    t0 = time.perf_counter()
+    for s in range(rest):
+        st = bt.timestep(p, t + s * DT, stats=(s == rest - 1))
+        log(f"batched step {compare_steps + s + 1}/{steps} done")
+    cb[0].sync()
+    t_rest = time.perf_counter() - t0
+    its = [sum(x.bicg_iterations) for x in st] if st else its0
+    bt.close()
+    for c in cb:
+        c.close()
+    ms = (t_b + t_rest) / max(compare_steps + rest, 1) * 1e3
+    return {
+        "cases": n_cases,
+        "grids": "random from 65^3, 97^3, 129x65x65, 193x97x97 points; radius U[0.2,0.35], stenosis U[0.3,0.7]",
+        "points": pts,
+        "form": "FP64 stencils; 200 FP32 JOR sweeps, BiCGstab to 1e-8 right-preconditioned by 8 FP32 sweeps",
+        "steps_timed": compare_steps + rest,
+        "ms_per_step": ms,
+        "instance_steps_per_s": n_cases / (ms * 1e-3),
+        "point_steps_per_s": pts / (ms * 1e-3),
+        "compared_steps": compare_steps,
+        "batched_ms_per_step": t_b / compare_steps * 1e3,
+        "sequential_ms_per_step": t_s / compare_steps * 1e3,
+        "speedup_vs_sequential": t_s / t_b,
+        "bit_identical_to_sequential": bool(same),
+        "bicgstab_iterations_per_step": {"min": int(min(its)), "max": int(max(its))},
+    }
+
+
+SERVE_SIZES = [(97, 97, 97), (129, 129, 129), (257, 129, 129), (385, 193, 193)]  # 96^3, 128^3, 256x128^2, 384x192^2 cells
+
+
+def serving(ih, n_requests=100, rate=4.0, seed=5, budget_gb=288, max_active=8, steps_div=100):
+    """Config 5: Poisson arrivals at `rate`/s of stenosed tubes (radius U[0.2,
+    0.35], stenosis U[0.3, 0.7]) of a size drawn from SERVE_SIZES and a step
+    count drawn from U{50..500}, divided by `steps_div` (at the stated counts the
+    offered load is tens of GPU-seconds a second, far past any single device),
+    admitted by a 288 GB HBM budget onto a pool of resident slots.  The step is
+    the FP64 projection step with BiCGstab preconditioned by the multigrid V
+    cycle (the reference's Poisson configuration).  Reports latency percentiles,
+    the measured capacity (point-steps/s while the trace runs), the offered load
+    as a fraction of it, and the regime."""
     rng = np.random.default_rng(seed)
     t, reqs = 0.0, []
     for q in range(n_requests):
         t += float(rng.exponential(1.0 / rate))
+        steps = int(rng.integers(50, 501))
         reqs.append(dict(arrival_s=t, N=SERVE_SIZES[int(rng.integers(len(SERVE_SIZES)))],
-                         steps=int(rng.integers(2, 6)), radius=float(rng.uniform(0.2, 0.35)),
+                         steps=max(1, -(-steps // steps_div)), radius=float(rng.uniform(0.2, 0.35)),
                          stenosis=float(rng.uniform(0.3, 0.7))))
     step = ih.StepParams(Re=RE, dtime=DT, jor_sweeps=JOR_SWEEPS, jor_omega=JOR_OMEGA, bicg_rel_tol=TOL,
                          bicg_max_it=MAXIT, level_pressure=1, inflow_kind=2, inflow_amp=1.0, use_mask=1,
-                         jor_fp32=0, precond_sweeps=0)
+                         jor_fp32=0, precond_sweeps=0, precond_poisson=1)
     res, summ, rc = ih.serve(reqs, step, hbm_budget=budget_gb << 30, max_active=max_active)
     if rc != 0:
         raise RuntimeError(f"serving failed: {rc}")
     lat = np.array([r.latency_s for r in res])
+    wait = np.array([r.admit_s - q["arrival_s"] for r, q in zip(res, reqs)])
+    run = np.array([r.finish_s - r.admit_s for r in res])
     steps = sum(r["steps"] for r in reqs)
     pts = sum(r["steps"] * r["N"][0] * r["N"][1] * r["N"][2] for r in reqs)
+    capacity = pts / summ.makespan_s
+    offered = rate * pts / n_requests  # point-steps a second the arrivals bring
     return {
         "requests": n_requests,
         "arrival_rate_per_s": rate,
-        "sizes": "96^3, 128^3, 256x128x128 cells, 2-5 steps each",
+        "sizes": "96^3, 128^3, 256x128x128, 384x192x192 cells",
+        "steps": f"U{{50..500}} / {steps_div}, rounded up ({steps} steps in all)",
+        "step_form": "FP64, 200 JOR sweeps + BiCGstab to 1e-8 preconditioned by the multigrid V cycle",
         "hbm_budget_gb": budget_gb,
         "max_active": max_active,
+        "pool_slots": summ.pool_slots,
+        "pool_gb": summ.pool_bytes / 2**30,
+        "setup_s": summ.setup_s,
         "p50_latency_s": float(np.percentile(lat, 50)),
         "p99_latency_s": float(np.percentile(lat, 99)),
+        "mean_admit_wait_s": float(wait.mean()),
+        "mean_run_s": float(run.mean()),
         "makespan_s": summ.makespan_s,
+        "trace_span_s": reqs[-1]["arrival_s"],
         "steps_per_s": steps / summ.makespan_s,
-        "point_steps_per_s": pts / summ.makespan_s,
+        "capacity_point_steps_per_s": capacity,
+        "offered_point_steps_per_s": offered,
+        "offered_over_capacity": offered / capacity,
+        "regime": "queue-dominated (arrivals outpace service)" if offered > capacity
+        else "service-dominated (the device keeps up with arrivals)",
         "peak_active": summ.peak_active,
         "peak_hbm_gb": summ.peak_bytes / 2**30,
     }
@@ -457,10 +564,14 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision run")
-    ap.add_argument("--no-batch", action="store_true", help="skip the batched-relaxation run")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched relaxation and batched steps")
+    ap.add_argument("--batch-steps", type=int, default=10, help="timed steps of the 32-case batch")
     ap.add_argument("--no-multigrid", action="store_true", help="skip the multigrid-preconditioned FP64 run")
     ap.add_argument("--no-ct", action="store_true", help="skip the CT-geometry load and forcing timing")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving trace")
+    ap.add_argument("--serve-requests", type=int, default=100, help="requests of the serving trace")
+    ap.add_argument("--serve-steps-div", type=int, default=200,
+                    help="the serving trace's step counts U{50..500} are divided by this (rounded up)")
     ap.add_argument("--no-hbm-case", action="store_true", help="skip the kernel table at 385x193x193")
     ap.add_argument("--precond-sweeps", type=int, default=8,
                     help="FP32 JOR sweeps of the mixed-precision BiCGstab preconditioner")
@@ -609,9 +720,11 @@ def main():
     if not args.no_batch:
         log("batched relaxation ...")
         result["batched_relaxation"] = batched_relaxation(ih)
+        log("batched steps ...")
+        result["batched_steps"] = batched_steps(ih, steps=args.batch_steps, compare_steps=min(2, args.batch_steps))
     if not args.no_serve:
         log("serving trace ...")
-        result["serving"] = serving(ih)
+        result["serving"] = serving(ih, n_requests=args.serve_requests, steps_div=args.serve_steps_div)
     if not args.no_hbm_case:
         log(f"kernels at {HBM_CASE} ...")
         t_big = kernel_table(ih, HBM_CASE)

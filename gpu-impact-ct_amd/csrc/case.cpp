@@ -45,8 +45,9 @@ int fail(int code, const char *fmt, ...) {
 const double aRK[3] = {8.0 / 15.0, 5.0 / 12.0, 3.0 / 4.0};
 const double bRK[3] = {0.0, -17.0 / 60.0, -5.0 / 12.0};
 
-// Scalar slots in ih_case::dscal / hscal
+// Scalar slots in ih_case::dscal / hscal (slot HSCAL_NOALLOC of hscal belongs to ih_dot_noalloc)
 enum { S_DOT0 = 0, S_DOT1, S_DOT2, S_DOT3, S_MAX, S_BETA, S_OM2, S_ALPHA, S_OMEGA, S_ALPHA2, S_SUM, S_NSLOTS = 32 };
+static_assert(HSCAL_NOALLOC > S_SUM + 1 && HSCAL_NOALLOC < S_NSLOTS, "a slot of its own");
 
 bool valid_field(int f) { return f >= 0 && f < IH_NFIELDS; }
 
@@ -310,6 +311,15 @@ int to_box(const ih_case *c, const int ss[3], const int nn[3], Box &b) {
     if (!box_ok(c->g, b)) return fail(IH_EINVAL, "index range outside the block");
     return IH_OK;
 }
+
+}  // namespace
+
+namespace ih {
+int set_fail(int code, const char *msg) { return fail(code, "%s", msg); }
+int box_from(const ih_case *c, const int ss[3], const int nn[3], Box &b) { return to_box(c, ss, nn, b); }
+}  // namespace ih
+
+namespace {
 
 double absmax_host(ih_case *c, const Box &b, const double *a, int *rc) {
     *rc = max_to(c, b, a, S_MAX);

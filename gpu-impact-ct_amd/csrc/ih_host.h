@@ -67,6 +67,13 @@ struct CtState {
     double dtime = 0.0;
 };
 
+// the error of the calling thread (ih_last_error) and ih_dot's index check, for
+// entry points outside case.cpp
+int set_fail(int code, const char *msg);
+int box_from(const ih_case *c, const int ss[3], const int nn[3], Box &b);
+// the slot of ih_case::hscal that ih_dot_noalloc reads its result through
+constexpr int HSCAL_NOALLOC = 31;
+
 struct KernelTiming {
     double total_ms = 0.0;
     int64_t launches = 0;

@@ -156,6 +156,7 @@ def load(path: os.PathLike | None = None):
         "ih_axpby": (C.c_int, [vp, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int]),
         "ih_product_div_grad": (C.c_int, [vp, C.c_int, C.c_int]),
         "ih_dot": (C.c_int, [vp, ip, ip, C.c_int, C.c_int, dp]),
+        "ih_dot_noalloc": (C.c_int, [vp, ip, ip, C.c_int, C.c_int, dp]),
         "ih_jor": (C.c_int, [vp, C.c_int, C.c_double, ip, ip, C.c_int, C.c_int]),
         "ih_residual_cdg": (C.c_int, [vp, ip, ip, C.c_int, C.c_int, dp]),
         "ih_bicgstab": (C.c_int, [vp, C.c_double, C.c_int, ip, ip, C.c_int, C.c_int, C.POINTER(BicgStats)]),
@@ -475,6 +476,12 @@ class Case:
     def dot(self, ss, nn, a, b) -> float:
         out = C.c_double(0.0)
         check(self.lib.ih_dot(self.h, _ints(ss), _ints(nn), a, b, C.byref(out)), "dot")
+        return out.value
+
+    def dot_noalloc(self, ss, nn, a, b) -> float:
+        """ih_dot_noalloc: the same product through the library's fixed scratch array"""
+        out = C.c_double(0.0)
+        check(self.lib.ih_dot_noalloc(self.h, _ints(ss), _ints(nn), a, b, C.byref(out)), "dot_noalloc")
         return out.value
 
     def jor(self, n, omega, ss, nn, rel=PRE, bb=RES):

@@ -148,6 +148,14 @@ int ih_helmholtz(ih_case *c, int component, double multL, ih_field vel_c, ih_fie
 int ih_axpby(ih_case *c, double a, double b, ih_field x, ih_field y, ih_field z);
 int ih_product_div_grad(ih_case *c, ih_field phi, ih_field lap);
 int ih_dot(ih_case *c, const int ss[3], const int nn[3], ih_field a, ih_field b, double *out);
+/* ih_dot without allocation.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same result, bit for
+ * bit, as ih_dot; the block partials and the sum live in one fixed 2 MiB device
+ * array per GPU instead of the case's partial buffer, so a box may need at most
+ * 262112 reduction partials (16 x 16 columns x 32-plane chunks) or the call
+ * fails with IH_EINVAL.  Calls that overlap on one GPU race on that array. */
+int ih_dot_noalloc(ih_case *c, const int ss[3], const int nn[3], ih_field a, ih_field b, double *out);
 
 typedef struct {
     int iterations;

@@ -714,6 +714,8 @@ int ih_field_upload(ih_case *c, ih_field f, const double *host) {
     HIPCHK(hipMemcpy2DAsync(c->fields[f] + g.lead, sizeof(double) * (size_t)g.px, host, w, w,
                             (size_t)(g.N[1] + 7) * (g.N[2] + 7), hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
+    c->n_up += 1;
+    c->bytes_up += (int64_t)w * (g.N[1] + 7) * (g.N[2] + 7);
     return IH_OK;
 }
 
@@ -725,6 +727,8 @@ int ih_field_download(ih_case *c, ih_field f, double *host) {
     HIPCHK(hipMemcpy2DAsync(host, w, c->fields[f] + g.lead, sizeof(double) * (size_t)g.px, w,
                             (size_t)(g.N[1] + 7) * (g.N[2] + 7), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
+    c->n_down += 1;
+    c->bytes_down += (int64_t)w * (g.N[1] + 7) * (g.N[2] + 7);
     return IH_OK;
 }
 
@@ -1987,70 +1991,192 @@ int ih_timestep(ih_case *c, const ih_step_params *p, double time, ih_step_stats 
 }
 
 // ---------------------------------------------------------------------------
-// Reference-signature entry points: host arrays in and out (PCIe each call),
-// as the reference's *_host.f90 wrappers do.
+// Reference-signature entry points: host arrays in and out, as the reference's
+// *_host.f90 wrappers take them.  A host array bound to a field (ih_bind_field)
+// is that field's mirror: it is uploaded only when the host copy is newer, the
+// call works on the field in place, and a result stays on the device until
+// ih_sync.  Any other array is staged through TMP1..TMP4 and crosses PCIe both
+// ways on every call, as the reference's wrappers do.
 // ---------------------------------------------------------------------------
-int ih_device_gradient(ih_case *c, int dimension_id, double *phi, double *grad) {
+namespace {
+
+// staging fields of the wrappers and the relaxation's ping-pong field: never bound
+bool bindable(int f) { return valid_field(f) && !(f >= IH_TMP1 && f <= IH_TMP4) && f != IH_COMP; }
+
+int bound_field(const ih_case *c, const double *host) {
+    if (!host) return -1;
+    for (int f = 0; f < IH_NFIELDS; ++f)
+        if (c->bind_state[f] != BIND_NONE && c->bind_host[f] == host) return f;
+    return -1;
+}
+
+// One host-array argument of a wrapper: the field the call works on.
+struct Arg {
+    double *host;
+    int field;
+    bool bound;
+};
+
+// resolves `host` to its bound field (uploading a newer host copy) or uploads it
+// into `staging`; `upload` false: an output whose old values are not read
+int arg_in(ih_case *c, double *host, ih_field staging, Arg &a, bool upload = true) {
+    if (!host) return fail(IH_EINVAL, "null array argument");
+    a.host = host;
+    const int f = bound_field(c, host);
+    a.bound = f >= 0;
+    a.field = a.bound ? f : (int)staging;
     int rc;
-    if ((rc = ih_field_upload(c, IH_TMP1, phi))) return rc;
-    if ((rc = ih_field_upload(c, IH_TMP2, grad))) return rc;
-    if ((rc = ih_gradient(c, dimension_id, IH_TMP1, IH_TMP2))) return rc;
-    if ((rc = ih_field_download(c, IH_TMP1, phi))) return rc;
-    return ih_field_download(c, IH_TMP2, grad);
+    if (a.bound) {
+        if (upload && c->bind_state[f] == BIND_HOST_NEWER) {
+            if ((rc = ih_field_upload(c, (ih_field)f, host))) return rc;
+            c->bind_state[f] = BIND_CLEAN;
+        }
+    } else if (upload) {
+        if ((rc = ih_field_upload(c, staging, host))) return rc;
+    }
+    return IH_OK;
+}
+
+// after the call: a written bound field is now the newer copy; a written staged
+// array goes back to the host
+int arg_out(ih_case *c, const Arg &a, bool written) {
+    if (!written) return IH_OK;
+    if (a.bound) {
+        c->bind_state[a.field] = BIND_DEVICE_NEWER;
+        return IH_OK;
+    }
+    return ih_field_download(c, (ih_field)a.field, a.host);
+}
+
+}  // namespace
+
+int ih_bind_field(ih_case *c, ih_field f, double *host) {
+    if (!c || !valid_field(f)) return fail(IH_EINVAL, "bad argument");
+    if (!bindable(f)) return fail(IH_EINVAL, "field %d is a staging field of the wrappers and cannot be bound", (int)f);
+    int rc;
+    if (host) {
+        const int g = bound_field(c, host);
+        if (g >= 0 && g != (int)f) return fail(IH_EINVAL, "that host array is already bound to field %d", g);
+    }
+    // an unbound or rebound mirror gives its newer device copy back first
+    if (c->bind_state[f] == BIND_DEVICE_NEWER && c->bind_host[f] != host) {
+        if ((rc = ih_field_download(c, f, const_cast<double *>(c->bind_host[f])))) return rc;
+    }
+    c->bind_host[f] = host;
+    c->bind_state[f] = host ? BIND_HOST_NEWER : BIND_NONE;
+    return IH_OK;
+}
+
+int ih_sync(ih_case *c, ih_field f, int dir) {
+    if (!c || !valid_field(f) || (dir != IH_SYNC_TO_HOST && dir != IH_SYNC_TO_DEVICE))
+        return fail(IH_EINVAL, "bad argument");
+    if (c->bind_state[f] == BIND_NONE) return fail(IH_ESTATE, "field %d is not bound", (int)f);
+    int rc;
+    if (dir == IH_SYNC_TO_HOST) {
+        if (c->bind_state[f] == BIND_DEVICE_NEWER) {
+            if ((rc = ih_field_download(c, f, const_cast<double *>(c->bind_host[f])))) return rc;
+            c->bind_state[f] = BIND_CLEAN;
+        }
+    } else {
+        // the host changed the array: its copy becomes the newer one
+        if ((rc = ih_field_upload(c, f, c->bind_host[f]))) return rc;
+        c->bind_state[f] = BIND_CLEAN;
+    }
+    return IH_OK;
+}
+
+int ih_sync_all(ih_case *c, int dir) {
+    if (!c) return fail(IH_EINVAL, "null case");
+    for (int f = 0; f < IH_NFIELDS; ++f)
+        if (c->bind_state[f] != BIND_NONE) {
+            const int rc = ih_sync(c, (ih_field)f, dir);
+            if (rc) return rc;
+        }
+    return IH_OK;
+}
+
+int ih_transfer_stats(ih_case *c, int64_t out[4]) {
+    if (!c || !out) return fail(IH_EINVAL, "bad argument");
+    out[0] = c->n_up;
+    out[1] = c->n_down;
+    out[2] = c->bytes_up;
+    out[3] = c->bytes_down;
+    return IH_OK;
+}
+
+int ih_device_gradient(ih_case *c, int dimension_id, double *phi, double *grad) {
+    if (!c) return fail(IH_EINVAL, "null case");
+    Arg p{}, g{};
+    int rc;
+    if ((rc = arg_in(c, phi, IH_TMP1, p)) || (rc = arg_in(c, grad, IH_TMP2, g))) return rc;
+    if ((rc = ih_gradient(c, dimension_id, (ih_field)p.field, (ih_field)g.field))) return rc;
+    // the ghost fill of phi is part of the call, as in the reference
+    if ((rc = arg_out(c, p, true)) || (rc = arg_out(c, g, true))) return rc;
+    return IH_OK;
 }
 
 int ih_device_divergence2(ih_case *c, double *phi3, double *div) {
     int rc;
     if (!c || !phi3) return fail(IH_EINVAL, "null argument");
     const int64_t n = ih_field_elems(c);
+    Arg v[3] = {}, d{};
     for (int q = 0; q < 3; ++q)
-        if ((rc = ih_field_upload(c, (ih_field)(IH_TMP1 + q), phi3 + q * n))) return rc;
-    if ((rc = ih_field_upload(c, IH_TMP4, div))) return rc;
-    if ((rc = ih_divergence2(c, IH_TMP1, IH_TMP2, IH_TMP3, IH_TMP4))) return rc;
+        if ((rc = arg_in(c, phi3 + q * n, (ih_field)(IH_TMP1 + q), v[q]))) return rc;
+    if ((rc = arg_in(c, div, IH_TMP4, d))) return rc;
+    if ((rc = ih_divergence2(c, (ih_field)v[0].field, (ih_field)v[1].field, (ih_field)v[2].field, (ih_field)d.field)))
+        return rc;
     for (int q = 0; q < 3; ++q)
-        if ((rc = ih_field_download(c, (ih_field)(IH_TMP1 + q), phi3 + q * n))) return rc;
-    return ih_field_download(c, IH_TMP4, div);
+        if ((rc = arg_out(c, v[q], true))) return rc;
+    return arg_out(c, d, true);
 }
 
 int ih_device_nonlinear_upwind(ih_case *c, int component_id, double *pp, double *vel_component, double *nl_component) {
+    if (!c) return fail(IH_EINVAL, "null case");
+    Arg a{}, v{}, nl{};
     int rc;
-    if ((rc = ih_field_upload(c, IH_TMP1, pp))) return rc;
-    if ((rc = ih_field_upload(c, IH_TMP2, vel_component))) return rc;
-    if ((rc = ih_field_upload(c, IH_TMP3, nl_component))) return rc;
-    if ((rc = ih_nonlinear_upwind(c, component_id, IH_TMP1, IH_TMP2, IH_TMP3))) return rc;
-    return ih_field_download(c, IH_TMP3, nl_component);
+    if ((rc = arg_in(c, pp, IH_TMP1, a)) || (rc = arg_in(c, vel_component, IH_TMP2, v)) ||
+        (rc = arg_in(c, nl_component, IH_TMP3, nl)))
+        return rc;
+    if ((rc = ih_nonlinear_upwind(c, component_id, (ih_field)a.field, (ih_field)v.field, (ih_field)nl.field))) return rc;
+    return arg_out(c, nl, true);
 }
 
 int ih_device_helmholtz(ih_case *c, int component_id, double multL, double *vel_component, double *helm_component) {
+    if (!c) return fail(IH_EINVAL, "null case");
+    Arg v{}, h{};
     int rc;
-    if ((rc = ih_field_upload(c, IH_TMP1, vel_component))) return rc;
-    if ((rc = ih_field_upload(c, IH_TMP2, helm_component))) return rc;
-    if ((rc = ih_helmholtz(c, component_id, multL, IH_TMP1, IH_TMP2))) return rc;
-    return ih_field_download(c, IH_TMP2, helm_component);
+    if ((rc = arg_in(c, vel_component, IH_TMP1, v)) || (rc = arg_in(c, helm_component, IH_TMP2, h))) return rc;
+    if ((rc = ih_helmholtz(c, component_id, multL, (ih_field)v.field, (ih_field)h.field))) return rc;
+    return arg_out(c, h, true);
 }
 
 int ih_device_axpby(ih_case *c, double a, double b, double *x, double *y, double *z) {
+    if (!c) return fail(IH_EINVAL, "null case");
+    Arg ax{}, ay{}, az{};
     int rc;
-    if ((rc = ih_field_upload(c, IH_TMP1, x))) return rc;
-    if ((rc = ih_field_upload(c, IH_TMP2, y))) return rc;
-    if ((rc = ih_axpby(c, a, b, IH_TMP1, IH_TMP2, IH_TMP3))) return rc;
-    return ih_field_download(c, IH_TMP3, z);
+    if ((rc = arg_in(c, x, IH_TMP1, ax)) || (rc = arg_in(c, y, IH_TMP2, ay)) ||
+        (rc = arg_in(c, z, IH_TMP3, az, false)))
+        return rc;
+    if ((rc = ih_axpby(c, a, b, (ih_field)ax.field, (ih_field)ay.field, (ih_field)az.field))) return rc;
+    return arg_out(c, az, true);
 }
 
 int ih_device_interpolate(ih_case *c, int dimension_id, double *phi, double *inter) {
+    if (!c) return fail(IH_EINVAL, "null case");
+    Arg p{}, i{};
     int rc;
-    if ((rc = ih_field_upload(c, IH_TMP1, phi))) return rc;
-    if ((rc = ih_field_upload(c, IH_TMP2, inter))) return rc;
-    if ((rc = ih_interpolate2_pre_vel(c, dimension_id, IH_TMP1, IH_TMP2))) return rc;
-    return ih_field_download(c, IH_TMP2, inter);
+    if ((rc = arg_in(c, phi, IH_TMP1, p)) || (rc = arg_in(c, inter, IH_TMP2, i))) return rc;
+    if ((rc = ih_interpolate2_pre_vel(c, dimension_id, (ih_field)p.field, (ih_field)i.field))) return rc;
+    return arg_out(c, i, true);
 }
 
 int ih_device_interpolate2(ih_case *c, int dimension_id, double *phi, double *inter) {
+    if (!c) return fail(IH_EINVAL, "null case");
+    Arg p{}, i{};
     int rc;
-    if ((rc = ih_field_upload(c, IH_TMP1, phi))) return rc;
-    if ((rc = ih_field_upload(c, IH_TMP2, inter))) return rc;
-    if ((rc = ih_interpolate2_vel_pre(c, dimension_id, IH_TMP1, IH_TMP2))) return rc;
-    return ih_field_download(c, IH_TMP2, inter);
+    if ((rc = arg_in(c, phi, IH_TMP1, p)) || (rc = arg_in(c, inter, IH_TMP2, i))) return rc;
+    if ((rc = ih_interpolate2_vel_pre(c, dimension_id, (ih_field)p.field, (ih_field)i.field))) return rc;
+    return arg_out(c, i, true);
 }
 
 // device_jacobi_solver (src/jacobi_solver_host_v4.f90:5-717) in the CPU
@@ -2073,8 +2199,9 @@ int ih_device_jacobi_solver(ih_case *c, int grid_id, double omega, int SS1, int 
     const int ss[3] = {SS1, SS2, SS3}, nn[3] = {NN1, NN2, NN3};
     Box b;
     if ((rc = to_box(c, ss, nn, b))) return rc;
-    if ((rc = ih_field_upload(c, IH_TMP1, rel))) return rc;
-    if ((rc = ih_field_upload(c, IH_TMP2, bb))) return rc;
+    Arg ar{}, ab{};
+    if ((rc = arg_in(c, rel, IH_TMP1, ar)) || (rc = arg_in(c, bb, IH_TMP2, ab))) return rc;
+    double *X = F(c, (ih_field)ar.field);
     const int mode_j = 20;
     const double thresh = 20. * mode_j * 8. * 1. / 3.0 * 0.8 * 3.1415 * 3.1415 *
                           (1. / (double)(c->g.N[0] * c->g.N[0]) + 1. / (double)(c->g.N[1] * c->g.N[1]) +
@@ -2088,9 +2215,9 @@ int ih_device_jacobi_solver(ih_case *c, int grid_id, double omega, int SS1, int 
     const int max_iter = 200000;
     bool stopped = false;
     for (int iter = 1; iter <= max_iter; ++iter) {
-        if ((rc = jacobi_pressure_bc(c, F(c, IH_TMP1), substep, dtime))) return rc;
+        if ((rc = jacobi_pressure_bc(c, X, substep, dtime))) return rc;
         if (iter % mode_j == 1) {
-            launch_absdiff_partial(c->stream, c->g, b, F(c, IH_TMP1), F(c, IH_TMP3), c->part);
+            launch_absdiff_partial(c->stream, c->g, b, X, F(c, IH_TMP3), c->part);
             launch_final_sum(c->stream, c->part, nb, c->dscal + S_SUM);
             LAUNCHCHK();
             if ((rc = read_scalars(c, S_SUM, 1))) return rc;
@@ -2099,16 +2226,16 @@ int ih_device_jacobi_solver(ih_case *c, int grid_id, double omega, int SS1, int 
                 stopped = true;
                 break;
             }
-            HIPCHK(hipMemcpyAsync(F(c, IH_TMP3), F(c, IH_TMP1), sizeof(double) * (size_t)c->g.fe,
+            HIPCHK(hipMemcpyAsync(F(c, IH_TMP3), X, sizeof(double) * (size_t)c->g.fe,
                                   hipMemcpyDeviceToDevice, c->stream));
             err_tot_old = err_tot;
         }
-        if ((rc = jor_run(c, 1, omega, b, IH_TMP1, IH_TMP2, 0, 0.0))) return rc;
+        if ((rc = jor_run(c, 1, omega, b, (ih_field)ar.field, (ih_field)ab.field, 0, 0.0))) return rc;
     }
-    if (!stopped && (rc = jacobi_pressure_bc(c, F(c, IH_TMP1), substep, dtime))) return rc;
-    launch_exchange_relax(c->stream, c->g, F(c, IH_TMP1));
+    if (!stopped && (rc = jacobi_pressure_bc(c, X, substep, dtime))) return rc;
+    launch_exchange_relax(c->stream, c->g, X);
     LAUNCHCHK();
-    if ((rc = ih_field_download(c, IH_TMP1, rel))) return rc;
+    if ((rc = arg_out(c, ar, true))) return rc;
     if (!stopped) return fail(IH_ENOTCONV, "device_jacobi_solver: no stop after %d sweeps", max_iter);
     return IH_OK;
 }

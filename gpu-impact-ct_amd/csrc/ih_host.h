@@ -71,6 +71,9 @@ struct CtState {
 // entry points outside case.cpp
 int set_fail(int code, const char *msg);
 int box_from(const ih_case *c, const int ss[3], const int nn[3], Box &b);
+// states of a bound field: the host copy is newer (uploaded before the next
+// device use), both agree, the device copy is newer (downloaded by ih_sync)
+enum { BIND_NONE = 0, BIND_HOST_NEWER = 1, BIND_CLEAN = 2, BIND_DEVICE_NEWER = 3 };
 // the slot of ih_case::hscal that ih_dot_noalloc reads its result through
 constexpr int HSCAL_NOALLOC = 31;
 
@@ -129,4 +132,9 @@ struct ih_case {
     int mg_cfg[4] = {15, 4, 4, 4};    // n_grids_limit, n_relax_down, n_relax_up, n_relax_bottom
     int64_t mg_bytes = 0;             // device bytes of the multigrid levels (in bytes while built)
     std::vector<ih::HostMgLevel> mgh;
+    // residency registry of the reference-signature entry points: the host array
+    // each field mirrors (or null) and which copy is newer (ih::BIND_*)
+    const double *bind_host[IH_NFIELDS] = {};
+    uint8_t bind_state[IH_NFIELDS] = {};
+    int64_t n_up = 0, n_down = 0, bytes_up = 0, bytes_down = 0;  // whole-field transfers
 };

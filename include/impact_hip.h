@@ -345,6 +345,27 @@ int ih_device_axpby(ih_case *c, double a, double b, double *x, double *y, double
 int ih_device_interpolate(ih_case *c, int dimension_id, double *phi, double *inter);
 int ih_device_interpolate2(ih_case *c, int dimension_id, double *phi, double *inter);
 
+/* Residency registry of the calls above.  Without it every host array crosses
+ * PCIe both ways on every call, as in the reference's *_host.f90 wrappers
+ * (e.g. src/helmholtz_explicit_host.f90:204-214, 386).  ih_bind_field makes
+ * field f the device mirror of the host array `host` (the reference's module
+ * array, identified by its address): a call given that array uploads it only
+ * when the host copy is newer, works on f in place, and leaves a result on the
+ * device until ih_sync copies it back.  The host copy starts as the newer one.
+ * host = NULL unbinds (a device copy newer than the host's is copied back
+ * first).  TMP1..TMP4 and COMP are the calls' staging fields and cannot be
+ * bound; one host array binds to one field.  Resident calls (ih_jor,
+ * ih_timestep, ...) on a bound field do not change its state: follow them with
+ * ih_sync as needed. */
+int ih_bind_field(ih_case *c, ih_field f, double *host);
+enum { IH_SYNC_TO_HOST = 0, IH_SYNC_TO_DEVICE = 1 };
+/* IH_SYNC_TO_HOST: copy f back if the device copy is newer; IH_SYNC_TO_DEVICE:
+ * the host changed the array, upload it now.  f must be bound. */
+int ih_sync(ih_case *c, ih_field f, int dir);
+int ih_sync_all(ih_case *c, int dir);
+/* Whole-field transfers of the case so far: uploads, downloads, bytes up, bytes down. */
+int ih_transfer_stats(ih_case *c, int64_t out[4]);
+
 /* ---- batched relaxation of many cases ------------------------------------ */
 /* Many cases (uneven grids, one device) relaxed together: each sweep is one
  * launch over the tiles of all cases.  No reference counterpart: the reference

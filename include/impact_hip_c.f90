@@ -21,6 +21,7 @@ module impact_hip_c
   integer(c_int), parameter :: IH_RR = 18, IH_RH = 19, IH_PK = 20, IH_AP = 21, IH_AR = 22
   integer(c_int), parameter :: IH_TMP1 = 23, IH_TMP2 = 24, IH_TMP3 = 25, IH_TMP4 = 26
   integer(c_int), parameter :: IH_Z1 = 27, IH_Z2 = 28
+  integer(c_int), parameter :: IH_SYNC_TO_HOST = 0, IH_SYNC_TO_DEVICE = 1
 
   type, bind(C) :: ih_grid_desc
      integer(c_int) :: version
@@ -168,5 +169,31 @@ module impact_hip_c
        integer(c_int), value :: dimension_id
        real(c_double) :: phi(*), inter(*)
      end function ih_device_interpolate2
+
+     ! residency registry: field f mirrors the host array (identified by its address)
+     integer(c_int) function ih_bind_field(ctx, f, host) bind(C, name="ih_bind_field")
+       import :: c_int, c_double, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: f
+       real(c_double) :: host(*)
+     end function ih_bind_field
+
+     integer(c_int) function ih_sync(ctx, f, dir) bind(C, name="ih_sync")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: f, dir
+     end function ih_sync
+
+     integer(c_int) function ih_sync_all(ctx, dir) bind(C, name="ih_sync_all")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: dir
+     end function ih_sync_all
+
+     integer(c_int) function ih_transfer_stats(ctx, out) bind(C, name="ih_transfer_stats")
+       import :: c_int, c_int64_t, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int64_t), intent(out) :: out(4)
+     end function ih_transfer_stats
   end interface
 end module impact_hip_c

@@ -123,7 +123,7 @@ def operator_roofline(table, grid, traffic=None):
     }
 
 
-def setup_case(ih, mixed=False, precond_sweeps=8, multigrid=False):
+def setup_case(ih, mixed=False, precond_sweeps=16, multigrid=False, fp64_precond=False):
     """The bench case; mixed=True runs the Poisson solve in the mixed-precision
     form (FP32 JOR defect correction and an FP32-JOR-preconditioned BiCGstab,
     residuals, inner products and updates in FP64); multigrid=True runs the FP64
@@ -141,7 +141,9 @@ def setup_case(ih, mixed=False, precond_sweeps=8, multigrid=False):
     c.fill(ih.PRE, 0.0)
     p = ih.StepParams(Re=RE, dtime=DT, jor_sweeps=JOR_SWEEPS, jor_omega=JOR_OMEGA, bicg_rel_tol=TOL,
                       bicg_max_it=MAXIT, level_pressure=1, inflow_kind=2, inflow_amp=1.0, use_mask=1,
-                      jor_fp32=1 if mixed else 0, precond_sweeps=precond_sweeps if (mixed and not multigrid) else 0,
+                      jor_fp32=1 if mixed else 0,
+                      precond_sweeps=precond_sweeps if ((mixed or fp64_precond) and not multigrid) else 0,
+                      precond_fp64=1 if fp64_precond else 0,
                       precond_poisson=1 if multigrid else 0, mg_fp32=1 if (mixed and multigrid) else 0)
     return c, p
 
@@ -573,7 +575,7 @@ def main():
     ap.add_argument("--serve-steps-div", type=int, default=200,
                     help="the serving trace's step counts U{50..500} are divided by this (rounded up)")
     ap.add_argument("--no-hbm-case", action="store_true", help="skip the kernel table at 385x193x193")
-    ap.add_argument("--precond-sweeps", type=int, default=8,
+    ap.add_argument("--precond-sweeps", type=int, default=16,
                     help="FP32 JOR sweeps of the mixed-precision BiCGstab preconditioner")
     ap.add_argument("--only", choices=("jor", "apply", "apply_w", "apply_self", "p_update", "s_update", "x_update"),
                     default=None,
@@ -679,6 +681,23 @@ def main():
             **{f"{k}_vs_fp64": v for k, v in err.items()},
         }
         cm.close()
+        # like for like: the same solver with the JOR sweeps (defect correction and
+        # preconditioner) in FP64, so the precision's own share of the speedup shows
+        cf, pf = setup_case(ih, precond_sweeps=args.precond_sweeps, fp64_precond=True)
+        ms_f, _, iters_f, _ = run_steps(ih, cf, pf, args.steps, args.warmup, "fp64 preconditioned")
+        # its distance from the plain FP64 solve is what the 1e-8 stop test alone
+        # leaves open, the yardstick for the mixed form's distance above
+        err_f = rel_errors(ih, cf, c)
+        result["mixed_precision"]["like_for_like_fp64"] = {
+            "form": (f"{JOR_SWEEPS} FP64 JOR sweeps, then BiCGstab right-preconditioned by {args.precond_sweeps} "
+                     "FP64 JOR sweeps"),
+            "steps_per_s": 1000.0 / ms_f,
+            "ms_per_step": ms_f,
+            "bicgstab_iterations_per_step": iters_f,
+            "fp32_speedup_over_it": ms_f / ms_m,
+            **{f"{k}_vs_fp64": v for k, v in err_f.items()},
+        }
+        cf.close()
     if not args.no_multigrid:
         cg, pg = setup_case(ih, multigrid=True)
         ms_g, fams_g, iters_g, _ = run_steps(ih, cg, pg, args.steps, args.warmup, "multigrid")

@@ -54,12 +54,14 @@ __device__ inline double wave_sum(double v) {
     return v;
 }
 
+// The larger of a and m, where a NaN in either wins: max-norms built from it are
+// NaN as soon as one entry is, so a non-finite residual cannot pass a stop test
+// as a small one.  For finite values it is the plain (a > m) ? a : m.
+__device__ inline double max_nan(double a, double m) { return (a > m || a != a) ? a : m; }
+
 __device__ inline double wave_max(double v) {
 #pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
-        const double o = __shfl_xor(v, off, 64);
-        v = (o > v) ? o : v;
-    }
+    for (int off = 32; off >= 1; off >>= 1) v = max_nan(__shfl_xor(v, off, 64), v);
     return v;
 }
 
@@ -81,7 +83,7 @@ __device__ inline double block_max(double v, double *sm) {
     double r = 0.0;
     if (tid == 0) {
         r = sm[0];
-        for (int q = 1; q < 4; ++q) r = (sm[q] > r) ? sm[q] : r;
+        for (int q = 1; q < 4; ++q) r = max_nan(sm[q], r);
     }
     return r;
 }

@@ -814,7 +814,7 @@ __global__ __launch_bounds__(256) void k_max_partial(Geo g, Range3 r, const doub
         red_chunk(r, (int)blockIdx.z, (int)gridDim.z, kc0, kc1);
         for (int k = kc0; k <= kc1; ++k) {
             const double v = fabs(a[gidx(g, i, j, k)]);
-            m = (v > m) ? v : m;
+            m = max_nan(v, m);
         }
     }
     const double s = block_max(m, sm);
@@ -834,7 +834,7 @@ __global__ __launch_bounds__(256) void k_final_max(const double *part, int64_t n
     __shared__ double sm[4];
     const int tid = threadIdx.x + threadIdx.y * blockDim.x;
     double m = 0.0;
-    for (int64_t q = tid; q < nb; q += RED_THREADS) m = (part[q] > m) ? part[q] : m;
+    for (int64_t q = tid; q < nb; q += RED_THREADS) m = max_nan(part[q], m);
     const double s = block_max(m, sm);
     if (tid == 0) *out = s;
 }
@@ -865,7 +865,7 @@ __global__ __launch_bounds__(256) void k_residual_cdg(Geo g, Coef c, Range3 r, c
             t = t - ((g1[1 * c0 + i] + g2[1 * c1 + j]) + g3[1 * c2 + k]) * rel[x];
             if (res_out) res_out[x] = t;
             const double a = fabs(t);
-            m = (a > m) ? a : m;
+            m = max_nan(a, m);
         }
     }
     const double s = block_max(m, sm);

@@ -396,7 +396,7 @@ __device__ inline void init_rr_block(const Geo &g, const Range3 &r, int bx, int 
             rr[x] = v;
             rh[x] = v;
             const double a = fabs(v);
-            m = (a > m) ? a : m;
+            m = max_nan(a, m);
             d = d + v * v;
         }
     }
@@ -436,7 +436,7 @@ __device__ inline double reduce_sum_parts(const double *part, int64_t nb, double
 __device__ inline double reduce_max_parts(const double *part, int64_t nb, double *sm) {
     const int tid = threadIdx.x + threadIdx.y * blockDim.x;
     double m = 0.0;
-    for (int64_t q = tid; q < nb; q += RED_THREADS) m = (part[q] > m) ? part[q] : m;
+    for (int64_t q = tid; q < nb; q += RED_THREADS) m = max_nan(part[q], m);
     return block_max(m, sm);
 }
 
@@ -632,7 +632,7 @@ __device__ inline void x_update_block(const Geo &g, const Range3 &r, int bx, int
             const double v = rr[x] - om * Ar[x];
             rr[x] = v;
             const double a = fabs(v);
-            m = (a > m) ? a : m;
+            m = max_nan(a, m);
             d = d + v * rh[x];
         }
     }
@@ -809,7 +809,7 @@ __global__ __launch_bounds__(256) void k_init_rr_batch(const KBatchCase *__restr
         red_chunk(r, bz, K.nbz, kc0, kc1);
         for (int k = kc0; k <= kc1; ++k) {
             const double a = fabs(K.bb[gidx(K.g, i, j, k)]);
-            m = (a > m) ? a : m;
+            m = max_nan(a, m);
         }
     }
     const double bm = block_max(m, sm);

@@ -392,7 +392,7 @@ double orc_reduce_max(const orc_case *c, const int ss[3], const int nn[3], orc_p
         for (int j = ss[1]; j <= nn[1]; ++j)
             for (int i = ss[0]; i <= nn[0]; ++i) {
                 const double v = f(ctx, ORC_IDX(c, i, j, k));
-                m = (v > m) ? v : m;
+                m = (v > m || v != v) ? v : m; /* a NaN wins */
             }
     return m;
 }

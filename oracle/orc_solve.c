@@ -135,7 +135,7 @@ double orc_residual_cdg(const orc_case *c, const int ss[3], const int nn[3], dou
                 t = t - CDG(g3, 1, k) * A(rel, i, j, k + 1);
                 t = t - ((CDG(g1, 0, i) + CDG(g2, 0, j)) + CDG(g3, 0, k)) * A(rel, i, j, k);
                 const double a = fabs(t);
-                m = (a > m) ? a : m;
+                m = (a > m || a != a) ? a : m; /* a NaN wins */
             }
     return m;
 }
@@ -247,7 +247,7 @@ static double absmax(const orc_case *c, const int ss[3], const int nn[3], const 
         for (int j = ss[1]; j <= nn[1]; ++j)
             for (int i = ss[0]; i <= nn[0]; ++i) {
                 const double v = fabs(A(a, i, j, k));
-                m = (v > m) ? v : m;
+                m = (v > m || v != v) ? v : m; /* a NaN wins */
             }
     return m;
 }

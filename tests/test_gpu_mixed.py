@@ -121,3 +121,25 @@ def test_timestep_fp64_preconditioned_bit_exact():
         t += 5e-4
         assert list(so.bicg_iterations) == list(sg.bicg_iterations)
         compare_state(oc, gc, vel, pre, nl, f"FP64-preconditioned step {step}")
+
+
+@pytest.mark.parametrize("spec", GRIDS)
+def test_nan_residual_stops_as_breakdown(spec):
+    """One NaN in the right-hand side: the max-norm of the residual is NaN, and
+    BiCGstab (plain, FP32-preconditioned, FP64-preconditioned) stops at its first
+    stop test with a breakdown (exit 2) instead of reading the NaN residual as a
+    small one and reporting convergence.  A mixed time step relies on that exit
+    to redo the solve in FP64."""
+    oc, gc = pair(spec)
+    bb = rnd(oc.shape, 194)
+    bb[tuple(n // 2 for n in oc.shape)] = np.nan
+    phi = oc.zeros()
+    gc.upload(ih.RES, bb)
+    gc.upload(ih.PRE, phi)
+    assert np.isnan(gc.residual_cdg(oc.Sp, oc.Np))
+    for solve in (lambda: gc.bicgstab(1e-8, 50, oc.Sp, oc.Np),
+                  lambda: gc.bicgstab_mixed(1e-8, 50, oc.Sp, oc.Np, 3, 0.8),
+                  lambda: gc.bicgstab_jor64(1e-8, 50, oc.Sp, oc.Np, 3, 0.8)):
+        gc.upload(ih.PRE, phi)
+        st = solve()
+        assert st.exit_reason == 2 and st.iterations == 0, (st.exit_reason, st.iterations)

@@ -198,3 +198,28 @@ def test_fp32_jor_correction_tracks_fp64_jor():
         c.jor_mixed(n, 0.8, c.Sp, c.Np, x32, bb)
         err = np.linalg.norm(x32[s] - x64[s]) / np.linalg.norm(x64[s])
         assert 0 < err < 5e-6 * n, (n, err)
+
+
+def test_nan_residual_stops_as_breakdown():
+    """The stop rule for a non-finite residual (no reference counterpart: the
+    reference's MAX(ABS(phi), normInf) leaves a NaN's fate to the compiler).
+    Max-norms let a NaN win, so one NaN in the right-hand side makes the residual
+    norm NaN, and every BiCGstab form stops at its first stop test with exit 2
+    instead of reading the NaN residual as converged."""
+    c = O.OracleCase((17, 13, 11), (1, 1, 1, 1, 1, 1), (2.0, 1.0, 1.0), (0.0, 1.5, 1.5))
+    bb = np.ascontiguousarray(np.random.default_rng(194).standard_normal(c.shape))
+    bb[tuple(n // 2 for n in c.shape)] = np.nan
+    assert np.isnan(c.residual_cdg(c.Sp, c.Np, c.zeros(), bb))
+    for solve in (lambda phi: c.bicgstab(1e-8, 50, c.Sp, c.Np, bb, phi),
+                  lambda phi: c.bicgstab_mixed(1e-8, 50, c.Sp, c.Np, bb, phi, 3, 0.8),
+                  lambda phi: c.bicgstab_jor64(1e-8, 50, c.Sp, c.Np, bb, phi, 3, 0.8)):
+        st = solve(c.zeros())
+        assert st.exit_reason == 2 and st.iterations == 0, (st.exit_reason, st.iterations)
+    # a finite right-hand side still converges (the rule changes nothing else)
+    bb[tuple(n // 2 for n in c.shape)] = 0.0
+    xt = c.zeros()
+    xt[3:-3, 3:-3, 3:-3] = np.random.default_rng(195).standard_normal(xt[3:-3, 3:-3, 3:-3].shape)
+    b2 = c.zeros()
+    c.product_div_grad(xt, b2, c.zeros())
+    st = c.bicgstab(1e-9 * np.abs(b2).max(), 500, c.Sp, c.Np, b2, c.zeros())
+    assert st.exit_reason == 0

@@ -100,6 +100,42 @@ def kernel_table(ih, grid, launches=40):
     return out
 
 
+# the substep's stencil kernels: algorithmic bytes per point of their range
+# (nl_helm_rhs<q>: read the three work fields, vel and nl, write nl and rhs;
+# divergence2: read u, v, w, write div; interp_vel_pre3: read u, v, w, write
+# three work fields; grad_correct<q>: read p and vel, write vel)
+STENCIL_BYTES = {"nl_helm_rhs0": 56, "nl_helm_rhs1": 56, "nl_helm_rhs2": 56, "divergence2": 32,
+                 "interp_vel_pre3": 48, "grad_correct0": 24}
+
+
+def stencil_points(name, grid):
+    """points of the kernel's range on a grid with walls on every side: the
+    pressure range is N per direction; velocity component q's interior is N-1
+    along q and N-2 across"""
+    if name.startswith(("nl_helm_rhs", "grad_correct")):
+        q = int(name[-1])
+        return math.prod((grid[d] - 1) if d == q else (grid[d] - 2) for d in range(3))
+    return math.prod(grid)
+
+
+def stencil_table(ih, grid, launches=20):
+    """ms per launch, algorithmic GB/s and fraction of HBM peak of each stencil
+    kernel of a substep on a case of the given grid (HIP events around
+    back-to-back launches on the case's stream)."""
+    c = ih.Case(grid, BC, LEN, STRETCH, device=0)
+    rng = np.random.default_rng(2)
+    for f in (ih.VEL1, ih.VEL2, ih.VEL3, ih.NL1, ih.NL2, ih.NL3, ih.WORK1, ih.WORK2, ih.WORK3, ih.PRE):
+        c.upload(f, np.ascontiguousarray(rng.standard_normal(c.shape)))
+    out = {}
+    for name, bpp in STENCIL_BYTES.items():
+        ms = c.bench_kernel(name, launches)
+        nbytes = bpp * stencil_points(name, grid)
+        out[name] = {"ms": ms, "bytes": nbytes, "GBs": nbytes / (ms * 1e-3) / 1e9,
+                     "frac": nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    c.close()
+    return out
+
+
 def operator_roofline(table, grid, traffic=None):
     """roofline of the fused Poisson operator k_apply in the two forms a BiCGstab
     iteration launches (Ap with <rh, Ap>: 24 B/point; As with <s, As>, <As, As>:
@@ -546,16 +582,23 @@ def kernel_only(ih, which, launches, grid=N):
     scaled by those of a copy of known size."""
     c = ih.Case(grid, BC, LEN, STRETCH, device=0)
     rng = np.random.default_rng(1)
-    for f in (ih.PRE, ih.RES, ih.RR, ih.RH, ih.PK, ih.AP, ih.AR, ih.TMP1, ih.TMP4):
+    fields = (ih.PRE, ih.RES, ih.RR, ih.RH, ih.PK, ih.AP, ih.AR, ih.TMP1, ih.TMP4)
+    if which in STENCIL_BYTES:
+        fields += (ih.VEL1, ih.VEL2, ih.VEL3, ih.NL1, ih.NL2, ih.NL3, ih.WORK1, ih.WORK2, ih.WORK3)
+    for f in fields:
         c.upload(f, np.ascontiguousarray(rng.standard_normal(c.shape)))
     ms = c.bench_kernel(which, launches)
     ih.check(c.lib.ih_field_copy(c.h, ih.TMP3, ih.TMP1), "field copy")
     c.sync()
     fe = int(np.prod(c.shape))
-    P = grid[0] * grid[1] * grid[2]
+    if which in STENCIL_BYTES:
+        P = stencil_points(which, grid)
+        nbytes = STENCIL_BYTES[which] * P
+    else:
+        P = grid[0] * grid[1] * grid[2]
+        nbytes = KERNEL_BYTES.get(which, 16) * P
     print(json.dumps({"kernel_only": which, "grid": list(grid), "launches": launches, "ms_per_launch": ms,
-                      "points": P, "algorithmic_bytes": KERNEL_BYTES.get(which, 16) * P,
-                      "copy_bytes_each_way": 8 * fe}), flush=True)
+                      "points": P, "algorithmic_bytes": nbytes, "copy_bytes_each_way": 8 * fe}), flush=True)
     c.close()
 
 
@@ -577,7 +620,8 @@ def main():
     ap.add_argument("--no-hbm-case", action="store_true", help="skip the kernel table at 385x193x193")
     ap.add_argument("--precond-sweeps", type=int, default=16,
                     help="FP32 JOR sweeps of the mixed-precision BiCGstab preconditioner")
-    ap.add_argument("--only", choices=("jor", "apply", "apply_w", "apply_self", "p_update", "s_update", "x_update"),
+    ap.add_argument("--only", choices=("jor", "apply", "apply_w", "apply_self", "p_update", "s_update", "x_update")
+                    + tuple(STENCIL_BYTES),
                     default=None,
                     help="launch one kernel on the bench grid (for counter profiles), followed by one plain device "
                          "copy of a field for calibration")
@@ -608,6 +652,8 @@ def main():
     traffic = read_traffic()
     log("kernel table ...")
     table = kernel_table(ih, N)
+    log("stencil table ...")
+    stencils = stencil_table(ih, N)
     it_ms = bi_ms / max(bi_n, 1)
     result = {
         "metric": METRIC,
@@ -655,6 +701,7 @@ def main():
             "method": "BiCGstab phase time of the timed steps (HIP events) over its iterations",
         },
         "kernels_257x129x129": table,
+        "stencils_257x129x129": stencils,
         "phases_ms_per_step": {
             "jor": jor_ms / args.steps,
             "bicgstab": bi_ms / args.steps,

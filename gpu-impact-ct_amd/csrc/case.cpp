@@ -2272,6 +2272,22 @@ int ih_bench_kernel(ih_case *c, const char *name, int launches, double *ms_per_l
         one = [&]() {
             launch_apply_A(s, c->g, c->c, b, F(c, IH_PK), F(c, IH_AP), nullptr, KDOT_NONE, nullptr, nullptr, nullptr);
         };
+    } else if (n == "nl_helm_rhs0" || n == "nl_helm_rhs1" || n == "nl_helm_rhs2") {
+        // one component of the substep's fused nonlinear + Helmholtz + rhs (with the nl term of the last substep)
+        const int q = n.back() - '0';
+        one = [&, q]() {
+            launch_nl_helm_rhs(s, c->g, c->c, q, F(c, IH_WORK1), F(c, IH_WORK2), F(c, IH_WORK3), F(c, IH_VEL1 + q),
+                               F(c, IH_NL1 + q), F(c, IH_RHS1 + q), 0.5, 0.25, 0.125, 1);
+        };
+    } else if (n == "divergence2") {
+        one = [&]() { launch_divergence2(s, c->g, c->c, F(c, IH_VEL1), F(c, IH_VEL2), F(c, IH_VEL3), F(c, IH_TMP1)); };
+    } else if (n == "interp_vel_pre3") {
+        one = [&]() {
+            launch_interp_vel_pre3(s, c->g, c->c, F(c, IH_VEL1), F(c, IH_VEL2), F(c, IH_VEL3), F(c, IH_WORK1),
+                                   F(c, IH_WORK2), F(c, IH_WORK3));
+        };
+    } else if (n == "grad_correct0") {
+        one = [&]() { launch_grad_correct(s, c->g, c->c, 0, F(c, IH_PRE), F(c, IH_VEL1)); };
     } else if (part >= 0) {
         if ((rc = ensure_part(c, red_blocks(b)))) return rc;
         for (int q = 0; q < KS_NSLOTS; ++q) c->hksc[q] = 0.5;
@@ -2285,7 +2301,9 @@ int ih_bench_kernel(ih_case *c, const char *name, int launches, double *ms_per_l
         if (part == KPART_X_UPDATE) launch_krylov_part(s, c->g, c->c, b, k, KPART_APPLY_SELF);
         one = [&]() { launch_krylov_part(s, c->g, c->c, b, k, part); };
     } else {
-        return fail(IH_EINVAL, "unknown kernel '%s' (jor, apply, apply_w, apply_self, p_update, s_update, x_update)",
+        return fail(IH_EINVAL,
+                    "unknown kernel '%s' (jor, apply, apply_w, apply_self, p_update, s_update, x_update, "
+                    "nl_helm_rhs0/1/2, divergence2, interp_vel_pre3, grad_correct0)",
                     name);
     }
     one();  // warm

@@ -7,6 +7,9 @@
 // so every field access is a coalesced 512-byte row segment.
 #include "ih_device.h"
 
+#include <algorithm>
+#include <atomic>
+
 namespace ih {
 
 namespace {
@@ -94,15 +97,6 @@ __device__ inline double interp_up(const Geo &g, const double *c, int cst, int q
     return v;
 }
 
-__global__ void k_interp_vel_pre3(Geo g, Coef c, Range3 r, const double *u, const double *v, const double *w,
-                                  double *w1, double *w2, double *w3) {
-    IH_POINT(r);
-    const int64_t x = gidx(g, i, j, k);
-    w1[x] = interp_up(g, c.Iu[0], g.cst[0], 0, u, x, i);
-    w2[x] = interp_up(g, c.Iu[1], g.cst[1], 1, v, x, j);
-    w3[x] = interp_up(g, c.Iu[2], g.cst[2], 2, w, x, k);
-}
-
 __global__ void k_interp_vel_pre(Geo g, Coef c, Range3 r, int q, const double *phi, double *inter) {
     IH_POINT(r);
     const int64_t x = gidx(g, i, j, k);
@@ -136,46 +130,370 @@ __device__ inline double tap7(const double *c, int cst, int idx, const double *f
 // ---------------------------------------------------------------------------
 // Fused nonlinear (upwind, src/mod_diff.f90:784-1424) + Helmholtz_explicit
 // (src/mod_diff.f90:549-762) + rhs assembly (src/mod_rhs.f90:56-130) for one
-// velocity component over its interior range.  The advection velocity
-// pp = I_m(work_d) is interpolated in registers instead of a 39 MB field.
+// velocity component Q over its interior range.  Per point, in this order:
+//   for d = 1, 2, 3: pp = I_Q(work_d) (6 taps -2..3 along Q, interpolate2_pre_vel);
+//     dd1 = the upwind stencil of vel_Q along d chosen by the sign of pp (7 taps
+//     -3..3); nl = dd1 pp (d = 1), nl + dd1 pp after;
+//   Helmholtz: the 7 x taps, then 7 y taps, then 7 z taps of vel_Q in one sum h;
+//   nl = nl - multL h; rhs = vel - mult_b nl_old (when use_b) - mult_a nl.
+// The advection velocity pp is interpolated in registers instead of a 39 MB
+// field.  2.5-D: a 16 x 16 column tile marches through a z chunk.  Per plane the component's
+// velocity (the tile and a ring of 3) is staged in LDS for its x and y taps,
+// and its z taps come from a per-thread queue of 7 planes.  The work fields
+// that pp is interpolated from are staged with their halo along Q (Q = 0, 1)
+// or queued along z (Q = 2).  Coefficients that depend on x or y are per-thread
+// registers (Helmholtz, interpolation) or LDS tables (the two upwind stencils,
+// chosen per point by the sign of pp); those that depend on z are staged for
+// the chunk.  Every value is formed with the expressions and order above, so
+// the result is bit-identical to a serial evaluation of the point loop, while
+// each field value is loaded from memory once per tile instead of once per tap.
 // ---------------------------------------------------------------------------
-__global__ void k_nl_helm_rhs(Geo g, Coef c, Range3 r, int q, const double *w1, const double *w2, const double *w3,
-                              const double *vel, double *nl, double *rhs, double multL, double mult_a, double mult_b,
-                              int use_b) {
-    IH_POINT(r);
-    const int64_t x = gidx(g, i, j, k);
-    const int iq = pick(q, i, j, k);
-    const double *work[3] = {w1, w2, w3};
-    double acc = 0.0;
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-        const double pp = interp_pu(g, c.Ip[q], g.cst[q], q, work[d], x, iq);
-        const double *cU = (d == q) ? c.NuU[d] : c.NpU[d];
-        const double *cD = (d == q) ? c.NuD[d] : c.NpD[d];
-        const double *cc = (pp >= 0.0) ? cU : cD;
-        const double dd1 = tap7(cc, g.cst[d], pick(d, i, j, k), vel, x, sh(g, d));
-        acc = (d == 0) ? dd1 * pp : acc + dd1 * pp;
+namespace nlh {
+constexpr int TX = 16, TY = 16, NT = TX * TY, AZ = 32;
+constexpr int VW = TX + 6, VS = VW * (TY + 6);  // velocity plane with a ring of 3
+constexpr int NVH = VS - NT;                    // 228 ring positions
+constexpr int WL = TX + 5;                      // work tile extent along Q (-2 .. +3 halo)
+constexpr int WS = WL * TY;                     // 336 (TX == TY)
+constexpr int NWH = 5 * TX;                     // halo positions of one work field
+static_assert(TX == TY, "the work tile layout assumes a square tile");
+static_assert(NVH <= NT && 3 * NWH <= NT, "one ring load and one work halo load per thread at most");
+}  // namespace nlh
+
+template <int Q>
+__global__ __launch_bounds__(nlh::NT) void k_nl_helm_rhs_t(Geo g, Coef c, Range3 r, int nbx, int nby, int nbz,
+                                                          const double *__restrict__ w1,
+                                                          const double *__restrict__ w2,
+                                                          const double *__restrict__ w3,
+                                                          const double *__restrict__ vel, double *__restrict__ nl,
+                                                          double *__restrict__ rhs, double multL, double mult_a,
+                                                          double mult_b, int use_b) {
+    using namespace nlh;
+    __shared__ double sV[2][VS];
+    __shared__ double sW[2][3][Q < 2 ? WS : 1];
+    __shared__ double sUx[2][7][TX], sUy[2][7][TY];  // [upwind U / D][tap][x or y]
+    __shared__ double sUz[2][7][AZ], sH3[7][AZ], sIz[6][AZ];
+    int bx, by, bz;
+    if (!xcd_tile(nbx, nby, nbz, bx, by, bz)) return;
+    const int tx = threadIdx.x, ty = threadIdx.y, tid = tx + TX * ty;
+    const int x0 = r.lo0 + bx * TX, y0 = r.lo1 + by * TY;
+    const int x = x0 + tx, y = y0 + ty;
+    const int xe = r.lo0 + r.n0 - 1, ye = r.lo1 + r.n1 - 1;
+    const bool own = x <= xe && y <= ye;
+    int kc0, kc1;
+    red_chunk(r, bz, nbz, kc0, kc1);
+    const int c0 = g.cst[0], c1s = g.cst[1], c2s = g.cst[2];
+    const int64_t pxy = g.pxy;
+    const double *zero = c.zero;
+    const double *wk[3] = {w1, w2, w3};
+
+    // coefficient tables: upwind along x and y for the tile, along z for the chunk
+    for (int e = tid; e < 2 * 7 * TX; e += NT) {
+        const int sel = e / (7 * TX), t = (e / TX) % 7, u = e % TX;
+        const double *cx = sel == 0 ? (Q == 0 ? c.NuU[0] : c.NpU[0]) : (Q == 0 ? c.NuD[0] : c.NpD[0]);
+        const double *cy = sel == 0 ? (Q == 1 ? c.NuU[1] : c.NpU[1]) : (Q == 1 ? c.NuD[1] : c.NpD[1]);
+        sUx[sel][t][u] = (x0 + u <= xe) ? cx[t * c0 + x0 + u] : 0.0;
+        sUy[sel][t][u] = (y0 + u <= ye) ? cy[t * c1s + y0 + u] : 0.0;
     }
-    // Helmholtz: 21 taps accumulated in one sum
-    {
-        const double *c1 = (q == 0) ? c.u2[0] : c.p2[0];
-        const double *c2 = (q == 1) ? c.u2[1] : c.p2[1];
-        const double *c3 = (q == 2) ? c.u2[2] : c.p2[2];
-        const int64_t s1 = 1, s2 = g.px, s3 = g.pxy;
-        double dd1 = c1[0 * g.cst[0] + i] * vel[x - 3 * s1];
-#pragma unroll
-        for (int t = -2; t <= 3; ++t) dd1 = dd1 + c1[(t + 3) * g.cst[0] + i] * vel[x + t * s1];
-#pragma unroll
-        for (int t = -3; t <= 3; ++t) dd1 = dd1 + c2[(t + 3) * g.cst[1] + j] * vel[x + t * s2];
-#pragma unroll
-        for (int t = -3; t <= 3; ++t) dd1 = dd1 + c3[(t + 3) * g.cst[2] + k] * vel[x + t * s3];
-        acc = acc - multL * dd1;
+    for (int e = tid; e < 7 * AZ; e += NT) {
+        const int t = e / AZ, u = e % AZ, kk = kc0 + u;
+        const bool v = kk <= kc1;
+        sUz[0][t][u] = v ? (Q == 2 ? c.NuU[2] : c.NpU[2])[t * c2s + kk] : 0.0;
+        sUz[1][t][u] = v ? (Q == 2 ? c.NuD[2] : c.NpD[2])[t * c2s + kk] : 0.0;
+        sH3[t][u] = v ? (Q == 2 ? c.u2[2] : c.p2[2])[t * c2s + kk] : 0.0;
+        if (Q == 2 && t < 6) sIz[t][u] = v ? c.Ip[2][t * c2s + kk] : 0.0;
     }
-    double rr = vel[x];
-    if (use_b) rr = rr - mult_b * nl[x];
-    rr = rr - mult_a * acc;
-    nl[x] = acc;
-    rhs[x] = rr;
+    // per-thread coefficients: Helmholtz along x and y, interpolation along Q (Q < 2)
+    double h1[7], h2[7], ip[6];
+#pragma unroll
+    for (int t = 0; t < 7; ++t) {
+        h1[t] = own ? (Q == 0 ? c.u2[0] : c.p2[0])[t * c0 + x] : 0.0;
+        h2[t] = own ? (Q == 1 ? c.u2[1] : c.p2[1])[t * c1s + y] : 0.0;
+    }
+#pragma unroll
+    for (int t = 0; t < 6; ++t) ip[t] = (own && Q < 2) ? c.Ip[Q][t * (Q == 0 ? c0 : c1s) + (Q == 0 ? x : y)] : 0.0;
+
+    // load positions: the velocity column (tile and ring reach xe+3, ye+3), one
+    // ring position, the own work value (Q < 2) and one work halo position
+    const int64_t base = gidx(g, 0, 0, 0);
+    const bool vcol = x <= xe + 3 && y <= ye + 3;
+    const int64_t colo = gidx(g, x, y, 0) - base;
+    int ringo = -1, rings = VS;  // offset from base in plane 0, LDS slot
+    if (tid < NVH) {
+        int row, col;
+        if (tid < 6 * VW) {
+            const int rr = tid / VW;
+            row = rr < 3 ? rr : rr + TY;
+            col = tid % VW;
+        } else {
+            const int e2 = tid - 6 * VW;
+            row = 3 + e2 / 6;
+            const int cc = e2 % 6;
+            col = cc < 3 ? cc : cc + TX;
+        }
+        const int hx = x0 - 3 + col, hy = y0 - 3 + row;
+        rings = row * VW + col;
+        if (hx <= xe + 3 && hy <= ye + 3) ringo = (int)(gidx(g, hx, hy, 0) - base);
+    }
+    // work: own value (valid where it serves as a tap of an own point) and halo
+    const bool wown = Q == 0 ? (x <= xe + 3 && y <= ye) : (Q == 1 ? (x <= xe && y <= ye + 3) : false);
+    const int wslot = Q == 0 ? ty * WL + tx + 2 : (ty + 2) * TX + tx;
+    int wh_d = 0, wh_o = -1, wh_s = 0;
+    if (Q < 2 && tid < 3 * NWH) {
+        wh_d = tid / NWH;
+        const int e = tid % NWH;
+        const int a = e / 5, bq = e % 5;  // along the tile's other direction, position in the halo
+        const int off = bq < 2 ? bq : bq + TX;  // halo columns -2, -1, 16, 17, 18 (+2 shift)
+        int hx, hy;
+        if (Q == 0) {
+            hx = x0 - 2 + off;
+            hy = y0 + a;
+            wh_s = a * WL + off;
+            if (hx <= xe + 3 && hy <= ye) wh_o = (int)(gidx(g, hx, hy, 0) - base);
+        } else {
+            hx = x0 + a;
+            hy = y0 - 2 + off;
+            wh_s = off * TX + a;
+            if (hx <= xe && hy <= ye + 3) wh_o = (int)(gidx(g, hx, hy, 0) - base);
+        }
+    }
+    __syncthreads();
+
+    auto ld = [&](const double *f, int64_t o, bool v) -> double { return *(v ? f + base + o : zero); };
+    // the z queues: vq = vel(k-3 .. k+3); wq[d] = work_d(k-2 .. k+3) (Q = 2)
+    double vq[7], wq[3][6];
+#pragma unroll
+    for (int s = 0; s < 7; ++s) vq[s] = ld(vel, colo + (int64_t)(kc0 - 3 + s) * pxy, vcol);
+    if (Q == 2) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+#pragma unroll
+            for (int s = 0; s < 6; ++s) wq[d][s] = ld(wk[d], colo + (int64_t)(kc0 - 2 + s) * pxy, own);
+    }
+    // values staged for plane kc0
+    double hv = ld(vel, ringo + (int64_t)kc0 * pxy, ringo >= 0);
+    double wo[3], whv = 0.0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) wo[d] = Q < 2 ? ld(wk[d], colo + (int64_t)kc0 * pxy, wown) : 0.0;
+    if (Q < 2) whv = ld(wk[wh_d], wh_o + (int64_t)kc0 * pxy, wh_o >= 0);
+    double nlv = use_b ? ld(nl, colo + (int64_t)kc0 * pxy, own) : 0.0;
+
+    for (int k = kc0; k <= kc1; ++k) {
+        const int buf = (k - kc0) & 1, kz = k - kc0;
+        double *V = sV[buf];
+        V[(ty + 3) * VW + tx + 3] = vq[3];
+        if (rings < VS) V[rings] = hv;
+        if (Q < 2) {
+#pragma unroll
+            for (int d = 0; d < 3; ++d) sW[buf][d][wslot] = wo[d];
+            if (tid < 3 * NWH) sW[buf][wh_d][wh_s] = whv;
+        }
+        __syncthreads();
+        // loads for plane k+1 and the queues' next planes
+        const int kn = k + 1;
+        const bool vn = kn <= kc1;
+        const double nlc = nlv;
+        hv = ld(vel, ringo + (int64_t)kn * pxy, vn && ringo >= 0);
+        if (Q < 2) {
+#pragma unroll
+            for (int d = 0; d < 3; ++d) wo[d] = ld(wk[d], colo + (int64_t)kn * pxy, vn && wown);
+            whv = ld(wk[wh_d], wh_o + (int64_t)kn * pxy, vn && wh_o >= 0);
+        }
+        if (use_b) nlv = ld(nl, colo + (int64_t)kn * pxy, vn && own);
+        const double vnext = ld(vel, colo + (int64_t)(k + 4) * pxy, vcol && k + 4 <= kc1 + 3);
+        double wnext[3];
+        if (Q == 2) {
+#pragma unroll
+            for (int d = 0; d < 3; ++d) wnext[d] = ld(wk[d], colo + (int64_t)(k + 4) * pxy, own && k + 4 <= kc1 + 3);
+        }
+
+        if (own) {
+            const int vc = (ty + 3) * VW + tx + 3;  // the point in the velocity plane
+            double acc = 0.0;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                double pp;
+                if (Q == 2) {
+                    pp = sIz[0][kz] * wq[d][0];
+#pragma unroll
+                    for (int t = -1; t <= 3; ++t) pp = pp + sIz[t + 2][kz] * wq[d][t + 2];
+                } else {
+                    const double *W = sW[buf][d];
+                    const int st = Q == 0 ? 1 : TX;
+                    const int w0 = Q == 0 ? ty * WL + tx : ty * TX + tx;  // tap -2
+                    pp = ip[0] * W[w0];
+#pragma unroll
+                    for (int t = -1; t <= 3; ++t) pp = pp + ip[t + 2] * W[w0 + (t + 2) * st];
+                }
+                const int sel = (pp >= 0.0) ? 0 : 1;
+                double dd1;
+                if (d == 0) {
+                    dd1 = sUx[sel][0][tx] * V[vc - 3];
+#pragma unroll
+                    for (int t = -2; t <= 3; ++t) dd1 = dd1 + sUx[sel][t + 3][tx] * V[vc + t];
+                } else if (d == 1) {
+                    dd1 = sUy[sel][0][ty] * V[vc - 3 * VW];
+#pragma unroll
+                    for (int t = -2; t <= 3; ++t) dd1 = dd1 + sUy[sel][t + 3][ty] * V[vc + t * VW];
+                } else {
+                    dd1 = sUz[sel][0][kz] * vq[0];
+#pragma unroll
+                    for (int t = -2; t <= 3; ++t) dd1 = dd1 + sUz[sel][t + 3][kz] * vq[t + 3];
+                }
+                acc = (d == 0) ? dd1 * pp : acc + dd1 * pp;
+            }
+            double hh = h1[0] * V[vc - 3];
+#pragma unroll
+            for (int t = -2; t <= 3; ++t) hh = hh + h1[t + 3] * V[vc + t];
+#pragma unroll
+            for (int t = -3; t <= 3; ++t) hh = hh + h2[t + 3] * V[vc + t * VW];
+#pragma unroll
+            for (int t = -3; t <= 3; ++t) hh = hh + sH3[t + 3][kz] * vq[t + 3];
+            acc = acc - multL * hh;
+            double rr = vq[3];
+            if (use_b) rr = rr - mult_b * nlc;
+            rr = rr - mult_a * acc;
+            const int64_t xo = base + colo + (int64_t)k * pxy;
+            nl[xo] = acc;
+            rhs[xo] = rr;
+        }
+        // advance the queues
+#pragma unroll
+        for (int s = 0; s < 6; ++s) vq[s] = vq[s + 1];
+        vq[6] = vnext;
+        if (Q == 2) {
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+#pragma unroll
+                for (int s = 0; s < 5; ++s) wq[d][s] = wq[d][s + 1];
+                wq[d][5] = wnext[d];
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Six-tap stencils (-3 .. 2) of u along x, v along y and w along z over the
+// pressure range, in 2.5-D: divergence2 (MODE XYZ_DIV, src/mod_diff.f90:130-215:
+// one sum of the three in that order) and interpolate2_vel_pre for all three
+// components (MODE XYZ_INTERP, src/mod_diff.f90:1740-1825: three outputs).
+// A 16 x 16 tile marches through a z chunk; per plane u is staged in LDS with
+// its x halo and v with its y halo, and w comes from a per-thread queue of 6
+// planes.  x and y coefficients are per-thread registers, z coefficients are
+// staged for the chunk.  divergence2 sums u's x taps, then v's y taps, then
+// w's z taps, each from tap -3 up, into one value; the interpolation forms each
+// output from tap -3 up as interp_up does, so the results are bit-identical to
+// serial point loops.
+// ---------------------------------------------------------------------------
+enum { XYZ_DIV = 0, XYZ_INTERP = 1 };
+namespace x6 {
+constexpr int TX = 16, TY = 16, NT = TX * TY, AZ = 32;
+constexpr int UW = TX + 5, US = UW * TY;  // u: x0-3 .. x0+17
+constexpr int VS = TX * (TY + 5);         // v: y0-3 .. y0+17
+constexpr int NH = 5 * TX;                // halo positions of u (and of v)
+static_assert(TX == TY && 2 * NH <= NT, "one halo load per thread at most");
+}  // namespace x6
+
+template <int MODE>
+__global__ __launch_bounds__(x6::NT) void k_xyz6_t(Geo g, Range3 r, int nbx, int nby, int nbz,
+                                                  const double *__restrict__ c1, const double *__restrict__ c2,
+                                                  const double *__restrict__ c3, const double *__restrict__ u,
+                                                  const double *__restrict__ v, const double *__restrict__ w,
+                                                  double *__restrict__ o1, double *__restrict__ o2,
+                                                  double *__restrict__ o3, const double *zero) {
+    using namespace x6;
+    __shared__ double sU[2][US], sV[2][VS];
+    __shared__ double sC3[6][AZ];
+    int bx, by, bz;
+    if (!xcd_tile(nbx, nby, nbz, bx, by, bz)) return;
+    const int tx = threadIdx.x, ty = threadIdx.y, tid = tx + TX * ty;
+    const int x0 = r.lo0 + bx * TX, y0 = r.lo1 + by * TY;
+    const int x = x0 + tx, y = y0 + ty;
+    const int xe = r.lo0 + r.n0 - 1, ye = r.lo1 + r.n1 - 1;
+    const bool own = x <= xe && y <= ye;
+    int kc0, kc1;
+    red_chunk(r, bz, nbz, kc0, kc1);
+    const int64_t pxy = g.pxy;
+    for (int e = tid; e < 6 * AZ; e += NT) {
+        const int t = e / AZ, kk = kc0 + e % AZ;
+        sC3[t][e % AZ] = kk <= kc1 ? c3[t * g.cst[2] + kk] : 0.0;
+    }
+    double a1[6], a2[6];
+#pragma unroll
+    for (int t = 0; t < 6; ++t) {
+        a1[t] = own ? c1[t * g.cst[0] + x] : 0.0;
+        a2[t] = own ? c2[t * g.cst[1] + y] : 0.0;
+    }
+    const int64_t base = gidx(g, 0, 0, 0);
+    const int colo = (int)(gidx(g, x, y, 0) - base);
+    // u at the thread's column serves taps up to x+2 of own points, v up to y+2
+    const bool uown = x <= xe + 2 && y <= ye, vown = x <= xe && y <= ye + 2;
+    // halo: u at x0-3 .. x0-1 and x0+16, x0+17 (rows y0 ..), v likewise along y
+    int ho = -1, hs = 0, hf = 0;
+    if (tid < 2 * NH) {
+        hf = tid / NH;
+        const int e = tid % NH, a = e / 5, bq = e % 5;
+        const int off = bq < 3 ? bq : bq + TX;  // index from -3
+        if (hf == 0) {
+            const int hx = x0 - 3 + off, hy = y0 + a;
+            hs = a * UW + off;
+            if (hx <= xe + 2 && hy <= ye) ho = (int)(gidx(g, hx, hy, 0) - base);
+        } else {
+            const int hx = x0 + a, hy = y0 - 3 + off;
+            hs = off * TX + a;
+            if (hx <= xe && hy <= ye + 2) ho = (int)(gidx(g, hx, hy, 0) - base);
+        }
+    }
+    __syncthreads();
+    auto ld = [&](const double *f, int64_t o, bool ok) -> double { return *(ok ? f + base + o : zero); };
+    double wq[6];
+#pragma unroll
+    for (int s = 0; s < 6; ++s) wq[s] = ld(w, colo + (int64_t)(kc0 - 3 + s) * pxy, own);
+    double uo = ld(u, colo + (int64_t)kc0 * pxy, uown), vo = ld(v, colo + (int64_t)kc0 * pxy, vown);
+    double hv = ld(hf == 0 ? u : v, ho + (int64_t)kc0 * pxy, ho >= 0);
+    for (int k = kc0; k <= kc1; ++k) {
+        const int buf = (k - kc0) & 1, kz = k - kc0;
+        sU[buf][ty * UW + tx + 3] = uo;
+        sV[buf][(ty + 3) * TX + tx] = vo;
+        if (tid < 2 * NH) (hf == 0 ? sU[buf] : sV[buf])[hs] = hv;
+        __syncthreads();
+        const int kn = k + 1;
+        const bool vn = kn <= kc1;
+        uo = ld(u, colo + (int64_t)kn * pxy, vn && uown);
+        vo = ld(v, colo + (int64_t)kn * pxy, vn && vown);
+        hv = ld(hf == 0 ? u : v, ho + (int64_t)kn * pxy, vn && ho >= 0);
+        const double wnext = ld(w, colo + (int64_t)(k + 3) * pxy, own && k + 3 <= kc1 + 2);
+        if (own) {
+            const double *U = sU[buf] + ty * UW + tx;  // tap -3
+            const double *V = sV[buf] + ty * TX + tx;  // tap -3
+            const int64_t xo = base + colo + (int64_t)k * pxy;
+            if (MODE == XYZ_DIV) {
+                double s = a1[0] * U[0];
+#pragma unroll
+                for (int t = -2; t <= 2; ++t) s = s + a1[t + 3] * U[t + 3];
+#pragma unroll
+                for (int t = -3; t <= 2; ++t) s = s + a2[t + 3] * V[(t + 3) * TX];
+#pragma unroll
+                for (int t = -3; t <= 2; ++t) s = s + sC3[t + 3][kz] * wq[t + 3];
+                o1[xo] = s;
+            } else {
+                double s1 = a1[0] * U[0];
+#pragma unroll
+                for (int t = -2; t <= 2; ++t) s1 = s1 + a1[t + 3] * U[t + 3];
+                double s2 = a2[0] * V[0];
+#pragma unroll
+                for (int t = -2; t <= 2; ++t) s2 = s2 + a2[t + 3] * V[(t + 3) * TX];
+                double s3 = sC3[0][kz] * wq[0];
+#pragma unroll
+                for (int t = -2; t <= 2; ++t) s3 = s3 + sC3[t + 3][kz] * wq[t + 3];
+                o1[xo] = s1;
+                o2[xo] = s2;
+                o3[xo] = s3;
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < 5; ++s) wq[s] = wq[s + 1];
+        wq[5] = wnext;
+    }
 }
 
 // one (component, direction) pass of nonlinear, for device_Nonlinear_upwind
@@ -265,23 +583,6 @@ __global__ void k_scale_box(Geo g, Range3 r, double *f, double a, double b, int 
     IH_POINT(r);
     const int64_t x = gidx(g, i, j, k);
     f[x] = divide ? f[x] / a / b : f[x] * a * b;
-}
-
-// divergence2, src/mod_diff.f90:130-215
-__global__ void k_divergence2(Geo g, Coef c, Range3 r, const double *u, const double *v, const double *w,
-                              double *div) {
-    IH_POINT(r);
-    const int64_t x = gidx(g, i, j, k);
-    const int c0 = g.cst[0], c1 = g.cst[1], c2 = g.cst[2];
-    const double *d1 = c.D[0], *d2 = c.D[1], *d3 = c.D[2];
-    double s = d1[0 * c0 + i] * u[x - 3];
-#pragma unroll
-    for (int t = -2; t <= 2; ++t) s = s + d1[(t + 3) * c0 + i] * u[x + t];
-#pragma unroll
-    for (int t = -3; t <= 2; ++t) s = s + d2[(t + 3) * c1 + j] * v[x + t * (int64_t)g.px];
-#pragma unroll
-    for (int t = -3; t <= 2; ++t) s = s + d3[(t + 3) * c2 + k] * w[x + t * g.pxy];
-    div[x] = s;
 }
 
 // gradient, src/mod_diff.f90:309-414: 6 taps (-2..3) along m.  For m = 1 the
@@ -983,6 +1284,20 @@ __global__ void k_apply_mask(Geo g, Range3 r, int q, const uint8_t *mask, double
 using namespace dev;
 static inline Range3 R(const Box &b) { return make_range(b); }
 
+static void launch_xyz6(hipStream_t s, const Geo &g, const Range3 &r, int mode, const double *const cf[3],
+                        const double *u, const double *v, const double *w, double *o1, double *o2, double *o3,
+                        const double *zero) {
+    const int nbx = (r.n0 + x6::TX - 1) / x6::TX, nby = (r.n1 + x6::TY - 1) / x6::TY;
+    const int nbz = (r.n2 + x6::AZ - 1) / x6::AZ;
+    const dim3 grid(xcd_grid((int64_t)nbx * nby * nbz)), block(x6::TX, x6::TY, 1);
+    if (mode == XYZ_DIV)
+        hipLaunchKernelGGL(k_xyz6_t<XYZ_DIV>, grid, block, 0, s, g, r, nbx, nby, nbz, cf[0], cf[1], cf[2], u, v, w, o1,
+                           o2, o3, zero);
+    else
+        hipLaunchKernelGGL(k_xyz6_t<XYZ_INTERP>, grid, block, 0, s, g, r, nbx, nby, nbz, cf[0], cf[1], cf[2], u, v,
+                           w, o1, o2, o3, zero);
+}
+
 void launch_exchange(hipStream_t s, const Geo &g, int dir, int nf, double *const *f, const int *vel_dir) {
     const int d = dir - 1;
     const int ea = d == 0 ? 1 : 0;
@@ -1031,7 +1346,7 @@ void launch_interp_vel_pre3(hipStream_t s, const Geo &g, const Coef &c, const do
                             const double *w, double *w1, double *w2, double *w3) {
     Range3 r = R(pbox(g));
     if (empty(r)) return;
-    hipLaunchKernelGGL(k_interp_vel_pre3, grid_for(r), dim3(BX, BY, 1), 0, s, g, c, r, u, v, w, w1, w2, w3);
+    launch_xyz6(s, g, r, XYZ_INTERP, c.Iu, u, v, w, w1, w2, w3, c.zero);
 }
 
 void launch_interp_vel_pre(hipStream_t s, const Geo &g, const Coef &c, int q, const double *phi, double *inter) {
@@ -1046,13 +1361,49 @@ void launch_interp_pre_vel(hipStream_t s, const Geo &g, const Coef &c, int q, co
     hipLaunchKernelGGL(k_interp_pre_vel, grid_for(r), dim3(BX, BY, 1), 0, s, g, c, r, q, phi, inter);
 }
 
+// z chunks of k_nl_helm_rhs_t: at most nlh::AZ planes each, and, where the tile
+// columns leave room, as many as fill exactly one round of resident blocks
+// (the kernel is latency-bound, and a second, partial round measured slowest),
+// down to chunks of 8 planes
+static int nlh_chunks(int q, int columns, int n2) {
+    // looked up once (per kernel, per device); concurrent first calls store the same value
+    static std::atomic<int> per_cu[3] = {};
+    static std::atomic<int> cus[64] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
+    if (!cus[dev].load()) {
+        int n = 0;
+        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess) cus[dev].store(n);
+    }
+    if (!per_cu[q].load()) {
+        const void *fn = q == 0 ? (const void *)k_nl_helm_rhs_t<0>
+                                : (q == 1 ? (const void *)k_nl_helm_rhs_t<1> : (const void *)k_nl_helm_rhs_t<2>);
+        int n = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, nlh::NT, 0) == hipSuccess) per_cu[q].store(n);
+    }
+    // chunks of at least 8 planes: shorter ones pay the 7-plane queue prologue on little work
+    const int least = (n2 + nlh::AZ - 1) / nlh::AZ, most = (n2 + 7) / 8, nc = cus[dev].load(), pc = per_cu[q].load();
+    if (columns <= 0 || nc <= 0 || pc <= 0) return least;
+    return std::max(least, std::min(most, pc * nc / columns));
+}
+
 void launch_nl_helm_rhs(hipStream_t s, const Geo &g, const Coef &c, int q, const double *w1, const double *w2,
                         const double *w3, const double *vel, double *nl, double *rhs, double multL, double mult_a,
                         double mult_b, int use_b) {
     Range3 r = R(ibox(g, q));
     if (empty(r)) return;
-    hipLaunchKernelGGL(k_nl_helm_rhs, grid_for(r), dim3(BX, BY, 1), 0, s, g, c, r, q, w1, w2, w3, vel, nl, rhs,
-                       multL, mult_a, mult_b, use_b);
+    const int nbx = (r.n0 + nlh::TX - 1) / nlh::TX, nby = (r.n1 + nlh::TY - 1) / nlh::TY;
+    const int nbz = nlh_chunks(q, nbx * nby, r.n2);
+    const dim3 grid(xcd_grid((int64_t)nbx * nby * nbz)), block(nlh::TX, nlh::TY, 1);
+    if (q == 0)
+        hipLaunchKernelGGL(k_nl_helm_rhs_t<0>, grid, block, 0, s, g, c, r, nbx, nby, nbz, w1, w2, w3, vel, nl, rhs,
+                           multL, mult_a, mult_b, use_b);
+    else if (q == 1)
+        hipLaunchKernelGGL(k_nl_helm_rhs_t<1>, grid, block, 0, s, g, c, r, nbx, nby, nbz, w1, w2, w3, vel, nl, rhs,
+                           multL, mult_a, mult_b, use_b);
+    else
+        hipLaunchKernelGGL(k_nl_helm_rhs_t<2>, grid, block, 0, s, g, c, r, nbx, nby, nbz, w1, w2, w3, vel, nl, rhs,
+                           multL, mult_a, mult_b, use_b);
 }
 
 void launch_nonlinear_one(hipStream_t s, const Geo &g, const Coef &c, int q, int d, const double *pp,
@@ -1092,7 +1443,7 @@ void launch_divergence2(hipStream_t s, const Geo &g, const Coef &c, const double
                         const double *w, double *div) {
     Range3 r = R(pbox(g));
     if (empty(r)) return;
-    hipLaunchKernelGGL(k_divergence2, grid_for(r), dim3(BX, BY, 1), 0, s, g, c, r, u, v, w, div);
+    launch_xyz6(s, g, r, XYZ_DIV, c.D, u, v, w, div, nullptr, nullptr, c.zero);
 }
 
 void launch_gradient(hipStream_t s, const Geo &g, const Coef &c, int q, const double *phi, double *grad) {

@@ -393,7 +393,8 @@ def serving(ih, n_requests=100, rate=4.0, seed=5, budget_gb=288, max_active=8, s
     step = ih.StepParams(Re=RE, dtime=DT, jor_sweeps=JOR_SWEEPS, jor_omega=JOR_OMEGA, bicg_rel_tol=TOL,
                          bicg_max_it=MAXIT, level_pressure=1, inflow_kind=2, inflow_amp=1.0, use_mask=1,
                          jor_fp32=0, precond_sweeps=0, precond_poisson=1)
-    res, summ, rc = ih.serve(reqs, step, hbm_budget=budget_gb << 30, max_active=max_active)
+    res, summ, rc = ih.serve(reqs, step, hbm_budget=budget_gb << 30, max_active=max_active,
+                             progress_every=max(1, n_requests // 20))
     if rc != 0:
         raise RuntimeError(f"serving failed: {rc}")
     lat = np.array([r.latency_s for r in res])

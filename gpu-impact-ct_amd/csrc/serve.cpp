@@ -11,6 +11,7 @@
 // waits behind one of another class); its thread resets the slot's fields,
 // builds its mask and advances it on the slot's stream, so the kernels of the
 // admitted requests overlap on the device.  Built on the public C ABI only.
+#include <cstdio>
 #include <algorithm>
 #include <atomic>
 #include <chrono>
@@ -190,6 +191,11 @@ extern "C" int ih_serve(const ih_serve_config *cfg, const ih_request *req, int n
                 s.index = -1;
                 --running;
                 ++finished;
+                if (cfg->progress_every > 0 && finished % cfg->progress_every == 0) {
+                    std::fprintf(stderr, "ih_serve: %d/%d requests finished, %.1f s into the trace\n", finished, n,
+                                 seconds_since(t0));
+                    std::fflush(stderr);
+                }
             }
         const double now = seconds_since(t0);
         while (next < n && req[order[next]].arrival_s * scale <= now) {

@@ -99,7 +99,7 @@ class StepStats(C.Structure):
 
 class ServeConfig(C.Structure):
     _fields_ = [("device", C.c_int), ("hbm_budget", C.c_int64), ("max_active", C.c_int), ("time_scale", C.c_double),
-                ("step", StepParams)]
+                ("step", StepParams), ("progress_every", C.c_int)]
 
 
 class ServeSummary(C.Structure):
@@ -714,9 +714,12 @@ class Batch:
         return list(st) if stats else None
 
 
-def serve(requests, step: StepParams, hbm_budget: int, max_active: int, time_scale: float = 1.0, device: int = 0):
+def serve(requests, step: StepParams, hbm_budget: int, max_active: int, time_scale: float = 1.0, device: int = 0,
+          progress_every: int = 0):
     """Run a request trace (list of dicts with arrival_s, N, steps, radius,
-    stenosis) through ih_serve; returns (results, summary, status)."""
+    stenosis) through ih_serve; returns (results, summary, status).
+    progress_every > 0 prints a line on stderr after every that many finished
+    requests."""
     L = load()
     n = len(requests)
     req = (Request * max(n, 1))()
@@ -728,7 +731,7 @@ def serve(requests, step: StepParams, hbm_budget: int, max_active: int, time_sca
         req[q].stenosis = float(r["stenosis"])
     out = (RequestResult * max(n, 1))()
     cfg = ServeConfig(device=device, hbm_budget=int(hbm_budget), max_active=int(max_active),
-                      time_scale=float(time_scale), step=step)
+                      time_scale=float(time_scale), step=step, progress_every=int(progress_every))
     summ = ServeSummary()
     rc = L.ih_serve(C.byref(cfg), req, n, out, C.byref(summ))
     return list(out[:n]), summ, rc

@@ -437,6 +437,7 @@ typedef struct {
     int max_active;      /* simulations running at once */
     double time_scale;   /* arrival times are multiplied by this (1 = as given) */
     ih_step_params step;
+    int progress_every;  /* > 0: one line on stderr after every that many finished requests */
 } ih_serve_config;
 
 typedef struct {

@@ -16,8 +16,17 @@ from test_gpu_parity import GRIDS, MULTI, box, compare_state, pair, rnd, same, t
 
 pytestmark = pytest.mark.gpu
 
+# walls on every side at ragged sizes: rows of 70, 131 and 261 points, so the
+# sweeps' 16-byte vectors, halo vectors and wavefront overlaps fall at every
+# offset, in FP32 and FP64
+WALLED = [
+    pytest.param(((70, 9, 40), (1, 1, 1, 1, 1, 1), (2.0, 1.0, 1.0), (0.0, 1.3, 1.5)), id="walls-70x9x40"),
+    pytest.param(((131, 23, 15), (1, 1, 1, 1, 1, 1), (1.0, 1.0, 1.0), (0.0, 0.0, 1.2)), id="walls-131x23x15"),
+    pytest.param(((261, 9, 13), (1, 1, 1, 1, 1, 1), (2.0, 1.0, 1.0), (0.0, 1.1, 0.0)), id="walls-261x9x13"),
+]
 
-@pytest.mark.parametrize("spec", GRIDS + MULTI)
+
+@pytest.mark.parametrize("spec", GRIDS + MULTI + WALLED)
 def test_bicgstab_mixed(spec):
     oc, gc = pair(spec)
     xt = rnd(oc.shape, 190)
@@ -91,7 +100,7 @@ def test_mixed_against_fp64():
     assert 2 * n32 < n64, (n32, n64)
 
 
-@pytest.mark.parametrize("spec", GRIDS + MULTI)
+@pytest.mark.parametrize("spec", GRIDS + MULTI + WALLED)
 def test_bicgstab_jor64(spec):
     """The like-for-like FP64 baseline: the same right-preconditioned BiCGstab
     with M = FP64 JOR sweeps, bit for bit against the oracle."""

@@ -160,14 +160,18 @@ __device__ inline void wg_edges(const Geo &g, T *f1, T *f2, int three, int tid, 
             cc[p] = ip, cc[q] = iq, cc[t] = s;
             const int64_t pe = gidx(g, e[0], e[1], e[2]), pa = gidx(g, a[0], a[1], a[2]),
                           pb = gidx(g, b[0], b[1], b[2]), pc = gidx(g, cc[0], cc[1], cc[2]);
-            for (int w = 0; w < 2; ++w) {
-                T *f = w ? f2 : f1;
-                if (!f) continue;
-                if (three)
-                    f[pe] = (f[pa] + f[pb] + f[pc]) / T(3);
-                else
-                    f[pe] = (f[pa] + f[pb]) / T(2);
+            // both fields' loads before either store: the loads of a pair never read a
+            // point that the pair writes, and f1 and f2 are different fields
+            T s1 = T(0), s2 = T(0);
+            if (three) {
+                if (f1) s1 = (f1[pa] + f1[pb] + f1[pc]) / T(3);
+                if (f2) s2 = (f2[pa] + f2[pb] + f2[pc]) / T(3);
+            } else {
+                if (f1) s1 = (f1[pa] + f1[pb]) / T(2);
+                if (f2) s2 = (f2[pa] + f2[pb]) / T(2);
             }
+            if (f1) f1[pe] = s1;
+            if (f2) f2[pe] = s2;
         }
         __syncthreads();
     }
@@ -212,8 +216,12 @@ __global__ void k_mg_lap(Geo g, Coef c, Range3 r, const T *__restrict__ x, T *__
     pt_lap<T>(g, dg_of<T>(c), i, j, k, x, lap);
 }
 
+// one workgroup of MG_EDGE_THREADS: the longest line set of the bench grid
+// (4 x 257 points) takes two passes
+constexpr int MG_EDGE_THREADS = 1024;
+
 template <typename T>
-__global__ __launch_bounds__(256) void k_mg_edges(Geo g, T *f1, T *f2, int three, const int *active) {
+__global__ __launch_bounds__(MG_EDGE_THREADS) void k_mg_edges(Geo g, T *f1, T *f2, int three, const int *active) {
     if (active && !*active) return;
     wg_edges<T>(g, f1, f2, three, (int)threadIdx.x, (int)blockDim.x);
 }
@@ -478,10 +486,11 @@ void mg_cycle(hipStream_t s, const MgDev &m, double *v, double *z, const int *ac
         hipLaunchKernelGGL(k_mg_lap<T>, grid_for(pr), dim3(BX, BY, 1), 0, s, L.g, L.c, pr, x[g], B_::C(L), act);
         const bool walls = !(L.g.per[0] && L.g.per[1] && L.g.per[2]);
         if (walls) {
-            hipLaunchKernelGGL(k_mg_edges<T>, dim3(1), dim3(256), 0, s, L.g, b[g], B_::C(L), 0, act);
+            hipLaunchKernelGGL(k_mg_edges<T>, dim3(1), dim3(MG_EDGE_THREADS), 0, s, L.g, b[g], B_::C(L), 0, act);
             // the FP32 cycle works on a copy of v; v's edges change as the reference's do
             if (!fp64 && g == 1)
-                hipLaunchKernelGGL(k_mg_edges<double>, dim3(1), dim3(256), 0, s, L.g, v, (double *)nullptr, 0, act);
+                hipLaunchKernelGGL(k_mg_edges<double>, dim3(1), dim3(MG_EDGE_THREADS), 0, s, L.g, v, (double *)nullptr,
+                                   0, act);
         }
         LevelW<T> R;
         int nmax[3];
@@ -512,7 +521,7 @@ void mg_cycle(hipStream_t s, const MgDev &m, double *v, double *z, const int *ac
     for (int g = gl; g >= 1; --g) {
         const MgLevel &L = m.L[g], &Lc = m.L[g + 1];
         if (!(Lc.g.per[0] && Lc.g.per[1] && Lc.g.per[2]))
-            hipLaunchKernelGGL(k_mg_edges<T>, dim3(1), dim3(256), 0, s, Lc.g, x[g + 1], (T *)nullptr, 1, act);
+            hipLaunchKernelGGL(k_mg_edges<T>, dim3(1), dim3(MG_EDGE_THREADS), 0, s, Lc.g, x[g + 1], (T *)nullptr, 1, act);
         LevelW<T> W;
         for (int d = 0; d < 3; ++d) {
             W.w[d] = B_::cI(L)[d];

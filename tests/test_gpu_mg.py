@@ -27,6 +27,11 @@ SPECS = [
     pytest.param(((20, 20, 20), (1, 1, 1, 1, 1, 1), (1.0, 1.0, 1.0), (0.0, 0.0, 0.0)), id="one-level"),
     # level 2 (33x33x17) above the fused-kernel size: launched level by level, levels 3.. fused
     pytest.param(((65, 65, 33), (1, 1, -1, -1, 1, 1), (1.0, 1.0, 0.5), (0.0, 0.0, 1.5)), id="launched-levels"),
+    # edge lines of 4 x 257 points on level 1: more than one workgroup's threads, so the
+    # launched edge averaging takes a second pass over a line set.  x is the stretched
+    # direction: tanh stretching 1.5 over 9 points puts the wall velocity coordinates
+    # outside atanh's domain (NaN, in the reference's coord_tanh as here)
+    pytest.param(((257, 9, 9), (1, 1, 1, 1, 1, 1), (4.0, 1.0, 1.0), (1.5, 0.0, 0.0)), id="long-lines"),
 ]
 
 

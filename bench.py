@@ -43,7 +43,8 @@ JOR32_BYTES_PER_POINT = 12  # the same in FP32
 # write Ap); s = r - alpha Ap (read r, Ap; write r); As, <s, As>, <As, As> (read s;
 # write As); phi += alpha p + omega s, r = s - omega As, |r|, <r, rh> (read phi, p, s,
 # As, rh; write phi, r) -- 152 bytes an iteration
-KERNEL_BYTES = {"jor": 24, "p_update": 32, "apply_w": 24, "s_update": 24, "apply_self": 16, "x_update": 56}
+KERNEL_BYTES = {"jor": 24, "jor_f32": 12, "p_update": 32, "apply_w": 24, "s_update": 24, "apply_self": 16,
+                "x_update": 56}
 ITERATION_BYTES = 152
 HBM_CASE = (385, 193, 193)  # 384x192x192 cells: every kernel's working set exceeds the 256 MB last-level cache
 
@@ -621,7 +622,8 @@ def main():
     ap.add_argument("--no-hbm-case", action="store_true", help="skip the kernel table at 385x193x193")
     ap.add_argument("--precond-sweeps", type=int, default=16,
                     help="FP32 JOR sweeps of the mixed-precision BiCGstab preconditioner")
-    ap.add_argument("--only", choices=("jor", "apply", "apply_w", "apply_self", "p_update", "s_update", "x_update")
+    ap.add_argument("--only", choices=("jor", "jor_f32", "apply", "apply_w", "apply_self", "p_update", "s_update",
+                                       "x_update")
                     + tuple(STENCIL_BYTES),
                     default=None,
                     help="launch one kernel on the bench grid (for counter profiles), followed by one plain device "

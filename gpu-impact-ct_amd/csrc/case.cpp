@@ -2268,6 +2268,12 @@ int ih_bench_kernel(ih_case *c, const char *name, int launches, double *ms_per_l
             launch_jor(s, c->g, c->c, b, 0.8, F(c, IH_PRE), F(c, IH_RES), F(c, IH_COMP), l);
         });
         one = [&]() { launch_jor(s, c->g, c->c, b, 0.8, F(c, IH_PRE), F(c, IH_RES), F(c, IH_COMP), sh); };
+    } else if (n == "jor_f32") {
+        // one FP32 sweep of the mixed-precision preconditioner, on the case's float fields
+        const JorLaunch sh = jor_shape(c, b, true, [&](JorLaunch l) {
+            launch_jor_f32(s, c->g, c->c, b, 0.8f, c->f32[0], c->f32[1], c->f32[2], nullptr, l);
+        });
+        one = [&]() { launch_jor_f32(s, c->g, c->c, b, 0.8f, c->f32[0], c->f32[1], c->f32[2], nullptr, sh); };
     } else if (n == "apply") {
         one = [&]() {
             launch_apply_A(s, c->g, c->c, b, F(c, IH_PK), F(c, IH_AP), nullptr, KDOT_NONE, nullptr, nullptr, nullptr);
@@ -2302,7 +2308,7 @@ int ih_bench_kernel(ih_case *c, const char *name, int launches, double *ms_per_l
         one = [&]() { launch_krylov_part(s, c->g, c->c, b, k, part); };
     } else {
         return fail(IH_EINVAL,
-                    "unknown kernel '%s' (jor, apply, apply_w, apply_self, p_update, s_update, x_update, "
+                    "unknown kernel '%s' (jor, jor_f32, apply, apply_w, apply_self, p_update, s_update, x_update, "
                     "nl_helm_rhs0/1/2, divergence2, interp_vel_pre3, grad_correct0)",
                     name);
     }

@@ -474,11 +474,13 @@ int ih_timer_stop(ih_case *c, double *ms);
 int ih_kernel_timing_enable(ih_case *c, int on);
 int ih_kernel_timing_get(ih_case *c, const char *family, double *total_ms, int64_t *launches);
 /* `launches` launches of one kernel back to back between two events on the
- * case's stream: "jor" (an FP64 sweep over the pressure range), "apply" (the
- * fused operator alone), and the five kernels of a BiCGstab iteration --
- * "p_update", "apply_w" (operator with <rh, Ap>), "s_update", "apply_self"
- * (operator with <r, Ar>, <Ar, Ar>), "x_update" -- on the case's Krylov
- * fields (their results are meaningless). */
+ * case's stream: "jor" (an FP64 sweep over the pressure range), "jor_f32" (an
+ * FP32 sweep of the mixed-precision preconditioner, on the case's float
+ * fields), "apply" (the fused operator alone), the five kernels of a BiCGstab
+ * iteration -- "p_update", "apply_w" (operator with <rh, Ap>), "s_update",
+ * "apply_self" (operator with <r, Ar>, <Ar, Ar>), "x_update" -- on the case's
+ * Krylov fields (their results are meaningless), and the substep stencils
+ * "nl_helm_rhs0/1/2", "divergence2", "interp_vel_pre3", "grad_correct0". */
 int ih_bench_kernel(ih_case *c, const char *name, int launches, double *ms_per_launch);
 
 #ifdef __cplusplus

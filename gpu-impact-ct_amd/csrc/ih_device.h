@@ -88,6 +88,67 @@ __device__ inline double block_max(double v, double *sm) {
     return r;
 }
 
+// One thread's share of a reduction over nb partials: part[tid], part[tid + RED_THREADS],
+// ... in that order.  The loads run RED_AHEAD strides ahead of the adds (one
+// after another they would cost a memory latency per stride), and a slot past
+// nb is skipped rather than added as 0.0, so the result is the plain loop's bit
+// for bit.  thread_sum_parts2 forms two such sums with all loads first.
+constexpr int RED_AHEAD = 4;
+
+__device__ inline double thread_sum_parts(const double *part, int64_t nb, int tid) {
+    double acc = 0.0;
+    for (int64_t q0 = tid; q0 < nb; q0 += (int64_t)RED_AHEAD * RED_THREADS) {
+        double v[RED_AHEAD];
+#pragma unroll
+        for (int u = 0; u < RED_AHEAD; ++u) {
+            const int64_t q = q0 + (int64_t)u * RED_THREADS;
+            v[u] = q < nb ? part[q] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < RED_AHEAD; ++u)
+            if (q0 + (int64_t)u * RED_THREADS < nb) acc = acc + v[u];
+    }
+    return acc;
+}
+
+__device__ inline void thread_sum_parts2(const double *p1, const double *p2, int64_t nb, int tid, double &s1,
+                                         double &s2) {
+    double a1 = 0.0, a2 = 0.0;
+    for (int64_t q0 = tid; q0 < nb; q0 += (int64_t)RED_AHEAD * RED_THREADS) {
+        double v1[RED_AHEAD], v2[RED_AHEAD];
+#pragma unroll
+        for (int u = 0; u < RED_AHEAD; ++u) {
+            const int64_t q = q0 + (int64_t)u * RED_THREADS;
+            v1[u] = q < nb ? p1[q] : 0.0;
+            v2[u] = q < nb ? p2[q] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < RED_AHEAD; ++u)
+            if (q0 + (int64_t)u * RED_THREADS < nb) {
+                a1 = a1 + v1[u];
+                a2 = a2 + v2[u];
+            }
+    }
+    s1 = a1;
+    s2 = a2;
+}
+
+__device__ inline double thread_max_parts(const double *part, int64_t nb, int tid) {
+    double m = 0.0;
+    for (int64_t q0 = tid; q0 < nb; q0 += (int64_t)RED_AHEAD * RED_THREADS) {
+        double v[RED_AHEAD];
+#pragma unroll
+        for (int u = 0; u < RED_AHEAD; ++u) {
+            const int64_t q = q0 + (int64_t)u * RED_THREADS;
+            v[u] = q < nb ? part[q] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < RED_AHEAD; ++u)
+            if (q0 + (int64_t)u * RED_THREADS < nb) m = max_nan(v[u], m);
+    }
+    return m;
+}
+
 __device__ inline int64_t part_index() {
     return (int64_t)blockIdx.x + (int64_t)gridDim.x * ((int64_t)blockIdx.y + (int64_t)gridDim.y * blockIdx.z);
 }

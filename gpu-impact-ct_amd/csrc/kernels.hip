@@ -1125,18 +1125,14 @@ __global__ __launch_bounds__(256) void k_max_partial(Geo g, Range3 r, const doub
 __global__ __launch_bounds__(256) void k_final_sum(const double *part, int64_t nb, double *out) {
     __shared__ double sm[4];
     const int tid = threadIdx.x + threadIdx.y * blockDim.x;
-    double acc = 0.0;
-    for (int64_t q = tid; q < nb; q += RED_THREADS) acc = acc + part[q];
-    const double s = block_sum(acc, sm);
+    const double s = block_sum(thread_sum_parts(part, nb, tid), sm);
     if (tid == 0) *out = s;
 }
 
 __global__ __launch_bounds__(256) void k_final_max(const double *part, int64_t nb, double *out) {
     __shared__ double sm[4];
     const int tid = threadIdx.x + threadIdx.y * blockDim.x;
-    double m = 0.0;
-    for (int64_t q = tid; q < nb; q += RED_THREADS) m = max_nan(part[q], m);
-    const double s = block_max(m, sm);
+    const double s = block_max(thread_max_parts(part, nb, tid), sm);
     if (tid == 0) *out = s;
 }
 

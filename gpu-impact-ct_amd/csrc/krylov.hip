@@ -428,16 +428,12 @@ __global__ void k_p_update(Geo g, Range3 r, const double *rr, double *pk, const 
 
 __device__ inline double reduce_sum_parts(const double *part, int64_t nb, double *sm) {
     const int tid = threadIdx.x + threadIdx.y * blockDim.x;
-    double acc = 0.0;
-    for (int64_t q = tid; q < nb; q += RED_THREADS) acc = acc + part[q];
-    return block_sum(acc, sm);
+    return block_sum(thread_sum_parts(part, nb, tid), sm);
 }
 
 __device__ inline double reduce_max_parts(const double *part, int64_t nb, double *sm) {
     const int tid = threadIdx.x + threadIdx.y * blockDim.x;
-    double m = 0.0;
-    for (int64_t q = tid; q < nb; q += RED_THREADS) m = max_nan(part[q], m);
-    return block_max(m, sm);
+    return block_max(thread_max_parts(part, nb, tid), sm);
 }
 
 // status_iteration + the scalar part of the iteration head (mod_solvers.f90:1589-1666)
@@ -602,9 +598,11 @@ __device__ inline void x_update_block(const Geo &g, const Range3 &r, int bx, int
     __shared__ double s_om;
     __shared__ int s_go;
     if (!wg_active(fl, &s_go)) return;
-    const double rAr = reduce_sum_parts(p1, nb, sm);
+    double t1, t2;
+    thread_sum_parts2(p1, p2, nb, (int)(threadIdx.x + threadIdx.y * blockDim.x), t1, t2);
+    const double rAr = block_sum(t1, sm);
     __syncthreads();
-    const double ArAr = reduce_sum_parts(p2, nb, sm + 4);
+    const double ArAr = block_sum(t2, sm + 4);
     if (threadIdx.x == 0 && threadIdx.y == 0) {
         const bool ok = !(fabs(ArAr) == 0.0);
         const double om = ok ? rAr / ArAr : 0.0;
@@ -626,14 +624,45 @@ __device__ inline void x_update_block(const Geo &g, const Range3 &r, int bx, int
     if (i < r.lo0 + r.n0 && j < r.lo1 + r.n1) {
         int kc0, kc1;
         red_chunk(r, bz, nbz, kc0, kc1);
-        for (int k = kc0; k <= kc1; ++k) {
-            const int64_t x = gidx(g, i, j, k);
-            phi[x] = phi[x] + om * ra[x] + alpha * pa[x];
-            const double v = rr[x] - om * Ar[x];
-            rr[x] = v;
-            const double a = fabs(v);
-            m = max_nan(a, m);
-            d = d + v * rh[x];
+        // two register sets: the loads of plane k+1 are in flight while plane k is
+        // updated (a thread's planes are distinct elements, so no store changes a
+        // value loaded for a later plane; ra is rr itself without a preconditioner,
+        // and both then read the old r).  No memory access sits on a branch, so the
+        // waits can count exactly: past the chunk, the loads read its last plane
+        // again, and the one update past it (odd chunk lengths) rewrites that plane
+        // from the values loaded before its first update, the same bits again,
+        // while the norm and the dot skip it.
+        const int64_t x0 = gidx(g, i, j, kc0);
+        const int64_t pxy = g.pxy;
+        struct In {
+            double ph, ra, pa, rr, Ar, rh;
+        };
+        In A, B;
+        auto ld = [&](int k, In &v) {
+            const int64_t x = x0 + (int64_t)((k < kc1 ? k : kc1) - kc0) * pxy;
+            v.ph = phi[x];
+            v.ra = ra[x];
+            v.pa = pa[x];
+            v.rr = rr[x];
+            v.Ar = Ar[x];
+            v.rh = rh[x];
+        };
+        auto up = [&](int k, const In &v) {
+            const int64_t x = x0 + (int64_t)((k < kc1 ? k : kc1) - kc0) * pxy;
+            phi[x] = v.ph + om * v.ra + alpha * v.pa;
+            const double w = v.rr - om * v.Ar;
+            rr[x] = w;
+            const double a = fabs(w);
+            const bool in = k <= kc1;
+            m = in ? max_nan(a, m) : m;
+            d = in ? d + w * v.rh : d;
+        };
+        if (kc0 <= kc1) ld(kc0, A);
+        for (int k = kc0; k <= kc1; k += 2) {
+            ld(k + 1, B);
+            up(k, A);
+            ld(k + 2, A);
+            up(k + 1, B);
         }
     }
     const double bm = block_max(m, sm);

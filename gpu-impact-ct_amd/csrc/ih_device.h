@@ -133,6 +133,29 @@ __device__ inline void thread_sum_parts2(const double *p1, const double *p2, int
     s2 = a2;
 }
 
+// thread_max_parts of pm and thread_sum_parts of ps, with all loads first
+__device__ inline void thread_max_sum_parts(const double *pm, const double *ps, int64_t nb, int tid, double &m_out,
+                                            double &s_out) {
+    double m = 0.0, a = 0.0;
+    for (int64_t q0 = tid; q0 < nb; q0 += (int64_t)RED_AHEAD * RED_THREADS) {
+        double vm[RED_AHEAD], vs[RED_AHEAD];
+#pragma unroll
+        for (int u = 0; u < RED_AHEAD; ++u) {
+            const int64_t q = q0 + (int64_t)u * RED_THREADS;
+            vm[u] = q < nb ? pm[q] : 0.0;
+            vs[u] = q < nb ? ps[q] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < RED_AHEAD; ++u)
+            if (q0 + (int64_t)u * RED_THREADS < nb) {
+                m = max_nan(vm[u], m);
+                a = a + vs[u];
+            }
+    }
+    m_out = m;
+    s_out = a;
+}
+
 __device__ inline double thread_max_parts(const double *part, int64_t nb, int tid) {
     double m = 0.0;
     for (int64_t q0 = tid; q0 < nb; q0 += (int64_t)RED_AHEAD * RED_THREADS) {

@@ -686,9 +686,11 @@ __global__ __launch_bounds__(256) void k_ctl_end(const double *pmax, const doubl
                                                  int *fl) {
     if (!fl[KF_ACTIVE]) return;
     __shared__ double sm[8];
-    const double nm = reduce_max_parts(pmax, nb, sm);
+    double tm, ts;
+    thread_max_sum_parts(pmax, pdot, nb, (int)(threadIdx.x + threadIdx.y * blockDim.x), tm, ts);
+    const double nm = block_max(tm, sm);
     __syncthreads();
-    const double dt = reduce_sum_parts(pdot, nb, sm + 4);
+    const double dt = block_sum(ts, sm + 4);
     if (threadIdx.x == 0 && threadIdx.y == 0) {
         sc[KS_NORM] = nm;
         sc[KS_RHR_NEXT] = dt;
@@ -808,9 +810,11 @@ __global__ __launch_bounds__(256) void k_ctl_end_batch(const KBatchCase *__restr
     const KBatchCase &K = cases[blockIdx.x];
     if (!K.fl[KF_ACTIVE]) return;
     __shared__ double sm[8];
-    const double nm = reduce_max_parts(K.part3, K.nb, sm);
+    double tm, ts;
+    thread_max_sum_parts(K.part3, K.part4, K.nb, (int)(threadIdx.x + threadIdx.y * blockDim.x), tm, ts);
+    const double nm = block_max(tm, sm);
     __syncthreads();
-    const double dt = reduce_sum_parts(K.part4, K.nb, sm + 4);
+    const double dt = block_sum(ts, sm + 4);
     if (threadIdx.x == 0 && threadIdx.y == 0) {
         K.sc[KS_NORM] = nm;
         K.sc[KS_RHR_NEXT] = dt;

@@ -415,15 +415,47 @@ __global__ __launch_bounds__(256) void k_init_rr(Geo g, Range3 r, const double *
                   pmax, pdot);
 }
 
+// The first FP32 preconditioner sweep at one point, from the zero start: b32 =
+// (float) v and e = the JOR update with every neighbour 0, in the operands and
+// order of the sweep kernels (jor_f32_first_block), so the values are the same
+// bits.  The update kernels that produce v call it when the preconditioner is
+// FP32 sweeps (FIRST), which saves the first sweep's own pass over v.
+struct FirstSweep {
+    const float *g1, *g2, *g3;
+    float omega;
+    float *b32, *e;
+};
+
+__device__ inline void first_sweep_point(const Geo &g, const FirstSweep &f, int i, int j, int k, int64_t x,
+                                         double v) {
+    const int c0 = g.cst[0], c1 = g.cst[1], c2 = g.cst[2];
+    const float z = 0.0f;
+    const float bv = (float)v;
+    float s = bv;
+    s = s - f.g1[0 * c0 + i] * z;
+    s = s - f.g1[2 * c0 + i] * z;
+    s = s - f.g2[0 * c1 + j] * z;
+    s = s - f.g2[2 * c1 + j] * z;
+    s = s - f.g3[0 * c2 + k] * z;
+    s = s - f.g3[2 * c2 + k] * z;
+    const float diag = (f.g1[1 * c0 + i] + f.g2[1 * c1 + j]) + f.g3[1 * c2 + k];
+    f.b32[x] = bv;
+    f.e[x] = f.omega * s / diag + (1.0f - f.omega) * z;
+}
+
+template <bool FIRST>
 __global__ void k_p_update(Geo g, Range3 r, const double *rr, double *pk, const double *Ap, const double *sc,
-                           const int *fl) {
+                           const int *fl, FirstSweep fs) {
     if (!fl[KF_ACTIVE]) return;
     IH_POINT(r);
     const int64_t x = gidx(g, i, j, k);
+    double p;
     if (fl[KF_MODE])
-        pk[x] = rr[x] + sc[KS_BETA] * pk[x] + sc[KS_OM2] * Ap[x];
+        p = rr[x] + sc[KS_BETA] * pk[x] + sc[KS_OM2] * Ap[x];
     else
-        pk[x] = rr[x];
+        p = rr[x];
+    pk[x] = p;
+    if (FIRST) first_sweep_point(g, fs, i, j, k, x, p);
 }
 
 __device__ inline double reduce_sum_parts(const double *part, int64_t nb, double *sm) {
@@ -545,9 +577,10 @@ __device__ inline void breakdown(int *fl) {
 
 // alpha = rhr/<rh, Ap> (a vanishing <rh, Ap> is a breakdown), then rr = rr - alpha*Ap
 // (src/mod_solvers.f90:1709-1740)
+template <bool FIRST = false>
 __device__ inline void s_update_block(const Geo &g, const Range3 &r, int bx, int by, int bz, int nbz, int64_t pi,
                                       double *rr, const double *Ap, const double *part, int64_t nb, double *sc,
-                                      int *fl) {
+                                      int *fl, const FirstSweep &fs = FirstSweep{}) {
     __shared__ double sm[4];
     __shared__ double s_alpha;
     __shared__ int s_go;
@@ -575,14 +608,18 @@ __device__ inline void s_update_block(const Geo &g, const Range3 &r, int bx, int
     red_chunk(r, bz, nbz, kc0, kc1);
     for (int k = kc0; k <= kc1; ++k) {
         const int64_t x = gidx(g, i, j, k);
-        rr[x] = rr[x] - alpha * Ap[x];
+        const double v = rr[x] - alpha * Ap[x];
+        rr[x] = v;
+        if (FIRST) first_sweep_point(g, fs, i, j, k, x, v);
     }
 }
 
+template <bool FIRST>
 __global__ __launch_bounds__(256) void k_s_update_alpha(Geo g, Range3 r, double *rr, const double *Ap,
-                                                        const double *part, int64_t nb, double *sc, int *fl) {
-    s_update_block(g, r, (int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z, (int)gridDim.z, part_index(), rr, Ap,
-                   part, nb, sc, fl);
+                                                        const double *part, int64_t nb, double *sc, int *fl,
+                                                        FirstSweep fs) {
+    s_update_block<FIRST>(g, r, (int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z, (int)gridDim.z, part_index(), rr,
+                          Ap, part, nb, sc, fl, fs);
 }
 
 // omega = <r, Ar>/<Ar, Ar> (a vanishing <Ar, Ar> is a breakdown), then
@@ -931,9 +968,10 @@ void launch_krylov_init(hipStream_t s, const Geo &g, const Box &b, const double 
 
 // z = M v: `sweeps` FP32 JOR sweeps on A_cdg e = (float)v from e = 0, z = (double)e.
 // The first sweep reads v itself (and stores b = (float)v), the last writes z
-// widened to double, so M costs sweeps passes over the box and no more.
+// widened to double, so M costs sweeps passes over the box and no more.  With
+// first_done the kernel that wrote v has formed the first sweep already.
 static void launch_precond(hipStream_t s, const Geo &g, const Coef &c, const Box &b, double *v, double *z,
-                           const KrylovPtrs &k) {
+                           const KrylovPtrs &k, bool first_done = false) {
     const int *act = k.fl + KF_ACTIVE;
     if (k.mg) {
         // multigridV(.TRUE.,1,v,z) (src/mod_solvers.f90:1688, 1751)
@@ -958,7 +996,7 @@ static void launch_precond(hipStream_t s, const Geo &g, const Coef &c, const Box
         launch_jor_f32_first(s, g, c, b, k.omega, v, k.b32, nullptr, z, act);
         return;
     }
-    launch_jor_f32_first(s, g, c, b, k.omega, v, k.b32, k.e0, nullptr, act);
+    if (!first_done) launch_jor_f32_first(s, g, c, b, k.omega, v, k.b32, k.e0, nullptr, act);
     float *x = k.e0, *y = k.e1;
     for (int q = 1; q < k.sweeps - 1; ++q) {
         launch_jor_f32(s, g, c, b, k.omega, x, k.b32, y, act, k.shape);
@@ -971,13 +1009,25 @@ void launch_krylov_iteration(hipStream_t s, const Geo &g, const Coef &c, const B
     Range3 r = Rk(b);
     const int64_t nb = red_blocks(b);
     const bool pre = k.sweeps > 0 || k.sweeps64 > 0 || k.mg;
-    hipLaunchKernelGGL(k_p_update, grid_for(r), dim3(BX, BY, 1), 0, s, g, r, k.rr, k.pk, k.Ap, k.sc, k.fl);
-    if (pre) launch_precond(s, g, c, b, k.pk, k.z1, k);
+    // FP32 sweeps as M (two or more): the p and s updates also form M's first sweep
+    const bool fuse = !k.mg && k.sweeps64 == 0 && k.sweeps >= 2;
+    const FirstSweep fs{c.dgf[0], c.dgf[1], c.dgf[2], k.omega, k.b32, k.e0};
+    if (fuse)
+        hipLaunchKernelGGL(k_p_update<true>, grid_for(r), dim3(BX, BY, 1), 0, s, g, r, k.rr, k.pk, k.Ap, k.sc, k.fl,
+                           fs);
+    else
+        hipLaunchKernelGGL(k_p_update<false>, grid_for(r), dim3(BX, BY, 1), 0, s, g, r, k.rr, k.pk, k.Ap, k.sc, k.fl,
+                           FirstSweep{});
+    if (pre) launch_precond(s, g, c, b, k.pk, k.z1, k, fuse);
     launch_apply_A(s, g, c, b, pre ? k.z1 : k.pk, k.Ap, k.rh, DOT_W, k.part1, nullptr, k.fl + KF_ACTIVE);
-    hipLaunchKernelGGL(k_s_update_alpha, red_grid(r), dim3(RED_TX, RED_TY, 1), 0, s, g, r, k.rr, k.Ap, k.part1, nb,
-                       k.sc, k.fl);
+    if (fuse)
+        hipLaunchKernelGGL(k_s_update_alpha<true>, red_grid(r), dim3(RED_TX, RED_TY, 1), 0, s, g, r, k.rr, k.Ap,
+                           k.part1, nb, k.sc, k.fl, fs);
+    else
+        hipLaunchKernelGGL(k_s_update_alpha<false>, red_grid(r), dim3(RED_TX, RED_TY, 1), 0, s, g, r, k.rr, k.Ap,
+                           k.part1, nb, k.sc, k.fl, FirstSweep{});
     if (pre) {
-        launch_precond(s, g, c, b, k.rr, k.z2, k);
+        launch_precond(s, g, c, b, k.rr, k.z2, k, fuse);
         launch_apply_A(s, g, c, b, k.z2, k.Ar, k.rr, DOT_PAIR, k.part1, k.part2, k.fl + KF_ACTIVE);
     } else {
         launch_apply_A(s, g, c, b, k.rr, k.Ar, nullptr, DOT_SELF, k.part1, k.part2, k.fl + KF_ACTIVE);
@@ -993,14 +1043,15 @@ void launch_krylov_part(hipStream_t s, const Geo &g, const Coef &c, const Box &b
     const int64_t nb = red_blocks(b);
     switch (which) {
     case KPART_P_UPDATE:
-        hipLaunchKernelGGL(k_p_update, grid_for(r), dim3(BX, BY, 1), 0, s, g, r, k.rr, k.pk, k.Ap, k.sc, k.fl);
+        hipLaunchKernelGGL(k_p_update<false>, grid_for(r), dim3(BX, BY, 1), 0, s, g, r, k.rr, k.pk, k.Ap, k.sc, k.fl,
+                           FirstSweep{});
         break;
     case KPART_APPLY_W:
         launch_apply_A(s, g, c, b, k.pk, k.Ap, k.rh, DOT_W, k.part1, nullptr, k.fl + KF_ACTIVE);
         break;
     case KPART_S_UPDATE:
-        hipLaunchKernelGGL(k_s_update_alpha, red_grid(r), dim3(RED_TX, RED_TY, 1), 0, s, g, r, k.rr, k.Ap, k.part1,
-                           nb, k.sc, k.fl);
+        hipLaunchKernelGGL(k_s_update_alpha<false>, red_grid(r), dim3(RED_TX, RED_TY, 1), 0, s, g, r, k.rr, k.Ap,
+                           k.part1, nb, k.sc, k.fl, FirstSweep{});
         break;
     case KPART_APPLY_SELF:
         launch_apply_A(s, g, c, b, k.rr, k.Ar, nullptr, DOT_SELF, k.part1, k.part2, k.fl + KF_ACTIVE);

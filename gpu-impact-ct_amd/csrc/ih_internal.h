@@ -170,6 +170,12 @@ int launch_jor_batch(hipStream_t s, const JorBatchItem *items, const int *tile_i
                      int nitems, int row_tiles, int layout, int tpb, bool fp32, double omega, bool any_periodic_x);
 // the last FP32 sweep written widened to double, and the first one from e = 0
 // on A e = (float)v (stores b32; writes e, or e64 widened when e64 != nullptr)
+// the residual product of a multigrid level (product_div_grad_relax) over box b
+// with the JOR sweep's vector lanes, in FP64 or FP32 (cdg or its FP32 copy)
+void launch_lap(hipStream_t s, const Geo &g, const Coef &c, const Box &b, const double *x, double *lap,
+                const int *active, JorLaunch shape);
+void launch_lap(hipStream_t s, const Geo &g, const Coef &c, const Box &b, const float *x, float *lap,
+                const int *active, JorLaunch shape);
 void launch_jor_f32_to64(hipStream_t s, const Geo &g, const Coef &c, const Box &b, float omega, const float *rel,
                          const float *bb, double *out, const int *active, JorLaunch shape = JorLaunch());
 void launch_jor_f32_first(hipStream_t s, const Geo &g, const Coef &c, const Box &b, float omega, const double *v,

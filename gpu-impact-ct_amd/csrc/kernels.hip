@@ -756,8 +756,11 @@ constexpr int JOR_VEC_NB = 2;
 // not periodic, 1 periodic, 2 read from g at run time.  A run-time choice
 // leaves loads on one branch only inside the plane loop, and the compiler then
 // drains every load in flight once a trip; launches that know the periodicity
-// pass it as a constant.
-template <typename T, typename TO, int TPB, int PXM, typename PC, typename PO>
+// pass it as a constant.  LAP: instead of the sweep, the residual product of
+// the multigrid levels (product_div_grad_relax, pt_lap in mg.hip): the six
+// neighbour products in the order x-, x+, y-, y+, z-, z+, then + diag x; bb
+// and omega are unused.
+template <typename T, typename TO, int TPB, int PXM, typename PC, typename PO, bool LAP = false>
 __device__ inline void jor_vec_block(const Geo &g, const Range3 &r, const JorVecShape &sh, int tw, int bz, int nbz,
                                      T omega, PC __restrict__ rel, PC __restrict__ bb, PO __restrict__ out, PC g1,
                                      PC g2, PC g3) {
@@ -815,7 +818,10 @@ __device__ inline void jor_vec_block(const Geo &g, const Range3 &r, const JorVec
     auto load = [&](int k, Plane &P) {
         const int kk = k <= kc1 ? k : kc1;  // past the chunk: reload its last plane (never used)
         const int64_t x = xy + (int64_t)(kk + 3) * g.pxy;
-        P.b = vload<VT>(bb + x);
+        if (LAP)
+            P.b = VT{};
+        else
+            P.b = vload<VT>(bb + x);
         P.ym = vload<VT>(rel + x + dym);
         P.yp = vload<VT>(rel + x + dyp);
         P.zp = vload<VT>(rel + zidx(kk + 1));
@@ -847,6 +853,16 @@ __device__ inline void jor_vec_block(const Geo &g, const Range3 &r, const JorVec
         TO res[V];
 #pragma unroll
         for (int t = 0; t < V; ++t) {
+            if (LAP) {
+                T l = a1m[t] * xm[t];
+                l = l + a1p[t] * xp[t];
+                l = l + a2m * P.ym[t];
+                l = l + a2p * P.yp[t];
+                l = l + g3m * zm[t];
+                l = l + g3p * P.zp[t];
+                res[t] = (TO)(l + ((a1d[t] + a2d) + g3d) * zc[t]);
+                continue;
+            }
             T s = P.b[t];
             s = s - a1m[t] * xm[t];
             s = s - a1p[t] * xp[t];
@@ -896,6 +912,18 @@ __global__ __launch_bounds__(TPB) void k_jor_vec(Geo g, Coef c, Range3 r, JorVec
     const T *g2 = (const T *)(sizeof(T) == 8 ? (const void *)c.dg[1] : (const void *)c.dgf[1]);
     const T *g3 = (const T *)(sizeof(T) == 8 ? (const void *)c.dg[2] : (const void *)c.dgf[2]);
     jor_vec_block<T, TO, TPB, PXM, const T *, TO *>(g, r, sh, tw, bz, nbz, omega, rel, bb, out, g1, g2, g3);
+}
+
+template <typename T, int TPB, int PXM>
+__global__ __launch_bounds__(TPB) void k_lap_vec(Geo g, Coef c, Range3 r, JorVecShape sh, int nbw, int nbz,
+                                                 const T *__restrict__ x, T *__restrict__ lap, const int *active) {
+    if (active && !*active) return;
+    int tw, dummy, bz;
+    if (!xcd_tile(nbw, 1, nbz, tw, dummy, bz)) return;
+    const T *g1 = (const T *)(sizeof(T) == 8 ? (const void *)c.dg[0] : (const void *)c.dgf[0]);
+    const T *g2 = (const T *)(sizeof(T) == 8 ? (const void *)c.dg[1] : (const void *)c.dgf[1]);
+    const T *g3 = (const T *)(sizeof(T) == 8 ? (const void *)c.dg[2] : (const void *)c.dgf[2]);
+    jor_vec_block<T, T, TPB, PXM, const T *, T *, true>(g, r, sh, tw, bz, nbz, T(0), x, x, lap, g1, g2, g3);
 }
 
 // First FP32 sweep from e = 0 on A e = (float)v: b32 = (float)v is stored for
@@ -1534,6 +1562,39 @@ void launch_jor_f32(hipStream_t s, const Geo &g, const Coef &c, const Box &b, fl
 void launch_jor_f32_to64(hipStream_t s, const Geo &g, const Coef &c, const Box &b, float omega, const float *rel,
                          const float *bb, double *out, const int *active, JorLaunch shape) {
     launch_jor_vec<float, double>(s, g, c, b, omega, rel, bb, out, active, shape);
+}
+
+// the residual product of a multigrid level with the sweep's vector lanes and launch shape
+template <typename T>
+static void launch_lap_vec(hipStream_t s, const Geo &g, const Coef &c, const Box &b, const T *x, T *lap,
+                           const int *active, JorLaunch shape) {
+    Range3 r = R(b);
+    if (empty(r)) return;
+    const int tpb = shape.tpb == 128 || shape.tpb == 512 ? shape.tpb : JOR_VEC_TPB;
+    int nbw, nbz;
+    const JorVecShape sh = jor_vec_shape(g, r, 16 / (int)sizeof(T), tpb, shape.kc, nbw, nbz);
+    const dim3 grid(xcd_grid((int64_t)nbw * nbz));
+#define IH_LV(TPBV, PXV)                                                                                        \
+    hipLaunchKernelGGL((k_lap_vec<T, TPBV, PXV>), grid, dim3(TPBV), 0, s, g, c, r, sh, nbw, nbz, x, lap, active)
+    const bool px = g.per[0] != 0;
+    if (tpb == 128) {
+        if (px) IH_LV(128, 1); else IH_LV(128, 0);
+    } else if (tpb == 512) {
+        if (px) IH_LV(512, 1); else IH_LV(512, 0);
+    } else {
+        if (px) IH_LV(256, 1); else IH_LV(256, 0);
+    }
+#undef IH_LV
+}
+
+void launch_lap(hipStream_t s, const Geo &g, const Coef &c, const Box &b, const double *x, double *lap,
+                const int *active, JorLaunch shape) {
+    launch_lap_vec<double>(s, g, c, b, x, lap, active, shape);
+}
+
+void launch_lap(hipStream_t s, const Geo &g, const Coef &c, const Box &b, const float *x, float *lap,
+                const int *active, JorLaunch shape) {
+    launch_lap_vec<float>(s, g, c, b, x, lap, active, shape);
 }
 
 void launch_jor_f32_first(hipStream_t s, const Geo &g, const Coef &c, const Box &b, float omega, const double *v,

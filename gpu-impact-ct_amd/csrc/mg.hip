@@ -209,12 +209,6 @@ __global__ void k_mg_init_f32_first(Geo g, Coef c, Range3 r, float omega, const 
     pt_init<float>(g, c.dgf, i, j, k, omega, b32, x);
 }
 
-template <typename T>
-__global__ void k_mg_lap(Geo g, Coef c, Range3 r, const T *__restrict__ x, T *__restrict__ lap, const int *active) {
-    if (active && !*active) return;
-    IH_POINT(r);
-    pt_lap<T>(g, dg_of<T>(c), i, j, k, x, lap);
-}
 
 // one workgroup of MG_EDGE_THREADS: the longest line set of the bench grid
 // (4 x 257 points) takes two passes
@@ -482,8 +476,9 @@ void mg_cycle(hipStream_t s, const MgDev &m, double *v, double *z, const int *ac
     for (int g = 1; g <= gl; ++g) {
         const MgLevel &L = m.L[g], &Lc = m.L[g + 1];
         relax(g, true, m.n_down, true, g >= 2);
-        const Range3 pr = make_range(L.pr);
-        hipLaunchKernelGGL(k_mg_lap<T>, grid_for(pr), dim3(BX, BY, 1), 0, s, L.g, L.c, pr, x[g], B_::C(L), act);
+        // the residual product with the JOR sweep's 16-byte lanes and launch shape
+        // (one point a lane with seven 8-byte loads is bound by address issue)
+        launch_lap(s, L.g, L.c, L.pr, (const T *)x[g], B_::C(L), act, B_::shape(L));
         const bool walls = !(L.g.per[0] && L.g.per[1] && L.g.per[2]);
         if (walls) {
             hipLaunchKernelGGL(k_mg_edges<T>, dim3(1), dim3(MG_EDGE_THREADS), 0, s, L.g, b[g], B_::C(L), 0, act);

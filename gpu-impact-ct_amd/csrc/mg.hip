@@ -42,17 +42,26 @@ __device__ inline int64_t wix(const Geo &g, int i, int j, int k) {
 // ---- per-point steps (T: the arithmetic type of the cycle) -----------------
 
 // first relaxation of a level (r == 1 with init_yes): rel = omega bb / diag
+// The per-point steps take their field pointers and coefficient tables as
+// template types: plain pointers from the launched kernels, global-qualified
+// ones from the fused kernel, which reads them from its level table (as plain
+// pointers the compiler could not tell they address global memory and would
+// emit flat accesses, which wait on both the vector-memory and LDS counters).
 template <typename T>
-__device__ inline void pt_init(const Geo &g, const T *const *dg, int i, int j, int k, T omega, const T *bb, T *x) {
+using gptr = __attribute__((address_space(1))) T *;
+template <typename T>
+using gcptr = const __attribute__((address_space(1))) T *;
+
+template <typename T, typename TAB, typename PC, typename PM>
+__device__ inline void pt_init(const Geo &g, TAB dg, int i, int j, int k, T omega, PC bb, PM x) {
     const int64_t p = gidx(g, i, j, k);
     const T d = (dg[0][g.cst[0] + i] + dg[1][g.cst[1] + j]) + dg[2][g.cst[2] + k];
     x[p] = omega * bb[p] / d;
 }
 
 // one Jacobi sweep point (src/mod_laplace.f90:514-518), out of place
-template <typename T>
-__device__ inline void pt_jor(const Geo &g, const T *const *dg, int i, int j, int k, T omega, const T *x,
-                              const T *bb, T *y) {
+template <typename T, typename TAB, typename PC, typename PM>
+__device__ inline void pt_jor(const Geo &g, TAB dg, int i, int j, int k, T omega, PC x, PC bb, PM y) {
     const int c0 = g.cst[0], c1 = g.cst[1], c2 = g.cst[2];
     const int64_t p = gidx(g, i, j, k);
     T t = bb[p];
@@ -67,8 +76,8 @@ __device__ inline void pt_jor(const Geo &g, const T *const *dg, int i, int j, in
 }
 
 // product_div_grad_relax (src/mod_laplace.f90:270-280)
-template <typename T>
-__device__ inline void pt_lap(const Geo &g, const T *const *dg, int i, int j, int k, const T *x, T *lap) {
+template <typename T, typename TAB, typename PC, typename PM>
+__device__ inline void pt_lap(const Geo &g, TAB dg, int i, int j, int k, PC x, PM lap) {
     const int c0 = g.cst[0], c1 = g.cst[1], c2 = g.cst[2];
     T t = dg[0][i] * x[wix(g, i - 1, j, k)];
     t = t + dg[0][2 * c0 + i] * x[wix(g, i + 1, j, k)];
@@ -83,9 +92,9 @@ __device__ inline void pt_lap(const Geo &g, const T *const *dg, int i, int j, in
 
 // restrict(.TRUE., g, coarse, bb, lap) at coarse point (ii, jj, kk)
 // (src/mod_solvers.f90:699-714); w: the coarse level's weights
-template <typename T>
-__device__ inline T pt_restrict(const Geo &gf, const Geo &gc, const T *const *w, const int *dd, int ii, int jj,
-                                int kk, const T *bb, const T *lap, T *out) {
+template <typename T, typename TAB, typename PC, typename PM>
+__device__ inline T pt_restrict(const Geo &gf, const Geo &gc, TAB w, const int *dd, int ii, int jj, int kk, PC bb,
+                                PC lap, PM out) {
     const int i = dd[0] * (ii - 1) + 1, j = dd[1] * (jj - 1) + 1, k = dd[2] * (kk - 1) + 1;
     auto df = [&](int a, int b, int c) {
         const int64_t p = wix(gf, a, b, c);
@@ -108,9 +117,9 @@ __device__ inline T pt_restrict(const Geo &gf, const Geo &gc, const T *const *w,
 // (src/mod_solvers.f90:1046-1141): the injected coarse values, then the z, y
 // and x passes, evaluated with the operands each pass reads, added to fine;
 // w: the fine level's weights, dn: coarsening ratios
-template <typename T>
-__device__ inline void pt_interp(const Geo &gf, const Geo &gc, const T *const *w, const int *dn, int i, int j, int k,
-                                 const T *coarse, T *x) {
+template <typename T, typename TAB, typename PC, typename PM>
+__device__ inline void pt_interp(const Geo &gf, const Geo &gc, TAB w, const int *dn, int i, int j, int k, PC coarse,
+                                 PM x) {
     auto lat = [&](int d, int t) { return dn[d] == 1 || ((t - 1) & 1) == 0; };
     auto cix = [&](int d, int t) { return (t - 1) / dn[d] + 1; };
     auto C = [&](int I, int J, int K) { return coarse[wix(gc, I, J, K)]; };
@@ -141,8 +150,8 @@ __device__ inline void pt_interp(const Geo &gf, const Geo &gc, const T *const *w
 // when every N >= 3, so a pair is one parallel step; pairs are ordered by
 // barriers (a later pair reads corners an earlier one wrote).  Runs on one
 // workgroup of nthr threads (tid); f2 may be null.
-template <typename T>
-__device__ inline void wg_edges(const Geo &g, T *f1, T *f2, int three, int tid, int nthr) {
+template <typename T, typename PM = T *>
+__device__ inline void wg_edges(const Geo &g, PM f1, PM f2, int three, int tid, int nthr) {
     const int pairs[3][3] = {{0, 1, 2}, {0, 2, 1}, {1, 2, 0}};
     for (int pr = 0; pr < 3; ++pr) {
         const int p = pairs[pr][0], q = pairs[pr][1], t = pairs[pr][2];
@@ -316,11 +325,20 @@ __global__ __launch_bounds__(MG_FUSED_THREADS) void k_mg_fused(const MgLevelDev<
                                                                const int *active) {
     if (active && !*active) return;
     const int tid = (int)threadIdx.x;
-    T *x[MG_LEVELS_MAX + 1], *y[MG_LEVELS_MAX + 1];
+    gptr<T> x[MG_LEVELS_MAX + 1], y[MG_LEVELS_MAX + 1];
     for (int g = g0; g <= G; ++g) {
-        x[g] = L[g].B;
-        y[g] = L[g].D;
+        x[g] = (gptr<T>)L[g].B;
+        y[g] = (gptr<T>)L[g].D;
     }
+    // a level's coefficient tables as global pointers
+    struct Tab {
+        gcptr<T> t[3];
+    };
+    auto tab = [](const T *const *w) {
+        Tab r;
+        for (int d = 0; d < 3; ++d) r.t[d] = (gcptr<T>)w[d];
+        return r;
+    };
     auto range = [&](int g, int *lo, int *n) {
         for (int d = 0; d < 3; ++d) {
             lo[d] = L[g].g.Sp[d];
@@ -331,17 +349,19 @@ __global__ __launch_bounds__(MG_FUSED_THREADS) void k_mg_fused(const MgLevelDev<
         const MgLevelDev<T> &l = L[g];
         int lo[3], n[3];
         range(g, lo, n);
+        const Tab dg = tab(l.dg);
+        const gcptr<T> A = (gcptr<T>)l.A;
         int q = 1;
         if (init && nr >= 1) {
-            T *xx = x[g];
-            wg_points(lo, n, [&](int i, int j, int k) { pt_init<T>(l.g, l.dg, i, j, k, omega, l.A, xx); });
+            gptr<T> xx = x[g];
+            wg_points(lo, n, [&](int i, int j, int k) { pt_init<T>(l.g, dg.t, i, j, k, omega, A, xx); });
             q = 2;
         }
         for (; q <= nr; ++q) {
-            const T *xx = x[g];
-            T *yy = y[g];
-            wg_points(lo, n, [&](int i, int j, int k) { pt_jor<T>(l.g, l.dg, i, j, k, omega, xx, l.A, yy); });
-            T *tmp = x[g];
+            const gcptr<T> xx = x[g];
+            gptr<T> yy = y[g];
+            wg_points(lo, n, [&](int i, int j, int k) { pt_jor<T>(l.g, dg.t, i, j, k, omega, xx, A, yy); });
+            gptr<T> tmp = x[g];
             x[g] = y[g];
             y[g] = tmp;
         }
@@ -351,24 +371,27 @@ __global__ __launch_bounds__(MG_FUSED_THREADS) void k_mg_fused(const MgLevelDev<
         relax(g, true, n_down);
         int lo[3], n[3];
         range(g, lo, n);
-        const T *xx = x[g];
-        wg_points(lo, n, [&](int i, int j, int k) { pt_lap<T>(l.g, l.dg, i, j, k, xx, l.C); });
-        wg_edges<T>(l.g, l.A, l.C, 0, tid, MG_FUSED_THREADS);
+        const gcptr<T> xx = x[g];
+        const Tab dg = tab(l.dg), cR = tab(lc.cR);
+        const gptr<T> A = (gptr<T>)l.A, Cl = (gptr<T>)l.C, Ac = (gptr<T>)lc.A;
+        wg_points(lo, n, [&](int i, int j, int k) { pt_lap<T>(l.g, dg.t, i, j, k, xx, Cl); });
+        wg_edges<T, gptr<T>>(l.g, A, Cl, 0, tid, MG_FUSED_THREADS);
         int clo[3] = {1, 1, 1}, cn[3];
         for (int d = 0; d < 3; ++d) cn[d] = lc.g.N[d] - (l.g.per[d] ? 1 : 0);
         wg_points(clo, cn, [&](int ii, int jj, int kk) {
-            pt_restrict<T>(l.g, lc.g, lc.cR, l.dn, ii, jj, kk, l.A, l.C, lc.A);
+            pt_restrict<T>(l.g, lc.g, cR.t, l.dn, ii, jj, kk, (gcptr<T>)A, (gcptr<T>)Cl, Ac);
         });
     }
     relax(G, true, n_bottom);
     relax(G, false, n_bottom);
     for (int g = G - 1; g >= g0; --g) {
         const MgLevelDev<T> &l = L[g], &lc = L[g + 1];
-        wg_edges<T>(lc.g, x[g + 1], (T *)nullptr, 1, tid, MG_FUSED_THREADS);
+        wg_edges<T, gptr<T>>(lc.g, x[g + 1], (gptr<T>)nullptr, 1, tid, MG_FUSED_THREADS);
         int flo[3] = {1, 1, 1}, fn[3] = {l.g.N[0], l.g.N[1], l.g.N[2]};
-        const T *co = x[g + 1];
-        T *xx = x[g];
-        wg_points(flo, fn, [&](int i, int j, int k) { pt_interp<T>(l.g, lc.g, l.cI, l.dn, i, j, k, co, xx); });
+        const gcptr<T> co = x[g + 1];
+        gptr<T> xx = x[g];
+        const Tab cI = tab(l.cI);
+        wg_points(flo, fn, [&](int i, int j, int k) { pt_interp<T>(l.g, lc.g, cI.t, l.dn, i, j, k, co, xx); });
         relax(g, false, n_up);
     }
 }

@@ -2830,6 +2830,8 @@ int batch_step_tables(ih_batch *b, int sweeps) {
         K.part4 = c->part4;
         K.sc = c->ksc;
         K.fl = c->kfl;
+        K.b32 = sweeps > 0 ? c->f32[1] : nullptr;
+        K.e32 = sweeps > 0 ? c->f32[0] : nullptr;
         (g.per[2] ? b->k_pz : b->k_nonpz) = true;
         b->k_px = b->k_px || g.per[0] != 0;
         if (sweeps > 0) {
@@ -2921,10 +2923,12 @@ int batch_step_tables(ih_batch *b, int sweeps) {
 }
 
 // z = M v for every case (v = pk -> Z1 with w = 0, v = rr -> Z2 with w = 1): the
-// FP32 sweeps of launch_precond, batched
-void batch_precond(ih_batch *b, int w, int sweeps, float omega) {
+// FP32 sweeps of launch_precond, batched; first_done: the update that produced v
+// has already formed the first sweep
+void batch_precond(ih_batch *b, int w, int sweeps, float omega, bool first_done) {
     hipStream_t s = b->stream;
-    launch_jor_f32_first_batch(s, b->pfirst[w], b->pfirst_tile, b->pfirst_ntiles, omega, sweeps == 1);
+    if (!first_done)
+        launch_jor_f32_first_batch(s, b->pfirst[w], b->pfirst_tile, b->pfirst_ntiles, omega, sweeps == 1);
     if (sweeps < 2) return;
     for (int q = 1; q < sweeps - 1; ++q)
         launch_jor_batch_f32(s, b->pmid[(q - 1) & 1], b->pjor_tile, b->pjor_ntiles, omega, false, b->k_px);
@@ -2935,12 +2939,14 @@ void batch_precond(ih_batch *b, int w, int sweeps, float omega) {
 void batch_iteration(ih_batch *b, int sweeps, float omega) {
     hipStream_t s = b->stream;
     const bool pre = sweeps > 0;
-    launch_p_update_batch(s, b->kcases, b->ktile, b->kntiles);
-    if (pre) batch_precond(b, 0, sweeps, omega);
+    // two or more sweeps: the p and s updates also form M's first sweep
+    const bool fuse = sweeps >= 2;
+    launch_p_update_batch(s, b->kcases, b->ktile, b->kntiles, fuse, omega);
+    if (pre) batch_precond(b, 0, sweeps, omega, fuse);
     launch_apply_batch(s, b->kcases, b->ktile, b->kntiles, KB_APPLY_W, pre, b->k_pz, b->k_nonpz);
-    launch_s_update_batch(s, b->kcases, b->ktile, b->kntiles);
+    launch_s_update_batch(s, b->kcases, b->ktile, b->kntiles, fuse, omega);
     if (pre) {
-        batch_precond(b, 1, sweeps, omega);
+        batch_precond(b, 1, sweeps, omega, fuse);
         launch_apply_batch(s, b->kcases, b->ktile, b->kntiles, KB_APPLY_PAIR, pre, b->k_pz, b->k_nonpz);
     } else {
         launch_apply_batch(s, b->kcases, b->ktile, b->kntiles, KB_APPLY_SELF, pre, b->k_pz, b->k_nonpz);

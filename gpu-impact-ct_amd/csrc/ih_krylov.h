@@ -70,6 +70,7 @@ struct KBatchCase {
     double *part1, *part2, *part3, *part4;
     double *sc;
     int *fl;
+    float *b32, *e32;  // FP32 preconditioner scratch: the sweeps' right side and first iterate
 };
 // batched operator forms: A phi -> rr (initial residual, no dot), A (z1 or pk) -> Ap
 // with <Ap, rh>, A rr -> Ar with <rr, Ar>, <Ar, Ar>, A z2 -> Ar with <rr, Ar>, <Ar, Ar>
@@ -83,8 +84,12 @@ void launch_krylov_init_batch(hipStream_t s, const KBatchCase *cases, int ncases
                               double tol, int nmax);
 // one batched iteration: p update, [M pk], A, alpha/s update, [M rr], A, omega/x update,
 // head of the next iteration.  M (pre) is applied by the caller-provided functor
-void launch_p_update_batch(hipStream_t s, const KBatchCase *cases, const int *tile_case, int ntiles);
-void launch_s_update_batch(hipStream_t s, const KBatchCase *cases, const int *tile_case, int ntiles);
+// first: the update also forms the first FP32 preconditioner sweep of its result
+// into b32/e32 (M = two or more FP32 sweeps), as the single-case iteration does
+void launch_p_update_batch(hipStream_t s, const KBatchCase *cases, const int *tile_case, int ntiles, bool first,
+                           float omega);
+void launch_s_update_batch(hipStream_t s, const KBatchCase *cases, const int *tile_case, int ntiles, bool first,
+                           float omega);
 void launch_x_update_batch(hipStream_t s, const KBatchCase *cases, const int *tile_case, int ntiles, bool pre);
 void launch_ctl_end_batch(hipStream_t s, const KBatchCase *cases, int ncases);
 // fl_out[q*KF_NSLOTS + s] = case q's flag s, sc_out[q*KS_NSLOTS + s] its scalar s

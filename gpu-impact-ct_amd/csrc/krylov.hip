@@ -795,9 +795,16 @@ __global__ __launch_bounds__(RED_TX *RED_TY, 3) void k_apply_batch(const KBatchC
                                                 K.part2);
 }
 
-// pk = rr + beta*pk + om2*Ap (rr in the first iteration) over a reduction tile
+// the first-sweep operands of a batched case
+__device__ inline FirstSweep batch_first(const KBatchCase &K, float omega) {
+    return FirstSweep{K.c.dgf[0], K.c.dgf[1], K.c.dgf[2], omega, K.b32, K.e32};
+}
+
+// pk = rr + beta*pk + om2*Ap (rr in the first iteration) over a reduction tile;
+// FIRST: with M's first FP32 sweep of pk
+template <bool FIRST>
 __global__ __launch_bounds__(256) void k_p_update_batch(const KBatchCase *__restrict__ cases,
-                                                        const int *__restrict__ tile_case, int ntiles) {
+                                                        const int *__restrict__ tile_case, int ntiles, float omega) {
     int bx, by, bz;
     int64_t pi;
     const int q = batch_tile(cases, tile_case, ntiles, bx, by, bz, pi);
@@ -812,23 +819,25 @@ __global__ __launch_bounds__(256) void k_p_update_batch(const KBatchCase *__rest
     red_chunk(r, bz, K.nbz, kc0, kc1);
     const bool mode = K.fl[KF_MODE] != 0;
     const double beta = K.sc[KS_BETA], om2 = K.sc[KS_OM2];
+    const FirstSweep fs = batch_first(K, omega);
     for (int k = kc0; k <= kc1; ++k) {
         const int64_t x = gidx(K.g, i, j, k);
-        if (mode)
-            K.pk[x] = K.rr[x] + beta * K.pk[x] + om2 * K.Ap[x];
-        else
-            K.pk[x] = K.rr[x];
+        const double p = mode ? K.rr[x] + beta * K.pk[x] + om2 * K.Ap[x] : K.rr[x];
+        K.pk[x] = p;
+        if (FIRST) first_sweep_point(K.g, fs, i, j, k, x, p);
     }
 }
 
+template <bool FIRST>
 __global__ __launch_bounds__(256) void k_s_update_batch(const KBatchCase *__restrict__ cases,
-                                                        const int *__restrict__ tile_case, int ntiles) {
+                                                        const int *__restrict__ tile_case, int ntiles, float omega) {
     int bx, by, bz;
     int64_t pi;
     const int q = batch_tile(cases, tile_case, ntiles, bx, by, bz, pi);
     if (q < 0) return;
     const KBatchCase &K = cases[q];
-    s_update_block(K.g, K.r, bx, by, bz, K.nbz, pi, K.rr, K.Ap, K.part1, K.nb, K.sc, K.fl);
+    s_update_block<FIRST>(K.g, K.r, bx, by, bz, K.nbz, pi, K.rr, K.Ap, K.part1, K.nb, K.sc, K.fl,
+                          batch_first(K, omega));
 }
 
 __global__ __launch_bounds__(256) void k_x_update_batch(const KBatchCase *__restrict__ cases,
@@ -1097,16 +1106,26 @@ void launch_krylov_init_batch(hipStream_t s, const KBatchCase *cases, int ncases
     hipLaunchKernelGGL(k_ctl_setup_batch, dim3(ncases), dim3(RED_TX, RED_TY, 1), 0, s, cases, tol, nmax);
 }
 
-void launch_p_update_batch(hipStream_t s, const KBatchCase *cases, const int *tile_case, int ntiles) {
-    if (ntiles > 0)
-        hipLaunchKernelGGL(k_p_update_batch, dim3(xcd_grid(ntiles)), dim3(RED_TX, RED_TY, 1), 0, s, cases, tile_case,
-                           ntiles);
+void launch_p_update_batch(hipStream_t s, const KBatchCase *cases, const int *tile_case, int ntiles, bool first,
+                           float omega) {
+    if (ntiles <= 0) return;
+    if (first)
+        hipLaunchKernelGGL(k_p_update_batch<true>, dim3(xcd_grid(ntiles)), dim3(RED_TX, RED_TY, 1), 0, s, cases,
+                           tile_case, ntiles, omega);
+    else
+        hipLaunchKernelGGL(k_p_update_batch<false>, dim3(xcd_grid(ntiles)), dim3(RED_TX, RED_TY, 1), 0, s, cases,
+                           tile_case, ntiles, omega);
 }
 
-void launch_s_update_batch(hipStream_t s, const KBatchCase *cases, const int *tile_case, int ntiles) {
-    if (ntiles > 0)
-        hipLaunchKernelGGL(k_s_update_batch, dim3(xcd_grid(ntiles)), dim3(RED_TX, RED_TY, 1), 0, s, cases, tile_case,
-                           ntiles);
+void launch_s_update_batch(hipStream_t s, const KBatchCase *cases, const int *tile_case, int ntiles, bool first,
+                           float omega) {
+    if (ntiles <= 0) return;
+    if (first)
+        hipLaunchKernelGGL(k_s_update_batch<true>, dim3(xcd_grid(ntiles)), dim3(RED_TX, RED_TY, 1), 0, s, cases,
+                           tile_case, ntiles, omega);
+    else
+        hipLaunchKernelGGL(k_s_update_batch<false>, dim3(xcd_grid(ntiles)), dim3(RED_TX, RED_TY, 1), 0, s, cases,
+                           tile_case, ntiles, omega);
 }
 
 void launch_x_update_batch(hipStream_t s, const KBatchCase *cases, const int *tile_case, int ntiles, bool pre) {

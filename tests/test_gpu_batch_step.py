@@ -74,6 +74,31 @@ def test_batched_steps_equal_single_case_steps(mixed):
         c.close()
 
 
+@pytest.mark.parametrize("sweeps", [1, 2])
+def test_batched_preconditioner_sweep_counts(sweeps):
+    """M = 1 FP32 sweep (its own batched kernel) and 2 (the batched p and s
+    updates form the first sweep, no middle sweep): each case of the batch
+    bit-identical to its own step."""
+    kw = dict(Re=500.0, dtime=5e-4, jor_sweeps=20, jor_omega=0.8, bicg_rel_tol=1e-8, bicg_max_it=4000,
+              level_pressure=1, inflow_kind=2, inflow_amp=1.0, use_mask=1, jor_fp32=1, precond_sweeps=sweeps)
+    p = ih.StepParams(**kw)
+    spec = CASES[3:]
+    print(f"batch step: {len(spec)} cases, {sweeps} preconditioner sweeps", flush=True)
+    batched = [tube(N, R, s, bc) for N, R, s, bc in spec]
+    single = [tube(N, R, s, bc) for N, R, s, bc in spec]
+    bt = ih.Batch(batched)
+    sb = bt.timestep(p, 0.0)
+    for q, c in enumerate(single):
+        ss = c.timestep(p, 0.0)
+        assert list(sb[q].bicg_iterations) == list(ss.bicg_iterations), q
+        assert list(sb[q].bicg_final) == list(ss.bicg_final), q
+        for a, b in zip(_state(batched[q]), _state(c)):
+            assert np.array_equal(a, b), q
+    bt.close()
+    for c in batched + single:
+        c.close()
+
+
 def test_batched_step_matches_the_oracle():
     """the smallest tube of a batch against the oracle's step"""
     N, R, sten, bc = (33, 17, 17), 0.3, 0.5, (1,) * 6

@@ -43,6 +43,26 @@ def test_bicgstab_mixed(spec):
     same(gc.download(ih.PRE), phi, "mixed bicgstab phi")
 
 
+@pytest.mark.parametrize("sweeps", [1, 2, 5])
+def test_bicgstab_mixed_sweep_counts(sweeps):
+    """M = 1, 2 and 5 FP32 sweeps: one sweep is its own kernel; from two on the
+    p and s updates form the first sweep, with no middle sweep at two and an
+    odd number of them at five.  Bit for bit against the oracle."""
+    oc, gc = pair(WALLED[0].values[0])
+    xt = rnd(oc.shape, 195)
+    bb = oc.zeros()
+    oc.product_div_grad(xt.copy(), bb, oc.zeros())
+    phi = np.ascontiguousarray(0.1 * rnd(oc.shape, 196))
+    gc.upload(ih.RES, bb)
+    gc.upload(ih.PRE, phi)
+    eps = 1e-9 * np.abs(bb).max()
+    so = oc.bicgstab_mixed(eps, 3000, oc.Sp, oc.Np, bb, phi, sweeps, 0.8)
+    sg = gc.bicgstab_mixed(eps, 3000, oc.Sp, oc.Np, sweeps, 0.8)
+    assert so.iterations == sg.iterations and so.exit_reason == sg.exit_reason == 0
+    assert so.norm_inf_final == sg.norm_inf_final
+    same(gc.download(ih.PRE), phi, f"mixed bicgstab phi, {sweeps} sweeps")
+
+
 def _tube_case(mixed):
     spec = ((33, 17, 17), (1, 1, 1, 1, 1, 1), (2.0, 1.0, 1.0), (0.0, 1.5, 1.5))
     oc, gc = pair(spec)
